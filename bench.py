@@ -1,0 +1,199 @@
+"""Benchmark: LETKF local analyses per second on one MI355X.
+
+A step is one full analysis sweep over the synthetic workload -- obs binning,
+local selection, Yb^T R^-1 Yb / Yb^T R^-1 d contractions, eigensolve,
+transform and ensemble update for every grid column -- with all inputs
+resident in HBM.  Default workload: 0.5 deg global grid (720 x 360 columns, 32
+levels, n = 97 rows), k = 64 members, 400 000 observations (70 % in six
+Gaussian clusters of 800 km), R scaled 0.1-10x, L = 600 km, RTPS alpha = 0.8.
+
+Prints one JSON line.  `value` is the analyses/s of the headline form; the
+plain FP64 form is timed beside it, and the C oracle (the reference CPU path
+restated) is timed on a strided sample of columns on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "dida-thhpca-sc23_amd")
+for _p in (ROOT, PKG):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+METRIC = ("local analyses/s on 1 MI355X: mixed-prec batched vs plain FP64, at stated rel err")
+FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def var_rows(nz):
+    return {"ps": [0], "u": list(range(1, 1 + nz)), "v": list(range(1 + nz, 1 + 2 * nz)),
+            "T": list(range(1 + 2 * nz, 1 + 3 * nz))}
+
+
+def make_engine(dl, cfg, prm, form, gram, nobs):
+    n = 1 + 3 * cfg.nz
+    p = dl.default_params(nanals=cfg.k, nrow=n, nlat=cfg.nlat, nlon=cfg.nlon, form=form,
+                          gram=gram, **prm)
+    return dl.Engine(p, nobs_max=nobs), p
+
+
+def time_form(dl, torch, data, cfg, prm, form, gram, steps, warmup):
+    eng, p = make_engine(dl, cfg, prm, form, gram, cfg.nobs)
+    eng.set_grid(data["grid_lat"], data["grid_lon"])
+    obs = dl.DeviceObs(data["lat"], data["lon"], data["yb"], data["dep"], data["rinv"])
+    xa = torch.empty_like(data["xb"])
+    axa = torch.empty_like(data["axb"])
+    for _ in range(max(warmup, 1)):
+        eng.sweep(obs, data["xb"], data["axb"], xa, axa)
+    st = eng.stats()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        eng.sweep(obs, data["xb"], data["axb"], xa, axa)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / max(steps, 1)
+    # one more sweep with per-stage HIP events on the sweep's stream
+    eng.set_timing(True)
+    eng.sweep(obs, data["xb"], data["axb"], xa, axa)
+    stages = eng.stage_ms()
+    eng.set_timing(False)
+    torch.cuda.synchronize()
+    return {"engine": eng, "params": p, "xa": xa, "axa": axa, "ms": ms, "stats": st,
+            "stages": stages}
+
+
+def transform_flops(k, nobsl_sum, nanal):
+    """Algorithmic flops of the transform stage: Gram 2k^2 p + b 2kp per
+    analysis, eigensolve 9k^3 + W 2k^3 + wbar 4k^2 (nominal counts)."""
+    return 2.0 * k * k * nobsl_sum + 2.0 * k * nobsl_sum + nanal * (11.0 * k ** 3 + 4.0 * k * k)
+
+
+def update_bytes(k, n, ncol, s=8):
+    return ncol * (2.0 * k * n * s + 2.0 * n * 8)
+
+
+def cpu_baseline(dl, O, data, cfg, prm, target_s):
+    import numpy as np
+    ncol = cfg.nlat * cfg.nlon
+    n = 1 + 3 * cfg.nz
+    h = {key: data[key].cpu().numpy() for key in ("lat", "lon", "yb", "dep", "rinv",
+                                                   "grid_lat", "grid_lon")}
+    obs = O.Obs(h["lat"], h["lon"], h["yb"], h["dep"], h["rinv"])
+    bin_deg = 2.7
+    perm, bs, nb, nl = O.bin_obs(obs, bin_deg, 90.0)
+    ob = obs.permuted(perm)
+    op = O.params(cfg.k, n, cfg.nlat, cfg.nlon, prm["local_dist"], prm["inflation_factor"])
+    try:
+        threads = len(os.sched_getaffinity(0))
+    except AttributeError:
+        threads = os.cpu_count() or 1
+    threads = min(threads, 16)
+
+    def run(m):
+        stride = max(1, ncol // m)
+        cols = np.arange(0, ncol, stride, dtype=np.int64)[:m].astype(np.int32)
+        idx = data["xb"].new_tensor(cols, dtype=__import__("torch").long)
+        xb_c = data["xb"][idx].cpu().numpy()
+        axb_c = data["axb"][idx].cpu().numpy()
+        t0 = time.perf_counter()
+        O.sweep_compact(op, h["grid_lat"], h["grid_lon"], ob, (bs, nb, nl, bin_deg), cols, xb_c,
+                        axb_c, nthreads=threads)
+        return len(cols), time.perf_counter() - t0
+
+    m, dt = run(threads * 4)
+    m2 = int(min(ncol, max(m, m * target_s / max(dt, 1e-3))))
+    if m2 > m:
+        m, dt = run(m2)
+    return {"value": m / dt, "unit": "local analyses/s", "cores": threads, "kind": "port",
+            "sample": f"{m} of {ncol} columns (every {max(1, ncol // m)}th), "
+                      f"{dt:.1f} s, rate extrapolated to the full grid"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--scale", type=float, default=1.0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(the analysis shards by latitude bands across GPUs by design, but this "
+              "build measures one GPU's share)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import dida_letkf as dl
+    from oracle import oracle as O
+
+    torch.cuda.set_device(0)
+    cfg, prm = dl.config_synth(args.config, args.scale)
+    ncol = cfg.nlat * cfg.nlon
+    n = 1 + 3 * cfg.nz
+    log(f"generating workload: {cfg.nlat}x{cfg.nlon}x{cfg.nz} k={cfg.k} nobs={cfg.nobs}")
+    t0 = time.perf_counter()
+    data = dl.synth_generate(cfg)
+    log(f"  generated in {time.perf_counter() - t0:.1f} s")
+
+    forms = {"plain_fp64": (dl.FORM_PLAIN, dl.GRAM_FP64)}
+    res = {}
+    for name, (form, gram) in forms.items():
+        log(f"timing {name}")
+        res[name] = time_form(dl, torch, data, cfg, prm, form, gram, args.steps, args.warmup)
+        log(f"  {res[name]['ms']:.1f} ms/step, stages {res[name]['stages']}")
+    head = "plain_fp64"
+    r = res[head]
+    st = r["stats"]
+    ms = r["ms"]
+    value = ncol / (ms / 1e3)
+
+    tf_ms = r["stages"]["transform"]
+    flops = transform_flops(cfg.k, st["nobsl_sum"], ncol)
+    achieved = flops / (tf_ms / 1e3) / 1e12 if tf_ms > 0 else None
+    roofline = {"bound": "mfma", "kernel": "k_letkf_core_f64 (transform)",
+                "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / FP64_PEAK_TFLOPS if achieved else None, "traffic": None}
+
+    out_forms = {}
+    for name, rr in res.items():
+        out_forms[name] = {"ms_per_step": rr["ms"], "analyses_per_s": ncol / (rr["ms"] / 1e3),
+                           "stage_ms": rr["stages"],
+                           "update_GBps": update_bytes(cfg.k, n, ncol) / (rr["stages"]["update"] / 1e3) / 1e9
+                           if rr["stages"]["update"] > 0 else None}
+    cpu = None
+    if not args.no_cpu:
+        log("timing the CPU oracle on a column sample")
+        cpu = cpu_baseline(dl, O, data, cfg, prm, args.cpu_seconds)
+    line = {
+        "metric": METRIC, "value": value, "unit": "local analyses/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"LETKF global {cfg.nlat}x{cfg.nlon} columns x {cfg.nz} levels "
+                               f"(n={n}), k={cfg.k}, {cfg.nobs} obs (70% in 6 clusters), "
+                               f"L={prm['local_dist']} km, RTPS 0.8, MAX_OBS 10000",
+                   "form": head, "columns": ncol, "nobsl_sum": st["nobsl_sum"],
+                   "nobsl_max": st["nobsl_max"], "nobsl_capped": st["nobsl_capped"],
+                   "jacobi_max_sweeps_used": st["max_sweeps_used"],
+                   "nonconverged": st["nonconverged"]},
+        "roofline": roofline, "cpu_baseline": cpu, "forms": out_forms,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,79 @@
+// Device-side helpers shared by the LETKF kernels (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dida {
+
+constexpr int kWave = 64;
+
+// Single-precision literals of the reference, widened exactly to double:
+// pro_info.F90:148 (a = 6371.393) and :156 (pi = atan(1.0)*4 in REAL(4)),
+// letkf_yb.F90:256-261 (1.666667, 0.6666667).
+constexpr double kEarthA = (double)6371.393f;
+constexpr double kPiF32 = (double)3.14159274101257324f;
+constexpr double kGc53 = (double)1.666667f;
+constexpr double kGc23 = (double)0.6666667f;
+
+// Gaspari-Cohn weight of letkf_yb.F90:245-269 (schr2) or the boxcar of
+// :270-285 (schr), evaluated in the reference's operation order.
+__device__ __forceinline__ double gc_weight(double ri, int gc_filter) {
+#pragma clang fp contract(off)
+  if (!gc_filter) return ri > 2.0 ? 0.0 : 1.0;
+  double rci, rj, rk;
+  if (ri > 2.0) {
+    rci = 0.0;
+  } else if (ri > 1.0) {
+    rj = ri / 12.0 - 0.5;
+    rj = rj * ri + 0.625;
+    rj = rj * ri + kGc53;
+    rci = (rj * ri - 5.0) * ri + 4.0 - kGc23 / ri;
+  } else if (ri > 0.0) {
+    rk = ri * ri;
+    rj = (-0.25 * ri + 0.5) * ri + 0.625;
+    rci = (rj * ri - kGc53) * rk + 1.0;
+  } else {
+    rci = 1.0;
+  }
+  return rci;
+}
+
+// Degrees to the reference's radians (x/180*pi with the REAL(4) pi).
+__device__ __forceinline__ double deg2rad_ref(double x) {
+#pragma clang fp contract(off)
+  return x / 180.0 * kPiF32;
+}
+
+// Cosine of the central angle, as latlon_dist_r8 forms it
+// (pro_info.F90:157): cos(x1)*cos(x2)*cos(y1-y2) + sin(x1)*sin(x2).
+__device__ __forceinline__ double central_cos(double cx1, double sx1, double y1r,
+                                              double cx2, double sx2, double y2r) {
+#pragma clang fp contract(off)
+  double v = cx1 * cx2 * cos(y1r - y2r) + sx1 * sx2;
+  return v;
+}
+
+__device__ __forceinline__ double dist_from_cos(double v) {
+#pragma clang fp contract(off)
+  if (!(v < 1.0)) v = 1.0;
+  return kEarthA * acos(v);
+}
+
+// Wave-wide helpers ---------------------------------------------------------
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ double shfl_xor_d(double v, int mask) {
+  return __shfl_xor(v, mask, kWave);
+}
+
+// FP64 MFMA 16x16x4: D(16x16) += A(16x4) * B(4x16).
+// Operand fragments: lane l holds A[l%16][l/16] and B[l/16][l%16]; the result
+// holds D[4*(l/16)+r][l%16] in element r.
+typedef double double4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ double4_t mfma_f64_16x16x4(double a, double b, double4_t c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+}  // namespace dida

@@ -1,0 +1,663 @@
+/*
+ * Host layer of the LETKF engine, in C: the public C ABI of dida_letkf.h on
+ * top of the thin kernel ABI of letkf_kernels.h.  It owns device workspaces,
+ * orders the kernel stages of a sweep on the caller's stream, and implements
+ * the reference's Fortran-callable letkf_core_ / letkf_core_batch_.
+ */
+#define __HIP_PLATFORM_AMD__ 1
+#include <hip/hip_runtime_api.h>
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../../include/dida_letkf.h"
+#include "letkf_kernels.h"
+
+/* ------------------------------------------------------------------------ */
+static _Thread_local char g_err[512];
+
+static int set_err(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+const char *dida_letkf_last_error(void) { return g_err; }
+
+#define HIP_TRY(expr)                                                                    \
+    do {                                                                                 \
+        int e_ = (int)(expr);                                                            \
+        if (e_ != 0)                                                                     \
+            return set_err(DIDA_ERR_HIP, "%s failed: %s (%s:%d)", #expr,                 \
+                           hipGetErrorString((hipError_t)e_), __FILE__, __LINE__);       \
+    } while (0)
+
+static const double kEarthA = (double)6371.393f;
+
+void dida_letkf_params_default(dida_letkf_params *p)
+{
+    memset(p, 0, sizeof *p);
+    p->nanals = 0;
+    p->max_obs = 10000;
+    p->gc_filter = 1;
+    p->local_dist = 1000.0;
+    p->obs_polar = 90.0;
+    p->inflation_factor = 1.0;
+    p->rtps = 0;
+    p->rtps_alpha = 0.0;
+    p->form = DIDA_FORM_PLAIN;
+    p->gram = DIDA_GRAM_FP64;
+    p->jacobi_max_sweeps = 30;
+    p->jacobi_tol = 1e-13;
+    p->bin_deg = 0.0;
+}
+
+static int check_params(const dida_letkf_params *p)
+{
+    if (p->nanals < 2 || p->nanals > 64)
+        return set_err(DIDA_ERR_UNSUPPORTED, "nanals = %d: supported range is 2..64", p->nanals);
+    if (p->nrow < 1 || p->nlat < 1 || p->nlon < 1)
+        return set_err(DIDA_ERR_ARG, "grid %d x %d x %d rows is empty", p->nlat, p->nlon, p->nrow);
+    if (p->max_obs < 0) return set_err(DIDA_ERR_ARG, "max_obs < 0");
+    if (!(p->local_dist > 0.0)) return set_err(DIDA_ERR_ARG, "local_dist must be > 0");
+    if (!(p->inflation_factor > 0.0)) return set_err(DIDA_ERR_ARG, "inflation_factor must be > 0");
+    if (p->jacobi_max_sweeps < 1) return set_err(DIDA_ERR_ARG, "jacobi_max_sweeps < 1");
+    if (!(p->jacobi_tol > 0.0)) return set_err(DIDA_ERR_ARG, "jacobi_tol must be > 0");
+    if (p->form != DIDA_FORM_PLAIN)
+        return set_err(DIDA_ERR_UNSUPPORTED, "form %d is not available", p->form);
+    if (p->gram != DIDA_GRAM_FP64)
+        return set_err(DIDA_ERR_UNSUPPORTED, "gram precision %d is not available", p->gram);
+    return DIDA_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+#define DIDA_MAX_MARKS 160
+
+struct dida_letkf_ctx {
+    dida_letkf_params prm;
+    int32_t nobs_max;
+    int64_t ncol;
+    const float *grid_lat, *grid_lon;
+    dk_bin_geom geom;
+    int end_bit;
+    double theta_deg, cos_far;
+    /* binning */
+    uint32_t *key, *key_sorted;
+    int32_t *iota, *perm, *bin_count, *bin_start;
+    void *sort_tmp;
+    size_t sort_tmp_bytes;
+    double *cx, *sx, *yr, *yb_b, *dep_b;
+    float *rinv_b;
+    /* selection */
+    int32_t *nobsl;
+    int64_t *offset;
+    void *scan_tmp;
+    size_t scan_tmp_bytes;
+    int32_t *csr_idx;
+    double *csr_w;
+    int64_t capacity;
+    int32_t *flags; /* [0] capacity overflow */
+    /* transform (plain form, chunked) */
+    int32_t chunk;
+    double *trans;
+    int32_t *info;
+    /* per-stage HIP event timing (eager sweeps only) */
+    int timing;
+    int nmark;
+    hipEvent_t ev[DIDA_MAX_MARKS];
+    int ev_stage[DIDA_MAX_MARKS];
+    /* graph */
+    hipGraph_t graph;
+    hipGraphExec_t gexec;
+    hipStream_t last_stream;
+    int32_t last_nobs;
+};
+
+static void dfree(void *p)
+{
+    if (p) (void)hipFree(p);
+}
+
+int dida_letkf_destroy(dida_letkf_ctx *c)
+{
+    if (!c) return DIDA_OK;
+    for (int i = 0; i < DIDA_MAX_MARKS; ++i)
+        if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    void *ptrs[] = {c->key, c->key_sorted, c->iota, c->perm, c->bin_count, c->bin_start,
+                    c->sort_tmp, c->cx, c->sx, c->yr, c->yb_b, c->dep_b, c->rinv_b,
+                    c->nobsl, c->offset, c->scan_tmp, c->csr_idx, c->csr_w, c->flags,
+                    c->trans, c->info};
+    for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; ++i) dfree(ptrs[i]);
+    free(c);
+    return DIDA_OK;
+}
+
+#define DMALLOC(ptr, bytes)                                                              \
+    do {                                                                                 \
+        size_t b_ = (size_t)(bytes);                                                     \
+        if (hipMalloc((void **)&(ptr), b_ ? b_ : 16) != hipSuccess) {                    \
+            dida_letkf_destroy(c);                                                       \
+            return set_err(DIDA_ERR_NOMEM, "hipMalloc(%zu) failed for %s", b_, #ptr);    \
+        }                                                                                \
+    } while (0)
+
+int dida_letkf_create(dida_letkf_ctx **out, const dida_letkf_params *prm, int32_t nobs_max,
+                      int64_t sel_capacity)
+{
+    if (!out || !prm) return set_err(DIDA_ERR_ARG, "null argument");
+    *out = NULL;
+    int rc = check_params(prm);
+    if (rc) return rc;
+    if (nobs_max < 0 || sel_capacity < 0) return set_err(DIDA_ERR_ARG, "negative size");
+    dida_letkf_ctx *c = (dida_letkf_ctx *)calloc(1, sizeof *c);
+    if (!c) return set_err(DIDA_ERR_NOMEM, "calloc");
+    c->prm = *prm;
+    c->nobs_max = nobs_max;
+    c->ncol = (int64_t)prm->nlat * prm->nlon;
+    const int k = prm->nanals;
+
+    /* support radius in degrees, widened so that no kept obs falls outside
+     * the visited bins (the distance formula uses a float-rounded pi) */
+    const double pi = 3.14159265358979323846;
+    double theta_rad = 2.0 * prm->local_dist / kEarthA;
+    c->theta_deg = theta_rad * (180.0 / pi) * (1.0 + 1e-6) + 1e-6;
+    c->cos_far = (theta_rad * (1.0 + 1e-7) < pi) ? cos(theta_rad * (1.0 + 1e-7)) : -2.0;
+    double bd = prm->bin_deg;
+    if (!(bd > 0.0)) {
+        bd = c->theta_deg / 4.0;
+        if (bd < 0.25) bd = 0.25;
+        if (bd > 15.0) bd = 15.0;
+    }
+    c->geom.obs_polar = prm->obs_polar;
+    c->geom.nband = (int32_t)ceil(180.0 / bd);
+    c->geom.nlonbin = (int32_t)ceil(360.0 / bd);
+    c->geom.dlat = 180.0 / c->geom.nband;
+    c->geom.dlon = 360.0 / c->geom.nlonbin;
+    c->geom.nbins = c->geom.nband * c->geom.nlonbin;
+    c->end_bit = 1;
+    while ((1u << c->end_bit) <= (unsigned)c->geom.nbins) ++c->end_bit;
+
+    const size_t no = (size_t)(nobs_max > 0 ? nobs_max : 1);
+    DMALLOC(c->key, no * sizeof(uint32_t));
+    DMALLOC(c->key_sorted, no * sizeof(uint32_t));
+    DMALLOC(c->iota, no * sizeof(int32_t));
+    DMALLOC(c->perm, no * sizeof(int32_t));
+    DMALLOC(c->bin_count, (size_t)(c->geom.nbins + 1) * sizeof(int32_t));
+    DMALLOC(c->bin_start, (size_t)(c->geom.nbins + 2) * sizeof(int32_t));
+    c->sort_tmp_bytes = 0;
+    if (dk_sort_pairs_u32(NULL, &c->sort_tmp_bytes, c->key, c->key_sorted, c->iota, c->perm,
+                          (int32_t)no, c->end_bit, NULL)) {
+        dida_letkf_destroy(c);
+        return set_err(DIDA_ERR_HIP, "sort size query failed");
+    }
+    DMALLOC(c->sort_tmp, c->sort_tmp_bytes);
+    DMALLOC(c->cx, no * sizeof(double));
+    DMALLOC(c->sx, no * sizeof(double));
+    DMALLOC(c->yr, no * sizeof(double));
+    DMALLOC(c->yb_b, no * (size_t)k * sizeof(double));
+    DMALLOC(c->dep_b, no * sizeof(double));
+    DMALLOC(c->rinv_b, no * sizeof(float));
+
+    DMALLOC(c->nobsl, (size_t)c->ncol * sizeof(int32_t));
+    DMALLOC(c->offset, (size_t)(c->ncol + 1) * sizeof(int64_t));
+    dk_scan_offsets(NULL, &c->scan_tmp_bytes, NULL, NULL, (int32_t)c->ncol, NULL);
+    DMALLOC(c->scan_tmp, c->scan_tmp_bytes);
+    DMALLOC(c->flags, 4 * sizeof(int32_t));
+    c->capacity = sel_capacity;
+    if (sel_capacity > 0) {
+        DMALLOC(c->csr_idx, (size_t)sel_capacity * sizeof(int32_t));
+        DMALLOC(c->csr_w, (size_t)sel_capacity * sizeof(double));
+    }
+    c->chunk = (int32_t)(c->ncol < 32768 ? c->ncol : 32768);
+    DMALLOC(c->trans, (size_t)c->chunk * k * k * sizeof(double));
+    DMALLOC(c->info, (size_t)c->ncol * sizeof(int32_t));
+    *out = c;
+    return DIDA_OK;
+}
+
+int dida_letkf_set_grid(dida_letkf_ctx *c, const float *grid_lat, const float *grid_lon)
+{
+    if (!c || !grid_lat || !grid_lon) return set_err(DIDA_ERR_ARG, "null argument");
+    c->grid_lat = grid_lat;
+    c->grid_lon = grid_lon;
+    return DIDA_OK;
+}
+
+/* Record a timing mark closing `stage` (-1: the start mark). */
+static void tmark(dida_letkf_ctx *c, int stage, hipStream_t s)
+{
+    if (!c->timing || c->nmark >= DIDA_MAX_MARKS) return;
+    if (!c->ev[c->nmark] && hipEventCreate(&c->ev[c->nmark]) != hipSuccess) return;
+    c->ev_stage[c->nmark] = stage;
+    (void)hipEventRecord(c->ev[c->nmark], s);
+    c->nmark++;
+}
+
+/* Stages up to and including the count pass and the offset scan. */
+static int sweep_select_count(dida_letkf_ctx *c, const dida_letkf_obs *o, hipStream_t s,
+                              dk_select_args *sa)
+{
+    const int k = c->prm.nanals;
+    c->nmark = 0;
+    tmark(c, -1, s);
+    HIP_TRY(dk_fill_i32(c->bin_count, 0, c->geom.nbins + 1, s));
+    HIP_TRY(dk_fill_i32(c->flags, 0, 4, s));
+    HIP_TRY(dk_bin_keys(o->nobs, o->lat, o->lon, c->geom, c->key, c->iota, c->bin_count, s));
+    if (o->nobs > 0) {
+        size_t tb = c->sort_tmp_bytes;
+        HIP_TRY(dk_sort_pairs_u32(c->sort_tmp, &tb, c->key, c->key_sorted, c->iota, c->perm,
+                                  o->nobs, c->end_bit, s));
+    }
+    HIP_TRY(dk_bin_scan(c->geom.nbins, c->bin_count, c->bin_start, s));
+    HIP_TRY(dk_obs_gather(o->nobs, k, c->perm, o->lat, o->lon, o->yb, o->dep, o->rinv, c->cx,
+                          c->sx, c->yr, c->yb_b, c->dep_b, c->rinv_b, s));
+    tmark(c, DIDA_STAGE_BIN, s);
+    memset(sa, 0, sizeof *sa);
+    sa->nlat = c->prm.nlat;
+    sa->nlon = c->prm.nlon;
+    sa->grid_lat = c->grid_lat;
+    sa->grid_lon = c->grid_lon;
+    sa->geom = c->geom;
+    sa->obs.cx = c->cx;
+    sa->obs.sx = c->sx;
+    sa->obs.yr = c->yr;
+    sa->obs.yb = c->yb_b;
+    sa->obs.dep = c->dep_b;
+    sa->obs.rinv = c->rinv_b;
+    sa->obs.bin_start = c->bin_start;
+    sa->local_dist = c->prm.local_dist;
+    sa->theta_deg = c->theta_deg;
+    sa->cos_far = c->cos_far;
+    sa->gc_filter = c->prm.gc_filter;
+    sa->max_obs = c->prm.max_obs;
+    sa->nobsl = c->nobsl;
+    sa->offset = c->offset;
+    sa->overflow = c->flags;
+    HIP_TRY(dk_select_count(sa, s));
+    size_t tb = c->scan_tmp_bytes;
+    HIP_TRY(dk_scan_offsets(c->scan_tmp, &tb, c->nobsl, c->offset, (int32_t)c->ncol, s));
+    return DIDA_OK;
+}
+
+static int ensure_capacity(dida_letkf_ctx *c, hipStream_t s)
+{
+    if (c->capacity > 0) return DIDA_OK;
+    int64_t total = 0;
+    HIP_TRY(hipMemcpyAsync(&total, c->offset + c->ncol, sizeof total, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    int64_t cap = total + total / 8 + 1024;
+    if (hipMalloc((void **)&c->csr_idx, (size_t)cap * sizeof(int32_t)) != hipSuccess ||
+        hipMalloc((void **)&c->csr_w, (size_t)cap * sizeof(double)) != hipSuccess)
+        return set_err(DIDA_ERR_NOMEM, "selection workspace of %lld entries", (long long)cap);
+    c->capacity = cap;
+    return DIDA_OK;
+}
+
+static int sweep_rest(dida_letkf_ctx *c, dk_select_args *sa, const double *xb,
+                      const double *axb, double *xa, double *axa, hipStream_t s)
+{
+    const int k = c->prm.nanals;
+    sa->csr_idx = c->csr_idx;
+    sa->csr_w = c->csr_w;
+    sa->capacity = c->capacity;
+    HIP_TRY(dk_select_fill(sa, s));
+    tmark(c, DIDA_STAGE_SELECT, s);
+
+    dk_core_args ca;
+    memset(&ca, 0, sizeof ca);
+    ca.k = k;
+    ca.offset = c->offset;
+    ca.nobsl = c->nobsl;
+    ca.idx = c->csr_idx;
+    ca.rloc = c->csr_w;
+    ca.capacity = c->capacity;
+    ca.max_p = c->prm.max_obs;
+    ca.yb = c->yb_b;
+    ca.rinv = c->rinv_b;
+    ca.dep = c->dep_b;
+    ca.rho = c->prm.inflation_factor;
+    ca.tol = c->prm.jacobi_tol;
+    ca.max_sweeps = c->prm.jacobi_max_sweeps;
+    ca.trans = c->trans;
+    ca.wbar = NULL;
+    ca.info = c->info;
+    dk_update_args ua;
+    memset(&ua, 0, sizeof ua);
+    ua.k = k;
+    ua.nrow = c->prm.nrow;
+    ua.trans = c->trans;
+    ua.xb = xb;
+    ua.axb = axb;
+    ua.xa = xa;
+    ua.axa = axa;
+    ua.rtps = c->prm.rtps;
+    ua.rtps_alpha = c->prm.rtps_alpha;
+    for (int64_t c0 = 0; c0 < c->ncol; c0 += c->chunk) {
+        int32_t n = (int32_t)((c->ncol - c0) < c->chunk ? (c->ncol - c0) : c->chunk);
+        ca.aid0 = (int32_t)c0;
+        ca.nanal = n;
+        HIP_TRY(dk_letkf_core_f64(&ca, s));
+        tmark(c, DIDA_STAGE_TRANSFORM, s);
+        ua.col0 = (int32_t)c0;
+        ua.ncol = n;
+        HIP_TRY(dk_update_f64(&ua, s));
+        tmark(c, DIDA_STAGE_UPDATE, s);
+    }
+    return DIDA_OK;
+}
+
+static int check_sweep_args(dida_letkf_ctx *c, const dida_letkf_obs *o, const double *xb,
+                            const double *axb, double *xa, double *axa)
+{
+    if (!c || !o || !xb || !axb || !xa || !axa) return set_err(DIDA_ERR_ARG, "null argument");
+    if (!c->grid_lat) return set_err(DIDA_ERR_ARG, "grid not set (dida_letkf_set_grid)");
+    if (o->nobs < 0 || o->nobs > c->nobs_max)
+        return set_err(DIDA_ERR_ARG, "nobs = %d outside [0, %d]", o->nobs, c->nobs_max);
+    if (o->nobs > 0 && (!o->lat || !o->lon || !o->yb || !o->dep || !o->rinv))
+        return set_err(DIDA_ERR_ARG, "null obs array");
+    return DIDA_OK;
+}
+
+int dida_letkf_sweep(dida_letkf_ctx *c, const dida_letkf_obs *o, const double *xb,
+                     const double *axb, double *xa, double *axa, void *stream)
+{
+    int rc = check_sweep_args(c, o, xb, axb, xa, axa);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    dk_select_args sa;
+    rc = sweep_select_count(c, o, s, &sa);
+    if (rc) return rc;
+    rc = ensure_capacity(c, s);
+    if (rc) return rc;
+    rc = sweep_rest(c, &sa, xb, axb, xa, axa, s);
+    c->last_stream = s;
+    c->last_nobs = o->nobs;
+    return rc;
+}
+
+int dida_letkf_graph_capture(dida_letkf_ctx *c, const dida_letkf_obs *o, const double *xb,
+                             const double *axb, double *xa, double *axa, void *stream)
+{
+    int rc = check_sweep_args(c, o, xb, axb, xa, axa);
+    if (rc) return rc;
+    if (c->capacity <= 0)
+        return set_err(DIDA_ERR_ARG, "run one sweep (or give sel_capacity) before capture");
+    hipStream_t s = (hipStream_t)stream;
+    if (c->gexec) {
+        (void)hipGraphExecDestroy(c->gexec);
+        c->gexec = NULL;
+    }
+    if (c->graph) {
+        (void)hipGraphDestroy(c->graph);
+        c->graph = NULL;
+    }
+    HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const int timing = c->timing;
+    c->timing = 0;
+    dk_select_args sa;
+    rc = sweep_select_count(c, o, s, &sa);
+    if (!rc) rc = sweep_rest(c, &sa, xb, axb, xa, axa, s);
+    c->timing = timing;
+    hipGraph_t g = NULL;
+    int e = (int)hipStreamEndCapture(s, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e) return set_err(DIDA_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString((hipError_t)e));
+    c->graph = g;
+    HIP_TRY(hipGraphInstantiate(&c->gexec, g, NULL, NULL, 0));
+    c->last_nobs = o->nobs;
+    return DIDA_OK;
+}
+
+int dida_letkf_graph_launch(dida_letkf_ctx *c, void *stream)
+{
+    if (!c || !c->gexec) return set_err(DIDA_ERR_ARG, "no captured graph");
+    HIP_TRY(hipGraphLaunch(c->gexec, (hipStream_t)stream));
+    c->last_stream = (hipStream_t)stream;
+    return DIDA_OK;
+}
+
+int dida_letkf_set_timing(dida_letkf_ctx *c, int32_t on)
+{
+    if (!c) return set_err(DIDA_ERR_ARG, "null context");
+    c->timing = on ? 1 : 0;
+    c->nmark = 0;
+    return DIDA_OK;
+}
+
+int dida_letkf_stage_ms(dida_letkf_ctx *c, double ms[DIDA_NSTAGES])
+{
+    if (!c || !ms) return set_err(DIDA_ERR_ARG, "null argument");
+    for (int i = 0; i < DIDA_NSTAGES; ++i) ms[i] = 0.0;
+    if (c->nmark < 2) return DIDA_OK;
+    HIP_TRY(hipEventSynchronize(c->ev[c->nmark - 1]));
+    for (int i = 1; i < c->nmark; ++i) {
+        float t = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&t, c->ev[i - 1], c->ev[i]));
+        int st = c->ev_stage[i];
+        if (st >= 0 && st < DIDA_NSTAGES) ms[st] += t;
+    }
+    return DIDA_OK;
+}
+
+int dida_letkf_stats_get(dida_letkf_ctx *c, dida_letkf_stats *st)
+{
+    if (!c || !st) return set_err(DIDA_ERR_ARG, "null argument");
+    memset(st, 0, sizeof *st);
+    HIP_TRY(hipStreamSynchronize(c->last_stream));
+    size_t n = (size_t)c->ncol;
+    int32_t *nb = (int32_t *)malloc(n * sizeof(int32_t));
+    int32_t *inf = (int32_t *)malloc(n * sizeof(int32_t));
+    int32_t flags[4];
+    if (!nb || !inf) {
+        free(nb);
+        free(inf);
+        return set_err(DIDA_ERR_NOMEM, "host buffers");
+    }
+    int rc = DIDA_OK;
+    if (hipMemcpy(nb, c->nobsl, n * sizeof(int32_t), hipMemcpyDeviceToHost) ||
+        hipMemcpy(inf, c->info, n * sizeof(int32_t), hipMemcpyDeviceToHost) ||
+        hipMemcpy(flags, c->flags, sizeof flags, hipMemcpyDeviceToHost)) {
+        rc = set_err(DIDA_ERR_HIP, "stats copy failed");
+    } else {
+        st->nanal = (int64_t)n;
+        for (size_t i = 0; i < n; ++i) {
+            st->nobsl_sum += nb[i];
+            if (nb[i] > st->nobsl_max) st->nobsl_max = nb[i];
+            if (nb[i] >= c->prm.max_obs) st->nobsl_capped++;
+            if (inf[i] < 0) st->nonconverged++;
+            else if (inf[i] > st->max_sweeps_used) st->max_sweeps_used = inf[i];
+        }
+        st->capacity_overflow = flags[0];
+        if (flags[0]) rc = set_err(DIDA_ERR_CAPACITY, "selection lists exceeded the workspace");
+        else if (st->nonconverged) rc = DIDA_WARN_NOCONV;
+    }
+    free(nb);
+    free(inf);
+    return rc;
+}
+
+int dida_letkf_debug_arrays(dida_letkf_ctx *c, const int32_t **nobsl, const int64_t **offset,
+                            const int32_t **csr_idx, const double **csr_w, const int32_t **perm,
+                            const int32_t **info)
+{
+    if (!c) return set_err(DIDA_ERR_ARG, "null context");
+    if (nobsl) *nobsl = c->nobsl;
+    if (offset) *offset = c->offset;
+    if (csr_idx) *csr_idx = c->csr_idx;
+    if (csr_w) *csr_w = c->csr_w;
+    if (perm) *perm = c->perm;
+    if (info) *info = c->info;
+    return DIDA_OK;
+}
+
+int dida_letkf_post(int32_t k, int32_t nrow, int32_t nlat, int32_t nlon, const float *grid_lat,
+                    double polar_lat, double gamma1, double gamma2, const double *xb,
+                    const double *axb, const double *xa, const double *axa, double *xa_out,
+                    double *spread_b, double *spread_a, void *stream)
+{
+    if (k < 1 || nrow < 1 || nlat < 1 || nlon < 1) return set_err(DIDA_ERR_ARG, "empty shape");
+    if (!grid_lat || !xb || !axb || !xa || !axa || !xa_out || !spread_b || !spread_a)
+        return set_err(DIDA_ERR_ARG, "null argument");
+    dk_post_args a = {k, nrow, nlat, nlon, grid_lat, polar_lat, gamma1, gamma2, xb, axb,
+                      xa, axa, xa_out, spread_b, spread_a};
+    HIP_TRY(dk_post(&a, stream));
+    return DIDA_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Dense batched core on device pointers. */
+static _Thread_local int64_t *tl_off = NULL;
+static _Thread_local int32_t tl_off_n = 0;
+static _Thread_local int32_t tl_off_stride = 0;
+
+int dida_letkf_core_batch_dev(int32_t batch, int32_t max_obs, int32_t nanals,
+                              const int32_t *nobsl, const double *hxens, const float *rdiaginv,
+                              const double *dep, const double *rloc, double *trans,
+                              double *overline_w, double inflation_factor, int32_t gram,
+                              int32_t *info, void *stream)
+{
+    if (batch < 0 || max_obs < 0) return set_err(DIDA_ERR_ARG, "negative size");
+    if (nanals < 2 || nanals > 64)
+        return set_err(DIDA_ERR_UNSUPPORTED, "nanals = %d: supported range is 2..64", nanals);
+    if (gram != DIDA_GRAM_FP64) return set_err(DIDA_ERR_UNSUPPORTED, "gram %d", gram);
+    if (!(inflation_factor > 0.0)) return set_err(DIDA_ERR_ARG, "inflation_factor must be > 0");
+    if (batch == 0) return DIDA_OK;
+    if (!nobsl || !trans || (max_obs > 0 && (!hxens || !rdiaginv || !dep || !rloc)))
+        return set_err(DIDA_ERR_ARG, "null argument");
+    hipStream_t s = (hipStream_t)stream;
+    /* per-analysis offsets a * max_obs, kept per thread */
+    if (tl_off_n < batch || tl_off_stride != max_obs) {
+        if (tl_off) (void)hipFree(tl_off);
+        tl_off = NULL;
+        tl_off_n = 0;
+        int64_t *h = (int64_t *)malloc((size_t)batch * sizeof(int64_t));
+        if (!h) return set_err(DIDA_ERR_NOMEM, "host offsets");
+        for (int32_t i = 0; i < batch; ++i) h[i] = (int64_t)i * max_obs;
+        if (hipMalloc((void **)&tl_off, (size_t)batch * sizeof(int64_t)) != hipSuccess) {
+            free(h);
+            return set_err(DIDA_ERR_NOMEM, "offsets");
+        }
+        int e = (int)hipMemcpy(tl_off, h, (size_t)batch * sizeof(int64_t), hipMemcpyHostToDevice);
+        free(h);
+        if (e) return set_err(DIDA_ERR_HIP, "offset copy: %s", hipGetErrorString((hipError_t)e));
+        tl_off_n = batch;
+        tl_off_stride = max_obs;
+    }
+    dk_core_args ca;
+    memset(&ca, 0, sizeof ca);
+    ca.k = nanals;
+    ca.nanal = batch;
+    ca.offset = tl_off;
+    ca.nobsl = nobsl;
+    ca.capacity = (int64_t)batch * max_obs;
+    ca.max_p = max_obs;
+    ca.idx = NULL;
+    ca.rloc = rloc;
+    ca.yb = hxens;
+    ca.rinv = rdiaginv;
+    ca.dep = dep;
+    ca.rho = inflation_factor;
+    ca.tol = 1e-13;
+    ca.max_sweeps = 30;
+    ca.trans = trans;
+    ca.wbar = overline_w;
+    ca.info = info;
+    HIP_TRY(dk_letkf_core_f64(&ca, s));
+    return DIDA_OK;
+}
+
+/* Host-pointer batch: stages through device buffers kept per thread. */
+static _Thread_local struct {
+    size_t c_hx, c_dep, c_rloc, c_ri, c_tr, c_wb, c_nb, c_info;
+    double *hx, *dep, *rloc, *tr, *wb;
+    float *ri;
+    int32_t *nb, *info;
+} tl;
+
+static int grow(void **p, size_t *cap, size_t need)
+{
+    if (need == 0) need = 16;
+    if (*cap >= need && *p) return 0;
+    if (*p) (void)hipFree(*p);
+    *p = NULL;
+    *cap = 0;
+    if (hipMalloc(p, need) != hipSuccess) return -1;
+    *cap = need;
+    return 0;
+}
+
+static int core_batch_host(int32_t batch, int32_t max_obs, int32_t k, const int32_t *nobsl,
+                           const double *hxens, const float *rdiaginv, const double *dep,
+                           const double *rloc, double *trans, double *wbar, double rho,
+                           int32_t *info_h)
+{
+    if (batch < 0 || max_obs < 0 || k < 1) return set_err(DIDA_ERR_ARG, "bad sizes");
+    const size_t nob = (size_t)batch * (size_t)max_obs;
+    if (grow((void **)&tl.hx, &tl.c_hx, nob * (size_t)k * sizeof(double)) ||
+        grow((void **)&tl.dep, &tl.c_dep, nob * sizeof(double)) ||
+        grow((void **)&tl.rloc, &tl.c_rloc, nob * sizeof(double)) ||
+        grow((void **)&tl.ri, &tl.c_ri, nob * sizeof(float)) ||
+        grow((void **)&tl.tr, &tl.c_tr, (size_t)batch * k * k * sizeof(double)) ||
+        grow((void **)&tl.wb, &tl.c_wb, (size_t)batch * k * sizeof(double)) ||
+        grow((void **)&tl.nb, &tl.c_nb, (size_t)batch * sizeof(int32_t)) ||
+        grow((void **)&tl.info, &tl.c_info, (size_t)batch * sizeof(int32_t)))
+        return set_err(DIDA_ERR_NOMEM, "device staging buffers");
+    if (nob) {
+        HIP_TRY(hipMemcpy(tl.hx, hxens, nob * (size_t)k * sizeof(double), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(tl.dep, dep, nob * sizeof(double), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(tl.rloc, rloc, nob * sizeof(double), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(tl.ri, rdiaginv, nob * sizeof(float), hipMemcpyHostToDevice));
+    }
+    if (batch == 0) return DIDA_OK;
+    HIP_TRY(hipMemcpy(tl.nb, nobsl, (size_t)batch * sizeof(int32_t), hipMemcpyHostToDevice));
+    int rc = dida_letkf_core_batch_dev(batch, max_obs, k, tl.nb, tl.hx, tl.ri, tl.dep, tl.rloc,
+                                       tl.tr, tl.wb, rho, DIDA_GRAM_FP64, tl.info, NULL);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(trans, tl.tr, (size_t)batch * k * k * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(wbar, tl.wb, (size_t)batch * k * sizeof(double), hipMemcpyDeviceToHost));
+    if (info_h)
+        HIP_TRY(hipMemcpy(info_h, tl.info, (size_t)batch * sizeof(int32_t), hipMemcpyDeviceToHost));
+    return DIDA_OK;
+}
+
+void letkf_core_(const int32_t *nanals, const int32_t *nobsl, const double *hxens,
+                 const float *rdiaginv, const double *dep, const double *rloc, double *trans,
+                 double *overline_w, const double *inflation_factor, const int32_t *pflag)
+{
+    int32_t info = 0;
+    int32_t p = *nobsl;
+    int rc = core_batch_host(1, p, *nanals, &p, hxens, rdiaginv, dep, rloc, trans, overline_w,
+                             *inflation_factor, &info);
+    if (rc) fprintf(stderr, "letkf_core_: %s\n", dida_letkf_last_error());
+    else if (info < 0) fprintf(stderr, "letkf_core_: eigensolver did not converge\n");
+    if (pflag && *pflag) {
+        printf("nobsl %d  jacobi sweeps %d\n", p, info);
+        printf("overline_w");
+        for (int i = 0; i < *nanals; ++i) printf(" %.6g", overline_w[i]);
+        printf("\n");
+    }
+}
+
+void letkf_core_batch_(const int32_t *batch, const int32_t *batch_large_matrix,
+                       const int32_t *max_obs, const int32_t *nanals, const int32_t *nobsl,
+                       double *hxens, const float *rdiaginv, const double *dep,
+                       const double *rloc, double *trans, double *overline_w,
+                       const double *inflation_factor, const int32_t *pflag)
+{
+    (void)batch_large_matrix;
+    int rc = core_batch_host(*batch, *max_obs, *nanals, nobsl, hxens, rdiaginv, dep, rloc, trans,
+                             overline_w, *inflation_factor, NULL);
+    if (rc) fprintf(stderr, "letkf_core_batch_: %s\n", dida_letkf_last_error());
+    if (pflag && *pflag) printf("letkf_core_batch_: batch %d done\n", *batch);
+}

@@ -1,0 +1,141 @@
+/*
+ * Thin C ABI between the C host layer (letkf_host.c) and the HIP kernels.
+ * Every launcher enqueues on `stream` (a hipStream_t) and returns a
+ * hipError_t value as int; nothing here synchronises or allocates, except
+ * where a function says it only queries a size.
+ */
+#ifndef DIDA_LETKF_KERNELS_H
+#define DIDA_LETKF_KERNELS_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Observation bins: nband bands of dlat = 180/nband degrees and nlonbin
+ * longitude bins of dlon = 360/nlonbin degrees (an exact tiling, so bin
+ * indices wrap at the date line); band = floor((lat + 90) / dlat), lon bin =
+ * floor(wrap360(lon) / dlon); |lat| > obs_polar goes to bin nbins. */
+typedef struct dk_bin_geom {
+    double dlat, dlon;
+    double obs_polar;
+    int32_t nband, nlonbin, nbins;
+    int32_t pad_;
+} dk_bin_geom;
+
+/* Observations in binned order, with the per-obs factors of the distance
+ * formula precomputed (cos/sin of the latitude in the reference's radians
+ * and the longitude in the reference's radians). */
+typedef struct dk_obs_binned {
+    const double *cx, *sx, *yr;   /* [nobs] */
+    const double *yb;             /* [nobs * k] ensemble fastest */
+    const double *dep;            /* [nobs] */
+    const float *rinv;            /* [nobs] */
+    const int32_t *bin_start;     /* [nbins + 2] */
+} dk_obs_binned;
+
+typedef struct dk_select_args {
+    int32_t nlat, nlon;
+    const float *grid_lat, *grid_lon; /* REAL(4) grid coordinates, degrees */
+    dk_bin_geom geom;
+    dk_obs_binned obs;
+    double local_dist;   /* L, km */
+    double theta_deg;    /* angular radius of the support plus margin, deg */
+    double cos_far;      /* central-angle cosine below which gc = 0 surely */
+    int32_t gc_filter;
+    int32_t max_obs;
+    int32_t *nobsl;      /* [ncol] count pass output / fill pass input */
+    const int64_t *offset; /* [ncol + 1] fill pass */
+    int32_t *csr_idx;    /* fill pass: binned obs index */
+    double *csr_w;       /* fill pass: rloc */
+    int64_t capacity;    /* fill pass: entries available in csr_idx / csr_w */
+    int32_t *overflow;   /* fill pass: set to 1 when a column does not fit */
+} dk_select_args;
+
+/* Fill n int32 with v (a kernel, so that captured graphs hold no memset). */
+int dk_fill_i32(int32_t *p, int32_t v, int64_t n, void *stream);
+
+/* Stage 1: keys, counts per bin and identity values for the sort. */
+int dk_bin_keys(int32_t nobs, const double *lat, const double *lon, dk_bin_geom g,
+                uint32_t *key, int32_t *iota, int32_t *bin_count, void *stream);
+/* Stable radix sort of (key, iota) by key.  tmp == NULL: *tmp_bytes is set. */
+int dk_sort_pairs_u32(void *tmp, size_t *tmp_bytes, const uint32_t *kin, uint32_t *kout,
+                      const int32_t *vin, int32_t *vout, int32_t n, int end_bit, void *stream);
+/* Exclusive scan of nbins + 1 counts into bin_start[nbins + 2] (one block). */
+int dk_bin_scan(int32_t nbins, const int32_t *count, int32_t *bin_start, void *stream);
+/* Gather obs into binned order and precompute the distance factors. */
+int dk_obs_gather(int32_t nobs, int32_t k, const int32_t *perm, const double *lat,
+                  const double *lon, const double *yb, const double *dep, const float *rinv,
+                  double *cx, double *sx, double *yr, double *yb_b, double *dep_b,
+                  float *rinv_b, void *stream);
+/* Selection passes, one wave per column. */
+int dk_select_count(const dk_select_args *a, void *stream);
+int dk_select_fill(const dk_select_args *a, void *stream);
+/* offset[0] = 0, offset[i+1] = offset[i] + nobsl[i]; tmp as dk_sort_pairs. */
+int dk_scan_offsets(void *tmp, size_t *tmp_bytes, const int32_t *nobsl, int64_t *offset,
+                    int32_t n, void *stream);
+
+/* Transform (letkf_core) for a set of analyses. */
+typedef struct dk_core_args {
+    int32_t k;              /* nanals */
+    int32_t nanal;          /* analyses in this launch */
+    const int32_t *list;    /* analysis ids (NULL: aid0 + block index) */
+    int32_t aid0;           /* first analysis id; trans/wbar hold id - aid0 */
+    int32_t pad0_;
+    const int64_t *offset;  /* obs offset of each analysis */
+    const int32_t *nobsl;   /* obs count of each analysis */
+    int64_t capacity;       /* entries readable at idx/rloc; an analysis whose
+                               range does not fit is treated as having no obs */
+    int32_t max_p;          /* obs counts are clamped to [0, max_p] */
+    int32_t pad1_;
+    const int32_t *idx;     /* CSR obs index (NULL: dense, index = offset + o) */
+    const double *rloc;     /* [offset + o] */
+    const double *yb;       /* obs ensemble perturbations, k per obs */
+    const float *rinv;      /* per obs index */
+    const double *dep;      /* per obs index */
+    double rho;             /* multiplicative inflation */
+    double tol;             /* Jacobi: rotate while |g| > tol sqrt(a b) */
+    int32_t max_sweeps;
+    int32_t pad_;
+    double *trans;          /* [(id - aid0) * k * k] trans(i, j) at i + j k */
+    double *wbar;           /* [(id - aid0) * k] (optional) */
+    int32_t *info;          /* [id]: sweeps, or -1 when not converged (optional) */
+} dk_core_args;
+
+int dk_letkf_core_f64(const dk_core_args *a, void *stream);
+
+/* Update (xb_2_xa) for a set of columns, with optional RTPS. */
+typedef struct dk_update_args {
+    int32_t k, nrow;
+    int32_t ncol;           /* columns in this launch */
+    int32_t col0;           /* first column id (identity list) */
+    const double *trans;    /* [(col - col0) * k * k] */
+    const double *xb;       /* [col * nrow * k] */
+    const double *axb;      /* [col * nrow] */
+    double *xa;
+    double *axa;
+    int32_t rtps;           /* 1: relax to prior spread with alpha */
+    int32_t pad_;
+    double rtps_alpha;
+} dk_update_args;
+
+int dk_update_f64(const dk_update_args *a, void *stream);
+
+/* Post-analysis blend of run_da_parallel (RTPP-style gamma blend inside
+ * |lat| <= polar_lat, background outside) and spreads. */
+typedef struct dk_post_args {
+    int32_t k, nrow, nlat, nlon;
+    const float *grid_lat;
+    double polar_lat, gamma1, gamma2;
+    const double *xb, *axb, *xa, *axa;
+    double *xa_out, *spread_b, *spread_a;
+} dk_post_args;
+
+int dk_post(const dk_post_args *a, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,342 @@
+// Device-side local observation selection (gen_yb_dis, letkf_yb.F90:3-133).
+//
+// Observations are binned on a regular lat/lon lattice (stable sort by bin
+// id), then one wavefront per grid column visits the candidate bins in
+// ascending bin id, evaluates the reference's distance and Gaspari-Cohn
+// weight per candidate, and compacts the kept obs with a ballot.  Two
+// passes: count (capped at MAX_OBS) and fill into a CSR list.  The kept
+// sequence is exactly the one a full scan of the binned list yields, so
+// MAX_OBS truncation matches the reference's in-order truncation.
+#include "letkf_device.h"
+#include "letkf_kernels.h"
+
+#include <rocprim/device/device_radix_sort.hpp>
+
+using namespace dida;
+
+namespace {
+
+__device__ __forceinline__ double wrap360(double lon) {
+  double w = fmod(lon, 360.0);
+  if (w < 0.0) w += 360.0;
+  if (w >= 360.0) w -= 360.0;
+  return w;
+}
+
+__global__ void k_fill_i32(int32_t* __restrict__ p, int32_t v, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_bin_keys(int32_t nobs, const double* __restrict__ lat,
+                           const double* __restrict__ lon, dk_bin_geom g,
+                           uint32_t* __restrict__ key, int32_t* __restrict__ iota,
+                           int32_t* __restrict__ bin_count) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nobs) return;
+  double la = lat[i];
+  uint32_t kk;
+  if (!(fabs(la) <= g.obs_polar)) {
+    kk = (uint32_t)g.nbins;
+  } else {
+    int b = (int)floor((la + 90.0) / g.dlat);
+    b = b < 0 ? 0 : (b > g.nband - 1 ? g.nband - 1 : b);
+    int c = (int)floor(wrap360(lon[i]) / g.dlon);
+    c = c < 0 ? 0 : (c > g.nlonbin - 1 ? g.nlonbin - 1 : c);
+    kk = (uint32_t)(b * g.nlonbin + c);
+  }
+  key[i] = kk;
+  iota[i] = i;
+  atomicAdd(&bin_count[kk], 1);
+}
+
+// One block of 1024 threads scans nbins + 1 counts in chunks.
+__global__ __launch_bounds__(1024) void k_bin_scan(int32_t nbins, const int32_t* __restrict__ cnt,
+                                                   int32_t* __restrict__ start) {
+  __shared__ int32_t s[1024];
+  __shared__ int32_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  const int n = nbins + 1;
+  for (int base = 0; base < n; base += 1024) {
+    int i = base + threadIdx.x;
+    int v = i < n ? cnt[i] : 0;
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {
+      int t = threadIdx.x >= d ? s[threadIdx.x - d] : 0;
+      __syncthreads();
+      s[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (i < n) start[i] = carry + s[threadIdx.x] - v;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry += s[1023];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) start[n] = carry;
+}
+
+__global__ void k_obs_gather(int32_t nobs, const int32_t* __restrict__ perm,
+                             const double* __restrict__ lat, const double* __restrict__ lon,
+                             const double* __restrict__ dep, const float* __restrict__ rinv,
+                             double* __restrict__ cx, double* __restrict__ sx,
+                             double* __restrict__ yr, double* __restrict__ dep_b,
+                             float* __restrict__ rinv_b) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nobs) return;
+  int s = perm[i];
+  double x1r = deg2rad_ref(lat[s]);
+  cx[i] = cos(x1r);
+  sx[i] = sin(x1r);
+  yr[i] = deg2rad_ref(lon[s]);
+  dep_b[i] = dep[s];
+  rinv_b[i] = rinv[s];
+}
+
+__global__ void k_obs_gather_yb(int32_t nobs, int32_t k, const int32_t* __restrict__ perm,
+                                const double* __restrict__ yb, double* __restrict__ yb_b) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t total = (int64_t)nobs * k;
+  if (t >= total) return;
+  int64_t i = t / k;
+  int e = (int)(t - i * k);
+  yb_b[t] = yb[(int64_t)perm[i] * k + e];
+}
+
+// Visit the candidate bins of one column; call f(l0, l1) per contiguous obs
+// range in ascending bin order; f returns false to stop.
+template <class F>
+__device__ __forceinline__ void for_candidate_ranges(const dk_select_args& a, double clat,
+                                                     double clon, F&& f) {
+  const double kPi = 3.14159265358979323846;
+  const double th = a.theta_deg;
+  const dk_bin_geom& g = a.geom;
+  int b0 = (int)floor((clat - th + 90.0) / g.dlat);
+  int b1 = (int)floor((clat + th + 90.0) / g.dlat);
+  b0 = b0 < 0 ? 0 : b0;
+  b1 = b1 > g.nband - 1 ? g.nband - 1 : b1;
+  bool full = false;
+  double dlon = 0.0;
+  if (fabs(clat) + th >= 90.0) {
+    full = true;
+  } else {
+    double sv = sin(th * kPi / 180.0) / cos(clat * kPi / 180.0);
+    if (sv >= 1.0) full = true;
+    else dlon = asin(sv) * 180.0 / kPi * (1.0 + 1e-6) + 1e-6;
+  }
+  int lo = 0, hi = g.nlonbin - 1;
+  if (!full) {
+    double cw = wrap360(clon);
+    lo = (int)floor((cw - dlon) / g.dlon);
+    hi = (int)floor((cw + dlon) / g.dlon);
+    if (hi - lo + 1 >= g.nlonbin) full = true;
+  }
+  const int32_t* bs = a.obs.bin_start;
+  for (int b = b0; b <= b1; ++b) {
+    int base = b * g.nlonbin;
+    int r0a, r0b, r1a = -1, r1b = -2;
+    if (full) {
+      r0a = base; r0b = base + g.nlonbin - 1;
+    } else if (lo < 0) {
+      r0a = base; r0b = base + hi;
+      r1a = base + lo + g.nlonbin; r1b = base + g.nlonbin - 1;
+    } else if (hi > g.nlonbin - 1) {
+      r0a = base; r0b = base + hi - g.nlonbin;
+      r1a = base + lo; r1b = base + g.nlonbin - 1;
+    } else {
+      r0a = base + lo; r0b = base + hi;
+    }
+    if (!f(bs[r0a], bs[r0b + 1])) return;
+    if (r1a <= r1b && !f(bs[r1a], bs[r1b + 1])) return;
+  }
+}
+
+template <bool kFill>
+__global__ __launch_bounds__(256) void k_select(dk_select_args a) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t col = (int64_t)blockIdx.x * 4 + wave;
+  const int64_t ncol = (int64_t)a.nlat * a.nlon;
+  if (col >= ncol) return;
+  const int ilat = (int)(col % a.nlat), ilon = (int)(col / a.nlat);
+  const double clat = (double)a.grid_lat[ilat];
+  const double clon = (double)a.grid_lon[ilon];
+  const double x2r = deg2rad_ref(clat);
+  const double cx2 = cos(x2r), sx2 = sin(x2r), y2r = deg2rad_ref(clon);
+  const int limit = kFill ? a.nobsl[col] : a.max_obs;
+  int64_t out = kFill ? a.offset[col] : 0;
+  if (kFill && out + limit > a.capacity && lane == 0) *a.overflow = 1;
+  int cnt = 0;
+  if (limit > 0) {
+    for_candidate_ranges(a, clat, clon, [&](int l0, int l1) -> bool {
+      for (int base = l0; base < l1; base += kWave) {
+        const int l = base + lane;
+        bool keep = false;
+        double gc = 0.0;
+        if (l < l1) {
+          double v = central_cos(a.obs.cx[l], a.obs.sx[l], a.obs.yr[l], cx2, sx2, y2r);
+          if (v >= a.cos_far) {
+#pragma clang fp contract(off)
+            double d = dist_from_cos(v);
+            double r = d * 1.0 / a.local_dist;
+            gc = gc_weight(r, a.gc_filter);
+            keep = gc > 0.0;
+          }
+        }
+        const uint64_t m = __ballot(keep);
+        if (kFill && keep) {
+          const int pos = cnt + (int)__builtin_amdgcn_mbcnt_hi(
+                                    (uint32_t)(m >> 32),
+                                    __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+          if (pos < limit && out + pos < a.capacity) {
+            a.csr_idx[out + pos] = l;
+            a.csr_w[out + pos] = gc;
+          }
+        }
+        cnt += __popcll(m);
+        if (cnt >= limit) return false;
+      }
+      return true;
+    });
+  }
+  if (!kFill && lane == 0) a.nobsl[col] = cnt < a.max_obs ? cnt : a.max_obs;
+}
+
+}  // namespace
+
+extern "C" int dk_fill_i32(int32_t* p, int32_t v, int64_t n, void* stream) {
+  if (n <= 0) return 0;
+  k_fill_i32<<<(unsigned)((n + 255) / 256), 256, 0, (hipStream_t)stream>>>(p, v, n);
+  return (int)hipGetLastError();
+}
+
+extern "C" int dk_bin_keys(int32_t nobs, const double* lat, const double* lon, dk_bin_geom g,
+                           uint32_t* key, int32_t* iota, int32_t* bin_count, void* stream) {
+  if (nobs <= 0) return 0;
+  k_bin_keys<<<(nobs + 255) / 256, 256, 0, (hipStream_t)stream>>>(nobs, lat, lon, g, key, iota,
+                                                                   bin_count);
+  return (int)hipGetLastError();
+}
+
+extern "C" int dk_sort_pairs_u32(void* tmp, size_t* tmp_bytes, const uint32_t* kin,
+                                 uint32_t* kout, const int32_t* vin, int32_t* vout, int32_t n,
+                                 int end_bit, void* stream) {
+  return (int)rocprim::radix_sort_pairs(tmp, *tmp_bytes, kin, kout, vin, vout, (size_t)n, 0,
+                                        end_bit, (hipStream_t)stream, false);
+}
+
+extern "C" int dk_bin_scan(int32_t nbins, const int32_t* count, int32_t* bin_start,
+                           void* stream) {
+  k_bin_scan<<<1, 1024, 0, (hipStream_t)stream>>>(nbins, count, bin_start);
+  return (int)hipGetLastError();
+}
+
+extern "C" int dk_obs_gather(int32_t nobs, int32_t k, const int32_t* perm, const double* lat,
+                             const double* lon, const double* yb, const double* dep,
+                             const float* rinv, double* cx, double* sx, double* yr,
+                             double* yb_b, double* dep_b, float* rinv_b, void* stream) {
+  if (nobs <= 0) return 0;
+  hipStream_t s = (hipStream_t)stream;
+  k_obs_gather<<<(nobs + 255) / 256, 256, 0, s>>>(nobs, perm, lat, lon, dep, rinv, cx, sx, yr,
+                                                  dep_b, rinv_b);
+  int64_t total = (int64_t)nobs * k;
+  k_obs_gather_yb<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(nobs, k, perm, yb, yb_b);
+  return (int)hipGetLastError();
+}
+
+extern "C" int dk_select_count(const dk_select_args* a, void* stream) {
+  int64_t ncol = (int64_t)a->nlat * a->nlon;
+  if (ncol <= 0) return 0;
+  k_select<false><<<(unsigned)((ncol + 3) / 4), 256, 0, (hipStream_t)stream>>>(*a);
+  return (int)hipGetLastError();
+}
+
+extern "C" int dk_select_fill(const dk_select_args* a, void* stream) {
+  int64_t ncol = (int64_t)a->nlat * a->nlon;
+  if (ncol <= 0) return 0;
+  k_select<true><<<(unsigned)((ncol + 3) / 4), 256, 0, (hipStream_t)stream>>>(*a);
+  return (int)hipGetLastError();
+}
+
+namespace {
+
+// Three-phase exclusive scan of per-column obs counts into int64 offsets:
+// per-block totals, a one-block scan of the totals, then per-block scans.
+constexpr int kScanBlock = 1024;
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_block_sums(const int32_t* __restrict__ v,
+                                                                int32_t n,
+                                                                int64_t* __restrict__ bsum) {
+  __shared__ int64_t s[kScanBlock / kWave];
+  int64_t i = (int64_t)blockIdx.x * kScanBlock + threadIdx.x;
+  int64_t x = i < n ? (int64_t)v[i] : 0;
+  for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d, kWave);
+  if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t t = 0;
+    for (int w = 0; w < kScanBlock / kWave; ++w) t += s[w];
+    bsum[blockIdx.x] = t;
+  }
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_sums(int64_t* __restrict__ bsum, int32_t nb) {
+  __shared__ int64_t s[kScanBlock];
+  __shared__ int64_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < nb; base += kScanBlock) {
+    int i = base + threadIdx.x;
+    int64_t v = i < nb ? bsum[i] : 0;
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int d = 1; d < kScanBlock; d <<= 1) {
+      int64_t t = threadIdx.x >= (unsigned)d ? s[threadIdx.x - d] : 0;
+      __syncthreads();
+      s[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (i < nb) bsum[i] = carry + s[threadIdx.x] - v;
+    __syncthreads();
+    if (threadIdx.x == kScanBlock - 1) carry += s[kScanBlock - 1];
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_apply(const int32_t* __restrict__ v, int32_t n,
+                                                           const int64_t* __restrict__ bsum,
+                                                           int64_t* __restrict__ out) {
+  __shared__ int64_t s[kScanBlock];
+  int64_t i = (int64_t)blockIdx.x * kScanBlock + threadIdx.x;
+  int64_t x = i < n ? (int64_t)v[i] : 0;
+  s[threadIdx.x] = x;
+  __syncthreads();
+  for (int d = 1; d < kScanBlock; d <<= 1) {
+    int64_t t = threadIdx.x >= (unsigned)d ? s[threadIdx.x - d] : 0;
+    __syncthreads();
+    s[threadIdx.x] += t;
+    __syncthreads();
+  }
+  int64_t incl = bsum[blockIdx.x] + s[threadIdx.x];
+  if (i < n) out[i] = incl - x;
+  if (i == n - 1) out[n] = incl;
+}
+
+}  // namespace
+
+// tmp must hold one int64 per block of 1024 counts.
+extern "C" int dk_scan_offsets(void* tmp, size_t* tmp_bytes, const int32_t* nobsl,
+                               int64_t* offset, int32_t n, void* stream) {
+  const int32_t nb = (n + kScanBlock - 1) / kScanBlock;
+  if (tmp == nullptr) {
+    *tmp_bytes = sizeof(int64_t) * (size_t)(nb > 0 ? nb : 1);
+    return 0;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  if (n <= 0) return 0;
+  int64_t* bsum = (int64_t*)tmp;
+  k_scan_block_sums<<<nb, kScanBlock, 0, s>>>(nobsl, n, bsum);
+  k_scan_sums<<<1, kScanBlock, 0, s>>>(bsum, nb);
+  k_scan_apply<<<nb, kScanBlock, 0, s>>>(nobsl, n, bsum, offset);
+  return (int)hipGetLastError();
+}

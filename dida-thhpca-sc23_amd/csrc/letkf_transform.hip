@@ -1,0 +1,297 @@
+// LETKF transform (letkf_core, letkf_core.F90:14-233), plain FP64 form.
+//
+// One workgroup (4 wavefronts) per local analysis:
+//   1. Gram C = Yhat Yhat^T on FP64 MFMA (16x16x4), Yhat = Yb sqrt(R^-1 rloc),
+//      and b = Yhat (sqrt(R^-1 rloc) dep), both accumulated straight from the
+//      gathered obs columns; the four waves split the obs and reduce in LDS.
+//   2. One-sided (Hestenes) Jacobi on C resident in LDS: column pairs of a
+//      round-robin ordering are orthogonalised in parallel until a sweep makes
+//      no rotation.  The converged columns are lambda_j v_j, so no separate
+//      eigenvector array is needed (C is symmetric positive semi-definite).
+//   3. W = sqrt(rho) I + sum_j (f(lambda_j) - sqrt(rho)) v_j v_j^T and
+//      wbar = Pa b with Pa = rho/(k-1) I + sum_j (1/(lambda_j + (k-1)/rho) -
+//      rho/(k-1)) v_j v_j^T, f(l) = sqrt((k-1)/(l + (k-1)/rho)).  This is the
+//      reference's V f(Lambda) V^T with the null-space terms written in closed
+//      form, so the null-space eigenvectors (which one-sided Jacobi does not
+//      deliver) are never needed.  trans = W + wbar 1^T.
+#include "letkf_device.h"
+#include "letkf_kernels.h"
+
+using namespace dida;
+
+namespace {
+
+constexpr int kThreads = 256;
+
+template <int KP>
+struct CoreSmem {
+  static constexpr int LDA = KP + 1;
+  double A[KP * LDA];
+  double b[KP];
+  double nrm[KP];
+  double sg[KP];
+  double sh[KP];
+  double u[KP];
+  double wb[KP];
+  int flag[2];
+};
+
+// Gram + b with the obs split across the four waves (KP <= 64: all lower
+// tiles of C fit in one wave's accumulators).
+template <int KP>
+__device__ __forceinline__ void gram_f64(const dk_core_args& a, int64_t off, int p,
+                                         CoreSmem<KP>& sm) {
+  constexpr int NT = KP / 16;
+  constexpr int NTILE = NT * (NT + 1) / 2;
+  constexpr int LDA = CoreSmem<KP>::LDA;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int r16 = lane & 15, q4 = lane >> 4;
+  const int k = a.k;
+  double4_t acc[NTILE];
+#pragma unroll
+  for (int t = 0; t < NTILE; ++t) acc[t] = double4_t{0.0, 0.0, 0.0, 0.0};
+  double bacc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) bacc[t] = 0.0;
+
+  for (int c = wave; c * 4 < p; c += 4) {
+    const int o = c * 4 + q4;
+    double f[NT];
+    double sd = 0.0;
+    if (o < p) {
+      const int64_t gi = a.idx ? (int64_t)a.idx[off + o] : off + o;
+      const double rr = sqrt((double)a.rinv[gi] * a.rloc[off + o]);
+      const double* y = a.yb + gi * k;
+#pragma unroll
+      for (int T = 0; T < NT; ++T) {
+        const int row = T * 16 + r16;
+        f[T] = row < k ? y[row] * rr : 0.0;
+      }
+      sd = rr * a.dep[gi];
+    } else {
+#pragma unroll
+      for (int T = 0; T < NT; ++T) f[T] = 0.0;
+    }
+#pragma unroll
+    for (int I = 0; I < NT; ++I)
+#pragma unroll
+      for (int J = 0; J <= I; ++J) acc[I * (I + 1) / 2 + J] = mfma_f64_16x16x4(f[I], f[J], acc[I * (I + 1) / 2 + J]);
+#pragma unroll
+    for (int T = 0; T < NT; ++T) bacc[T] += f[T] * sd;
+  }
+#pragma unroll
+  for (int T = 0; T < NT; ++T) {
+    bacc[T] += shfl_xor_d(bacc[T], 16);
+    bacc[T] += shfl_xor_d(bacc[T], 32);
+  }
+  for (int w = 0; w < 4; ++w) {
+    if (wave == w) {
+#pragma unroll
+      for (int I = 0; I < NT; ++I)
+#pragma unroll
+        for (int J = 0; J <= I; ++J) {
+          const double4_t v = acc[I * (I + 1) / 2 + J];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int i = I * 16 + 4 * q4 + r, j = J * 16 + r16;
+            if (w == 0) {
+              sm.A[i + j * LDA] = v[r];
+              if (I != J) sm.A[j + i * LDA] = v[r];
+            } else {
+              sm.A[i + j * LDA] += v[r];
+              if (I != J) sm.A[j + i * LDA] += v[r];
+            }
+          }
+        }
+      if (q4 == 0) {
+#pragma unroll
+        for (int T = 0; T < NT; ++T) {
+          if (w == 0) sm.b[T * 16 + r16] = bacc[T];
+          else sm.b[T * 16 + r16] += bacc[T];
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// One-sided Jacobi on the first k rows / m = k rounded up to even columns of
+// sm.A.  Returns the number of sweeps; *converged tells whether the last
+// sweep made no rotation.
+template <int KP>
+__device__ int jacobi_one_sided(CoreSmem<KP>& sm, int k, double tol, int max_sweeps,
+                                bool* converged) {
+  constexpr int LDA = CoreSmem<KP>::LDA;
+  const int m = (k + 1) & ~1;
+  const int npair = m >> 1;
+  int tpp = 1;
+  while (tpp * 2 * npair <= kThreads && tpp < 32) tpp <<= 1;
+  const int tid = threadIdx.x;
+  const int pair = tid / tpp, sub = tid % tpp;
+  const bool active = pair < npair;
+  if (tid == 0) sm.flag[0] = 0;
+  int sweep = 0;
+  *converged = false;
+  for (; sweep < max_sweeps; ++sweep) {
+    for (int j = tid; j < m; j += kThreads) {
+      double s = 0.0;
+      for (int i = 0; i < k; ++i) s += sm.A[i + j * LDA] * sm.A[i + j * LDA];
+      sm.nrm[j] = s;
+    }
+    __syncthreads();
+    bool rotated = false;
+    for (int r = 0; r < m - 1; ++r) {
+      if (active) {
+        int p, q;
+        if (pair == 0) {
+          p = m - 1;
+          q = r;
+        } else {
+          p = (r - pair + (m - 1)) % (m - 1);
+          q = (r + pair) % (m - 1);
+        }
+        if (p > q) {
+          const int t = p;
+          p = q;
+          q = t;
+        }
+        double* ap = sm.A + p * LDA;
+        double* aq = sm.A + q * LDA;
+        double g = 0.0;
+        for (int i = sub; i < k; i += tpp) g += ap[i] * aq[i];
+        for (int s = 1; s < tpp; s <<= 1) g += shfl_xor_d(g, s);
+        const double al = sm.nrm[p], be = sm.nrm[q];
+        if (g != 0.0 && fabs(g) > tol * sqrt(al * be)) {
+          const double zeta = (be - al) / (2.0 * g);
+          double t;
+          if (fabs(zeta) > 1e100) t = 0.5 / zeta;
+          else t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+          const double c = 1.0 / sqrt(1.0 + t * t);
+          const double s = c * t;
+          for (int i = sub; i < k; i += tpp) {
+            const double x = ap[i], y = aq[i];
+            ap[i] = c * x - s * y;
+            aq[i] = s * x + c * y;
+          }
+          if (sub == 0) {
+            sm.nrm[p] = al - t * g;
+            sm.nrm[q] = be + t * g;
+          }
+          rotated = true;
+        }
+      }
+      __syncthreads();
+    }
+    if (rotated) sm.flag[sweep & 1] = 1;
+    __syncthreads();
+    if (sm.flag[sweep & 1] == 0) {
+      *converged = true;
+      ++sweep;
+      break;
+    }
+    if (tid == 0) sm.flag[(sweep + 1) & 1] = 0;
+  }
+  return sweep;
+}
+
+template <int KP>
+__global__ __launch_bounds__(kThreads) void k_letkf_core_f64(dk_core_args a) {
+  constexpr int NT = KP / 16;
+  constexpr int LDA = CoreSmem<KP>::LDA;
+  __shared__ CoreSmem<KP> sm;
+  const int aid = a.list ? a.list[blockIdx.x] : a.aid0 + (int)blockIdx.x;
+  const int64_t slot = (int64_t)aid - a.aid0;
+  const int k = a.k;
+  const int64_t off = a.offset[aid];
+  int p = a.nobsl[aid];
+  p = p < 0 ? 0 : (p > a.max_p ? a.max_p : p);
+  if (off < 0 || off + p > a.capacity) p = 0;
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63, r16 = lane & 15, q4 = lane >> 4;
+
+  gram_f64<KP>(a, off, p, sm);
+
+  bool conv;
+  const int sweeps = jacobi_one_sided<KP>(sm, k, a.tol, a.max_sweeps, &conv);
+  const int m = (k + 1) & ~1;
+
+  // exact column norms: alpha_j = lambda_j^2
+  for (int j = tid; j < m; j += kThreads) {
+    double s = 0.0;
+    for (int i = 0; i < k; ++i) s += sm.A[i + j * LDA] * sm.A[i + j * LDA];
+    sm.nrm[j] = s;
+  }
+  __syncthreads();
+  const double km1 = (double)(k - 1);
+  const double a0 = km1 / a.rho;
+  const double sa0 = sqrt(a0);
+  for (int j = tid; j < KP; j += kThreads) {
+    const double al = j < m ? sm.nrm[j] : 0.0;
+    double g = 0.0, h = 0.0;
+    if (al > 0.0) {
+      const double lam = sqrt(al);
+      const double sl = sqrt(lam + a0);
+      // f(lam) - f(0) and 1/(lam + a0) - 1/a0 in cancellation-free form
+      g = -sqrt(km1) * lam / (sl * sa0 * (sa0 + sl)) / al;
+      h = -lam / (a0 * (lam + a0)) / al;
+    }
+    sm.sg[j] = g;
+    sm.sh[j] = h;
+  }
+  __syncthreads();
+  for (int j = tid; j < KP; j += kThreads) {
+    double s = 0.0;
+    if (j < m)
+      for (int i = 0; i < k; ++i) s += sm.A[i + j * LDA] * sm.b[i];
+    sm.u[j] = s * sm.sh[j];
+  }
+  __syncthreads();
+  const double rk = a.rho / km1;
+  for (int i = tid; i < KP; i += kThreads) {
+    double s = 0.0;
+    if (i < k) {
+      s = rk * sm.b[i];
+      for (int j = 0; j < m; ++j) s += sm.A[i + j * LDA] * sm.u[j];
+    }
+    sm.wb[i] = s;
+  }
+  __syncthreads();
+
+  // W = sqrt(rho) I + (A diag(sg)) A^T on MFMA, written with wbar as trans.
+  const double srho = sqrt(a.rho);
+  double* trans = a.trans + slot * k * k;
+  for (int t = wave; t < NT * NT; t += 4) {
+    const int I = t / NT, I2 = t % NT;
+    double4_t d = double4_t{0.0, 0.0, 0.0, 0.0};
+    for (int j0 = 0; j0 < m; j0 += 4) {
+      const int jj = j0 + q4;
+      double fa = 0.0, fb = 0.0;
+      if (jj < m) {
+        fa = sm.A[I * 16 + r16 + jj * LDA] * sm.sg[jj];
+        fb = sm.A[I2 * 16 + r16 + jj * LDA];
+      }
+      d = mfma_f64_16x16x4(fa, fb, d);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = I * 16 + 4 * q4 + r, i2 = I2 * 16 + r16;
+      if (i < k && i2 < k) trans[i + (int64_t)i2 * k] = (i == i2 ? srho : 0.0) + d[r] + sm.wb[i];
+    }
+  }
+  if (a.wbar)
+    for (int i = tid; i < k; i += kThreads) a.wbar[slot * k + i] = sm.wb[i];
+  if (a.info && tid == 0) a.info[aid] = conv ? sweeps : -1;
+}
+
+}  // namespace
+
+extern "C" int dk_letkf_core_f64(const dk_core_args* a, void* stream) {
+  if (a->nanal <= 0) return 0;
+  hipStream_t s = (hipStream_t)stream;
+  const int k = a->k;
+  if (k < 2 || k > 64) return (int)hipErrorInvalidValue;
+  if (k <= 16) k_letkf_core_f64<16><<<a->nanal, kThreads, 0, s>>>(*a);
+  else if (k <= 32) k_letkf_core_f64<32><<<a->nanal, kThreads, 0, s>>>(*a);
+  else k_letkf_core_f64<64><<<a->nanal, kThreads, 0, s>>>(*a);
+  return (int)hipGetLastError();
+}

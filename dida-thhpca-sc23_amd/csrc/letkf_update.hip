@@ -1,0 +1,130 @@
+// Ensemble update (xb_2_xa, letkf_compute.F90:231-301) and the post-analysis
+// blend of run_da_parallel (module_da_parallel.F90:216-259).
+//
+// Update, one workgroup per column: trans is staged in LDS as T[i][e] (member
+// e contiguous, conflict-free), the column's Xb' rows stream through LDS a few
+// rows at a time (ensemble-fastest, so every row is one coalesced load), and
+// each lane forms Xa(e) = sum_i trans(i, e) Xb'(i) + xbar_b, the row mean, and
+// the re-centred perturbation.  Optional RTPS (Whitaker & Hamill 2012; not in
+// the reference): Xa' <- Xa' (alpha sigma_b / sigma_a + 1 - alpha).
+#include "letkf_device.h"
+#include "letkf_kernels.h"
+
+using namespace dida;
+
+namespace {
+
+constexpr int kThreads = 256;
+
+// Sum of v over the KP lanes of this thread's row slot.
+template <int KP>
+__device__ __forceinline__ double slot_sum(double v, double* sred) {
+  if (KP <= 64) {
+#pragma unroll
+    for (int s = 1; s < KP; s <<= 1) v += shfl_xor_d(v, s);
+    return v;
+  } else {
+#pragma unroll
+    for (int s = 1; s < 64; s <<= 1) v += shfl_xor_d(v, s);
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sred[w] = v;
+    __syncthreads();
+    const int slot0 = (w / (KP / 64)) * (KP / 64);
+    double t = 0.0;
+#pragma unroll
+    for (int q = 0; q < KP / 64; ++q) t += sred[slot0 + q];
+    return t;
+  }
+}
+
+template <int KP>
+__global__ __launch_bounds__(kThreads) void k_update_f64(dk_update_args u) {
+  constexpr int RP = kThreads / KP;
+  constexpr int LDT = KP + 1;
+  __shared__ double sT[KP * LDT];
+  __shared__ double sX[RP * KP];
+  __shared__ double sred[kThreads / 64];
+  const int c = blockIdx.x;
+  const int64_t col = (int64_t)u.col0 + c;
+  const int k = u.k, nrow = u.nrow;
+  const double* T = u.trans + (int64_t)c * k * k;
+  for (int t = threadIdx.x; t < k * k; t += kThreads) {
+    const int i = t % k, e = t / k;
+    sT[i * LDT + e] = T[t];
+  }
+  const int e = threadIdx.x % KP, slot = threadIdx.x / KP;
+  const double rk = 1.0 / (double)k;
+  for (int row0 = 0; row0 < nrow; row0 += RP) {
+    const int row = row0 + slot;
+    const bool rowok = row < nrow;
+    const bool ok = rowok && e < k;
+    const int64_t cr = col * nrow + row;
+    const double xv = ok ? u.xb[cr * k + e] : 0.0;
+    __syncthreads();
+    sX[slot * KP + e] = xv;
+    __syncthreads();
+    double acc = 0.0;
+    if (rowok)
+      for (int i = 0; i < k; ++i) acc += sT[i * LDT + e] * sX[slot * KP + i];
+    const double xbar = rowok ? u.axb[cr] : 0.0;
+    const double v = ok ? acc + xbar : 0.0;
+    const double mean = slot_sum<KP>(v, sred) / (double)k;
+    double xa = ok ? v - mean : 0.0;
+    if (u.rtps) {
+      const double sb2 = slot_sum<KP>(xv * xv, sred) * rk;
+      const double sa2 = slot_sum<KP>(xa * xa, sred) * rk;
+      if (sa2 > 0.0) xa *= u.rtps_alpha * sqrt(sb2) / sqrt(sa2) + 1.0 - u.rtps_alpha;
+    }
+    if (ok) u.xa[cr * k + e] = xa;
+    if (rowok && e == 0) u.axa[cr] = mean;
+  }
+}
+
+__global__ void k_post(dk_post_args a) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t ncr = (int64_t)a.nlat * a.nlon * a.nrow;
+  if (t >= ncr) return;
+  const int64_t col = t / a.nrow;
+  const int ilat = (int)(col % a.nlat);
+  const int k = a.k;
+  const double* b = a.xb + t * k;
+  const double* x = a.xa + t * k;
+  double* o = a.xa_out + t * k;
+  const double axb = a.axb[t], axa = a.axa[t];
+  double sb = 0.0, sa = 0.0;
+  for (int e = 0; e < k; ++e) sb += b[e] * b[e];
+  if (fabs((double)a.grid_lat[ilat]) <= a.polar_lat) {
+    for (int e = 0; e < k; ++e) {
+      const double v = a.gamma1 * (axb + b[e]) + a.gamma2 * (axa + x[e]);
+      o[e] = v;
+      sa += (v - axa) * (v - axa);
+    }
+  } else {
+    for (int e = 0; e < k; ++e) o[e] = b[e] + axb;
+    sa = sb;
+  }
+  a.spread_b[t] = sqrt(sb / k);
+  a.spread_a[t] = sqrt(sa / k);
+}
+
+}  // namespace
+
+extern "C" int dk_update_f64(const dk_update_args* u, void* stream) {
+  if (u->ncol <= 0) return 0;
+  hipStream_t s = (hipStream_t)stream;
+  const int k = u->k;
+  if (k < 2 || k > 128) return (int)hipErrorInvalidValue;
+  if (k <= 16) k_update_f64<16><<<u->ncol, kThreads, 0, s>>>(*u);
+  else if (k <= 32) k_update_f64<32><<<u->ncol, kThreads, 0, s>>>(*u);
+  else if (k <= 64) k_update_f64<64><<<u->ncol, kThreads, 0, s>>>(*u);
+  else k_update_f64<128><<<u->ncol, kThreads, 0, s>>>(*u);
+  return (int)hipGetLastError();
+}
+
+extern "C" int dk_post(const dk_post_args* a, void* stream) {
+  const int64_t ncr = (int64_t)a->nlat * a->nlon * a->nrow;
+  if (ncr <= 0) return 0;
+  k_post<<<(unsigned)((ncr + 255) / 256), 256, 0, (hipStream_t)stream>>>(*a);
+  return (int)hipGetLastError();
+}

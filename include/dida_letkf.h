@@ -1,0 +1,194 @@
+/*
+ * dida_letkf.h -- C ABI of the MI355X LETKF local-analysis engine.
+ *
+ * Drop-in boundary for the per-grid-point local analysis of the DIDA DA
+ * cycle.  All entry points are extern "C", take plain pointers and sizes,
+ * and return an int status (DIDA_OK = 0) unless they replace a Fortran
+ * subroutine, in which case they keep its void signature.
+ *
+ * Reference interfaces replaced (paths relative to the reference tree):
+ *   letkf_core_        code/x86/TradDA/DA/LETKF/src/letkf_core.F90:14-15
+ *   letkf_core_batch_  code/sw/TradDA/DA/LETKF/src/letkf_core.F90:16-17
+ *   dida_letkf_sweep   the per-cube loop of letkf_compute_nooverlap,
+ *                      code/x86/TradDA/DA/LETKF/src/letkf_compute.F90:12-126
+ *                      (gen_yb_dis :72, letkf_core :97, xb_2_xa :111)
+ *   dida_letkf_post    the post-analysis blend and spreads of run_da_parallel,
+ *                      code/x86/TradDA/DA/EnDNN/src/module_da_parallel.F90:216-259
+ */
+#ifndef DIDA_LETKF_H
+#define DIDA_LETKF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+#define DIDA_OK 0
+#define DIDA_WARN_NOCONV 1      /* some eigensolves hit the sweep cap */
+#define DIDA_ERR_ARG (-1)       /* invalid argument */
+#define DIDA_ERR_HIP (-2)       /* HIP runtime error */
+#define DIDA_ERR_NOMEM (-3)     /* device allocation failed */
+#define DIDA_ERR_CAPACITY (-4)  /* selection lists exceeded the workspace */
+#define DIDA_ERR_UNSUPPORTED (-5)
+
+/* Precision of the Yb^T R^-1 Yb contraction. */
+#define DIDA_GRAM_FP64 0        /* FP64 MFMA (the plain drop-in) */
+#define DIDA_GRAM_BF16X3 1      /* BF16 MFMA, FP32 accumulate, split hi/lo x3 */
+#define DIDA_GRAM_FP16X3 2      /* FP16 MFMA, FP32 accumulate, split hi/lo x3 */
+#define DIDA_GRAM_BF16 3        /* BF16 MFMA, FP32 accumulate, one pass */
+
+/* Execution form of a sweep. */
+#define DIDA_FORM_PLAIN 0       /* separate transform / update kernels, grid order */
+#define DIDA_FORM_BATCHED 1     /* obs-count-sorted work queue, fused persistent kernel */
+
+typedef struct dida_letkf_params {
+    int32_t nanals;             /* k, ensemble size (letkf.F90:24 ens) */
+    int32_t nrow;               /* n, state rows per column (letkf.F90:26) */
+    int32_t nlat, nlon;         /* grid: column c = ilat + nlat * ilon (nx, ny) */
+    int32_t max_obs;            /* MAX_OBS (letkf.F90:155; 10000) */
+    int32_t gc_filter;          /* 1: Gaspari-Cohn schr2, 0: boxcar schr */
+    double local_dist;          /* L in km (da_loc_distance) */
+    double obs_polar;           /* obs with |lat| > obs_polar are discarded */
+    double inflation_factor;    /* rho, multiplicative (da_inflation_factor) */
+    int32_t rtps;               /* 1: relax-to-prior-spread on Xa' with rtps_alpha */
+    int32_t form;               /* DIDA_FORM_* */
+    double rtps_alpha;
+    int32_t gram;               /* DIDA_GRAM_* */
+    int32_t jacobi_max_sweeps;  /* eigensolver sweep cap */
+    double jacobi_tol;          /* rotate while |a_p.a_q| > tol |a_p||a_q| */
+    double bin_deg;             /* obs bin size in degrees (0: automatic) */
+} dida_letkf_params;
+
+/* Observations, device pointers, one entry per scalar observation (the
+ * reference's obs_list flattened in its (obs, level, variable) order,
+ * read_obs_mod.F90:15-27). */
+typedef struct dida_letkf_obs {
+    int32_t nobs;
+    int32_t pad_;
+    const double *lat, *lon;    /* degrees */
+    const double *yb;           /* [nobs * k] H(Xb) - mean, ensemble fastest (pro_yb) */
+    const double *dep;          /* [nobs] yo - mean H(Xb) (dis_yo) */
+    const float *rinv;          /* [nobs] R^-1, REAL(4) as in the reference */
+} dida_letkf_obs;
+
+/* Diagnostics of the last sweep (dida_letkf_stats synchronises). */
+typedef struct dida_letkf_stats {
+    int64_t nanal;              /* analyses in the sweep */
+    int64_t nobsl_sum;          /* sum of local obs counts */
+    int32_t nobsl_max;
+    int32_t nobsl_capped;       /* analyses truncated at MAX_OBS */
+    int32_t nonconverged;       /* eigensolves that hit the sweep cap */
+    int32_t max_sweeps_used;
+    int32_t capacity_overflow;  /* 1 if the selection lists did not fit */
+    int32_t pad_;
+} dida_letkf_stats;
+
+typedef struct dida_letkf_ctx dida_letkf_ctx;
+
+/* Defaults: max_obs 10000, gc_filter 1, obs_polar 90, rho 1, plain FP64,
+ * jacobi_tol 1e-13, 30 sweeps, automatic bins. */
+void dida_letkf_params_default(dida_letkf_params *prm);
+
+/* ------------------------------------------------------------------------ */
+/* letkf_core_ : per-analysis drop-in for the external Fortran subroutine
+ * letkf_core (code/x86/TradDA/DA/LETKF/src/letkf_core.F90:14-15), same
+ * by-reference arguments and meaning.  Host pointers.  hxens is
+ * (nanals, nobsl) ensemble fastest; trans(i, j) = W_ij + wbar_i at
+ * trans[i + j * nanals].  Runs on the current HIP device; errors are printed
+ * to stderr (the reference has no status argument). */
+void letkf_core_(const int32_t *nanals, const int32_t *nobsl, const double *hxens,
+                 const float *rdiaginv, const double *dep, const double *rloc, double *trans,
+                 double *overline_w, const double *inflation_factor, const int32_t *pflag);
+
+/* letkf_core_batch_ : drop-in for the Sunway batched core
+ * (code/sw/TradDA/DA/LETKF/src/letkf_core.F90:16-17).  Host pointers:
+ * hxens (nanals, max_obs, batch), rdiaginv/dep/rloc (max_obs, batch),
+ * trans (nanals, nanals, batch), overline_w (nanals, batch).  Unlike the
+ * reference, hxens is left unmodified. */
+void letkf_core_batch_(const int32_t *batch, const int32_t *batch_large_matrix,
+                       const int32_t *max_obs, const int32_t *nanals, const int32_t *nobsl,
+                       double *hxens, const float *rdiaginv, const double *dep,
+                       const double *rloc, double *trans, double *overline_w,
+                       const double *inflation_factor, const int32_t *pflag);
+
+/* Status-returning batched core on device pointers (same layout as
+ * letkf_core_batch_), enqueued on `stream` (a hipStream_t, NULL = default).
+ * info[batch] (optional) receives the Jacobi sweep count or -1. */
+int dida_letkf_core_batch_dev(int32_t batch, int32_t max_obs, int32_t nanals,
+                              const int32_t *nobsl, const double *hxens, const float *rdiaginv,
+                              const double *dep, const double *rloc, double *trans,
+                              double *overline_w, double inflation_factor, int32_t gram,
+                              int32_t *info, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Whole-sweep engine: selection -> contractions -> eigensolve -> transform ->
+ * update for every column of the grid, the per-cube loop of
+ * letkf_compute_nooverlap (letkf_compute.F90:12-126) with cube_n = 1.
+ *
+ * nobs_max bounds the observation count of later sweeps; sel_capacity is the
+ * number of (column, obs) selection entries to reserve (0: sized by the first
+ * sweep, which then synchronises once). */
+int dida_letkf_create(dida_letkf_ctx **ctx, const dida_letkf_params *prm, int32_t nobs_max,
+                      int64_t sel_capacity);
+int dida_letkf_destroy(dida_letkf_ctx *ctx);
+
+/* Grid coordinates (REAL(4), degrees, device pointers): grid_lat[nlat],
+ * grid_lon[nlon] (pro_info.F90:11-12). */
+int dida_letkf_set_grid(dida_letkf_ctx *ctx, const float *grid_lat, const float *grid_lon);
+
+/* One analysis sweep.  Device pointers: xb = Xb' (k, n, nlat, nlon) ensemble
+ * fastest, axb = xbar_b (n, nlat, nlon); outputs xa = Xa' (re-centred) and
+ * axa = xbar_a in the same layouts (letkf_run, letkf.F90:157).  Enqueued on
+ * `stream`; no host synchronisation once the selection workspace is sized. */
+int dida_letkf_sweep(dida_letkf_ctx *ctx, const dida_letkf_obs *obs, const double *xb,
+                     const double *axb, double *xa, double *axa, void *stream);
+
+/* Capture one sweep into a HIP graph owned by the context (inputs and
+ * outputs fixed at capture), then replay it with dida_letkf_graph_launch. */
+int dida_letkf_graph_capture(dida_letkf_ctx *ctx, const dida_letkf_obs *obs, const double *xb,
+                             const double *axb, double *xa, double *axa, void *stream);
+int dida_letkf_graph_launch(dida_letkf_ctx *ctx, void *stream);
+
+/* Per-stage device time of the last eager sweep, from HIP events recorded on
+ * the sweep's stream when timing is on (the reference's GPTL regions
+ * "Generate Yb matrix", "Lektf computation core", "Convert xb to xa",
+ * letkf_compute.F90:69-120).  ms[DIDA_STAGE_*]; synchronises. */
+#define DIDA_STAGE_BIN 0        /* obs binning and gather */
+#define DIDA_STAGE_SELECT 1     /* local selection: count, scan, fill */
+#define DIDA_STAGE_TRANSFORM 2  /* contractions, eigensolve, W and wbar */
+#define DIDA_STAGE_UPDATE 3     /* ensemble update */
+#define DIDA_NSTAGES 4
+int dida_letkf_set_timing(dida_letkf_ctx *ctx, int32_t on);
+int dida_letkf_stage_ms(dida_letkf_ctx *ctx, double ms[DIDA_NSTAGES]);
+
+/* Diagnostics of the last sweep; synchronises the context's last stream. */
+int dida_letkf_stats_get(dida_letkf_ctx *ctx, dida_letkf_stats *st);
+
+/* Device arrays of the last sweep (valid until the next sweep):
+ * nobsl[ncol], offset[ncol + 1] (int64), csr_idx / csr_w (binned obs index
+ * and Gaspari-Cohn weight per selection entry), perm[nobs] (binned position
+ * -> caller's obs index), info[ncol] (Jacobi sweeps or -1). */
+int dida_letkf_debug_arrays(dida_letkf_ctx *ctx, const int32_t **nobsl, const int64_t **offset,
+                            const int32_t **csr_idx, const double **csr_w, const int32_t **perm,
+                            const int32_t **info);
+
+/* ------------------------------------------------------------------------ */
+/* Post-analysis (run_da_parallel, module_da_parallel.F90:216-259): inside
+ * |grid_lat| <= polar_lat, xa_out = gamma1 (xbar_b + Xb') + gamma2 (xbar_a +
+ * Xa'); outside, the background; spread_b/a = sqrt(sum_e (.)^2 / k).
+ * Device pointers; xa_out is (k, n, nlat, nlon), spreads (n, nlat, nlon). */
+int dida_letkf_post(int32_t k, int32_t nrow, int32_t nlat, int32_t nlon, const float *grid_lat,
+                    double polar_lat, double gamma1, double gamma2, const double *xb,
+                    const double *axb, const double *xa, const double *axa, double *xa_out,
+                    double *spread_b, double *spread_a, void *stream);
+
+/* Last error message of this thread (static storage). */
+const char *dida_letkf_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,165 @@
+/*
+ * CPU oracle for the LETKF local-analysis path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: the tests, the
+ * smoke entry point and the CPU-baseline leg of the benchmark link it; the
+ * product library never does.
+ *
+ * It is a plain-C FP64 restatement of the reference x86 path:
+ *   orc_latlon_dist   code/x86/TradDA/DA/utils/pro_info.F90:148-161
+ *   orc_schr2/schr    code/x86/TradDA/DA/LETKF/src/letkf_yb.F90:245-285
+ *   orc_select_*      code/x86/TradDA/DA/LETKF/src/letkf_yb.F90:3-133 (gen_yb_dis)
+ *   orc_letkf_core    code/x86/TradDA/DA/LETKF/src/letkf_core.F90:14-233
+ *   orc_xb_2_xa       code/x86/TradDA/DA/LETKF/src/letkf_compute.F90:231-301
+ *   orc_sweep         code/x86/TradDA/DA/LETKF/src/letkf_compute.F90:12-126
+ *   orc_centre_obs    code/x86/TradDA/DA/LETKF/src/letkf_obs.F90:156-178
+ *   orc_post          code/x86/TradDA/DA/EnDNN/src/module_da_parallel.F90:216-259
+ * and keeps the reference's numeric quirks: single-precision literals in the
+ * distance and Gaspari-Cohn code, REAL(4) R^-1 and grid coordinates, strict
+ * ">" thresholds and MAX_OBS truncation in scan order.
+ *
+ * Pinning (tests/test_oracle_*.py): Gaspari-Cohn against its exact rational
+ * closed form and the values the single-precision literals fix; distance
+ * against the haversine formula; letkf_core against p = 0 (W = sqrt(rho) I),
+ * p = 1 (Sherman-Morrison closed form), the invariants W = W^T,
+ * W^2 = (k-1) Pa, Pa^-1 = (k-1)/rho I + C, W 1 = sqrt(rho) 1, 1^T wbar = 0,
+ * and scipy eigh / long-double brute force; selection against a haversine
+ * brute force; the update against a dense product.
+ */
+#ifndef DIDA_LETKF_ORACLE_H
+#define DIDA_LETKF_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Great-circle distance in km, FP64 arithmetic with the reference's REAL(4)
+ * constants (a = 6371.393f, pi = 4*atan(1.0f)).  x = latitude, y = longitude,
+ * degrees. */
+double orc_latlon_dist(double x1, double y1, double x2, double y2);
+
+/* Gaspari-Cohn fifth-order compact support (schr2) and the boxcar (schr). */
+double orc_schr2(double r);
+double orc_schr(double r);
+
+/* Symmetric eigensolver (cyclic two-sided Jacobi, FP64).  a: n*n column-major,
+ * only read as symmetric; on return holds eigenvectors (columns), w ascending
+ * eigenvalues.  Returns the number of sweeps, or -1 if not converged. */
+int orc_syev_jacobi(int n, double *a, double *w, int max_sweeps);
+
+/* letkf_core: trans[i + j*k] = W_ij + wbar_i, wbar[k].  hxens: k*p, ensemble
+ * fastest.  Returns eigensolver sweeps (or -1). */
+int orc_letkf_core(int k, int p, const double *hxens, const float *rdiaginv,
+                   const double *dep, const double *rloc, double *trans,
+                   double *wbar, double inflation_factor);
+
+/* xb_2_xa for ncells rows of one cube: xa[e + c*k] = sum_i trans[i + e*k] *
+ * xb[i + c*k] + axb[c], then re-centred; axa[c] = ensemble mean. */
+void orc_xb_2_xa(int k, int ncells, const double *xb, const double *axb,
+                 const double *trans, double *xa, double *axa);
+
+/* Observation arrays, scalar records in scan order (one entry per
+ * (obs, level, variable) of the reference's obs_list, flattened in its
+ * (l, m, n) loop order). */
+typedef struct orc_obs {
+    int32_t nobs;
+    const double *lat;      /* degrees */
+    const double *lon;      /* degrees */
+    const double *yb;       /* k * nobs, ensemble fastest (pro_yb) */
+    const double *dep;      /* nobs (dis_yo, already centred) */
+    const float *rinv;      /* nobs (obs_rinv, REAL(4)) */
+} orc_obs;
+
+typedef struct orc_params {
+    int32_t k;              /* ensemble size (nanals) */
+    int32_t nrow;           /* state rows per column (n) */
+    int32_t nlat, nlon;     /* grid; column c = ilat + nlat*ilon */
+    int32_t max_obs;        /* MAX_OBS */
+    int32_t gc_filter;      /* 1: schr2, 0: schr */
+    double local_dist;      /* L, km */
+    double obs_polar;       /* |lat| <= obs_polar */
+    double inflation;       /* rho */
+} orc_params;
+
+/* One column's local selection by a scan of every obs in the given order
+ * (gen_yb_dis with a single neighbour block).  Writes up to max_obs indices
+ * and weights; returns nobsl. */
+int orc_select_column(const orc_params *prm, const orc_obs *obs,
+                      double center_lat, double center_lon,
+                      int32_t *idx, double *rloc);
+
+/* The observation bins used to prune the scan (shared geometry with the
+ * device path): nband = ceil(180 / bin_deg) bands of 180 / nband degrees,
+ * nlonbin = ceil(360 / bin_deg) longitude bins of 360 / nlonbin degrees, so
+ * the bins tile the sphere exactly; band = floor((lat + 90) / dlat), lon bin
+ * = floor(lon_wrapped / dlon); obs with |lat| > obs_polar go to the sentinel
+ * bin nbins.  Fills key[nobs]; returns nbins. */
+int orc_bin_keys(const orc_obs *obs, double bin_deg, double obs_polar,
+                 int32_t *key, int32_t *nband, int32_t *nlonbin);
+
+/* Stable counting sort of obs indices by key: perm[i] = original index of
+ * the i-th obs in binned order; bin_start[nbins + 2]. */
+void orc_bin_sort(int nobs, const int32_t *key, int nbins, int32_t *perm,
+                  int32_t *bin_start);
+
+/* Candidate bin ranges of a column (pairs of inclusive bin ids, ascending);
+ * ranges must hold 4 * nband ints.  Returns the number of ranges. */
+int orc_candidate_ranges(double clat, double clon, double local_dist,
+                         double bin_deg, int nband, int nlonbin, int32_t *ranges);
+
+/* Selection over the binned list (obs already permuted into binned order),
+ * visiting only candidate bins; same result as orc_select_column on the
+ * binned list. */
+int orc_select_column_binned(const orc_params *prm, const orc_obs *obs,
+                             const int32_t *bin_start, int nband, int nlonbin,
+                             double bin_deg, double center_lat,
+                             double center_lon, int32_t *idx, double *rloc);
+
+typedef struct orc_bins {
+    const int32_t *bin_start; /* nbins + 2 */
+    int32_t nband, nlonbin;
+    double bin_deg;
+} orc_bins;
+
+/* Whole sweep (letkf_compute_nooverlap with cube_n = 1): one analysis per
+ * column, OpenMP over columns.  If cols != NULL only the ncols listed
+ * columns are analysed (others untouched).  Outputs may be NULL except xa,
+ * axa.  nobsl_out[col], trans_out[col*k*k], wbar_out[col*k] optional.
+ * bins != NULL: obs are in binned order and only candidate bins are scanned.
+ * Returns the number of analyses whose eigensolve did not converge. */
+int orc_sweep(const orc_params *prm, const float *grid_lat,
+              const float *grid_lon, const orc_obs *obs, const double *xb,
+              const double *axb, double *xa, double *axa, const int32_t *cols,
+              int32_t ncols, int32_t *nobsl_out, double *trans_out,
+              double *wbar_out, const orc_bins *bins, int nthreads);
+
+/* The sweep restricted to the listed columns with compact state arrays:
+ * xb/xa [ncols * nrow * k] and axb/axa [ncols * nrow] hold column cols[w] at
+ * position w.  Used for sampled CPU baselines. */
+int orc_sweep_compact(const orc_params *prm, const float *grid_lat, const float *grid_lon,
+                      const orc_obs *obs, const orc_bins *bins, const int32_t *cols,
+                      int32_t ncols, const double *xb, const double *axb, double *xa,
+                      double *axa, int32_t *nobsl_out, int nthreads);
+
+/* Observation centring: ybar = mean_e hxb; dep = yo - ybar; yb = hxb - ybar. */
+void orc_centre_obs(int k, int nobs, const double *hxb, const double *yo,
+                    double *yb, double *dep);
+
+/* Post-analysis blend and spreads (run_da_parallel):
+ *   mode 0: xa_out = gamma1 (xb + axb) + gamma2 (xa + axa) inside
+ *           |lat| <= polar_lat, background outside; spreads as reference.
+ *   mode 1: RTPS (not in the reference): xa' <- xa' (alpha sb/sa + 1 - alpha),
+ *           xa_out = axa + xa'.
+ * xb, xa: k*nrow*ncol; axb, axa: nrow*ncol; spread_b/a: nrow*ncol. */
+void orc_post(int mode, int k, int nrow, int nlat, int nlon,
+              const float *grid_lat, double polar_lat, double gamma1,
+              double gamma2, double alpha, const double *xb,
+              const double *axb, const double *xa, const double *axa,
+              double *xa_out, double *spread_b, double *spread_a);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
