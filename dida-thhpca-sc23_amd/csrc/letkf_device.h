@@ -69,7 +69,8 @@ __device__ __forceinline__ double shfl_xor_d(double v, int mask) {
 
 // FP64 MFMA 16x16x4: D(16x16) += A(16x4) * B(4x16).
 // Operand fragments: lane l holds A[l%16][l/16] and B[l/16][l%16]; the result
-// holds D[4*(l/16)+r][l%16] in element r.
+// holds D[4*r + l/16][l%16] in element r (rows interleaved across the four
+// 16-lane groups; measured on gfx950).
 typedef double double4_t __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ double4_t mfma_f64_16x16x4(double a, double b, double4_t c) {

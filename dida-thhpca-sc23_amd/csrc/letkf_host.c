@@ -54,8 +54,16 @@ void dida_letkf_params_default(dida_letkf_params *p)
     p->form = DIDA_FORM_PLAIN;
     p->gram = DIDA_GRAM_FP64;
     p->jacobi_max_sweeps = 30;
-    p->jacobi_tol = 1e-13;
+    p->jacobi_tol = 0.0;
     p->bin_deg = 0.0;
+    p->chain_len = 0;
+    p->warm_start = 1;
+}
+
+static double jacobi_tol_of(const dida_letkf_params *p)
+{
+    if (p->jacobi_tol > 0.0) return p->jacobi_tol;
+    return p->gram == DIDA_GRAM_FP64 ? 1e-13 : 1e-9;
 }
 
 static int check_params(const dida_letkf_params *p)
@@ -68,11 +76,14 @@ static int check_params(const dida_letkf_params *p)
     if (!(p->local_dist > 0.0)) return set_err(DIDA_ERR_ARG, "local_dist must be > 0");
     if (!(p->inflation_factor > 0.0)) return set_err(DIDA_ERR_ARG, "inflation_factor must be > 0");
     if (p->jacobi_max_sweeps < 1) return set_err(DIDA_ERR_ARG, "jacobi_max_sweeps < 1");
-    if (!(p->jacobi_tol > 0.0)) return set_err(DIDA_ERR_ARG, "jacobi_tol must be > 0");
-    if (p->form != DIDA_FORM_PLAIN)
+    if (p->jacobi_tol < 0.0) return set_err(DIDA_ERR_ARG, "jacobi_tol must be >= 0");
+    if (p->chain_len < 0) return set_err(DIDA_ERR_ARG, "chain_len must be >= 0");
+    if (p->form != DIDA_FORM_PLAIN && p->form != DIDA_FORM_BATCHED)
         return set_err(DIDA_ERR_UNSUPPORTED, "form %d is not available", p->form);
-    if (p->gram != DIDA_GRAM_FP64)
+    if (p->gram < DIDA_GRAM_FP64 || p->gram > DIDA_GRAM_BF16)
         return set_err(DIDA_ERR_UNSUPPORTED, "gram precision %d is not available", p->gram);
+    if (p->form == DIDA_FORM_PLAIN && p->gram != DIDA_GRAM_FP64)
+        return set_err(DIDA_ERR_UNSUPPORTED, "the plain form computes the Gram in FP64 only");
     return DIDA_OK;
 }
 
@@ -107,6 +118,13 @@ struct dida_letkf_ctx {
     int32_t chunk;
     double *trans;
     int32_t *info;
+    /* batched form: chain queue */
+    int32_t chain_len, chains_per_lon, nchain, nblocks;
+    uint32_t *chain_key, *chain_key_sorted;
+    int32_t *chain_id, *chain_order, *queue_head;
+    void *chain_tmp;
+    size_t chain_tmp_bytes;
+    double *ymax;
     /* per-stage HIP event timing (eager sweeps only) */
     int timing;
     int nmark;
@@ -117,6 +135,7 @@ struct dida_letkf_ctx {
     hipGraphExec_t gexec;
     hipStream_t last_stream;
     int32_t last_nobs;
+    int32_t last_nobs_pending; /* obs count of the sweep being enqueued */
 };
 
 static void dfree(void *p)
@@ -134,7 +153,8 @@ int dida_letkf_destroy(dida_letkf_ctx *c)
     void *ptrs[] = {c->key, c->key_sorted, c->iota, c->perm, c->bin_count, c->bin_start,
                     c->sort_tmp, c->cx, c->sx, c->yr, c->yb_b, c->dep_b, c->rinv_b,
                     c->nobsl, c->offset, c->scan_tmp, c->csr_idx, c->csr_w, c->flags,
-                    c->trans, c->info};
+                    c->trans, c->info, c->chain_key, c->chain_key_sorted, c->chain_id,
+                    c->chain_order, c->queue_head, c->chain_tmp, c->ymax};
     for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; ++i) dfree(ptrs[i]);
     free(c);
     return DIDA_OK;
@@ -216,9 +236,34 @@ int dida_letkf_create(dida_letkf_ctx **out, const dida_letkf_params *prm, int32_
         DMALLOC(c->csr_idx, (size_t)sel_capacity * sizeof(int32_t));
         DMALLOC(c->csr_w, (size_t)sel_capacity * sizeof(double));
     }
-    c->chunk = (int32_t)(c->ncol < 32768 ? c->ncol : 32768);
-    DMALLOC(c->trans, (size_t)c->chunk * k * k * sizeof(double));
     DMALLOC(c->info, (size_t)c->ncol * sizeof(int32_t));
+    if (prm->form == DIDA_FORM_PLAIN) {
+        c->chunk = (int32_t)(c->ncol < 32768 ? c->ncol : 32768);
+        DMALLOC(c->trans, (size_t)c->chunk * k * k * sizeof(double));
+    } else {
+        c->chain_len = prm->chain_len > 0 ? prm->chain_len : 8;
+        if (c->chain_len > prm->nlat) c->chain_len = prm->nlat;
+        c->chains_per_lon = (prm->nlat + c->chain_len - 1) / c->chain_len;
+        c->nchain = c->chains_per_lon * prm->nlon;
+        DMALLOC(c->chain_key, (size_t)c->nchain * sizeof(uint32_t));
+        DMALLOC(c->chain_key_sorted, (size_t)c->nchain * sizeof(uint32_t));
+        DMALLOC(c->chain_id, (size_t)c->nchain * sizeof(int32_t));
+        DMALLOC(c->chain_order, (size_t)c->nchain * sizeof(int32_t));
+        DMALLOC(c->queue_head, 4 * sizeof(int32_t));
+        DMALLOC(c->ymax, no * sizeof(double));
+        if (dk_sort_pairs_u32(NULL, &c->chain_tmp_bytes, c->chain_key, c->chain_key_sorted,
+                              c->chain_id, c->chain_order, c->nchain, 32, NULL)) {
+            dida_letkf_destroy(c);
+            return set_err(DIDA_ERR_HIP, "chain sort size query failed");
+        }
+        DMALLOC(c->chain_tmp, c->chain_tmp_bytes);
+        int dev = 0, ncu = 0;
+        (void)hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            ncu <= 0)
+            ncu = 256;
+        c->nblocks = ncu * dk_batched_blocks_per_cu(k, prm->gram);
+    }
     *out = c;
     return DIDA_OK;
 }
@@ -246,6 +291,7 @@ static int sweep_select_count(dida_letkf_ctx *c, const dida_letkf_obs *o, hipStr
                               dk_select_args *sa)
 {
     const int k = c->prm.nanals;
+    c->last_nobs_pending = o->nobs;
     c->nmark = 0;
     tmark(c, -1, s);
     HIP_TRY(dk_fill_i32(c->bin_count, 0, c->geom.nbins + 1, s));
@@ -301,6 +347,56 @@ static int ensure_capacity(dida_letkf_ctx *c, hipStream_t s)
     return DIDA_OK;
 }
 
+/* Batched form: chain queue by cost, then the persistent fused kernel. */
+static int sweep_batched(dida_letkf_ctx *c, const double *xb, const double *axb, double *xa,
+                         double *axa, hipStream_t s)
+{
+    dk_batched_args ba;
+    memset(&ba, 0, sizeof ba);
+    ba.k = c->prm.nanals;
+    ba.nrow = c->prm.nrow;
+    ba.nlat = c->prm.nlat;
+    ba.nlon = c->prm.nlon;
+    ba.nchain = c->nchain;
+    ba.chains_per_lon = c->chains_per_lon;
+    ba.chain_len = c->chain_len;
+    ba.gram = c->prm.gram;
+    ba.warm_start = c->prm.warm_start ? 1 : 0;
+    ba.rtps = c->prm.rtps;
+    ba.max_sweeps = c->prm.jacobi_max_sweeps;
+    ba.max_p = c->prm.max_obs;
+    ba.rtps_alpha = c->prm.rtps_alpha;
+    ba.rho = c->prm.inflation_factor;
+    ba.tol = jacobi_tol_of(&c->prm);
+    ba.capacity = c->capacity;
+    ba.offset = c->offset;
+    ba.nobsl = c->nobsl;
+    ba.idx = c->csr_idx;
+    ba.rloc = c->csr_w;
+    ba.yb = c->yb_b;
+    ba.rinv = c->rinv_b;
+    ba.dep = c->dep_b;
+    ba.ymax = c->ymax;
+    ba.xb = xb;
+    ba.axb = axb;
+    ba.xa = xa;
+    ba.axa = axa;
+    ba.info = c->info;
+    ba.queue = c->chain_order;
+    ba.queue_head = c->queue_head;
+    if (c->prm.gram == DIDA_GRAM_FP16X3)
+        HIP_TRY(dk_obs_ymax(c->last_nobs_pending, ba.k, c->yb_b, c->rinv_b, c->ymax, s));
+    HIP_TRY(dk_chain_keys(&ba, c->chain_key, c->chain_id, s));
+    size_t tb = c->chain_tmp_bytes;
+    HIP_TRY(dk_sort_pairs_u32(c->chain_tmp, &tb, c->chain_key, c->chain_key_sorted, c->chain_id,
+                              c->chain_order, c->nchain, 32, s));
+    HIP_TRY(dk_fill_i32(c->queue_head, 0, 1, s));
+    tmark(c, DIDA_STAGE_SELECT, s);
+    HIP_TRY(dk_letkf_batched(&ba, c->nblocks, s));
+    tmark(c, DIDA_STAGE_TRANSFORM, s);
+    return DIDA_OK;
+}
+
 static int sweep_rest(dida_letkf_ctx *c, dk_select_args *sa, const double *xb,
                       const double *axb, double *xa, double *axa, hipStream_t s)
 {
@@ -310,6 +406,8 @@ static int sweep_rest(dida_letkf_ctx *c, dk_select_args *sa, const double *xb,
     sa->capacity = c->capacity;
     HIP_TRY(dk_select_fill(sa, s));
     tmark(c, DIDA_STAGE_SELECT, s);
+
+    if (c->prm.form == DIDA_FORM_BATCHED) return sweep_batched(c, xb, axb, xa, axa, s);
 
     dk_core_args ca;
     memset(&ca, 0, sizeof ca);
@@ -324,7 +422,7 @@ static int sweep_rest(dida_letkf_ctx *c, dk_select_args *sa, const double *xb,
     ca.rinv = c->rinv_b;
     ca.dep = c->dep_b;
     ca.rho = c->prm.inflation_factor;
-    ca.tol = c->prm.jacobi_tol;
+    ca.tol = jacobi_tol_of(&c->prm);
     ca.max_sweeps = c->prm.jacobi_max_sweeps;
     ca.trans = c->trans;
     ca.wbar = NULL;

@@ -123,6 +123,45 @@ typedef struct dk_update_args {
 
 int dk_update_f64(const dk_update_args *a, void *stream);
 
+/* Precision of the Gram in the batched form (values of DIDA_GRAM_*). */
+#define DK_GRAM_FP64 0
+#define DK_GRAM_BF16X3 1
+#define DK_GRAM_FP16X3 2
+#define DK_GRAM_BF16 3
+
+/* Batched form: chains of chain_len adjacent columns along a meridian,
+ * drained from a queue by a persistent kernel. */
+typedef struct dk_batched_args {
+    int32_t k, nrow, nlat, nlon;
+    int32_t nchain, chains_per_lon, chain_len, gram;
+    int32_t warm_start, rtps, max_sweeps, max_p;
+    double rtps_alpha, rho, tol;
+    int64_t capacity;
+    const int64_t *offset;
+    const int32_t *nobsl;
+    const int32_t *idx;     /* CSR binned obs index */
+    const double *rloc;
+    const double *yb;       /* binned obs, k per obs */
+    const float *rinv;
+    const double *dep;
+    const double *ymax;     /* per binned obs: max |yb| sqrt(R^-1) (FP16 scaling) */
+    const double *xb, *axb;
+    double *xa, *axa;
+    int32_t *info;          /* [col] Jacobi sweeps or -1 */
+    double *wbar_out;       /* [col * k] (optional) */
+    const int32_t *queue;   /* chain ids in processing order (NULL: identity) */
+    int32_t *queue_head;    /* work counter, zero before the launch */
+} dk_batched_args;
+
+/* Resident workgroups per CU of the batched kernel for (k, gram). */
+int dk_batched_blocks_per_cu(int k, int gram);
+/* Sort keys of the chains: 2^32-1 - cost (ascending sort = largest first). */
+int dk_chain_keys(const dk_batched_args *a, uint32_t *key, int32_t *id, void *stream);
+int dk_letkf_batched(const dk_batched_args *a, int nblocks, void *stream);
+/* ymax[i] = max_e |yb[i k + e]| sqrt(rinv[i]). */
+int dk_obs_ymax(int32_t nobs, int32_t k, const double *yb, const float *rinv, double *ymax,
+                void *stream);
+
 /* Post-analysis blend of run_da_parallel (RTPP-style gamma blend inside
  * |lat| <= polar_lat, background outside) and spreads. */
 typedef struct dk_post_args {

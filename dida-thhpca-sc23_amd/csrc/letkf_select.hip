@@ -104,6 +104,15 @@ __global__ void k_obs_gather_yb(int32_t nobs, int32_t k, const int32_t* __restri
   yb_b[t] = yb[(int64_t)perm[i] * k + e];
 }
 
+__global__ void k_obs_ymax(int32_t nobs, int32_t k, const double* __restrict__ yb,
+                           const float* __restrict__ rinv, double* __restrict__ ymax) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nobs) return;
+  double m = 0.0;
+  for (int e = 0; e < k; ++e) m = fmax(m, fabs(yb[(int64_t)i * k + e]));
+  ymax[i] = m * sqrt((double)rinv[i]);
+}
+
 // Visit the candidate bins of one column; call f(l0, l1) per contiguous obs
 // range in ascending bin order; f returns false to stop.
 template <class F>
@@ -241,6 +250,13 @@ extern "C" int dk_obs_gather(int32_t nobs, int32_t k, const int32_t* perm, const
                                                   dep_b, rinv_b);
   int64_t total = (int64_t)nobs * k;
   k_obs_gather_yb<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(nobs, k, perm, yb, yb_b);
+  return (int)hipGetLastError();
+}
+
+extern "C" int dk_obs_ymax(int32_t nobs, int32_t k, const double* yb, const float* rinv,
+                            double* ymax, void* stream) {
+  if (nobs <= 0) return 0;
+  k_obs_ymax<<<(nobs + 255) / 256, 256, 0, (hipStream_t)stream>>>(nobs, k, yb, rinv, ymax);
   return (int)hipGetLastError();
 }
 

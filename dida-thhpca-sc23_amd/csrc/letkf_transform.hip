@@ -15,6 +15,7 @@
 //      form, so the null-space eigenvectors (which one-sided Jacobi does not
 //      deliver) are never needed.  trans = W + wbar 1^T.
 #include "letkf_device.h"
+#include "letkf_jacobi.h"
 #include "letkf_kernels.h"
 
 using namespace dida;
@@ -93,7 +94,7 @@ __device__ __forceinline__ void gram_f64(const dk_core_args& a, int64_t off, int
           const double4_t v = acc[I * (I + 1) / 2 + J];
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            const int i = I * 16 + 4 * q4 + r, j = J * 16 + r16;
+            const int i = I * 16 + 4 * r + q4, j = J * 16 + r16;
             if (w == 0) {
               sm.A[i + j * LDA] = v[r];
               if (I != J) sm.A[j + i * LDA] = v[r];
@@ -115,85 +116,6 @@ __device__ __forceinline__ void gram_f64(const dk_core_args& a, int64_t off, int
   }
 }
 
-// One-sided Jacobi on the first k rows / m = k rounded up to even columns of
-// sm.A.  Returns the number of sweeps; *converged tells whether the last
-// sweep made no rotation.
-template <int KP>
-__device__ int jacobi_one_sided(CoreSmem<KP>& sm, int k, double tol, int max_sweeps,
-                                bool* converged) {
-  constexpr int LDA = CoreSmem<KP>::LDA;
-  const int m = (k + 1) & ~1;
-  const int npair = m >> 1;
-  int tpp = 1;
-  while (tpp * 2 * npair <= kThreads && tpp < 32) tpp <<= 1;
-  const int tid = threadIdx.x;
-  const int pair = tid / tpp, sub = tid % tpp;
-  const bool active = pair < npair;
-  if (tid == 0) sm.flag[0] = 0;
-  int sweep = 0;
-  *converged = false;
-  for (; sweep < max_sweeps; ++sweep) {
-    for (int j = tid; j < m; j += kThreads) {
-      double s = 0.0;
-      for (int i = 0; i < k; ++i) s += sm.A[i + j * LDA] * sm.A[i + j * LDA];
-      sm.nrm[j] = s;
-    }
-    __syncthreads();
-    bool rotated = false;
-    for (int r = 0; r < m - 1; ++r) {
-      if (active) {
-        int p, q;
-        if (pair == 0) {
-          p = m - 1;
-          q = r;
-        } else {
-          p = (r - pair + (m - 1)) % (m - 1);
-          q = (r + pair) % (m - 1);
-        }
-        if (p > q) {
-          const int t = p;
-          p = q;
-          q = t;
-        }
-        double* ap = sm.A + p * LDA;
-        double* aq = sm.A + q * LDA;
-        double g = 0.0;
-        for (int i = sub; i < k; i += tpp) g += ap[i] * aq[i];
-        for (int s = 1; s < tpp; s <<= 1) g += shfl_xor_d(g, s);
-        const double al = sm.nrm[p], be = sm.nrm[q];
-        if (g != 0.0 && fabs(g) > tol * sqrt(al * be)) {
-          const double zeta = (be - al) / (2.0 * g);
-          double t;
-          if (fabs(zeta) > 1e100) t = 0.5 / zeta;
-          else t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-          const double c = 1.0 / sqrt(1.0 + t * t);
-          const double s = c * t;
-          for (int i = sub; i < k; i += tpp) {
-            const double x = ap[i], y = aq[i];
-            ap[i] = c * x - s * y;
-            aq[i] = s * x + c * y;
-          }
-          if (sub == 0) {
-            sm.nrm[p] = al - t * g;
-            sm.nrm[q] = be + t * g;
-          }
-          rotated = true;
-        }
-      }
-      __syncthreads();
-    }
-    if (rotated) sm.flag[sweep & 1] = 1;
-    __syncthreads();
-    if (sm.flag[sweep & 1] == 0) {
-      *converged = true;
-      ++sweep;
-      break;
-    }
-    if (tid == 0) sm.flag[(sweep + 1) & 1] = 0;
-  }
-  return sweep;
-}
-
 template <int KP>
 __global__ __launch_bounds__(kThreads) void k_letkf_core_f64(dk_core_args a) {
   constexpr int NT = KP / 16;
@@ -212,7 +134,8 @@ __global__ __launch_bounds__(kThreads) void k_letkf_core_f64(dk_core_args a) {
   gram_f64<KP>(a, off, p, sm);
 
   bool conv;
-  const int sweeps = jacobi_one_sided<KP>(sm, k, a.tol, a.max_sweeps, &conv);
+  const int sweeps = jacobi_lds<LDA, kThreads>(sm.A, sm.nrm, sm.flag, k, a.tol, a.max_sweeps,
+                                               &conv);
   const int m = (k + 1) & ~1;
 
   // exact column norms: alpha_j = lambda_j^2
@@ -274,7 +197,7 @@ __global__ __launch_bounds__(kThreads) void k_letkf_core_f64(dk_core_args a) {
     }
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int i = I * 16 + 4 * q4 + r, i2 = I2 * 16 + r16;
+      const int i = I * 16 + 4 * r + q4, i2 = I2 * 16 + r16;
       if (i < k && i2 < k) trans[i + (int64_t)i2 * k] = (i == i2 ? srho : 0.0) + d[r] + sm.wb[i];
     }
   }

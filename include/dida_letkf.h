@@ -58,8 +58,13 @@ typedef struct dida_letkf_params {
     double rtps_alpha;
     int32_t gram;               /* DIDA_GRAM_* */
     int32_t jacobi_max_sweeps;  /* eigensolver sweep cap */
-    double jacobi_tol;          /* rotate while |a_p.a_q| > tol |a_p||a_q| */
+    double jacobi_tol;          /* rotate while |a_p.a_q| > tol |a_p||a_q|
+                                   (0: 1e-13 with an FP64 Gram, 1e-9 otherwise) */
     double bin_deg;             /* obs bin size in degrees (0: automatic) */
+    int32_t chain_len;          /* batched form: adjacent columns per work item,
+                                   each warm-starting the next one's eigensolve
+                                   (0: automatic) */
+    int32_t warm_start;         /* batched form: 1 to warm-start along chains */
 } dida_letkf_params;
 
 /* Observations, device pointers, one entry per scalar observation (the
@@ -89,7 +94,8 @@ typedef struct dida_letkf_stats {
 typedef struct dida_letkf_ctx dida_letkf_ctx;
 
 /* Defaults: max_obs 10000, gc_filter 1, obs_polar 90, rho 1, plain FP64,
- * jacobi_tol 1e-13, 30 sweeps, automatic bins. */
+ * automatic Jacobi tolerance, 30 sweeps, automatic bins, chains of 8 with
+ * warm start (batched form). */
 void dida_letkf_params_default(dida_letkf_params *prm);
 
 /* ------------------------------------------------------------------------ */
