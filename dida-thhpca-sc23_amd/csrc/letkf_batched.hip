@@ -49,7 +49,33 @@ struct BSmem {
   double red[4];
   int flag[2];
   int item;
+  // per-obs scalars of the current block of kObsBlock local obs
+  int32_t ogi[256];
+  double orr[256];
+  double osd[256];
 };
+
+constexpr int kObsBlock = 256;
+
+// Stage obs [base, base + kObsBlock) of the analysis: binned index,
+// sqrt(R^-1 rloc) and sqrt(R^-1 rloc) dep.  Ends with a barrier.
+template <int KP>
+__device__ __forceinline__ void stage_obs(const dk_batched_args& a, int64_t off, int p, int base,
+                                          BSmem<KP>& sm) {
+  const int o = base + threadIdx.x;
+  int gi = 0;
+  double rr = 0.0, sd = 0.0;
+  if (o < p) {
+    gi = a.idx[off + o];
+    rr = sqrt((double)a.rinv[gi] * a.rloc[off + o]);
+    sd = rr * a.dep[gi];
+  }
+  __syncthreads();
+  sm.ogi[threadIdx.x] = gi;
+  sm.orr[threadIdx.x] = rr;
+  sm.osd[threadIdx.x] = sd;
+  __syncthreads();
+}
 
 template <int GRAM>
 struct Half;
@@ -118,47 +144,49 @@ __device__ void gram_half(const dk_batched_args& a, int64_t off, int p, BSmem<KP
 #pragma unroll
   for (int I = 0; I < NB; ++I) bacc[I] = 0.0;
 
-  for (int c = wave; c * 16 < p; c += 4) {
-    hvec hi[NB], lo[NB];
+  for (int base = 0; base < p; base += kObsBlock) {
+    stage_obs<KP>(a, off, p, base, sm);
+    const int nb = min(kObsBlock, p - base);
+    // chunks of 16 obs; wave w takes chunks w, w + 4, ...
+    for (int c = wave; c * 16 < nb; c += 4) {
+      hvec hi[NB], lo[NB];
+      double x[8][NB];
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const int o = c * 16 + 8 * g2 + e;
-      double x[NB];
-      double sd = 0.0;
-      if (o < p) {
-        const int64_t gi = a.idx[off + o];
-        const double rr = sqrt((double)a.rinv[gi] * a.rloc[off + o]);
-        const double* y = a.yb + gi * k;
+      for (int e = 0; e < 8; ++e) {
+        const int ol = c * 16 + 8 * g2 + e;
+        const double rr = sm.orr[ol];
+        const double* y = a.yb + (int64_t)sm.ogi[ol] * k;
 #pragma unroll
         for (int I = 0; I < NB; ++I) {
           const int row = I * 32 + r32;
-          x[I] = row < k ? y[row] * rr : 0.0;
+          x[e][I] = (row < k && ol < nb) ? y[row] * rr : 0.0;
         }
-        sd = rr * a.dep[gi];
-      } else {
-#pragma unroll
-        for (int I = 0; I < NB; ++I) x[I] = 0.0;
       }
 #pragma unroll
-      for (int I = 0; I < NB; ++I) {
-        bacc[I] += x[I] * sd;
-        const float xs = (float)(x[I] * scale);
-        const helt h = (helt)xs;
-        hi[I][e] = h;
-        if (kSplit) lo[I][e] = (helt)(xs - (float)h);
+      for (int e = 0; e < 8; ++e) {
+        const int ol = c * 16 + 8 * g2 + e;
+        const double sd = ol < nb ? sm.osd[ol] : 0.0;
+#pragma unroll
+        for (int I = 0; I < NB; ++I) {
+          bacc[I] += x[e][I] * sd;
+          const float xs = (float)(x[e][I] * scale);
+          const helt h = (helt)xs;
+          hi[I][e] = h;
+          if (kSplit) lo[I][e] = (helt)(xs - (float)h);
+        }
       }
+#pragma unroll
+      for (int I = 0; I < NB; ++I)
+#pragma unroll
+        for (int J = 0; J <= I; ++J) {
+          const int t = I * (I + 1) / 2 + J;
+          acc[t] = Half<GRAM>::mfma(hi[I], hi[J], acc[t]);
+          if (kSplit) {
+            acc[t] = Half<GRAM>::mfma(hi[I], lo[J], acc[t]);
+            acc[t] = Half<GRAM>::mfma(lo[I], hi[J], acc[t]);
+          }
+        }
     }
-#pragma unroll
-    for (int I = 0; I < NB; ++I)
-#pragma unroll
-      for (int J = 0; J <= I; ++J) {
-        const int t = I * (I + 1) / 2 + J;
-        acc[t] = Half<GRAM>::mfma(hi[I], hi[J], acc[t]);
-        if (kSplit) {
-          acc[t] = Half<GRAM>::mfma(hi[I], lo[J], acc[t]);
-          acc[t] = Half<GRAM>::mfma(lo[I], hi[J], acc[t]);
-        }
-      }
   }
 #pragma unroll
   for (int I = 0; I < NB; ++I) bacc[I] += shfl_xor_d(bacc[I], 32);
@@ -210,31 +238,36 @@ __device__ void gram_f64(const dk_batched_args& a, int64_t off, int p, BSmem<KP>
   double bacc[NT];
 #pragma unroll
   for (int t = 0; t < NT; ++t) bacc[t] = 0.0;
-  for (int c = wave; c * 4 < p; c += 4) {
-    const int o = c * 4 + q4;
-    double f[NT];
-    double sd = 0.0;
-    if (o < p) {
-      const int64_t gi = a.idx[off + o];
-      const double rr = sqrt((double)a.rinv[gi] * a.rloc[off + o]);
-      const double* y = a.yb + gi * k;
+  for (int base = 0; base < p; base += kObsBlock) {
+    stage_obs<KP>(a, off, p, base, sm);
+    const int nb = min(kObsBlock, p - base);
+    // chunks of 4 obs, four chunks per step; wave w takes steps w, w + 4, ...
+    for (int c0 = wave * 4; c0 * 4 < nb; c0 += 16) {
+      double f[4][NT], sd[4];
 #pragma unroll
-      for (int T = 0; T < NT; ++T) {
-        const int row = T * 16 + r16;
-        f[T] = row < k ? y[row] * rr : 0.0;
+      for (int u = 0; u < 4; ++u) {
+        const int ol = (c0 + u) * 4 + q4;
+        const bool ok = ol < nb;
+        const double rr = ok ? sm.orr[ol] : 0.0;
+        sd[u] = ok ? sm.osd[ol] : 0.0;
+        const double* y = a.yb + (int64_t)(ok ? sm.ogi[ol] : 0) * k;
+#pragma unroll
+        for (int T = 0; T < NT; ++T) {
+          const int row = T * 16 + r16;
+          f[u][T] = (ok && row < k) ? y[row] * rr : 0.0;
+        }
       }
-      sd = rr * a.dep[gi];
-    } else {
 #pragma unroll
-      for (int T = 0; T < NT; ++T) f[T] = 0.0;
+      for (int u = 0; u < 4; ++u) {
+#pragma unroll
+        for (int I = 0; I < NT; ++I)
+#pragma unroll
+          for (int J = 0; J <= I; ++J)
+            acc[I * (I + 1) / 2 + J] = mfma_f64_16x16x4(f[u][I], f[u][J], acc[I * (I + 1) / 2 + J]);
+#pragma unroll
+        for (int T = 0; T < NT; ++T) bacc[T] += f[u][T] * sd[u];
+      }
     }
-#pragma unroll
-    for (int I = 0; I < NT; ++I)
-#pragma unroll
-      for (int J = 0; J <= I; ++J)
-        acc[I * (I + 1) / 2 + J] = mfma_f64_16x16x4(f[I], f[J], acc[I * (I + 1) / 2 + J]);
-#pragma unroll
-    for (int T = 0; T < NT; ++T) bacc[T] += f[T] * sd;
   }
 #pragma unroll
   for (int T = 0; T < NT; ++T) {
@@ -307,8 +340,25 @@ __device__ void lds_gemm_av(BSmem<KP>& sm) {
   __syncthreads();
 }
 
+// Per-phase clock accounting (s_memtime based), one thread per block.
+struct PhaseClock {
+  unsigned long long* out;
+  unsigned long long t;
+  __device__ explicit PhaseClock(unsigned long long* o) : out(threadIdx.x == 0 ? o : nullptr) {
+    if (out) t = wall_clock64();
+  }
+  __device__ void mark(int phase) {
+    if (out) {
+      const unsigned long long n = wall_clock64();
+      atomicAdd(out + phase, n - t);
+      t = n;
+    }
+  }
+};
+
 template <int KP, int GRAM>
 __device__ void analyse_column(const dk_batched_args& a, int64_t col, bool warm, BSmem<KP>& sm) {
+  PhaseClock clk(a.phase_clk);
   constexpr int NT = KP / 16;
   constexpr int LDA = BSmem<KP>::LDA;
   const int tid = threadIdx.x;
@@ -322,6 +372,7 @@ __device__ void analyse_column(const dk_batched_args& a, int64_t col, bool warm,
 
   if constexpr (GRAM == DK_GRAM_FP64) gram_f64<KP>(a, off, p, sm);
   else gram_half<KP, GRAM>(a, off, p, sm);
+  clk.mark(DK_PHASE_GRAM);
 
   // M = C + (k-1)/rho I, then the warm start A0 = M V_prev
   const double km1 = (double)(k - 1);
@@ -329,10 +380,12 @@ __device__ void analyse_column(const dk_batched_args& a, int64_t col, bool warm,
   for (int i = tid; i < k; i += kThreads) sm.A[i + i * LDA] += a0;
   __syncthreads();
   if (warm) lds_gemm_av<KP>(sm);
+  clk.mark(DK_PHASE_WARM);
 
   bool conv;
-  const int sweeps = jacobi_lds<LDA, kThreads>(sm.A, sm.nrm, sm.flag, k, a.tol, a.max_sweeps,
-                                               &conv);
+  const int sweeps = jacobi_lds<KP, LDA, kThreads>(sm.A, sm.nrm, sm.flag, a.tol, a.max_sweeps,
+                                                   &conv);
+  clk.mark(DK_PHASE_JACOBI);
   // lambda_j = |a_j|, v_j = a_j / lambda_j
   for (int j = tid; j < KP; j += kThreads) {
     double s = 0.0;
@@ -390,6 +443,7 @@ __device__ void analyse_column(const dk_batched_args& a, int64_t col, bool warm,
   if (a.wbar_out)
     for (int i = tid; i < k; i += kThreads) a.wbar_out[col * k + i] = sm.wb[i];
 
+  clk.mark(DK_PHASE_TRANSFORM);
   // Update: per 16-row block, D(e, row) = sum_i trans(i, e) Xb'(i, row)
   const int nrow = a.nrow;
   const int nrb = (nrow + 15) / 16;
@@ -455,10 +509,11 @@ __device__ void analyse_column(const dk_batched_args& a, int64_t col, bool warm,
     }
   }
   __syncthreads();
+  clk.mark(DK_PHASE_UPDATE);
 }
 
 template <int KP, int GRAM>
-__global__ __launch_bounds__(kThreads) void k_letkf_batched(dk_batched_args a) {
+__global__ __launch_bounds__(kThreads, 2) void k_letkf_batched(dk_batched_args a) {
   __shared__ BSmem<KP> sm;
   const int tid = threadIdx.x;
   for (;;) {

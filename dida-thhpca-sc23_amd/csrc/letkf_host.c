@@ -125,6 +125,7 @@ struct dida_letkf_ctx {
     void *chain_tmp;
     size_t chain_tmp_bytes;
     double *ymax;
+    unsigned long long *phase_clk; /* [DK_NPHASE], allocated when timing is on */
     /* per-stage HIP event timing (eager sweeps only) */
     int timing;
     int nmark;
@@ -154,7 +155,7 @@ int dida_letkf_destroy(dida_letkf_ctx *c)
                     c->sort_tmp, c->cx, c->sx, c->yr, c->yb_b, c->dep_b, c->rinv_b,
                     c->nobsl, c->offset, c->scan_tmp, c->csr_idx, c->csr_w, c->flags,
                     c->trans, c->info, c->chain_key, c->chain_key_sorted, c->chain_id,
-                    c->chain_order, c->queue_head, c->chain_tmp, c->ymax};
+                    c->chain_order, c->queue_head, c->chain_tmp, c->ymax, c->phase_clk};
     for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; ++i) dfree(ptrs[i]);
     free(c);
     return DIDA_OK;
@@ -384,6 +385,10 @@ static int sweep_batched(dida_letkf_ctx *c, const double *xb, const double *axb,
     ba.info = c->info;
     ba.queue = c->chain_order;
     ba.queue_head = c->queue_head;
+    if (c->timing && c->phase_clk) {
+        HIP_TRY(hipMemsetAsync(c->phase_clk, 0, DK_NPHASE * sizeof(unsigned long long), s));
+        ba.phase_clk = c->phase_clk;
+    }
     if (c->prm.gram == DIDA_GRAM_FP16X3)
         HIP_TRY(dk_obs_ymax(c->last_nobs_pending, ba.k, c->yb_b, c->rinv_b, c->ymax, s));
     HIP_TRY(dk_chain_keys(&ba, c->chain_key, c->chain_id, s));
@@ -530,6 +535,21 @@ int dida_letkf_set_timing(dida_letkf_ctx *c, int32_t on)
     if (!c) return set_err(DIDA_ERR_ARG, "null context");
     c->timing = on ? 1 : 0;
     c->nmark = 0;
+    if (c->timing && !c->phase_clk && c->prm.form == DIDA_FORM_BATCHED &&
+        hipMalloc((void **)&c->phase_clk, DK_NPHASE * sizeof(unsigned long long)) != hipSuccess)
+        return set_err(DIDA_ERR_NOMEM, "phase clocks");
+    return DIDA_OK;
+}
+
+int dida_letkf_phase_cycles(dida_letkf_ctx *c, double cyc[DIDA_NPHASES])
+{
+    if (!c || !cyc) return set_err(DIDA_ERR_ARG, "null argument");
+    for (int i = 0; i < DIDA_NPHASES; ++i) cyc[i] = 0.0;
+    if (!c->phase_clk) return DIDA_OK;
+    unsigned long long h[DK_NPHASE];
+    HIP_TRY(hipStreamSynchronize(c->last_stream));
+    HIP_TRY(hipMemcpy(h, c->phase_clk, sizeof h, hipMemcpyDeviceToHost));
+    for (int i = 0; i < DK_NPHASE && i < DIDA_NPHASES; ++i) cyc[i] = (double)h[i];
     return DIDA_OK;
 }
 
@@ -574,7 +594,10 @@ int dida_letkf_stats_get(dida_letkf_ctx *c, dida_letkf_stats *st)
             if (nb[i] > st->nobsl_max) st->nobsl_max = nb[i];
             if (nb[i] >= c->prm.max_obs) st->nobsl_capped++;
             if (inf[i] < 0) st->nonconverged++;
-            else if (inf[i] > st->max_sweeps_used) st->max_sweeps_used = inf[i];
+            else {
+                st->sweeps_sum += inf[i];
+                if (inf[i] > st->max_sweeps_used) st->max_sweeps_used = inf[i];
+            }
         }
         st->capacity_overflow = flags[0];
         if (flags[0]) rc = set_err(DIDA_ERR_CAPACITY, "selection lists exceeded the workspace");

@@ -151,7 +151,16 @@ typedef struct dk_batched_args {
     double *wbar_out;       /* [col * k] (optional) */
     const int32_t *queue;   /* chain ids in processing order (NULL: identity) */
     int32_t *queue_head;    /* work counter, zero before the launch */
+    unsigned long long *phase_clk; /* optional [DK_NPHASE] device clock sums */
 } dk_batched_args;
+
+/* Phases of the batched kernel timed by phase_clk (wave 0 of each block). */
+#define DK_PHASE_GRAM 0
+#define DK_PHASE_WARM 1
+#define DK_PHASE_JACOBI 2
+#define DK_PHASE_TRANSFORM 3
+#define DK_PHASE_UPDATE 4
+#define DK_NPHASE 5
 
 /* Resident workgroups per CU of the batched kernel for (k, gram). */
 int dk_batched_blocks_per_cu(int k, int gram);

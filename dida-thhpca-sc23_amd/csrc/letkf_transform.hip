@@ -134,8 +134,8 @@ __global__ __launch_bounds__(kThreads) void k_letkf_core_f64(dk_core_args a) {
   gram_f64<KP>(a, off, p, sm);
 
   bool conv;
-  const int sweeps = jacobi_lds<LDA, kThreads>(sm.A, sm.nrm, sm.flag, k, a.tol, a.max_sweeps,
-                                               &conv);
+  const int sweeps = jacobi_lds<KP, LDA, kThreads>(sm.A, sm.nrm, sm.flag, a.tol, a.max_sweeps,
+                                                   &conv);
   const int m = (k + 1) & ~1;
 
   // exact column norms: alpha_j = lambda_j^2

@@ -28,7 +28,8 @@ EXPORTED = ["dida_letkf_params_default", "letkf_core_", "letkf_core_batch_",
             "dida_letkf_set_grid", "dida_letkf_sweep", "dida_letkf_graph_capture",
             "dida_letkf_graph_launch", "dida_letkf_stats_get", "dida_letkf_debug_arrays",
             "dida_letkf_post", "dida_letkf_last_error", "dida_letkf_set_timing",
-            "dida_letkf_stage_ms"]
+            "dida_letkf_stage_ms", "dida_letkf_phase_cycles"]
+PHASES = ["gram", "warm", "jacobi", "transform", "update"]
 STAGES = ["bin", "select", "transform", "update"]
 
 
@@ -52,7 +53,7 @@ class StatsC(C.Structure):
     _fields_ = [("nanal", C.c_int64), ("nobsl_sum", C.c_int64), ("nobsl_max", C.c_int32),
                 ("nobsl_capped", C.c_int32), ("nonconverged", C.c_int32),
                 ("max_sweeps_used", C.c_int32), ("capacity_overflow", C.c_int32),
-                ("pad_", C.c_int32)]
+                ("pad_", C.c_int32), ("sweeps_sum", C.c_int64)]
 
 
 class SynthCfg(C.Structure):
@@ -98,6 +99,7 @@ def lib():
                                       C.c_double, C.c_double, vp, vp, vp, vp, vp, vp, vp, vp]
         L.dida_letkf_set_timing.argtypes = [vp, C.c_int32]
         L.dida_letkf_stage_ms.argtypes = [vp, C.POINTER(C.c_double)]
+        L.dida_letkf_phase_cycles.argtypes = [vp, C.POINTER(C.c_double)]
         L.letkf_core_.argtypes = [vp] * 10
         L.letkf_core_.restype = None
         L.letkf_core_batch_.argtypes = [vp] * 13
@@ -229,6 +231,11 @@ class Engine:
         ms = (C.c_double * len(STAGES))()
         check(lib().dida_letkf_stage_ms(self.h, ms), "stage_ms")
         return {name: ms[i] for i, name in enumerate(STAGES)}
+
+    def phase_cycles(self) -> dict:
+        cyc = (C.c_double * len(PHASES))()
+        check(lib().dida_letkf_phase_cycles(self.h, cyc), "phase_cycles")
+        return {name: cyc[i] for i, name in enumerate(PHASES)}
 
     def stats(self) -> dict:
         st = StatsC()

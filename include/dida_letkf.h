@@ -89,6 +89,7 @@ typedef struct dida_letkf_stats {
     int32_t max_sweeps_used;
     int32_t capacity_overflow;  /* 1 if the selection lists did not fit */
     int32_t pad_;
+    int64_t sweeps_sum;         /* Jacobi sweeps summed over converged analyses */
 } dida_letkf_stats;
 
 typedef struct dida_letkf_ctx dida_letkf_ctx;
@@ -169,6 +170,13 @@ int dida_letkf_graph_launch(dida_letkf_ctx *ctx, void *stream);
 #define DIDA_NSTAGES 4
 int dida_letkf_set_timing(dida_letkf_ctx *ctx, int32_t on);
 int dida_letkf_stage_ms(dida_letkf_ctx *ctx, double ms[DIDA_NSTAGES]);
+
+/* Batched form with timing on: device clock cycles summed over workgroups
+ * per phase of the fused kernel (Gram, warm start, Jacobi, transform,
+ * update; the reference's "eig decomposition" and core timers,
+ * letkf_core.F90:94-117).  Synchronises. */
+#define DIDA_NPHASES 5
+int dida_letkf_phase_cycles(dida_letkf_ctx *ctx, double cyc[DIDA_NPHASES]);
 
 /* Diagnostics of the last sweep; synchronises the context's last stream. */
 int dida_letkf_stats_get(dida_letkf_ctx *ctx, dida_letkf_stats *st);
