@@ -340,22 +340,6 @@ __device__ void lds_gemm_av(BSmem<KP>& sm) {
   __syncthreads();
 }
 
-// Per-phase clock accounting (s_memtime based), one thread per block.
-struct PhaseClock {
-  unsigned long long* out;
-  unsigned long long t;
-  __device__ explicit PhaseClock(unsigned long long* o) : out(threadIdx.x == 0 ? o : nullptr) {
-    if (out) t = wall_clock64();
-  }
-  __device__ void mark(int phase) {
-    if (out) {
-      const unsigned long long n = wall_clock64();
-      atomicAdd(out + phase, n - t);
-      t = n;
-    }
-  }
-};
-
 template <int KP, int GRAM>
 __device__ void analyse_column(const dk_batched_args& a, int64_t col, bool warm, BSmem<KP>& sm) {
   PhaseClock clk(a.phase_clk);

@@ -77,4 +77,20 @@ __device__ __forceinline__ double4_t mfma_f64_16x16x4(double a, double b, double
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+// Per-phase clock accounting (wall_clock64), one thread per workgroup.
+struct PhaseClock {
+  unsigned long long* out;
+  unsigned long long t;
+  __device__ explicit PhaseClock(unsigned long long* o) : out(threadIdx.x == 0 ? o : nullptr) {
+    if (out) t = wall_clock64();
+  }
+  __device__ void mark(int phase) {
+    if (out) {
+      const unsigned long long n = wall_clock64();
+      atomicAdd(out + phase, n - t);
+      t = n;
+    }
+  }
+};
+
 }  // namespace dida

@@ -263,7 +263,7 @@ int dida_letkf_create(dida_letkf_ctx **out, const dida_letkf_params *prm, int32_
         if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
             ncu <= 0)
             ncu = 256;
-        c->nblocks = ncu * dk_batched_blocks_per_cu(k, prm->gram);
+        c->nblocks = ncu * dk_wave_blocks_per_cu(k, prm->gram);
     }
     *out = c;
     return DIDA_OK;
@@ -397,7 +397,7 @@ static int sweep_batched(dida_letkf_ctx *c, const double *xb, const double *axb,
                               c->chain_order, c->nchain, 32, s));
     HIP_TRY(dk_fill_i32(c->queue_head, 0, 1, s));
     tmark(c, DIDA_STAGE_SELECT, s);
-    HIP_TRY(dk_letkf_batched(&ba, c->nblocks, s));
+    HIP_TRY(dk_letkf_wave(&ba, c->nblocks, s));
     tmark(c, DIDA_STAGE_TRANSFORM, s);
     return DIDA_OK;
 }

@@ -167,6 +167,9 @@ int dk_batched_blocks_per_cu(int k, int gram);
 /* Sort keys of the chains: 2^32-1 - cost (ascending sort = largest first). */
 int dk_chain_keys(const dk_batched_args *a, uint32_t *key, int32_t *id, void *stream);
 int dk_letkf_batched(const dk_batched_args *a, int nblocks, void *stream);
+/* Wavefront-per-analysis variant of the batched kernel (64-thread groups). */
+int dk_wave_blocks_per_cu(int k, int gram);
+int dk_letkf_wave(const dk_batched_args *a, int nblocks, void *stream);
 /* ymax[i] = max_e |yb[i k + e]| sqrt(rinv[i]). */
 int dk_obs_ymax(int32_t nobs, int32_t k, const double *yb, const float *rinv, double *ymax,
                 void *stream);

@@ -55,30 +55,50 @@ __device__ __forceinline__ void gram_f64(const dk_core_args& a, int64_t off, int
 #pragma unroll
   for (int t = 0; t < NT; ++t) bacc[t] = 0.0;
 
-  for (int c = wave; c * 4 < p; c += 4) {
-    const int o = c * 4 + q4;
-    double f[NT];
-    double sd = 0.0;
-    if (o < p) {
-      const int64_t gi = a.idx ? (int64_t)a.idx[off + o] : off + o;
-      const double rr = sqrt((double)a.rinv[gi] * a.rloc[off + o]);
-      const double* y = a.yb + gi * k;
+  // four chunks of 4 obs per step; obs past p read entry 0 of the analysis
+  // (in bounds: p > 0 inside the loop) with weight 0, so every load is
+  // unconditional: indices first, then the obs rows and scalars
+  constexpr int U = 4;
+  for (int c0 = wave * U; c0 * 4 < p; c0 += 4 * U) {
+    int64_t gi[U];
+    bool ok[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int o = (c0 + u) * 4 + q4;
+      ok[u] = o < p;
+      const int oc = ok[u] ? o : 0;
+      gi[u] = a.idx ? (int64_t)a.idx[off + oc] : off + oc;
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    double f[U][NT], ri[U], rl[U], dp[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int oc = ok[u] ? (c0 + u) * 4 + q4 : 0;
+      ri[u] = (double)a.rinv[gi[u]];
+      rl[u] = a.rloc[off + oc];
+      dp[u] = a.dep[gi[u]];
+      const double* y = a.yb + gi[u] * k;
 #pragma unroll
       for (int T = 0; T < NT; ++T) {
         const int row = T * 16 + r16;
-        f[T] = row < k ? y[row] * rr : 0.0;
+        f[u][T] = y[row < k ? row : k - 1];
       }
-      sd = rr * a.dep[gi];
-    } else {
-#pragma unroll
-      for (int T = 0; T < NT; ++T) f[T] = 0.0;
     }
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int I = 0; I < NT; ++I)
+    for (int u = 0; u < U; ++u) {
+      const double rr = ok[u] ? sqrt(ri[u] * rl[u]) : 0.0;
+      const double sd = rr * dp[u];
 #pragma unroll
-      for (int J = 0; J <= I; ++J) acc[I * (I + 1) / 2 + J] = mfma_f64_16x16x4(f[I], f[J], acc[I * (I + 1) / 2 + J]);
+      for (int T = 0; T < NT; ++T) f[u][T] = (T * 16 + r16 < k) ? f[u][T] * rr : 0.0;
 #pragma unroll
-    for (int T = 0; T < NT; ++T) bacc[T] += f[T] * sd;
+      for (int I = 0; I < NT; ++I)
+#pragma unroll
+        for (int J = 0; J <= I; ++J)
+          acc[I * (I + 1) / 2 + J] = mfma_f64_16x16x4(f[u][I], f[u][J], acc[I * (I + 1) / 2 + J]);
+#pragma unroll
+      for (int T = 0; T < NT; ++T) bacc[T] += f[u][T] * sd;
+    }
   }
 #pragma unroll
   for (int T = 0; T < NT; ++T) {
