@@ -208,7 +208,7 @@ def main():
         flops += 2.0 * cfg.k * cfg.k * n * ncol        # the fused update
     achieved = flops / (tf_ms / 1e3) / 1e12 if tf_ms > 0 else None
     kname = "k_letkf_core_f64 (transform)" if head == "plain_fp64" else \
-        "k_letkf_batched (fused transform + update)"
+        "k_letkf_wave (fused transform + update)"
     roofline = {"bound": "mfma", "kernel": kname, "achieved": achieved,
                 "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / FP64_PEAK_TFLOPS if achieved else None, "traffic": None,

@@ -162,12 +162,10 @@ typedef struct dk_batched_args {
 #define DK_PHASE_UPDATE 4
 #define DK_NPHASE 5
 
-/* Resident workgroups per CU of the batched kernel for (k, gram). */
-int dk_batched_blocks_per_cu(int k, int gram);
 /* Sort keys of the chains: 2^32-1 - cost (ascending sort = largest first). */
 int dk_chain_keys(const dk_batched_args *a, uint32_t *key, int32_t *id, void *stream);
-int dk_letkf_batched(const dk_batched_args *a, int nblocks, void *stream);
-/* Wavefront-per-analysis variant of the batched kernel (64-thread groups). */
+/* Batched kernel (one 64-thread wavefront per analysis): resident
+ * workgroups per CU for (k, gram), and the persistent launch. */
 int dk_wave_blocks_per_cu(int k, int gram);
 int dk_letkf_wave(const dk_batched_args *a, int nblocks, void *stream);
 /* ymax[i] = max_e |yb[i k + e]| sqrt(rinv[i]). */

@@ -26,7 +26,6 @@
 //   update    Xa = trans^T Xb' + xbar_b, mean, re-centre, RTPS (xb_2_xa,
 //             letkf_compute.F90:261-299) on FP64 MFMA.
 #include "letkf_device.h"
-#include "letkf_jacobi.h"
 #include "letkf_kernels.h"
 
 using namespace dida;
@@ -76,10 +75,54 @@ __device__ __forceinline__ int row_of(bool patB, int I, int r, int g) {
   return patB ? rowB(I, r, g) : rowA(I, r, g);
 }
 
-__device__ __forceinline__ double sum_g(double v) {
-  v += shfl_xor_d(v, 16);
-  v += shfl_xor_d(v, 32);
-  return v;
+// v + v(lane ^ 32) and v + v(lane ^ 16) on the VALU (gfx950 row swaps:
+// v_permlane32_swap exchanges lanes 32-63 of one operand with lanes 0-31 of
+// the other, v_permlane16_swap odd 16-lane rows with even ones; with both
+// operands equal to v the two results hold v and its partner, in either
+// order, so their sum is the butterfly).
+__device__ __forceinline__ double add_xor32(double v) {
+  const unsigned lo = __double2loint(v), hi = __double2hiint(v);
+  const auto a = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+  const auto b = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+  return __hiloint2double(b[0], a[0]) + __hiloint2double(b[1], a[1]);
+}
+__device__ __forceinline__ double add_xor16(double v) {
+  const unsigned lo = __double2loint(v), hi = __double2hiint(v);
+  const auto a = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+  const auto b = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+  return __hiloint2double(b[0], a[0]) + __hiloint2double(b[1], a[1]);
+}
+// sum over the four lane groups g (lanes c, c + 16, c + 32, c + 48)
+__device__ __forceinline__ double sum_g(double v) { return add_xor16(add_xor32(v)); }
+
+// v of lane (l ^ 1), same 16-lane row (ds_swizzle, bit mode: and 0x1f, xor 1)
+__device__ __forceinline__ double swz_xor1(double v) {
+  const int lo = __builtin_amdgcn_ds_swizzle(__double2loint(v), 0x1F | (1 << 10));
+  const int hi = __builtin_amdgcn_ds_swizzle(__double2hiint(v), 0x1F | (1 << 10));
+  return __hiloint2double(hi, lo);
+}
+// v of the lane whose byte address (4 * lane) is addr
+__device__ __forceinline__ double bperm_d(int addr, double v) {
+  const int lo = __builtin_amdgcn_ds_bpermute(addr, __double2loint(v));
+  const int hi = __builtin_amdgcn_ds_bpermute(addr, __double2hiint(v));
+  return __hiloint2double(hi, lo);
+}
+
+// 1 / sqrt(x) and 1 / x for positive normal x: the hardware estimates
+// (relative error about 2^-23) refined by two Newton steps, y (3 - x y^2) / 2
+// and y (2 - x y), to a few ulp.
+__device__ __forceinline__ double rsq_nr(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double hx = 0.5 * x;
+  y = y * fma(-hx * y, y, 1.5);
+  y = y * fma(-hx * y, y, 1.5);
+  return y;
+}
+__device__ __forceinline__ double rcp_nr(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  y = fma(y, fma(-x, y, 1.0), y);
+  y = fma(y, fma(-x, y, 1.0), y);
+  return y;
 }
 
 // Stage obs [base, base + 64) of the analysis in LDS (binned index,
@@ -310,145 +353,208 @@ struct Wave {
     return sum_g(s);
   }
 
-  // Exchange the bottom column of pair P with lane-group distance d (xor).
-  __device__ void xswap(int P, int d, bool first, double* nb, double* nt) {
+  // ---- one-sided Jacobi with scaled rotations ------------------------------
+  // Column j of the working matrix is sc[j] X[j]; nrm[j] is its squared norm
+  // (kept by alpha' = alpha - t gamma, beta' = beta + t gamma and recomputed
+  // at the start of each sweep, when the scales are folded back into X).  The
+  // rotation p' = c p - s q, q' = s p + c q (t = s / c) is applied as
+  // x_p' = x_p - t (sc_q / sc_p) x_q, x_q' = x_q + t (sc_p / sc_q) x_p with
+  // sc_p' = c sc_p, sc_q' = c sc_q: two FMAs a row instead of four.  |t| <= 1,
+  // so c >= 1/sqrt(2) and a sweep (KP - 1 rotations per column) changes a
+  // scale by at most 2^((KP - 1) / 2).
+  double nrm[NJ], sc[NJ], isc[NJ];
+
+  template <int P, int Q>
+  __device__ __forceinline__ double dot() const {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+#pragma unroll
+    for (int q = 0; q < NQ; q += 4) {
+      s0 = fma(X[P][q], X[Q][q], s0);
+      s1 = fma(X[P][q + 1], X[Q][q + 1], s1);
+      s2 = fma(X[P][q + 2], X[Q][q + 2], s2);
+      s3 = fma(X[P][q + 3], X[Q][q + 3], s3);
+    }
+    return (s0 + s1) + (s2 + s3);
+  }
+
+  // Rotate columns P, Q given gs = X[P] . X[Q] (summed over the wave).  A
+  // pair is rotated while gamma^2 > tol^2 alpha beta; *big records a pair with
+  // gamma^2 > tol alpha beta (the next sweep would still rotate, the test of
+  // predicted convergence).  Pairs below the threshold take t = 0, c = 1.
+  template <int P, int Q>
+  __device__ __forceinline__ void rot(double gs, double tol2, double tolp, bool& rotated,
+                                      bool& big) {
+    const double al = nrm[P], be = nrm[Q];
+    const double gm = gs * (sc[P] * sc[Q]);
+    const double gg = gm * gm, ab = al * be;
+    const bool need = gg > tol2 * ab;
+    big |= gg > tolp * ab;
+    rotated |= need;
+    // t = sign(d e) |e| / (|d| + sqrt(d^2 + e^2)), d = beta - alpha, e = 2 gamma
+    const double d = be - al, e = 2.0 * gm;
+    const double r2 = fma(d, d, e * e);
+    const double den = fabs(d) + r2 * rsq_nr(r2);
+    double t = fabs(e) * rcp_nr(den);
+    if ((d < 0.0) != (e < 0.0) && d != 0.0) t = -t;
+    t = need ? t : 0.0;
+    const double u = fma(t, t, 1.0);
+    double cs = rsq_nr(u);  // cos = 1 / sqrt(1 + t^2)
+    double ic = u * cs;     // 1 / cos
+    cs = need ? cs : 1.0;
+    ic = need ? ic : 1.0;
+    const double ta = t * (sc[Q] * isc[P]);
+    const double tb = t * (sc[P] * isc[Q]);
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-      const double v = first ? X[2 * P + 1][q] : X[2 * P][q];
-      const double w = shfl_xor_d(v, d);
-      if (first) X[2 * P + 1][q] = w;
-      else X[2 * P][q] = w;
+      const double x = X[P][q], y = X[Q][q];
+      X[P][q] = fma(-ta, y, x);
+      X[Q][q] = fma(tb, x, y);
     }
-    const double v = first ? nb[P] : nt[P];
-    const double w = shfl_xor_d(v, d);
-    if (first) nb[P] = w;
-    else nt[P] = w;
+    nrm[P] = fma(-t, gm, al);
+    nrm[Q] = fma(t, gm, be);
+    sc[P] *= cs;
+    sc[Q] *= cs;
+    isc[P] *= ic;
+    isc[Q] *= ic;
+  }
+
+  // One round: pairs (T0, B0) and, with two pairs a lane, (T1, B1).
+  template <int T0, int B0, int T1, int B1>
+  __device__ __forceinline__ void round(double tol2, double tolp, bool& rotated, bool& big) {
+    if constexpr (NP == 2) {
+      double g0 = dot<T0, B0>(), g1 = dot<T1, B1>();
+      g0 = sum_g(g0);
+      g1 = sum_g(g1);
+      rot<T0, B0>(g0, tol2, tolp, rotated, big);
+      rot<T1, B1>(g1, tol2, tolp, rotated, big);
+    } else {
+      rot<T0, B0>(sum_g(dot<T0, B0>()), tol2, tolp, rotated, big);
+    }
+  }
+
+  // Column J takes the column J of lane l ^ 1 (with its state).
+  template <int J>
+  __device__ __forceinline__ void move_xor1() {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) X[J][q] = swz_xor1(X[J][q]);
+    nrm[J] = swz_xor1(nrm[J]);
+    sc[J] = swz_xor1(sc[J]);
+    isc[J] = swz_xor1(isc[J]);
+  }
+  // Column J takes the column J of the lane at byte address addr.
+  template <int J>
+  __device__ __forceinline__ void move_perm(int addr) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) X[J][q] = bperm_d(addr, X[J][q]);
+    nrm[J] = bperm_d(addr, nrm[J]);
+    sc[J] = bperm_d(addr, sc[J]);
+    isc[J] = bperm_d(addr, isc[J]);
+  }
+  // Lanes of the first half (first) take the top T of their partner (byte
+  // address addr) into their bottom B; the partner takes their bottom into
+  // its top.  Each lane therefore sends what its partner takes.
+  template <int T, int B>
+  __device__ __forceinline__ void xswap(int addr, bool first) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const double w = bperm_d(addr, first ? X[B][q] : X[T][q]);
+      X[B][q] = first ? w : X[B][q];
+      X[T][q] = first ? X[T][q] : w;
+    }
+    double w = bperm_d(addr, first ? nrm[B] : nrm[T]);
+    nrm[B] = first ? w : nrm[B];
+    nrm[T] = first ? nrm[T] : w;
+    w = bperm_d(addr, first ? sc[B] : sc[T]);
+    sc[B] = first ? w : sc[B];
+    sc[T] = first ? sc[T] : w;
+    w = bperm_d(addr, first ? isc[B] : isc[T]);
+    isc[B] = first ? w : isc[B];
+    isc[T] = first ? isc[T] : w;
+  }
+
+  __device__ __forceinline__ void fold_scales() {
+#pragma unroll
+    for (int J = 0; J < NJ; ++J) {
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) X[J][q] *= sc[J];
+      sc[J] = 1.0;
+      isc[J] = 1.0;
+    }
   }
 
   // One-sided Jacobi on X.  Returns sweeps; *conv when a sweep rotated
-  // nothing, or (predict) when the largest relative off-diagonal of a sweep
-  // was below sqrt(tol), so that the next sweep would rotate nothing.
+  // nothing, or (predict) when no pair of a sweep had gamma^2 > tol alpha
+  // beta, so that the next sweep would rotate nothing.
+  //
+  // Order of a sweep (KP - 1 rounds): pair slots s = c NP + P hold a top
+  // (column 2P of lane group c) and a bottom (2P + 1).  For G = KP/2, KP/4,
+  // ..., 1, each block of G slots runs G rounds in which the top of slot s
+  // meets the bottom that started the level in slot s ^ gray(r); between
+  // rounds the bottoms move by the one bit in which gray(r) and gray(r + 1)
+  // differ (bit 0 within a lane when NP = 2, handled by renaming; higher bits
+  // across lane groups).  Then the bottom of every slot in the first half of
+  // its block swaps with the top of slot s + G/2, and the two halves recurse.
   __device__ int jacobi(double tol, int max_sweeps, bool predict, bool* conv) {
     const int lane = threadIdx.x, c = lane & 15;
-    constexpr int NPS = KP / 2;  // pair slots
-    double nt[NP], nb[NP];
+    const double tol2 = tol * tol;
+    const double tolp = predict ? tol : __builtin_inf();
+#pragma unroll
+    for (int J = 0; J < NJ; ++J) {
+      sc[J] = 1.0;
+      isc[J] = 1.0;
+    }
     int sweep = 0;
     *conv = false;
     for (; sweep < max_sweeps; ++sweep) {
+      fold_scales();
 #pragma unroll
-      for (int P = 0; P < NP; ++P) {
-        nt[P] = colnorm(2 * P);
-        nb[P] = colnorm(2 * P + 1);
-      }
-      bool rotated = false;
-      double dmax = 0.0;
-      for (int G = NPS; G >= 1; G >>= 1) {
-        // G rounds pairing the two halves of each block of 2G positions
-        for (int rr = 0; rr < G; ++rr) {
-          // all pairs of the lane are handled branch-free and side by side:
-          // dot products in four partial sums, then the rotation, applied
-          // as the identity (c = 1, s = t = 0) where none is needed
-          double gmv[NP];
-#pragma unroll
-          for (int P = 0; P < NP; ++P) {
-            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-#pragma unroll
-            for (int q = 0; q < NQ; q += 4) {
-              s0 += X[2 * P][q] * X[2 * P + 1][q];
-              s1 += X[2 * P][q + 1] * X[2 * P + 1][q + 1];
-              s2 += X[2 * P][q + 2] * X[2 * P + 1][q + 2];
-              s3 += X[2 * P][q + 3] * X[2 * P + 1][q + 3];
-            }
-            gmv[P] = (s0 + s1) + (s2 + s3);
-          }
-#pragma unroll
-          for (int P = 0; P < NP; ++P) gmv[P] = sum_g(gmv[P]);
-#pragma unroll
-          for (int P = 0; P < NP; ++P) {
-            const double gm = gmv[P];
-            const double al = nt[P], be = nb[P];
-            const double den = sqrt(al * be);
-            dmax = fmax(dmax, den > 0.0 ? fabs(gm) / den : 0.0);
-            const bool need = gm != 0.0 && fabs(gm) > tol * den;
-            rotated |= need;
-            double cs, sn, t;
-            jacobi_rotation(need ? al : 0.0, need ? be : 1.0, need ? gm : 0.0, &cs, &sn, &t);
-#pragma unroll
-            for (int q = 0; q < NQ; ++q) {
-              const double x = X[2 * P][q], y = X[2 * P + 1][q];
-              X[2 * P][q] = cs * x - sn * y;
-              X[2 * P + 1][q] = sn * x + cs * y;
-            }
-            nt[P] = al - t * gm;
-            nb[P] = be + t * gm;
-          }
-          if (rr + 1 < G) {
-            // bottoms move one position down within the block of G positions:
-            // position (c, P) takes the bottom of (c, P + 1), and the last pair
-            // of a lane group the bottom of pair 0 of the next lane group
-            // (cyclic within G / NP lane groups)
-            if (G >= NP) {
-              const int nlg = G / NP;  // lane groups per block
-              const int cn = (c & ~(nlg - 1)) | ((c + 1) & (nlg - 1));
-              const int src = (lane & ~15) | cn;
-              double first[NQ];
-              double n0 = nb[0];
-#pragma unroll
-              for (int q = 0; q < NQ; ++q) first[q] = X[1][q];
-#pragma unroll
-              for (int P = 0; P + 1 < NP; ++P) {
-#pragma unroll
-                for (int q = 0; q < NQ; ++q) X[2 * P + 1][q] = X[2 * P + 3][q];
-                nb[P] = nb[P + 1];
-              }
-              if (nlg > 1) {
-#pragma unroll
-                for (int q = 0; q < NQ; ++q) first[q] = __shfl(first[q], src, kWave);
-                n0 = __shfl(n0, src, kWave);
-              }
-#pragma unroll
-              for (int q = 0; q < NQ; ++q) X[2 * NP - 1][q] = first[q];
-              nb[NP - 1] = n0;
+      for (int J = 0; J < NJ; ++J) nrm[J] = colnorm(J);
+      bool rotated = false, big = false;
+      if constexpr (NP == 2) {
+        for (int G = KP / 2; G >= 4; G >>= 1) {
+          for (int r = 0; r < G; r += 4) {
+            round<0, 1, 2, 3>(tol2, tolp, rotated, big);
+            round<0, 3, 2, 1>(tol2, tolp, rotated, big);
+            move_xor1<1>();
+            move_xor1<3>();
+            round<0, 3, 2, 1>(tol2, tolp, rotated, big);
+            round<0, 1, 2, 3>(tol2, tolp, rotated, big);
+            if (r + 4 < G) {
+              const int addr = (lane ^ (((r + 4) & -(r + 4)) >> 1)) << 2;
+              move_perm<1>(addr);
+              move_perm<3>(addr);
             }
           }
+          const int d = G >> 2;
+          const int addr = (lane ^ d) << 2;
+          const bool first = (c & d) == 0;
+          xswap<0, 1>(addr, first);
+          xswap<2, 3>(addr, first);
         }
-        if (G > 1) {
-          // next level: position p (first half of its block of 2G) swaps its
-          // bottom with the top of p + G/2
-          const int half = G / 2;  // in pair slots
-          if (half >= NP) {
-            const int d = half / NP;  // lane groups
-            const bool first = ((c / d) & 1) == 0;
-#pragma unroll
-            for (int P = 0; P < NP; ++P) xswap(P, d, first, nb, nt);
-          } else {
-            // half < NP: partners are pairs P and P + half of the same lane
-#pragma unroll
-            for (int P = 0; P < NP; ++P) {
-              if ((P / half) % 2 == 0) {
-                const int P2 = P + half;
-#pragma unroll
-                for (int q = 0; q < NQ; ++q) {
-                  const double tmp = X[2 * P + 1][q];
-                  X[2 * P + 1][q] = X[2 * P2][q];
-                  X[2 * P2][q] = tmp;
-                }
-                const double tn = nb[P];
-                nb[P] = nt[P2];
-                nt[P2] = tn;
-              }
-            }
+        round<0, 1, 2, 3>(tol2, tolp, rotated, big);
+        round<0, 3, 2, 1>(tol2, tolp, rotated, big);
+        round<0, 2, 3, 1>(tol2, tolp, rotated, big);
+      } else {
+        for (int G = KP / 2; G >= 1; G >>= 1) {
+          for (int r = 0; r < G; ++r) {
+            round<0, 1, 0, 1>(tol2, tolp, rotated, big);
+            if (r + 1 < G) move_perm<1>((lane ^ ((r + 1) & -(r + 1))) << 2);
+          }
+          if (G > 1) {
+            const int d = G >> 1;
+            xswap<0, 1>((lane ^ d) << 2, (c & d) == 0);
           }
         }
       }
       const bool any_rot = __any(rotated);
-      for (int s = 32; s >= 1; s >>= 1) dmax = fmax(dmax, __shfl_xor(dmax, s, kWave));
-      if (!any_rot || (predict && dmax * dmax <= tol)) {
+      const bool any_big = __any(big);
+      if (!any_rot || (predict && !any_big)) {
         *conv = true;
         ++sweep;
         break;
       }
     }
+    fold_scales();
     return sweep;
   }
 };
@@ -635,6 +741,22 @@ __global__ __launch_bounds__(64, 1) void k_letkf_wave(dk_batched_args a) {
   }
 }
 
+// Sort key of chain c: 2^32 - 1 - cost, so that an ascending radix sort puts
+// the most expensive chains first.  Cost of a column ~ its local obs count
+// (Gram) + 10 k (eigensolve, transform and update).
+__global__ void k_chain_keys(dk_batched_args a, uint32_t* key, int32_t* id) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.nchain) return;
+  const int ilon = c / a.chains_per_lon;
+  const int lat0 = (c % a.chains_per_lon) * a.chain_len;
+  const int lat1 = min(lat0 + a.chain_len, a.nlat);
+  double cost = 0.0;
+  for (int ilat = lat0; ilat < lat1; ++ilat)
+    cost += (double)a.nobsl[(int64_t)ilon * a.nlat + ilat] + 10.0 * a.k;
+  key[c] = 0xFFFFFFFFu - (uint32_t)fmin(cost, 4.0e9);
+  id[c] = c;
+}
+
 template <int KP, int GRAM>
 int wlaunch(const dk_batched_args* a, int nblocks, hipStream_t s) {
   k_letkf_wave<KP, GRAM><<<nblocks, 64, 0, s>>>(*a);
@@ -675,6 +797,12 @@ int wocc_dispatch(int gram) {
 extern "C" int dk_wave_blocks_per_cu(int k, int gram) {
   if (k <= 32) return wocc_dispatch<32>(gram);
   return wocc_dispatch<64>(gram);
+}
+
+extern "C" int dk_chain_keys(const dk_batched_args* a, uint32_t* key, int32_t* id, void* stream) {
+  if (a->nchain <= 0) return 0;
+  k_chain_keys<<<(a->nchain + 255) / 256, 256, 0, (hipStream_t)stream>>>(*a, key, id);
+  return (int)hipGetLastError();
 }
 
 extern "C" int dk_letkf_wave(const dk_batched_args* a, int nblocks, void* stream) {
