@@ -27,6 +27,7 @@ for _p in (ROOT, PKG):
 
 METRIC = ("local analyses/s on 1 MI355X: mixed-prec batched vs plain FP64, at stated rel err")
 FP64_PEAK_TFLOPS = 78.6
+HALF_PEAK_TFLOPS = 2500.0
 HBM_PEAK_GBS = 8000.0
 
 
@@ -47,6 +48,9 @@ def make_engine(dl, cfg, prm, form, gram, nobs):
 
 
 def time_form(dl, torch, data, cfg, prm, form, gram, steps, warmup):
+    """Warm up, then time `steps` sweeps with HIP events around each stage on
+    the sweep's stream (the batched kernel's launch is the "transform" stage),
+    then one untimed sweep with the in-kernel phase clocks."""
     eng, p = make_engine(dl, cfg, prm, form, gram, cfg.nobs)
     eng.set_grid(data["grid_lat"], data["grid_lon"])
     obs = dl.DeviceObs(data["lat"], data["lon"], data["yb"], data["dep"], data["rinv"])
@@ -55,17 +59,22 @@ def time_form(dl, torch, data, cfg, prm, form, gram, steps, warmup):
     for _ in range(max(warmup, 1)):
         eng.sweep(obs, data["xb"], data["axb"], xa, axa)
     st = eng.stats()
+    eng.set_timing(True)
+    stage_sum = {}
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
         eng.sweep(obs, data["xb"], data["axb"], xa, axa)
+        for key, v in eng.stage_ms().items():
+            stage_sum[key] = stage_sum.get(key, 0.0) + v
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / max(steps, 1)
-    # one more sweep with per-stage HIP events on the sweep's stream
-    eng.set_timing(True)
-    eng.sweep(obs, data["xb"], data["axb"], xa, axa)
-    stages = eng.stage_ms()
-    phases = eng.phase_cycles() if form != dl.FORM_PLAIN else {}
+    stages = {key: v / max(steps, 1) for key, v in stage_sum.items()}
+    phases = {}
+    if form != dl.FORM_PLAIN:
+        eng.set_timing(True, phases=True)
+        eng.sweep(obs, data["xb"], data["axb"], xa, axa)
+        phases = eng.phase_cycles()
     eng.set_timing(False)
     torch.cuda.synchronize()
     tot = sum(phases.values())
@@ -97,6 +106,69 @@ def transform_flops(k, nobsl_sum, nanal):
 
 def update_bytes(k, n, ncol, s=8):
     return ncol * (2.0 * k * n * s + 2.0 * n * 8)
+
+
+def kernel_roofline(cfg, st, head, kernel_ms, config, scale):
+    """Roofline of the dominant kernel (the batched form's fused kernel, or
+    the plain form's transform kernel) from algorithmic flops per sweep:
+    Gram 2k^2 p (three half-precision MFMA passes in the split-x3 forms),
+    b 2kp, eigensolve 9k^3 + W 2k^3 + wbar 4k^2 per analysis (nominal), and,
+    when fused, the update 2k^2 n + 2kn.  The peak is the rate at which the
+    mix would run with each part at its own published peak (BF16/FP16 dense
+    MFMA 2.5 PFLOP/s, FP64 78.6 TFLOP/s), so frac = (time at peak) / time."""
+    k, n, ncol = cfg.k, 1 + 3 * cfg.nz, cfg.nlat * cfg.nlon
+    p_sum = float(st["nobsl_sum"])
+    gram = 2.0 * k * k * p_sum
+    fp64 = 2.0 * k * p_sum + ncol * (11.0 * k ** 3 + 4.0 * k * k)
+    fused = head != "plain_fp64"
+    if fused:
+        fp64 += ncol * (2.0 * k * k * n + 2.0 * k * n)
+    if head.endswith("fp64"):
+        half, fp64 = 0.0, fp64 + gram
+    else:
+        half = (3.0 if head.endswith("x3") else 1.0) * gram
+    t_peak = half / (HALF_PEAK_TFLOPS * 1e12) + fp64 / (FP64_PEAK_TFLOPS * 1e12)
+    flops = half + fp64
+    peak = flops / t_peak / 1e12
+    achieved = flops / (kernel_ms / 1e3) / 1e12 if kernel_ms > 0 else None
+    # bytes the kernel must move: obs rows gathered per analysis p (8k + 24)
+    # (mostly served from L2, neighbouring columns share obs) and, fused, the
+    # update's 2kn f64 + 2n f64 per column
+    gathered = p_sum * (8.0 * k + 24.0)
+    upd = ncol * (2.0 * k * n * 8 + 2.0 * n * 8) if fused else 0.0
+    traffic = measured_traffic(head, config, scale)
+    return {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+            "frac": achieved / peak if achieved else None, "traffic": traffic,
+            "kernel": "k_letkf_wave (fused Gram, eigensolve, transform, update)" if fused
+            else "k_letkf_core_f64 (Gram, eigensolve, transform)",
+            "kernel_ms": kernel_ms, "flops_per_launch": flops, "half_flops": half,
+            "fp64_flops": fp64,
+            "algorithmic_bytes": {"obs_gathered": gathered, "update": upd},
+            "hbm_frac_algorithmic": (gathered + upd) / (kernel_ms / 1e3) / (HBM_PEAK_GBS * 1e9)
+            if kernel_ms > 0 else None}
+
+
+def measured_traffic(head, config, scale):
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled by a copy kernel's
+    reading), for this exact workload; None when not measured."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        table = json.load(f)
+    e = table.get(f"{head}/config{config}/scale{scale:g}")
+    return e["hbm_bytes_per_launch"] if e else None
+
+
+def copy_probe(torch, nbytes=1 << 30, reps=3):
+    """A plain device copy of known size (the FETCH_SIZE / WRITE_SIZE scale)."""
+    src = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"copy probe: {reps} copies of {nbytes} bytes")
 
 
 def cpu_baseline(dl, O, data, cfg, prm, target_s):
@@ -146,6 +218,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="", help="comma list of forms to time")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="skip the plain FP64 form (profiling runs)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a 1 GiB device copy (counter scale for profiles)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -160,6 +236,9 @@ def main():
     from oracle import oracle as O
 
     torch.cuda.set_device(0)
+    if args.copy_probe:
+        copy_probe(torch)
+        return
     cfg, prm = dl.config_synth(args.config, args.scale)
     ncol = cfg.nlat * cfg.nlon
     n = 1 + 3 * cfg.nz
@@ -174,6 +253,8 @@ def main():
              "batched_bf16x3": (dl.FORM_BATCHED, dl.GRAM_BF16X3, {})}
     if args.forms:
         forms = {k: v for k, v in forms.items() if k in args.forms.split(",") or k == "plain_fp64"}
+    if args.no_plain:
+        forms.pop("plain_fp64")
     res = {}
     ref = None
     for name, (form, gram, extra) in forms.items():
@@ -196,30 +277,20 @@ def main():
     # headline: fastest batched mixed-precision form within 1e-4 on every variable
     ok = [n for n, r in res.items() if n.startswith("batched_") and "fp64" not in n
           and r["err"] and max(r["err"][v] for v in ("ps", "u", "v", "T")) <= 1e-4]
-    head = min(ok, key=lambda n: res[n]["ms"]) if ok else "plain_fp64"
+    head = min(ok, key=lambda n: res[n]["ms"]) if ok else next(iter(res))
+    plain_ms = res["plain_fp64"]["ms"] if "plain_fp64" in res else None
     r = res[head]
     st = r["stats"]
     ms = r["ms"]
     value = ncol / (ms / 1e3)
 
-    tf_ms = r["stages"]["transform"]
-    flops = transform_flops(cfg.k, st["nobsl_sum"], ncol)
-    if head != "plain_fp64":
-        flops += 2.0 * cfg.k * cfg.k * n * ncol        # the fused update
-    achieved = flops / (tf_ms / 1e3) / 1e12 if tf_ms > 0 else None
-    kname = "k_letkf_core_f64 (transform)" if head == "plain_fp64" else \
-        "k_letkf_wave (fused transform + update)"
-    roofline = {"bound": "mfma", "kernel": kname, "achieved": achieved,
-                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / FP64_PEAK_TFLOPS if achieved else None, "traffic": None,
-                "note": "algorithmic flops (Gram 2k^2p + 2kp, eigensolve 9k^3, W 2k^3, "
-                        "wbar 4k^2 per analysis) over the kernel time; FP64 peak since the "
-                        "eigensolve, W and update are FP64"}
+    roofline = kernel_roofline(cfg, st, head, r["stages"]["transform"], args.config,
+                               args.scale)
 
     out_forms = {}
     for name, rr in res.items():
         out_forms[name] = {"ms_per_step": rr["ms"], "analyses_per_s": ncol / (rr["ms"] / 1e3),
-                           "speedup_vs_plain_fp64": res["plain_fp64"]["ms"] / rr["ms"],
+                           "speedup_vs_plain_fp64": plain_ms / rr["ms"] if plain_ms else None,
                            "rel_err_vs_plain_fp64": rr["err"],
                            "jacobi_max_sweeps_used": rr["stats"]["max_sweeps_used"],
                            "jacobi_mean_sweeps": rr["stats"]["sweeps_sum"] / max(1, rr["stats"]["nanal"]),
@@ -245,7 +316,7 @@ def main():
                    "jacobi_max_sweeps_used": st["max_sweeps_used"],
                    "nonconverged": st["nonconverged"],
                    "rel_increment_error_vs_fp64": r["err"] or None,
-                   "speedup_vs_plain_fp64": res["plain_fp64"]["ms"] / r["ms"]},
+                   "speedup_vs_plain_fp64": plain_ms / r["ms"] if plain_ms else None},
         "roofline": roofline, "cpu_baseline": cpu, "forms": out_forms,
     }
     print(json.dumps(line), flush=True)

@@ -385,7 +385,7 @@ static int sweep_batched(dida_letkf_ctx *c, const double *xb, const double *axb,
     ba.info = c->info;
     ba.queue = c->chain_order;
     ba.queue_head = c->queue_head;
-    if (c->timing && c->phase_clk) {
+    if ((c->timing & 2) && c->phase_clk) {
         HIP_TRY(hipMemsetAsync(c->phase_clk, 0, DK_NPHASE * sizeof(unsigned long long), s));
         ba.phase_clk = c->phase_clk;
     }
@@ -533,9 +533,9 @@ int dida_letkf_graph_launch(dida_letkf_ctx *c, void *stream)
 int dida_letkf_set_timing(dida_letkf_ctx *c, int32_t on)
 {
     if (!c) return set_err(DIDA_ERR_ARG, "null context");
-    c->timing = on ? 1 : 0;
+    c->timing = on & 3;
     c->nmark = 0;
-    if (c->timing && !c->phase_clk && c->prm.form == DIDA_FORM_BATCHED &&
+    if ((c->timing & 2) && !c->phase_clk && c->prm.form == DIDA_FORM_BATCHED &&
         hipMalloc((void **)&c->phase_clk, DK_NPHASE * sizeof(unsigned long long)) != hipSuccess)
         return set_err(DIDA_ERR_NOMEM, "phase clocks");
     return DIDA_OK;

@@ -21,6 +21,7 @@ DIDA_OK = 0
 DIDA_WARN_NOCONV = 1
 GRAM_FP64, GRAM_BF16X3, GRAM_FP16X3, GRAM_BF16 = 0, 1, 2, 3
 FORM_PLAIN, FORM_BATCHED = 0, 1
+TIMING_STAGES, TIMING_PHASES = 1, 2
 
 # Every entry point declared in include/dida_letkf.h.
 EXPORTED = ["dida_letkf_params_default", "letkf_core_", "letkf_core_batch_",
@@ -224,8 +225,10 @@ class Engine:
     def graph_launch(self, stream=None) -> int:
         return check(lib().dida_letkf_graph_launch(self.h, _stream_ptr(stream)), "graph_launch")
 
-    def set_timing(self, on: bool = True):
-        check(lib().dida_letkf_set_timing(self.h, 1 if on else 0), "set_timing")
+    def set_timing(self, on: bool = True, phases: bool = False):
+        """Stage events (on) and, for the batched form, in-kernel phase clocks."""
+        mode = (TIMING_STAGES if on else 0) | (TIMING_PHASES if phases else 0)
+        check(lib().dida_letkf_set_timing(self.h, mode), "set_timing")
 
     def stage_ms(self) -> dict:
         ms = (C.c_double * len(STAGES))()

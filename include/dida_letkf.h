@@ -159,10 +159,15 @@ int dida_letkf_graph_capture(dida_letkf_ctx *ctx, const dida_letkf_obs *obs, con
                              const double *axb, double *xa, double *axa, void *stream);
 int dida_letkf_graph_launch(dida_letkf_ctx *ctx, void *stream);
 
-/* Per-stage device time of the last eager sweep, from HIP events recorded on
- * the sweep's stream when timing is on (the reference's GPTL regions
+/* Timing of eager sweeps.  `on` bit 0 (DIDA_TIMING_STAGES): HIP events on
+ * the sweep's stream around each stage (the reference's GPTL regions
  * "Generate Yb matrix", "Lektf computation core", "Convert xb to xa",
- * letkf_compute.F90:69-120).  ms[DIDA_STAGE_*]; synchronises. */
+ * letkf_compute.F90:69-120), read by dida_letkf_stage_ms for the last sweep
+ * (ms[DIDA_STAGE_*]; synchronises).  Bit 1 (DIDA_TIMING_PHASES): device
+ * clocks per phase inside the batched kernel (dida_letkf_phase_cycles);
+ * these add atomics to the kernel, so leave them off where it is timed. */
+#define DIDA_TIMING_STAGES 1
+#define DIDA_TIMING_PHASES 2
 #define DIDA_STAGE_BIN 0        /* obs binning and gather */
 #define DIDA_STAGE_SELECT 1     /* local selection: count, scan, fill */
 #define DIDA_STAGE_TRANSFORM 2  /* contractions, eigensolve, W and wbar */
@@ -171,7 +176,7 @@ int dida_letkf_graph_launch(dida_letkf_ctx *ctx, void *stream);
 int dida_letkf_set_timing(dida_letkf_ctx *ctx, int32_t on);
 int dida_letkf_stage_ms(dida_letkf_ctx *ctx, double ms[DIDA_NSTAGES]);
 
-/* Batched form with timing on: device clock cycles summed over workgroups
+/* Batched form with DIDA_TIMING_PHASES: device clock cycles summed over workgroups
  * per phase of the fused kernel (Gram, warm start, Jacobi, transform,
  * update; the reference's "eig decomposition" and core timers,
  * letkf_core.F90:94-117).  Synchronises. */

@@ -60,3 +60,24 @@ def test_post_blend_and_rtps(oracle):
     sa0 = np.sqrt((xa ** 2).mean(axis=2))
     assert np.allclose(sa, 0.8 * sb + 0.2 * sa0, atol=1e-13)
     assert np.allclose(out.mean(axis=2), axa, atol=1e-13)
+
+
+def test_post_blend_special_cases(oracle):
+    """gamma = (0, 1) returns the analysis, (1, 0) the background; outside the
+    polar limit the background is kept whatever gamma is."""
+    rng = np.random.default_rng(5)
+    k, nrow, nlat, nlon = 6, 2, 3, 4
+    glat = np.array([-70, 0, 70], np.float32)
+    ncol = nlat * nlon
+    xb = rng.normal(size=(ncol, nrow, k))
+    xa = rng.normal(size=(ncol, nrow, k))
+    axb, axa = rng.normal(size=(ncol, nrow)), rng.normal(size=(ncol, nrow))
+    full_b, full_a = axb[..., None] + xb, axa[..., None] + xa
+    out, _, _ = oracle.post(0, glat, nlat, nlon, 90.0, 0.0, 1.0, 0.0, xb, axb, xa, axa)
+    assert np.array_equal(out, full_a)
+    out, _, _ = oracle.post(0, glat, nlat, nlon, 90.0, 1.0, 0.0, 0.0, xb, axb, xa, axa)
+    assert np.array_equal(out, full_b)
+    out, _, _ = oracle.post(0, glat, nlat, nlon, 50.0, 0.0, 1.0, 0.0, xb, axb, xa, axa)
+    polar = np.abs(glat[np.arange(ncol) % nlat]) > 50
+    assert np.array_equal(out[polar], full_b[polar])
+    assert np.array_equal(out[~polar], full_a[~polar])
