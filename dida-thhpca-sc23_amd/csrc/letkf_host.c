@@ -58,6 +58,7 @@ void dida_letkf_params_default(dida_letkf_params *p)
     p->bin_deg = 0.0;
     p->chain_len = 0;
     p->warm_start = 1;
+    p->jacobi_predict = -1;
 }
 
 static double jacobi_tol_of(const dida_letkf_params *p)
@@ -68,8 +69,12 @@ static double jacobi_tol_of(const dida_letkf_params *p)
 
 static int check_params(const dida_letkf_params *p)
 {
-    if (p->nanals < 2 || p->nanals > 64)
-        return set_err(DIDA_ERR_UNSUPPORTED, "nanals = %d: supported range is 2..64", p->nanals);
+    if (p->nanals < 2 || p->nanals > 128)
+        return set_err(DIDA_ERR_UNSUPPORTED, "nanals = %d: supported range is 2..128", p->nanals);
+    if (p->form == DIDA_FORM_BATCHED && p->nanals > 64)
+        return set_err(DIDA_ERR_UNSUPPORTED,
+                       "nanals = %d: the batched form supports 2..64 (plain form: 2..128)",
+                       p->nanals);
     if (p->nrow < 1 || p->nlat < 1 || p->nlon < 1)
         return set_err(DIDA_ERR_ARG, "grid %d x %d x %d rows is empty", p->nlat, p->nlon, p->nrow);
     if (p->max_obs < 0) return set_err(DIDA_ERR_ARG, "max_obs < 0");
@@ -363,6 +368,8 @@ static int sweep_batched(dida_letkf_ctx *c, const double *xb, const double *axb,
     ba.chain_len = c->chain_len;
     ba.gram = c->prm.gram;
     ba.warm_start = c->prm.warm_start ? 1 : 0;
+    ba.predict = c->prm.jacobi_predict < 0 ? c->prm.gram != DIDA_GRAM_FP64
+                                            : c->prm.jacobi_predict != 0;
     ba.rtps = c->prm.rtps;
     ba.max_sweeps = c->prm.jacobi_max_sweeps;
     ba.max_p = c->prm.max_obs;
@@ -649,8 +656,8 @@ int dida_letkf_core_batch_dev(int32_t batch, int32_t max_obs, int32_t nanals,
                               int32_t *info, void *stream)
 {
     if (batch < 0 || max_obs < 0) return set_err(DIDA_ERR_ARG, "negative size");
-    if (nanals < 2 || nanals > 64)
-        return set_err(DIDA_ERR_UNSUPPORTED, "nanals = %d: supported range is 2..64", nanals);
+    if (nanals < 2 || nanals > 128)
+        return set_err(DIDA_ERR_UNSUPPORTED, "nanals = %d: supported range is 2..128", nanals);
     if (gram != DIDA_GRAM_FP64) return set_err(DIDA_ERR_UNSUPPORTED, "gram %d", gram);
     if (!(inflation_factor > 0.0)) return set_err(DIDA_ERR_ARG, "inflation_factor must be > 0");
     if (batch == 0) return DIDA_OK;

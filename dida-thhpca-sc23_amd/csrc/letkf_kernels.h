@@ -135,6 +135,7 @@ typedef struct dk_batched_args {
     int32_t k, nrow, nlat, nlon;
     int32_t nchain, chains_per_lon, chain_len, gram;
     int32_t warm_start, rtps, max_sweeps, max_p;
+    int32_t predict, pad_;
     double rtps_alpha, rho, tol;
     int64_t capacity;
     const int64_t *offset;

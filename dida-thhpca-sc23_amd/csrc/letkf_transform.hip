@@ -37,11 +37,96 @@ struct CoreSmem {
   int flag[2];
 };
 
-// Gram + b with the obs split across the four waves (KP <= 64: all lower
-// tiles of C fit in one wave's accumulators).
+// KP = 128 Gram: wave w accumulates lower tiles t = w, w + 4, ... (I >= J)
+// over all obs; 4 obs a step per lane group, loads issued before the MFMAs.
 template <int KP>
-__device__ __forceinline__ void gram_f64(const dk_core_args& a, int64_t off, int p,
-                                         CoreSmem<KP>& sm) {
+__device__ __forceinline__ void gram_f64_tiles(const dk_core_args& a, int64_t off, int p,
+                                               CoreSmem<KP>& sm) {
+  constexpr int NT = KP / 16;
+  constexpr int NTILE = NT * (NT + 1) / 2;
+  constexpr int NW = (NTILE + 3) / 4;  // tiles per wave
+  constexpr int LDA = CoreSmem<KP>::LDA;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int r16 = lane & 15, q4 = lane >> 4;
+  const int k = a.k;
+  int ti[NW], tj[NW];
+#pragma unroll
+  for (int w = 0; w < NW; ++w) {
+    int t = wave + 4 * w, I = 0;
+    if (t >= NTILE) t = NTILE - 1;  // duplicate of a real tile, not stored
+    while ((I + 1) * (I + 2) / 2 <= t) ++I;
+    ti[w] = I;
+    tj[w] = t - I * (I + 1) / 2;
+  }
+  double4_t acc[NW];
+#pragma unroll
+  for (int w = 0; w < NW; ++w) acc[w] = double4_t{0.0, 0.0, 0.0, 0.0};
+  double bacc[NT];
+#pragma unroll
+  for (int T = 0; T < NT; ++T) bacc[T] = 0.0;
+  for (int o0 = 0; o0 < p; o0 += 4) {
+    const int o = o0 + q4;
+    const bool ok = o < p;
+    const int oc = ok ? o : 0;
+    const int64_t gi = a.idx ? (int64_t)a.idx[off + oc] : off + oc;
+    __builtin_amdgcn_sched_barrier(0);
+    const double* y = a.yb + gi * k;
+    double f[NT];
+#pragma unroll
+    for (int T = 0; T < NT; ++T) {
+      const int row = T * 16 + r16;
+      f[T] = y[row < k ? row : k - 1];
+    }
+    const double rr = ok ? sqrt((double)a.rinv[gi] * a.rloc[off + oc]) : 0.0;
+    const double sd = rr * a.dep[gi];
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int T = 0; T < NT; ++T) f[T] = (T * 16 + r16 < k) ? f[T] * rr : 0.0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      double fa = 0.0, fb = 0.0;
+#pragma unroll
+      for (int T = 0; T < NT; ++T) {
+        fa = ti[w] == T ? f[T] : fa;
+        fb = tj[w] == T ? f[T] : fb;
+      }
+      acc[w] = mfma_f64_16x16x4(fa, fb, acc[w]);
+    }
+    if (wave == 0)
+#pragma unroll
+      for (int T = 0; T < NT; ++T) bacc[T] += f[T] * sd;
+  }
+#pragma unroll
+  for (int w = 0; w < NW; ++w) {
+    if (wave + 4 * w >= NTILE) continue;
+    const int I = ti[w], J = tj[w];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = I * 16 + 4 * r + q4, j = J * 16 + r16;
+      sm.A[i + j * LDA] = acc[w][r];
+      if (I != J) sm.A[j + i * LDA] = acc[w][r];
+    }
+  }
+  if (wave == 0) {
+#pragma unroll
+    for (int T = 0; T < NT; ++T) {
+      bacc[T] += shfl_xor_d(bacc[T], 16);
+      bacc[T] += shfl_xor_d(bacc[T], 32);
+    }
+    if (q4 == 0)
+#pragma unroll
+      for (int T = 0; T < NT; ++T) sm.b[T * 16 + r16] = bacc[T];
+  }
+  __syncthreads();
+}
+
+// Gram + b.  KP <= 64: the obs are split across the four waves (all lower
+// tiles of C fit in one wave's accumulators) and the partial sums meet in
+// LDS.  KP = 128: the 36 lower tiles are split across the waves (tile t to
+// wave t % 4), every wave reads every obs, and wave 0 also forms b.
+template <int KP>
+__device__ __forceinline__ void gram_f64_obs(const dk_core_args& a, int64_t off, int p,
+                                             CoreSmem<KP>& sm) {
   constexpr int NT = KP / 16;
   constexpr int NTILE = NT * (NT + 1) / 2;
   constexpr int LDA = CoreSmem<KP>::LDA;
@@ -134,6 +219,13 @@ __device__ __forceinline__ void gram_f64(const dk_core_args& a, int64_t off, int
     }
     __syncthreads();
   }
+}
+
+template <int KP>
+__device__ __forceinline__ void gram_f64(const dk_core_args& a, int64_t off, int p,
+                                         CoreSmem<KP>& sm) {
+  if constexpr (KP > 64) gram_f64_tiles<KP>(a, off, p, sm);
+  else gram_f64_obs<KP>(a, off, p, sm);
 }
 
 template <int KP>
@@ -232,9 +324,10 @@ extern "C" int dk_letkf_core_f64(const dk_core_args* a, void* stream) {
   if (a->nanal <= 0) return 0;
   hipStream_t s = (hipStream_t)stream;
   const int k = a->k;
-  if (k < 2 || k > 64) return (int)hipErrorInvalidValue;
+  if (k < 2 || k > 128) return (int)hipErrorInvalidValue;
   if (k <= 16) k_letkf_core_f64<16><<<a->nanal, kThreads, 0, s>>>(*a);
   else if (k <= 32) k_letkf_core_f64<32><<<a->nanal, kThreads, 0, s>>>(*a);
-  else k_letkf_core_f64<64><<<a->nanal, kThreads, 0, s>>>(*a);
+  else if (k <= 64) k_letkf_core_f64<64><<<a->nanal, kThreads, 0, s>>>(*a);
+  else k_letkf_core_f64<128><<<a->nanal, kThreads, 0, s>>>(*a);
   return (int)hipGetLastError();
 }

@@ -580,7 +580,7 @@ __device__ void wave_column(const dk_batched_args& a, int64_t col, bool warm, WS
   if (warm) w.warm(sm);
   clk.mark(DK_PHASE_WARM);
   bool conv;
-  const int sweeps = w.jacobi(a.tol, a.max_sweeps, GRAM != DK_GRAM_FP64, &conv);
+  const int sweeps = w.jacobi(a.tol, a.max_sweeps, a.predict != 0, &conv);
   clk.mark(DK_PHASE_JACOBI);
 
   // lambda_j = |a_j|; V = A / lambda (to LDS); f = sqrt((k-1)/lambda);

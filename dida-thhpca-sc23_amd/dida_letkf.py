@@ -41,7 +41,8 @@ class Params(C.Structure):
                 ("inflation_factor", C.c_double), ("rtps", C.c_int32), ("form", C.c_int32),
                 ("rtps_alpha", C.c_double), ("gram", C.c_int32),
                 ("jacobi_max_sweeps", C.c_int32), ("jacobi_tol", C.c_double),
-                ("bin_deg", C.c_double), ("chain_len", C.c_int32), ("warm_start", C.c_int32)]
+                ("bin_deg", C.c_double), ("chain_len", C.c_int32), ("warm_start", C.c_int32),
+                ("jacobi_predict", C.c_int32)]
 
 
 class ObsC(C.Structure):

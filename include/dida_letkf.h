@@ -65,6 +65,12 @@ typedef struct dida_letkf_params {
                                    each warm-starting the next one's eigensolve
                                    (0: automatic) */
     int32_t warm_start;         /* batched form: 1 to warm-start along chains */
+    int32_t jacobi_predict;     /* batched form: 1 ends the eigensolve after a sweep
+                                   whose largest |a_p.a_q|^2 / (|a_p|^2 |a_q|^2) is
+                                   <= tol (the next sweep would rotate nothing);
+                                   0 runs until a sweep rotates nothing; -1
+                                   (default): 1 with a half-precision Gram, 0 with
+                                   the FP64 Gram */
 } dida_letkf_params;
 
 /* Observations, device pointers, one entry per scalar observation (the

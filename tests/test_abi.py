@@ -48,3 +48,11 @@ def test_params_default_and_errors(dl):
     h = ctypes.c_void_p()
     rc = dl.lib().dida_letkf_create(ctypes.byref(h), ctypes.byref(bad), 10, 0)
     assert rc < 0 and b"nanals" in dl.lib().dida_letkf_last_error()
+    # k up to 128 in the plain form, up to 64 in the batched form
+    bad = dl.default_params(nanals=129, nrow=1, nlat=2, nlon=2)
+    rc = dl.lib().dida_letkf_create(ctypes.byref(h), ctypes.byref(bad), 10, 0)
+    assert rc == -5 and b"2..128" in dl.lib().dida_letkf_last_error()
+    bad = dl.default_params(nanals=96, nrow=1, nlat=2, nlon=2, form=dl.FORM_BATCHED,
+                            gram=dl.GRAM_BF16X3)
+    rc = dl.lib().dida_letkf_create(ctypes.byref(h), ctypes.byref(bad), 10, 0)
+    assert rc == -5 and b"batched" in dl.lib().dida_letkf_last_error()

@@ -43,7 +43,7 @@ def _batch(rng, k, counts, max_obs):
     return hx, ri, dep, rl
 
 
-@pytest.mark.parametrize("k", [2, 3, 16, 20, 32, 48, 64])
+@pytest.mark.parametrize("k", [2, 3, 16, 20, 32, 48, 64, 65, 96, 128])
 def test_batch_matches_oracle(cuda, dl, oracle, k):
     rng = np.random.default_rng(k)
     counts = [0, 1, 2, 5, k - 1, k, k + 3, 300, 1000]

@@ -62,13 +62,17 @@ def _check_selection(oracle, dbg, perm, bin_start, nband, nlonbin, ob, prm, bin_
         assert np.abs(dbg["csr_w"][off:off + p] - w).max(initial=0.0) <= 1e-12, col
 
 
-@pytest.mark.parametrize("case", ["uniform", "clustered_capped"])
+@pytest.mark.parametrize("case", ["uniform", "clustered_capped", "k128"])
 def test_sweep_matches_oracle(cuda, dl, oracle, case):
     torch = cuda
     bin_deg = 3.0
     if case == "uniform":
         cfg = dl.synth_cfg(36, 72, 4, 20, 2500, seed=0xD1DA0000)
         over = dict(local_dist=1000.0, inflation_factor=1.1, bin_deg=bin_deg)
+    elif case == "k128":
+        cfg = dl.synth_cfg(12, 24, 2, 128, 1500, seed=0xD1DA0004, cluster_frac=0.5,
+                           nclusters=2, cluster_radius_km=800.0, rscale=(0.5, 3.0))
+        over = dict(local_dist=1500.0, inflation_factor=1.05, bin_deg=bin_deg)
     else:
         cfg = dl.synth_cfg(45, 90, 3, 32, 6000, seed=0xD1DA0001, cluster_frac=0.7, nclusters=3,
                            cluster_radius_km=600.0, rscale=(0.1, 10.0))
