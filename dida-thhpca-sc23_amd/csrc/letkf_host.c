@@ -64,7 +64,13 @@ void dida_letkf_params_default(dida_letkf_params *p)
 static double jacobi_tol_of(const dida_letkf_params *p)
 {
     if (p->jacobi_tol > 0.0) return p->jacobi_tol;
-    return p->gram == DIDA_GRAM_FP64 ? 1e-13 : 1e-9;
+    return p->gram == DIDA_GRAM_FP64 ? 1e-13 : 1e-7;
+}
+
+static int predict_of(const dida_letkf_params *p)
+{
+    if (p->jacobi_predict < 0) return p->gram != DIDA_GRAM_FP64;
+    return p->jacobi_predict != 0;
 }
 
 static int check_params(const dida_letkf_params *p)
@@ -368,8 +374,7 @@ static int sweep_batched(dida_letkf_ctx *c, const double *xb, const double *axb,
     ba.chain_len = c->chain_len;
     ba.gram = c->prm.gram;
     ba.warm_start = c->prm.warm_start ? 1 : 0;
-    ba.predict = c->prm.jacobi_predict < 0 ? c->prm.gram != DIDA_GRAM_FP64
-                                            : c->prm.jacobi_predict != 0;
+    ba.predict = predict_of(&c->prm);
     ba.rtps = c->prm.rtps;
     ba.max_sweeps = c->prm.jacobi_max_sweeps;
     ba.max_p = c->prm.max_obs;
@@ -436,6 +441,7 @@ static int sweep_rest(dida_letkf_ctx *c, dk_select_args *sa, const double *xb,
     ca.rho = c->prm.inflation_factor;
     ca.tol = jacobi_tol_of(&c->prm);
     ca.max_sweeps = c->prm.jacobi_max_sweeps;
+    ca.predict = predict_of(&c->prm);
     ca.trans = c->trans;
     ca.wbar = NULL;
     ca.info = c->info;
@@ -698,6 +704,7 @@ int dida_letkf_core_batch_dev(int32_t batch, int32_t max_obs, int32_t nanals,
     ca.rho = inflation_factor;
     ca.tol = 1e-13;
     ca.max_sweeps = 30;
+    ca.predict = 0;
     ca.trans = trans;
     ca.wbar = overline_w;
     ca.info = info;

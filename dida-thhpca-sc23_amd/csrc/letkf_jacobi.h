@@ -8,7 +8,10 @@
 // a round are issued together).  A pair (p, q) is rotated while
 // |a_p . a_q| > tol |a_p| |a_q|; column norms are recomputed at the start of
 // each sweep and updated exactly in between (alpha' = alpha - t g,
-// beta' = beta + t g).  A sweep without any rotation ends the iteration.
+// beta' = beta + t g).  A sweep without any rotation ends the iteration, or,
+// with predict, a sweep in which no pair had g^2 > tol alpha beta (Jacobi
+// converges quadratically, so the next sweep would rotate nothing).
+// flag[4] is LDS scratch.
 // For a symmetric positive (semi)definite input the converged columns are
 // lambda_j v_j (Hestenes 1958; Demmel & Veselic 1992).
 #pragma once
@@ -36,7 +39,7 @@ __device__ __forceinline__ void jacobi_rotation(double al, double be, double g, 
 
 template <int KP, int LDA, int NTHREADS>
 __device__ int jacobi_lds(double* A, double* nrm, int* flag, double tol, int max_sweeps,
-                          bool* converged) {
+                          bool predict, bool* converged) {
   constexpr int NPAIR = KP / 2;
   constexpr int TPP0 = NTHREADS / NPAIR;
   constexpr int TPP = TPP0 > 32 ? 32 : TPP0;   // threads per pair (power of two)
@@ -44,7 +47,10 @@ __device__ int jacobi_lds(double* A, double* nrm, int* flag, double tol, int max
   const int tid = threadIdx.x;
   const int pair = tid / TPP, sub = tid % TPP;
   const bool active = pair < NPAIR;
-  if (tid == 0) flag[0] = 0;
+  if (tid == 0) {
+    flag[0] = 0;
+    flag[2] = 0;
+  }
   int sweep = 0;
   *converged = false;
   for (; sweep < max_sweeps; ++sweep) {
@@ -55,7 +61,7 @@ __device__ int jacobi_lds(double* A, double* nrm, int* flag, double tol, int max
       nrm[j] = s;
     }
     __syncthreads();
-    bool rotated = false;
+    bool rotated = false, big = false;
     for (int r = 0; r < KP - 1; ++r) {
       if (active) {
         int p, q;
@@ -86,6 +92,7 @@ __device__ int jacobi_lds(double* A, double* nrm, int* flag, double tol, int max
 #pragma unroll
         for (int s = 1; s < TPP; s <<= 1) g += shfl_xor_d(g, s);
         const double al = nrm[p], be = nrm[q];
+        big |= g * g > tol * (al * be);
         if (g != 0.0 && fabs(g) > tol * sqrt(al * be)) {
           double c, s, t;
           jacobi_rotation(al, be, g, &c, &s, &t);
@@ -107,13 +114,17 @@ __device__ int jacobi_lds(double* A, double* nrm, int* flag, double tol, int max
       __syncthreads();
     }
     if (rotated) flag[sweep & 1] = 1;
+    if (big) flag[2 + (sweep & 1)] = 1;
     __syncthreads();
-    if (flag[sweep & 1] == 0) {
+    if (flag[sweep & 1] == 0 || (predict && flag[2 + (sweep & 1)] == 0)) {
       *converged = true;
       ++sweep;
       break;
     }
-    if (tid == 0) flag[(sweep + 1) & 1] = 0;
+    if (tid == 0) {
+      flag[(sweep + 1) & 1] = 0;
+      flag[2 + ((sweep + 1) & 1)] = 0;
+    }
   }
   return sweep;
 }

@@ -98,7 +98,7 @@ typedef struct dk_core_args {
     double rho;             /* multiplicative inflation */
     double tol;             /* Jacobi: rotate while |g| > tol sqrt(a b) */
     int32_t max_sweeps;
-    int32_t pad_;
+    int32_t predict;        /* Jacobi: stop when the next sweep would rotate nothing */
     double *trans;          /* [(id - aid0) * k * k] trans(i, j) at i + j k */
     double *wbar;           /* [(id - aid0) * k] (optional) */
     int32_t *info;          /* [id]: sweeps, or -1 when not converged (optional) */

@@ -4,16 +4,14 @@
 //   1. Gram C = Yhat Yhat^T on FP64 MFMA (16x16x4), Yhat = Yb sqrt(R^-1 rloc),
 //      and b = Yhat (sqrt(R^-1 rloc) dep), both accumulated straight from the
 //      gathered obs columns; the four waves split the obs and reduce in LDS.
-//   2. One-sided (Hestenes) Jacobi on C resident in LDS: column pairs of a
-//      round-robin ordering are orthogonalised in parallel until a sweep makes
-//      no rotation.  The converged columns are lambda_j v_j, so no separate
-//      eigenvector array is needed (C is symmetric positive semi-definite).
-//   3. W = sqrt(rho) I + sum_j (f(lambda_j) - sqrt(rho)) v_j v_j^T and
-//      wbar = Pa b with Pa = rho/(k-1) I + sum_j (1/(lambda_j + (k-1)/rho) -
-//      rho/(k-1)) v_j v_j^T, f(l) = sqrt((k-1)/(l + (k-1)/rho)).  This is the
-//      reference's V f(Lambda) V^T with the null-space terms written in closed
-//      form, so the null-space eigenvectors (which one-sided Jacobi does not
-//      deliver) are never needed.  trans = W + wbar 1^T.
+//   2. One-sided (Hestenes) Jacobi, resident in LDS, on M = C + (k-1)/rho I
+//      (the reference shifts the eigenvalues of C by (k-1)/rho,
+//      letkf_core.F90:110-113): column pairs of a round-robin ordering are
+//      orthogonalised in parallel until a sweep makes no rotation.  M is
+//      symmetric positive definite, so the converged columns are lambda_j v_j
+//      with every lambda_j > 0 and no separate eigenvector array is needed.
+//   3. W = V diag(sqrt((k-1)/lambda)) V^T and wbar = V diag(1/lambda) V^T b
+//      (letkf_core.F90:129-210), trans = W + wbar 1^T.
 #include "letkf_device.h"
 #include "letkf_jacobi.h"
 #include "letkf_kernels.h"
@@ -34,7 +32,7 @@ struct CoreSmem {
   double sh[KP];
   double u[KP];
   double wb[KP];
-  int flag[2];
+  int flag[4];
 };
 
 // KP = 128 Gram: wave w accumulates lower tiles t = w, w + 4, ... (I >= J)
@@ -244,31 +242,36 @@ __global__ __launch_bounds__(kThreads) void k_letkf_core_f64(dk_core_args a) {
   const int wave = tid >> 6, lane = tid & 63, r16 = lane & 15, q4 = lane >> 4;
 
   gram_f64<KP>(a, off, p, sm);
+  const double km1 = (double)(k - 1);
+  const double a0 = km1 / a.rho;
+  // M = C + (k-1)/rho I on the first k diagonal entries (letkf_core.F90:110-113
+  // applies the shift to the eigenvalues; M is positive definite, so every
+  // converged column is a non-null lambda_j v_j)
+  for (int i = tid; i < k; i += kThreads) sm.A[i + i * LDA] += a0;
+  __syncthreads();
 
   bool conv;
   const int sweeps = jacobi_lds<KP, LDA, kThreads>(sm.A, sm.nrm, sm.flag, a.tol, a.max_sweeps,
-                                                   &conv);
+                                                   a.predict != 0, &conv);
   const int m = (k + 1) & ~1;
 
-  // exact column norms: alpha_j = lambda_j^2
+  // exact column norms: alpha_j = lambda_j^2, lambda_j eigenvalues of M
   for (int j = tid; j < m; j += kThreads) {
     double s = 0.0;
     for (int i = 0; i < k; ++i) s += sm.A[i + j * LDA] * sm.A[i + j * LDA];
     sm.nrm[j] = s;
   }
   __syncthreads();
-  const double km1 = (double)(k - 1);
-  const double a0 = km1 / a.rho;
-  const double sa0 = sqrt(a0);
+  // with v_j = a_j / lambda_j: W = sum_j sqrt((k-1)/lambda_j) v_j v_j^T
+  // = A diag(g) A^T, g_j = sqrt((k-1)/lambda_j) / alpha_j, and
+  // wbar = sum_j v_j (v_j^T b) / lambda_j = A diag(h) A^T b, h_j = 1/(lambda_j alpha_j)
   for (int j = tid; j < KP; j += kThreads) {
     const double al = j < m ? sm.nrm[j] : 0.0;
     double g = 0.0, h = 0.0;
     if (al > 0.0) {
       const double lam = sqrt(al);
-      const double sl = sqrt(lam + a0);
-      // f(lam) - f(0) and 1/(lam + a0) - 1/a0 in cancellation-free form
-      g = -sqrt(km1) * lam / (sl * sa0 * (sa0 + sl)) / al;
-      h = -lam / (a0 * (lam + a0)) / al;
+      g = sqrt(km1 / lam) / al;
+      h = 1.0 / (lam * al);
     }
     sm.sg[j] = g;
     sm.sh[j] = h;
@@ -281,19 +284,15 @@ __global__ __launch_bounds__(kThreads) void k_letkf_core_f64(dk_core_args a) {
     sm.u[j] = s * sm.sh[j];
   }
   __syncthreads();
-  const double rk = a.rho / km1;
   for (int i = tid; i < KP; i += kThreads) {
     double s = 0.0;
-    if (i < k) {
-      s = rk * sm.b[i];
+    if (i < k)
       for (int j = 0; j < m; ++j) s += sm.A[i + j * LDA] * sm.u[j];
-    }
     sm.wb[i] = s;
   }
   __syncthreads();
 
-  // W = sqrt(rho) I + (A diag(sg)) A^T on MFMA, written with wbar as trans.
-  const double srho = sqrt(a.rho);
+  // W = (A diag(sg)) A^T on MFMA, written with wbar as trans.
   double* trans = a.trans + slot * k * k;
   for (int t = wave; t < NT * NT; t += 4) {
     const int I = t / NT, I2 = t % NT;
@@ -310,7 +309,7 @@ __global__ __launch_bounds__(kThreads) void k_letkf_core_f64(dk_core_args a) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int i = I * 16 + 4 * r + q4, i2 = I2 * 16 + r16;
-      if (i < k && i2 < k) trans[i + (int64_t)i2 * k] = (i == i2 ? srho : 0.0) + d[r] + sm.wb[i];
+      if (i < k && i2 < k) trans[i + (int64_t)i2 * k] = d[r] + sm.wb[i];
     }
   }
   if (a.wbar)

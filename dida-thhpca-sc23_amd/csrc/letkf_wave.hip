@@ -64,9 +64,9 @@ struct WSmem {
   double b[KP];
   double f[KP];
   double u[KP];
-  int32_t ogi[64];
-  double orr[64];
-  double osd[64];
+  int32_t ogi[64];  // staged obs block: binned index,
+  double orr[64];   // sqrt(R^-1 rloc),
+  double ow[64];    // R^-1 rloc dep
 };
 
 __device__ __forceinline__ int rowA(int I, int r, int g) { return I * 16 + 4 * g + r; }
@@ -125,26 +125,37 @@ __device__ __forceinline__ double rcp_nr(double x) {
   return y;
 }
 
-// Stage obs [base, base + 64) of the analysis in LDS (binned index,
-// sqrt(R^-1 rloc), sqrt(R^-1 rloc) dep).
-template <int KP>
-__device__ __forceinline__ void stage64(const dk_batched_args& a, int64_t off, int p, int base,
-                                        WSmem<KP>& sm) {
-  const int l = threadIdx.x;
-  const int o = base + l;
-  int gi = 0;
-  double rr = 0.0, sd = 0.0;
-  if (o < p) {
-    gi = a.idx[off + o];
-    rr = sqrt((double)a.rinv[gi] * a.rloc[off + o]);
-    sd = rr * a.dep[gi];
+// Metadata of one 64-obs block of an analysis, lane l holding obs base + l:
+// binned index, sqrt(R^-1 rloc) and R^-1 rloc dep.  Entries past p take the
+// index of obs 0 and zero weights, so every load is unconditional and in
+// bounds (callers only use blocks with base < p, p > 0).  The loads are split
+// in two so the dependent ones can be issued once the indices have arrived.
+struct ObsMeta {
+  int gi;
+  bool ok;
+  double rl, ri, dp;
+  __device__ __forceinline__ void load_index(const dk_batched_args& a, int64_t off, int p,
+                                             int base) {
+    const int o = base + (int)threadIdx.x;
+    ok = o < p;
+    const int oc = ok ? o : 0;
+    gi = a.idx[off + oc];
+    rl = a.rloc[off + oc];
   }
-  __syncthreads();
-  sm.ogi[l] = gi;
-  sm.orr[l] = rr;
-  sm.osd[l] = sd;
-  __syncthreads();
-}
+  __device__ __forceinline__ void load_obs(const dk_batched_args& a) {
+    ri = (double)a.rinv[gi];
+    dp = a.dep[gi];
+  }
+  template <int KP>
+  __device__ __forceinline__ void store(WSmem<KP>& sm) const {
+    const int l = threadIdx.x;
+    const double r2 = ok ? ri * rl : 0.0;
+    const double rr = sqrt(r2);
+    sm.ogi[l] = gi;
+    sm.orr[l] = rr;
+    sm.ow[l] = rr * (rr * dp);
+  }
+};
 
 template <int KP, int GRAM>
 struct Wave {
@@ -157,142 +168,151 @@ struct Wave {
   bool patB;         // current row pattern
 
   // ---- Gram ---------------------------------------------------------------
+  // C = Yhat Yhat^T and b = Yhat^T (sqrt(R^-1 rloc) d) over 64-obs blocks:
+  // the metadata of block i + 1 is loaded while block i computes, and the Yb
+  // rows of both 32-obs halves of a block are issued before any arithmetic.
+  // Rows past k read row k - 1 (in bounds) and are masked once at the end
+  // (a row of Yhat only reaches the row and column of C it indexes).
   __device__ void gram(const dk_batched_args& a, int64_t off, int p, WSmem<KP>& sm) {
     const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
     const int k = a.k;
+    constexpr bool kF64 = GRAM == DK_GRAM_FP64;
+    typedef typename H16<kF64 ? DK_GRAM_BF16X3 : GRAM>::vec hvec;
+    typedef typename H16<kF64 ? DK_GRAM_BF16X3 : GRAM>::elt helt;
+    constexpr bool kSplit = GRAM != DK_GRAM_BF16;
     double bacc[NJ];
 #pragma unroll
     for (int I = 0; I < NJ; ++I) bacc[I] = 0.0;
-    if constexpr (GRAM == DK_GRAM_FP64) {
-      double4_t acc[NJ][NJ];
+    double scale = 1.0, inv2 = 1.0;
+    if (GRAM == DK_GRAM_FP16X3) {
+      // power-of-two scale so the largest |Yhat| entry is ~2^11 in FP16
+      double mx = 0.0;
+      for (int o = lane; o < p; o += 64) {
+        const int gi = a.idx[off + o];
+        mx = fmax(mx, a.ymax[gi] * sqrt(a.rloc[off + o]));
+      }
+      for (int s = 32; s >= 1; s >>= 1) mx = fmax(mx, __shfl_xor(mx, s, kWave));
+      if (mx > 0.0) {
+        int e;
+        frexp(mx, &e);
+        scale = ldexp(1.0, 11 - e);
+        inv2 = ldexp(1.0, 2 * (e - 11));
+      }
+    }
+    double4_t accd[kF64 ? NJ : 1][kF64 ? NJ : 1];
+    f32x4_t acch[kF64 ? 1 : NJ][kF64 ? 1 : NJ];
+    if constexpr (kF64) {
 #pragma unroll
       for (int I = 0; I < NJ; ++I)
 #pragma unroll
-        for (int J = 0; J < NJ; ++J) acc[I][J] = double4_t{0.0, 0.0, 0.0, 0.0};
-      for (int base = 0; base < p; base += 64) {
-        stage64<KP>(a, off, p, base, sm);
-        const int nb = min(64, p - base);
-        for (int c0 = 0; c0 * 4 < nb; c0 += 4) {
-          double fr[4][NJ], sd[4];
-          // loads first (unconditional, in bounds), then the arithmetic
+        for (int J = 0; J < NJ; ++J) accd[I][J] = double4_t{0.0, 0.0, 0.0, 0.0};
+    } else {
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            const double* y = a.yb + (int64_t)sm.ogi[(c0 + u) * 4 + g] * k;
+      for (int I = 0; I < NJ; ++I)
+#pragma unroll
+        for (int J = 0; J < NJ; ++J) acch[I][J] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+    }
+    if (p > 0) {
+      ObsMeta m;
+      m.load_index(a, off, p, 0);
+      m.load_obs(a);
+      for (int base = 0; base < p; base += 64) {
+        __syncthreads();
+        m.store<KP>(sm);
+        __syncthreads();
+        ObsMeta n;
+        n.load_index(a, off, p, base + 64);
+        // obs e of half h in the MFMA operand layout: FP64 16x16x4 k-steps of
+        // 4 obs (obs 4e + g), half 16x16x32 (obs 8g + e)
+        double raw[2][8][NJ];
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+          for (int e = 0; e < 8; ++e) {
+            const int ol = h * 32 + (kF64 ? 4 * e + g : 8 * g + e);
+            const double* y = a.yb + (int64_t)sm.ogi[ol] * k;
 #pragma unroll
             for (int I = 0; I < NJ; ++I) {
               const int row = I * 16 + c;
-              fr[u][I] = y[row < k ? row : k - 1];
+              raw[h][e][I] = y[row < k ? row : k - 1];
             }
           }
-          __builtin_amdgcn_sched_barrier(0);
+        __builtin_amdgcn_sched_barrier(0);
+        const int nb = min(64, p - base);
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            const int ol = (c0 + u) * 4 + g;
-            const double rr = sm.orr[ol];
-            sd[u] = sm.osd[ol];
+        for (int h = 0; h < 2; ++h) {
+          if (h == 1) n.load_obs(a);
+          if (h * 32 >= nb) break;  // uniform across the wave
+          if constexpr (kF64) {
 #pragma unroll
-            for (int I = 0; I < NJ; ++I) fr[u][I] = (I * 16 + c < k) ? fr[u][I] * rr : 0.0;
-          }
+            for (int e = 0; e < 8; ++e) {
+              const int ol = h * 32 + 4 * e + g;
+              const double rr = sm.orr[ol], w = sm.ow[ol];
+              double f[NJ];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
+              for (int I = 0; I < NJ; ++I) {
+                f[I] = raw[h][e][I] * rr;
+                bacc[I] = fma(raw[h][e][I], w, bacc[I]);
+              }
 #pragma unroll
-            for (int I = 0; I < NJ; ++I) {
-              bacc[I] += fr[u][I] * sd[u];
+              for (int I = 0; I < NJ; ++I)
 #pragma unroll
-              for (int J = 0; J < NJ; ++J) acc[I][J] = mfma_f64_16x16x4(fr[u][I], fr[u][J], acc[I][J]);
+                for (int J = 0; J < NJ; ++J) accd[I][J] = mfma_f64_16x16x4(f[I], f[J], accd[I][J]);
             }
+          } else {
+            hvec hi[NJ], lo[NJ];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+              const int ol = h * 32 + 8 * g + e;
+              const double rr = sm.orr[ol], w = sm.ow[ol];
+              const float rs = (float)(rr * scale);
+#pragma unroll
+              for (int I = 0; I < NJ; ++I) {
+                bacc[I] = fma(raw[h][e][I], w, bacc[I]);
+                const float xs = (float)raw[h][e][I] * rs;
+                const helt hv = (helt)xs;
+                hi[I][e] = hv;
+                if (kSplit) lo[I][e] = (helt)(xs - (float)hv);
+              }
+            }
+#pragma unroll
+            for (int I = 0; I < NJ; ++I)
+#pragma unroll
+              for (int J = 0; J < NJ; ++J) {
+                acch[I][J] = H16<GRAM>::mfma(hi[I], hi[J], acch[I][J]);
+                if (kSplit) {
+                  acch[I][J] = H16<GRAM>::mfma(hi[I], lo[J], acch[I][J]);
+                  acch[I][J] = H16<GRAM>::mfma(lo[I], hi[J], acch[I][J]);
+                }
+              }
           }
         }
+        m = n;
       }
+    }
+    if constexpr (kF64) {
       // D[4r + g][c] of tile (I, J) = C(I*16 + 4r + g, J*16 + c)
 #pragma unroll
       for (int I = 0; I < NJ; ++I)
 #pragma unroll
         for (int J = 0; J < NJ; ++J)
 #pragma unroll
-          for (int r = 0; r < 4; ++r) X[J][I * 4 + r] = acc[I][J][r];
+          for (int r = 0; r < 4; ++r) {
+            const bool in = I * 16 + 4 * r + g < k && J * 16 + c < k;
+            X[J][I * 4 + r] = in ? accd[I][J][r] : 0.0;
+          }
       patB = true;
     } else {
-      typedef typename H16<GRAM>::vec hvec;
-      typedef typename H16<GRAM>::elt helt;
-      constexpr bool kSplit = GRAM != DK_GRAM_BF16;
-      double scale = 1.0, inv2 = 1.0;
-      if (GRAM == DK_GRAM_FP16X3) {
-        double mx = 0.0;
-        for (int o = lane; o < p; o += 64) {
-          const int gi = a.idx[off + o];
-          mx = fmax(mx, a.ymax[gi] * sqrt(a.rloc[off + o]));
-        }
-        for (int s = 32; s >= 1; s >>= 1) mx = fmax(mx, __shfl_xor(mx, s, kWave));
-        if (mx > 0.0) {
-          int e;
-          frexp(mx, &e);
-          scale = ldexp(1.0, 11 - e);
-          inv2 = ldexp(1.0, 2 * (e - 11));
-        }
-      }
-      f32x4_t acc[NJ][NJ];
-#pragma unroll
-      for (int I = 0; I < NJ; ++I)
-#pragma unroll
-        for (int J = 0; J < NJ; ++J) acc[I][J] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
-      for (int base = 0; base < p; base += 64) {
-        stage64<KP>(a, off, p, base, sm);
-        const int nb = min(64, p - base);
-#pragma unroll
-        for (int ch = 0; ch < 2; ++ch) {
-          if (ch * 32 >= nb) break;  // uniform across the wave
-          // all 8 x NJ loads of the chunk first (staged entries past p carry
-          // index 0 and weight 0, so they are unconditional and in bounds),
-          // then the arithmetic
-          double raw[8][NJ];
-#pragma unroll
-          for (int e = 0; e < 8; ++e) {
-            const double* y = a.yb + (int64_t)sm.ogi[ch * 32 + 8 * g + e] * k;
-#pragma unroll
-            for (int I = 0; I < NJ; ++I) {
-              const int row = I * 16 + c;
-              raw[e][I] = y[row < k ? row : k - 1];
-            }
-          }
-          __builtin_amdgcn_sched_barrier(0);
-          hvec hi[NJ], lo[NJ];
-#pragma unroll
-          for (int e = 0; e < 8; ++e) {
-            const int ol = ch * 32 + 8 * g + e;
-            const double rr = sm.orr[ol];
-            const double sd = sm.osd[ol];
-            const float rs = (float)(rr * scale);
-#pragma unroll
-            for (int I = 0; I < NJ; ++I) {
-              const bool rowok = I * 16 + c < k;
-              const double xd = rowok ? raw[e][I] * rr : 0.0;
-              bacc[I] += xd * sd;
-              const float xs = rowok ? (float)raw[e][I] * rs : 0.0f;
-              const helt h = (helt)xs;
-              hi[I][e] = h;
-              if (kSplit) lo[I][e] = (helt)(xs - (float)h);
-            }
-          }
-#pragma unroll
-          for (int I = 0; I < NJ; ++I)
-#pragma unroll
-            for (int J = 0; J < NJ; ++J) {
-              acc[I][J] = H16<GRAM>::mfma(hi[I], hi[J], acc[I][J]);
-              if (kSplit) {
-                acc[I][J] = H16<GRAM>::mfma(hi[I], lo[J], acc[I][J]);
-                acc[I][J] = H16<GRAM>::mfma(lo[I], hi[J], acc[I][J]);
-              }
-            }
-        }
-      }
       // D[4g + r][c] of tile (I, J) = C(I*16 + 4g + r, J*16 + c)
 #pragma unroll
       for (int I = 0; I < NJ; ++I)
 #pragma unroll
         for (int J = 0; J < NJ; ++J)
 #pragma unroll
-          for (int r = 0; r < 4; ++r) X[J][I * 4 + r] = (double)acc[I][J][r] * inv2;
+          for (int r = 0; r < 4; ++r) {
+            const bool in = I * 16 + 4 * g + r < k && J * 16 + c < k;
+            X[J][I * 4 + r] = in ? (double)acch[I][J][r] * inv2 : 0.0;
+          }
       patB = false;
     }
     // b: lanes hold partials of rows I*16 + c over their obs; sum over g
@@ -301,7 +321,7 @@ struct Wave {
     __syncthreads();
     if (g == 0)
 #pragma unroll
-      for (int I = 0; I < NJ; ++I) sm.b[I * 16 + c] = bacc[I];
+      for (int I = 0; I < NJ; ++I) sm.b[I * 16 + c] = I * 16 + c < k ? bacc[I] : 0.0;
   }
 
   // M = C + a0 I on the first k diagonal entries

@@ -59,18 +59,20 @@ typedef struct dida_letkf_params {
     int32_t gram;               /* DIDA_GRAM_* */
     int32_t jacobi_max_sweeps;  /* eigensolver sweep cap */
     double jacobi_tol;          /* rotate while |a_p.a_q| > tol |a_p||a_q|
-                                   (0: 1e-13 with an FP64 Gram, 1e-9 otherwise) */
+                                   (0: 1e-13 with an FP64 Gram, 1e-7 otherwise) */
     double bin_deg;             /* obs bin size in degrees (0: automatic) */
     int32_t chain_len;          /* batched form: adjacent columns per work item,
                                    each warm-starting the next one's eigensolve
                                    (0: automatic) */
     int32_t warm_start;         /* batched form: 1 to warm-start along chains */
-    int32_t jacobi_predict;     /* batched form: 1 ends the eigensolve after a sweep
-                                   whose largest |a_p.a_q|^2 / (|a_p|^2 |a_q|^2) is
-                                   <= tol (the next sweep would rotate nothing);
-                                   0 runs until a sweep rotates nothing; -1
-                                   (default): 1 with a half-precision Gram, 0 with
-                                   the FP64 Gram */
+    int32_t jacobi_predict;     /* 1 ends the eigensolve after a sweep whose largest
+                                   |a_p.a_q|^2 / (|a_p|^2 |a_q|^2) is <= tol (the
+                                   next sweep would rotate nothing if convergence
+                                   were quadratic); 0 runs until a sweep rotates
+                                   nothing; -1 (default): 1 with a half-precision
+                                   Gram, 0 with the FP64 Gram, whose 1e-10 bar
+                                   the prediction can miss on cold starts with
+                                   clustered eigenvalues */
 } dida_letkf_params;
 
 /* Observations, device pointers, one entry per scalar observation (the
