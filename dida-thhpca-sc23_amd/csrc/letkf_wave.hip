@@ -64,9 +64,9 @@ struct WSmem {
   double b[KP];
   double f[KP];
   double u[KP];
-  int32_t ogi[64];  // staged obs block: binned index,
-  double orr[64];   // sqrt(R^-1 rloc),
-  double ow[64];    // R^-1 rloc dep
+  int32_t ogi[2][64];  // staged obs blocks (two buffers): binned index,
+  double orr[2][64];   // sqrt(R^-1 rloc),
+  double ow[2][64];    // R^-1 rloc dep
 };
 
 __device__ __forceinline__ int rowA(int I, int r, int g) { return I * 16 + 4 * g + r; }
@@ -94,6 +94,33 @@ __device__ __forceinline__ double add_xor16(double v) {
 }
 // sum over the four lane groups g (lanes c, c + 16, c + 32, c + 48)
 __device__ __forceinline__ double sum_g(double v) { return add_xor16(add_xor32(v)); }
+
+// Sums of x and y over the four lane groups, transposed: lanes of groups 0
+// and 1 get sum_g(x), lanes of groups 2 and 3 get sum_g(y).  The row swap
+// of x with y leaves x's groups 0, 1 and y's groups 0, 1 in one register and
+// x's groups 2, 3 and y's groups 2, 3 in the other (in either order), so one
+// add forms the partials; a row-pair butterfly finishes them.
+__device__ __forceinline__ double sum_g_pair(double x, double y) {
+  const auto a = __builtin_amdgcn_permlane32_swap(__double2loint(x), __double2loint(y), false,
+                                                  false);
+  const auto b = __builtin_amdgcn_permlane32_swap(__double2hiint(x), __double2hiint(y), false,
+                                                  false);
+  const double s = __hiloint2double(b[0], a[0]) + __hiloint2double(b[1], a[1]);
+  return add_xor16(s);
+}
+
+// v0 = v of lane groups 0 and 1, v1 = v of lane groups 2 and 3, in every lane
+// (for v equal within each of those pairs of groups): v_permlane32_swap of v
+// with itself returns lanes 0-31 of v twice in its first result and lanes
+// 32-63 twice in its second.
+__device__ __forceinline__ void split_g(double v, double& v0, double& v1) {
+  const auto a = __builtin_amdgcn_permlane32_swap(__double2loint(v), __double2loint(v), false,
+                                                  false);
+  const auto b = __builtin_amdgcn_permlane32_swap(__double2hiint(v), __double2hiint(v), false,
+                                                  false);
+  v0 = __hiloint2double(b[0], a[0]);
+  v1 = __hiloint2double(b[1], a[1]);
+}
 
 // v of lane (l ^ 1), same 16-lane row (ds_swizzle, bit mode: and 0x1f, xor 1)
 __device__ __forceinline__ double swz_xor1(double v) {
@@ -147,13 +174,16 @@ struct ObsMeta {
     dp = a.dep[gi];
   }
   template <int KP>
-  __device__ __forceinline__ void store(WSmem<KP>& sm) const {
+  __device__ __forceinline__ void store_index(WSmem<KP>& sm, int buf) const {
+    sm.ogi[buf][threadIdx.x] = gi;
+  }
+  template <int KP>
+  __device__ __forceinline__ void store_weights(WSmem<KP>& sm, int buf) const {
     const int l = threadIdx.x;
     const double r2 = ok ? ri * rl : 0.0;
     const double rr = sqrt(r2);
-    sm.ogi[l] = gi;
-    sm.orr[l] = rr;
-    sm.ow[l] = rr * (rr * dp);
+    sm.orr[buf][l] = rr;
+    sm.ow[buf][l] = rr * (rr * dp);
   }
 };
 
@@ -167,19 +197,91 @@ struct Wave {
   double X[NJ][NQ];  // working matrix (see header)
   bool patB;         // current row pattern
 
+  // obs e (< 8) of half h of a 64-obs block in the Gram MFMA operand layout:
+  // FP64 16x16x4 k-steps of 4 obs (obs 4e + g), half 16x16x32 (obs 8g + e)
+  static __device__ __forceinline__ int half_obs(int h, int e, int g) {
+    return h * 32 + (GRAM == DK_GRAM_FP64 ? 4 * e + g : 8 * g + e);
+  }
+
+  // Yb rows I*16 + c (row k - 1 past k) of the 8 obs of half h in buffer buf
+  __device__ __forceinline__ void load_rows(const dk_batched_args& a, const WSmem<KP>& sm,
+                                            int buf, int h, double (&raw)[8][NJ]) const {
+    const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
+    const int k = a.k;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const double* y = a.yb + (int64_t)sm.ogi[buf][half_obs(h, e, g)] * k;
+#pragma unroll
+      for (int I = 0; I < NJ; ++I) {
+        const int row = I * 16 + c;
+        raw[e][I] = y[row < k ? row : k - 1];
+      }
+    }
+  }
+
+  // Accumulate half h of the staged block: C on MFMA, b in FP64
+  template <typename AccD, typename AccH>
+  __device__ __forceinline__ void gram_half(const WSmem<KP>& sm, int buf, int h,
+                                            const double (&raw)[8][NJ], double scale,
+                                            double (&bacc)[NJ], AccD& accd, AccH& acch) const {
+    const int g = threadIdx.x >> 4;
+    if constexpr (GRAM == DK_GRAM_FP64) {
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int ol = half_obs(h, e, g);
+        const double rr = sm.orr[buf][ol], w = sm.ow[buf][ol];
+        double f[NJ];
+#pragma unroll
+        for (int I = 0; I < NJ; ++I) {
+          f[I] = raw[e][I] * rr;
+          bacc[I] = fma(raw[e][I], w, bacc[I]);
+        }
+#pragma unroll
+        for (int I = 0; I < NJ; ++I)
+#pragma unroll
+          for (int J = 0; J < NJ; ++J) accd[I][J] = mfma_f64_16x16x4(f[I], f[J], accd[I][J]);
+      }
+    } else {
+      typedef typename H16<GRAM>::vec hvec;
+      typedef typename H16<GRAM>::elt helt;
+      constexpr bool kSplit = GRAM != DK_GRAM_BF16;
+      hvec hi[NJ], lo[NJ];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int ol = half_obs(h, e, g);
+        const double rr = sm.orr[buf][ol], w = sm.ow[buf][ol];
+        const float rs = (float)(rr * scale);
+#pragma unroll
+        for (int I = 0; I < NJ; ++I) {
+          bacc[I] = fma(raw[e][I], w, bacc[I]);
+          const float xs = (float)raw[e][I] * rs;
+          const helt hv = (helt)xs;
+          hi[I][e] = hv;
+          if (kSplit) lo[I][e] = (helt)(xs - (float)hv);
+        }
+      }
+#pragma unroll
+      for (int I = 0; I < NJ; ++I)
+#pragma unroll
+        for (int J = 0; J < NJ; ++J) {
+          acch[I][J] = H16<GRAM>::mfma(hi[I], hi[J], acch[I][J]);
+          if (kSplit) {
+            acch[I][J] = H16<GRAM>::mfma(hi[I], lo[J], acch[I][J]);
+            acch[I][J] = H16<GRAM>::mfma(lo[I], hi[J], acch[I][J]);
+          }
+        }
+    }
+  }
+
   // ---- Gram ---------------------------------------------------------------
-  // C = Yhat Yhat^T and b = Yhat^T (sqrt(R^-1 rloc) d) over 64-obs blocks:
-  // the metadata of block i + 1 is loaded while block i computes, and the Yb
-  // rows of both 32-obs halves of a block are issued before any arithmetic.
-  // Rows past k read row k - 1 (in bounds) and are masked once at the end
-  // (a row of Yhat only reaches the row and column of C it indexes).
+  // C = Yhat Yhat^T and b = Yhat^T (sqrt(R^-1 rloc) d) over 64-obs blocks of
+  // two 32-obs halves.  Rows past k read row k - 1 (in bounds) and are masked
+  // once at the end (a row of Yhat only reaches the row and column of C it
+  // indexes).
   __device__ void gram(const dk_batched_args& a, int64_t off, int p, WSmem<KP>& sm) {
     const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
     const int k = a.k;
     constexpr bool kF64 = GRAM == DK_GRAM_FP64;
-    typedef typename H16<kF64 ? DK_GRAM_BF16X3 : GRAM>::vec hvec;
-    typedef typename H16<kF64 ? DK_GRAM_BF16X3 : GRAM>::elt helt;
-    constexpr bool kSplit = GRAM != DK_GRAM_BF16;
     double bacc[NJ];
 #pragma unroll
     for (int I = 0; I < NJ; ++I) bacc[I] = 0.0;
@@ -213,81 +315,32 @@ struct Wave {
         for (int J = 0; J < NJ; ++J) acch[I][J] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
     }
     if (p > 0) {
+      // software pipeline over 32-obs halves: the rows of a half are issued
+      // while the previous half computes; the next block's indices are
+      // staged in the other LDS buffer during the current block
       ObsMeta m;
       m.load_index(a, off, p, 0);
       m.load_obs(a);
+      m.store_index<KP>(sm, 0);
+      m.store_weights<KP>(sm, 0);
+      __syncthreads();
+      double rawA[8][NJ], rawB[8][NJ];
+      load_rows(a, sm, 0, 0, rawA);
       for (int base = 0; base < p; base += 64) {
-        __syncthreads();
-        m.store<KP>(sm);
-        __syncthreads();
+        const int buf = (base >> 6) & 1;
         ObsMeta n;
         n.load_index(a, off, p, base + 64);
-        // obs e of half h in the MFMA operand layout: FP64 16x16x4 k-steps of
-        // 4 obs (obs 4e + g), half 16x16x32 (obs 8g + e)
-        double raw[2][8][NJ];
-#pragma unroll
-        for (int h = 0; h < 2; ++h)
-#pragma unroll
-          for (int e = 0; e < 8; ++e) {
-            const int ol = h * 32 + (kF64 ? 4 * e + g : 8 * g + e);
-            const double* y = a.yb + (int64_t)sm.ogi[ol] * k;
-#pragma unroll
-            for (int I = 0; I < NJ; ++I) {
-              const int row = I * 16 + c;
-              raw[h][e][I] = y[row < k ? row : k - 1];
-            }
-          }
+        load_rows(a, sm, buf, 1, rawB);
         __builtin_amdgcn_sched_barrier(0);
-        const int nb = min(64, p - base);
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          if (h == 1) n.load_obs(a);
-          if (h * 32 >= nb) break;  // uniform across the wave
-          if constexpr (kF64) {
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-              const int ol = h * 32 + 4 * e + g;
-              const double rr = sm.orr[ol], w = sm.ow[ol];
-              double f[NJ];
-#pragma unroll
-              for (int I = 0; I < NJ; ++I) {
-                f[I] = raw[h][e][I] * rr;
-                bacc[I] = fma(raw[h][e][I], w, bacc[I]);
-              }
-#pragma unroll
-              for (int I = 0; I < NJ; ++I)
-#pragma unroll
-                for (int J = 0; J < NJ; ++J) accd[I][J] = mfma_f64_16x16x4(f[I], f[J], accd[I][J]);
-            }
-          } else {
-            hvec hi[NJ], lo[NJ];
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-              const int ol = h * 32 + 8 * g + e;
-              const double rr = sm.orr[ol], w = sm.ow[ol];
-              const float rs = (float)(rr * scale);
-#pragma unroll
-              for (int I = 0; I < NJ; ++I) {
-                bacc[I] = fma(raw[h][e][I], w, bacc[I]);
-                const float xs = (float)raw[h][e][I] * rs;
-                const helt hv = (helt)xs;
-                hi[I][e] = hv;
-                if (kSplit) lo[I][e] = (helt)(xs - (float)hv);
-              }
-            }
-#pragma unroll
-            for (int I = 0; I < NJ; ++I)
-#pragma unroll
-              for (int J = 0; J < NJ; ++J) {
-                acch[I][J] = H16<GRAM>::mfma(hi[I], hi[J], acch[I][J]);
-                if (kSplit) {
-                  acch[I][J] = H16<GRAM>::mfma(hi[I], lo[J], acch[I][J]);
-                  acch[I][J] = H16<GRAM>::mfma(lo[I], hi[J], acch[I][J]);
-                }
-              }
-          }
-        }
-        m = n;
+        gram_half(sm, buf, 0, rawA, scale, bacc, accd, acch);
+        n.load_obs(a);
+        n.store_index<KP>(sm, buf ^ 1);
+        __syncthreads();
+        load_rows(a, sm, buf ^ 1, 0, rawA);
+        __builtin_amdgcn_sched_barrier(0);
+        if (base + 32 < p) gram_half(sm, buf, 1, rawB, scale, bacc, accd, acch);
+        n.store_weights<KP>(sm, buf ^ 1);
+        __syncthreads();
       }
     }
     if constexpr (kF64) {
@@ -438,15 +491,69 @@ struct Wave {
     isc[Q] *= ic;
   }
 
-  // One round: pairs (T0, B0) and, with two pairs a lane, (T1, B1).
+  // Angle of one pair from its state (al, be squared norms, sp, sq scales)
+  // and gs = x_p . x_q: t = tan(theta), gm = gamma, cs = cos(theta), ic =
+  // 1/cos(theta); t = 0, cs = ic = 1 below the threshold.
+  static __device__ __forceinline__ void angle(double gs, double al, double be, double sp,
+                                               double sq, double tol2, double tolp,
+                                               bool& rotated, bool& big, double& t, double& gm,
+                                               double& cs, double& ic) {
+    gm = gs * (sp * sq);
+    const double gg = gm * gm, ab = al * be;
+    const bool need = gg > tol2 * ab;
+    big |= gg > tolp * ab;
+    rotated |= need;
+    const double d = be - al, e = 2.0 * gm;
+    const double r2 = fma(d, d, e * e);
+    const double den = fabs(d) + r2 * rsq_nr(r2);
+    t = fabs(e) * rcp_nr(den);
+    if ((d < 0.0) != (e < 0.0) && d != 0.0) t = -t;
+    t = need ? t : 0.0;
+    const double u = fma(t, t, 1.0);
+    cs = rsq_nr(u);
+    ic = u * cs;
+    cs = need ? cs : 1.0;
+    ic = need ? ic : 1.0;
+  }
+
+  // Apply the rotation (t, gm, cs, ic) to columns P, Q and their state.
+  template <int P, int Q>
+  __device__ __forceinline__ void apply(double t, double gm, double cs, double ic) {
+    const double ta = t * (sc[Q] * isc[P]);
+    const double tb = t * (sc[P] * isc[Q]);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const double x = X[P][q], y = X[Q][q];
+      X[P][q] = fma(-ta, y, x);
+      X[Q][q] = fma(tb, x, y);
+    }
+    nrm[P] = fma(-t, gm, nrm[P]);
+    nrm[Q] = fma(t, gm, nrm[Q]);
+    sc[P] *= cs;
+    sc[Q] *= cs;
+    isc[P] *= ic;
+    isc[Q] *= ic;
+  }
+
+  // One round: pairs (T0, B0) and, with two pairs a lane, (T1, B1).  With two
+  // pairs, lane groups 0-1 compute the angle of the first and groups 2-3 that
+  // of the second, and the angles are then broadcast.
   template <int T0, int B0, int T1, int B1>
   __device__ __forceinline__ void round(double tol2, double tolp, bool& rotated, bool& big) {
     if constexpr (NP == 2) {
-      double g0 = dot<T0, B0>(), g1 = dot<T1, B1>();
-      g0 = sum_g(g0);
-      g1 = sum_g(g1);
-      rot<T0, B0>(g0, tol2, tolp, rotated, big);
-      rot<T1, B1>(g1, tol2, tolp, rotated, big);
+      const double gs = sum_g_pair(dot<T0, B0>(), dot<T1, B1>());
+      const bool second = threadIdx.x >= 32;
+      double t, gm, cs, ic;
+      angle(gs, second ? nrm[T1] : nrm[T0], second ? nrm[B1] : nrm[B0],
+            second ? sc[T1] : sc[T0], second ? sc[B1] : sc[B0], tol2, tolp, rotated, big, t,
+            gm, cs, ic);
+      double t0, t1, gm0, gm1, cs0, cs1, ic0, ic1;
+      split_g(t, t0, t1);
+      split_g(gm, gm0, gm1);
+      split_g(cs, cs0, cs1);
+      split_g(ic, ic0, ic1);
+      apply<T0, B0>(t0, gm0, cs0, ic0);
+      apply<T1, B1>(t1, gm1, cs1, ic1);
     } else {
       rot<T0, B0>(sum_g(dot<T0, B0>()), tol2, tolp, rotated, big);
     }
