@@ -105,7 +105,15 @@ def transform_flops(k, nobsl_sum, nanal):
 
 
 def update_bytes(k, n, ncol, s=8):
+    """Update: read Xb' and xbar_b, write Xa' and xbar_a, per column."""
     return ncol * (2.0 * k * n * s + 2.0 * n * 8)
+
+
+def selection_bytes(k, nobs, nobsl_sum):
+    """Binning + selection: each obs record (Yb row of k f64, lat, lon, d,
+    R^-1) read once, and per selection entry an int32 index and an f64
+    weight written."""
+    return nobs * (8.0 * k + 32.0) + 12.0 * nobsl_sum
 
 
 def kernel_roofline(cfg, st, head, kernel_ms, config, scale):
@@ -297,7 +305,9 @@ def main():
                            "kernel_phase_fraction": rr["phase_frac"],
                            "stage_ms": rr["stages"],
                            "update_GBps": update_bytes(cfg.k, n, ncol) / (rr["stages"]["update"] / 1e3) / 1e9
-                           if rr["stages"]["update"] > 0 else None}
+                           if rr["stages"]["update"] > 0 else None,
+                           "selection_GBps": selection_bytes(cfg.k, cfg.nobs, rr["stats"]["nobsl_sum"])
+                           / ((rr["stages"]["bin"] + rr["stages"]["select"]) / 1e3) / 1e9}
     cpu = None
     if not args.no_cpu:
         log("timing the CPU oracle on a column sample")

@@ -108,13 +108,13 @@ struct dida_letkf_ctx {
     const float *grid_lat, *grid_lon;
     dk_bin_geom geom;
     int end_bit;
-    double theta_deg, cos_far;
+    double theta_deg, cos_far, cos_near;
     /* binning */
     uint32_t *key, *key_sorted;
     int32_t *iota, *perm, *bin_count, *bin_start;
     void *sort_tmp;
     size_t sort_tmp_bytes;
-    double *cx, *sx, *yr, *yb_b, *dep_b;
+    double *cx, *sx, *yr, *ux, *uy, *uz, *yb_b, *dep_b;
     float *rinv_b;
     /* selection */
     int32_t *nobsl;
@@ -163,7 +163,8 @@ int dida_letkf_destroy(dida_letkf_ctx *c)
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
     void *ptrs[] = {c->key, c->key_sorted, c->iota, c->perm, c->bin_count, c->bin_start,
-                    c->sort_tmp, c->cx, c->sx, c->yr, c->yb_b, c->dep_b, c->rinv_b,
+                    c->sort_tmp, c->cx, c->sx, c->yr, c->ux, c->uy, c->uz, c->yb_b,
+                    c->dep_b, c->rinv_b,
                     c->nobsl, c->offset, c->scan_tmp, c->csr_idx, c->csr_w, c->flags,
                     c->trans, c->info, c->chain_key, c->chain_key_sorted, c->chain_id,
                     c->chain_order, c->queue_head, c->chain_tmp, c->ymax, c->phase_clk};
@@ -202,6 +203,8 @@ int dida_letkf_create(dida_letkf_ctx **out, const dida_letkf_params *prm, int32_
     double theta_rad = 2.0 * prm->local_dist / kEarthA;
     c->theta_deg = theta_rad * (180.0 / pi) * (1.0 + 1e-6) + 1e-6;
     c->cos_far = (theta_rad * (1.0 + 1e-7) < pi) ? cos(theta_rad * (1.0 + 1e-7)) : -2.0;
+    /* candidates closer than theta (1 - 1e-7) have gc > 0 (r < 2) for sure */
+    c->cos_near = (theta_rad * (1.0 - 1e-7) < pi) ? cos(theta_rad * (1.0 - 1e-7)) : -1.0;
     double bd = prm->bin_deg;
     if (!(bd > 0.0)) {
         bd = c->theta_deg / 4.0;
@@ -234,6 +237,9 @@ int dida_letkf_create(dida_letkf_ctx **out, const dida_letkf_params *prm, int32_
     DMALLOC(c->cx, no * sizeof(double));
     DMALLOC(c->sx, no * sizeof(double));
     DMALLOC(c->yr, no * sizeof(double));
+    DMALLOC(c->ux, no * sizeof(double));
+    DMALLOC(c->uy, no * sizeof(double));
+    DMALLOC(c->uz, no * sizeof(double));
     DMALLOC(c->yb_b, no * (size_t)k * sizeof(double));
     DMALLOC(c->dep_b, no * sizeof(double));
     DMALLOC(c->rinv_b, no * sizeof(float));
@@ -316,7 +322,7 @@ static int sweep_select_count(dida_letkf_ctx *c, const dida_letkf_obs *o, hipStr
     }
     HIP_TRY(dk_bin_scan(c->geom.nbins, c->bin_count, c->bin_start, s));
     HIP_TRY(dk_obs_gather(o->nobs, k, c->perm, o->lat, o->lon, o->yb, o->dep, o->rinv, c->cx,
-                          c->sx, c->yr, c->yb_b, c->dep_b, c->rinv_b, s));
+                          c->sx, c->yr, c->ux, c->uy, c->uz, c->yb_b, c->dep_b, c->rinv_b, s));
     tmark(c, DIDA_STAGE_BIN, s);
     memset(sa, 0, sizeof *sa);
     sa->nlat = c->prm.nlat;
@@ -327,6 +333,9 @@ static int sweep_select_count(dida_letkf_ctx *c, const dida_letkf_obs *o, hipStr
     sa->obs.cx = c->cx;
     sa->obs.sx = c->sx;
     sa->obs.yr = c->yr;
+    sa->obs.ux = c->ux;
+    sa->obs.uy = c->uy;
+    sa->obs.uz = c->uz;
     sa->obs.yb = c->yb_b;
     sa->obs.dep = c->dep_b;
     sa->obs.rinv = c->rinv_b;
@@ -334,6 +343,7 @@ static int sweep_select_count(dida_letkf_ctx *c, const dida_letkf_obs *o, hipStr
     sa->local_dist = c->prm.local_dist;
     sa->theta_deg = c->theta_deg;
     sa->cos_far = c->cos_far;
+    sa->cos_near = c->cos_near;
     sa->gc_filter = c->prm.gc_filter;
     sa->max_obs = c->prm.max_obs;
     sa->nobsl = c->nobsl;

@@ -27,9 +27,11 @@ typedef struct dk_bin_geom {
 
 /* Observations in binned order, with the per-obs factors of the distance
  * formula precomputed (cos/sin of the latitude in the reference's radians
- * and the longitude in the reference's radians). */
+ * and the longitude in the reference's radians) and the unit vector
+ * (cx cos yr, cx sin yr, sx) used to decide far-from-threshold candidates. */
 typedef struct dk_obs_binned {
     const double *cx, *sx, *yr;   /* [nobs] */
+    const double *ux, *uy, *uz;   /* [nobs] */
     const double *yb;             /* [nobs * k] ensemble fastest */
     const double *dep;            /* [nobs] */
     const float *rinv;            /* [nobs] */
@@ -44,6 +46,7 @@ typedef struct dk_select_args {
     double local_dist;   /* L, km */
     double theta_deg;    /* angular radius of the support plus margin, deg */
     double cos_far;      /* central-angle cosine below which gc = 0 surely */
+    double cos_near;     /* central-angle cosine above which gc > 0 surely */
     int32_t gc_filter;
     int32_t max_obs;
     int32_t *nobsl;      /* [ncol] count pass output / fill pass input */
@@ -68,8 +71,8 @@ int dk_bin_scan(int32_t nbins, const int32_t *count, int32_t *bin_start, void *s
 /* Gather obs into binned order and precompute the distance factors. */
 int dk_obs_gather(int32_t nobs, int32_t k, const int32_t *perm, const double *lat,
                   const double *lon, const double *yb, const double *dep, const float *rinv,
-                  double *cx, double *sx, double *yr, double *yb_b, double *dep_b,
-                  float *rinv_b, void *stream);
+                  double *cx, double *sx, double *yr, double *ux, double *uy, double *uz,
+                  double *yb_b, double *dep_b, float *rinv_b, void *stream);
 /* Selection passes, one wave per column. */
 int dk_select_count(const dk_select_args *a, void *stream);
 int dk_select_fill(const dk_select_args *a, void *stream);

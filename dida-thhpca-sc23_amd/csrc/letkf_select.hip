@@ -81,15 +81,20 @@ __global__ void k_obs_gather(int32_t nobs, const int32_t* __restrict__ perm,
                              const double* __restrict__ lat, const double* __restrict__ lon,
                              const double* __restrict__ dep, const float* __restrict__ rinv,
                              double* __restrict__ cx, double* __restrict__ sx,
-                             double* __restrict__ yr, double* __restrict__ dep_b,
-                             float* __restrict__ rinv_b) {
+                             double* __restrict__ yr, double* __restrict__ ux,
+                             double* __restrict__ uy, double* __restrict__ uz,
+                             double* __restrict__ dep_b, float* __restrict__ rinv_b) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nobs) return;
   int s = perm[i];
   double x1r = deg2rad_ref(lat[s]);
-  cx[i] = cos(x1r);
-  sx[i] = sin(x1r);
-  yr[i] = deg2rad_ref(lon[s]);
+  const double c1 = cos(x1r), s1 = sin(x1r), y1 = deg2rad_ref(lon[s]);
+  cx[i] = c1;
+  sx[i] = s1;
+  yr[i] = y1;
+  ux[i] = c1 * cos(y1);
+  uy[i] = c1 * sin(y1);
+  uz[i] = s1;
   dep_b[i] = dep[s];
   rinv_b[i] = rinv[s];
 }
@@ -172,6 +177,11 @@ __global__ __launch_bounds__(256) void k_select(dk_select_args a) {
   const double clon = (double)a.grid_lon[ilon];
   const double x2r = deg2rad_ref(clat);
   const double cx2 = cos(x2r), sx2 = sin(x2r), y2r = deg2rad_ref(clon);
+  const double vx = cx2 * cos(y2r), vy = cx2 * sin(y2r), vz = sx2;
+  // the unit-vector dot product differs from the reference's cosine by a few
+  // ulp; with these margins it decides every candidate outside a thin shell
+  // around the support radius exactly as the reference formula would
+  const double far_lo = a.cos_far - 1e-12, near_hi = a.cos_near + 1e-12;
   const int limit = kFill ? a.nobsl[col] : a.max_obs;
   int64_t out = kFill ? a.offset[col] : 0;
   if (kFill && out + limit > a.capacity && lane == 0) *a.overflow = 1;
@@ -183,13 +193,20 @@ __global__ __launch_bounds__(256) void k_select(dk_select_args a) {
         bool keep = false;
         double gc = 0.0;
         if (l < l1) {
-          double v = central_cos(a.obs.cx[l], a.obs.sx[l], a.obs.yr[l], cx2, sx2, y2r);
-          if (v >= a.cos_far) {
+          const double vd = a.obs.ux[l] * vx + a.obs.uy[l] * vy + a.obs.uz[l] * vz;
+          if (vd >= far_lo) {
+            if (!kFill && vd > near_hi) {
+              keep = true;  // the count pass needs no weight
+            } else {
+              double v = central_cos(a.obs.cx[l], a.obs.sx[l], a.obs.yr[l], cx2, sx2, y2r);
+              if (v >= a.cos_far) {
 #pragma clang fp contract(off)
-            double d = dist_from_cos(v);
-            double r = d * 1.0 / a.local_dist;
-            gc = gc_weight(r, a.gc_filter);
-            keep = gc > 0.0;
+                double d = dist_from_cos(v);
+                double r = d * 1.0 / a.local_dist;
+                gc = gc_weight(r, a.gc_filter);
+                keep = gc > 0.0;
+              }
+            }
           }
         }
         const uint64_t m = __ballot(keep);
@@ -243,11 +260,12 @@ extern "C" int dk_bin_scan(int32_t nbins, const int32_t* count, int32_t* bin_sta
 extern "C" int dk_obs_gather(int32_t nobs, int32_t k, const int32_t* perm, const double* lat,
                              const double* lon, const double* yb, const double* dep,
                              const float* rinv, double* cx, double* sx, double* yr,
-                             double* yb_b, double* dep_b, float* rinv_b, void* stream) {
+                             double* ux, double* uy, double* uz, double* yb_b, double* dep_b,
+                             float* rinv_b, void* stream) {
   if (nobs <= 0) return 0;
   hipStream_t s = (hipStream_t)stream;
   k_obs_gather<<<(nobs + 255) / 256, 256, 0, s>>>(nobs, perm, lat, lon, dep, rinv, cx, sx, yr,
-                                                  dep_b, rinv_b);
+                                                  ux, uy, uz, dep_b, rinv_b);
   int64_t total = (int64_t)nobs * k;
   k_obs_gather_yb<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(nobs, k, perm, yb, yb_b);
   return (int)hipGetLastError();
