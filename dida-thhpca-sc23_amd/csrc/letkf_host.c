@@ -64,7 +64,7 @@ void dida_letkf_params_default(dida_letkf_params *p)
 static double jacobi_tol_of(const dida_letkf_params *p)
 {
     if (p->jacobi_tol > 0.0) return p->jacobi_tol;
-    return p->gram == DIDA_GRAM_FP64 ? 1e-13 : 1e-7;
+    return p->gram == DIDA_GRAM_FP64 ? 1e-13 : 1e-9;
 }
 
 static int predict_of(const dida_letkf_params *p)
