@@ -545,6 +545,117 @@ int dida_letkf_graph_capture(dida_letkf_ctx *c, const dida_letkf_obs *o, const d
     return DIDA_OK;
 }
 
+struct dida_letkf_group {
+    int32_t n;
+    hipStream_t *branch;
+    hipEvent_t *ev;      /* [n + 1]: fork, then one join event per branch */
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+};
+
+int dida_letkf_group_destroy(dida_letkf_group *g)
+{
+    if (!g) return DIDA_OK;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    for (int i = 0; g->ev && i <= g->n; ++i)
+        if (g->ev[i]) (void)hipEventDestroy(g->ev[i]);
+    for (int i = 0; g->branch && i < g->n; ++i)
+        if (g->branch[i]) (void)hipStreamDestroy(g->branch[i]);
+    free(g->ev);
+    free(g->branch);
+    free(g);
+    return DIDA_OK;
+}
+
+int dida_letkf_group_capture(dida_letkf_group **out, int32_t n, dida_letkf_ctx *const *ctx,
+                             const dida_letkf_obs *obs, const double *const *xb,
+                             const double *const *axb, double *const *xa, double *const *axa,
+                             void *stream)
+{
+    if (!out || n < 1 || !ctx || !obs || !xb || !axb || !xa || !axa)
+        return set_err(DIDA_ERR_ARG, "group capture: null argument or n < 1");
+    *out = NULL;
+    for (int i = 0; i < n; ++i) {
+        int rc = check_sweep_args(ctx[i], &obs[i], xb[i], axb[i], xa[i], axa[i]);
+        if (rc) return rc;
+        if (ctx[i]->capacity <= 0)
+            return set_err(DIDA_ERR_ARG, "group capture: context %d has not run a sweep", i);
+        for (int j = 0; j < i; ++j)
+            if (ctx[j] == ctx[i]) return set_err(DIDA_ERR_ARG, "group capture: context repeated");
+    }
+    dida_letkf_group *g = (dida_letkf_group *)calloc(1, sizeof *g);
+    if (!g) return set_err(DIDA_ERR_NOMEM, "group");
+    g->n = n;
+    g->branch = (hipStream_t *)calloc((size_t)n, sizeof(hipStream_t));
+    g->ev = (hipEvent_t *)calloc((size_t)n + 1, sizeof(hipEvent_t));
+    if (!g->branch || !g->ev) {
+        dida_letkf_group_destroy(g);
+        return set_err(DIDA_ERR_NOMEM, "group");
+    }
+    int rc = DIDA_OK;
+    for (int i = 0; i < n && !rc; ++i)
+        if (hipStreamCreateWithFlags(&g->branch[i], hipStreamNonBlocking) != hipSuccess)
+            rc = set_err(DIDA_ERR_HIP, "group: stream");
+    for (int i = 0; i <= n && !rc; ++i)
+        if (hipEventCreateWithFlags(&g->ev[i], hipEventDisableTiming) != hipSuccess)
+            rc = set_err(DIDA_ERR_HIP, "group: event");
+    if (rc) {
+        dida_letkf_group_destroy(g);
+        return rc;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        dida_letkf_group_destroy(g);
+        return set_err(DIDA_ERR_HIP, "group: begin capture");
+    }
+    /* fork: every branch waits for the origin stream's point of capture */
+    if (hipEventRecord(g->ev[0], s) != hipSuccess) rc = set_err(DIDA_ERR_HIP, "group: fork");
+    for (int i = 0; i < n && !rc; ++i) {
+        dida_letkf_ctx *c = ctx[i];
+        hipStream_t b = g->branch[i];
+        if (hipStreamWaitEvent(b, g->ev[0], 0) != hipSuccess) {
+            rc = set_err(DIDA_ERR_HIP, "group: fork wait");
+            break;
+        }
+        const int timing = c->timing;
+        c->timing = 0;
+        dk_select_args sa;
+        rc = sweep_select_count(c, &obs[i], b, &sa);
+        if (!rc) rc = sweep_rest(c, &sa, xb[i], axb[i], xa[i], axa[i], b);
+        c->timing = timing;
+        c->last_nobs = obs[i].nobs;
+        c->last_stream = s;
+        if (!rc && hipEventRecord(g->ev[i + 1], b) != hipSuccess)
+            rc = set_err(DIDA_ERR_HIP, "group: join record");
+        if (!rc && hipStreamWaitEvent(s, g->ev[i + 1], 0) != hipSuccess)
+            rc = set_err(DIDA_ERR_HIP, "group: join wait");
+    }
+    hipGraph_t graph = NULL;
+    int e = (int)hipStreamEndCapture(s, &graph);
+    if (!rc && e) rc = set_err(DIDA_ERR_HIP, "group: end capture: %s",
+                               hipGetErrorString((hipError_t)e));
+    if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        dida_letkf_group_destroy(g);
+        return rc;
+    }
+    g->graph = graph;
+    if (hipGraphInstantiate(&g->exec, graph, NULL, NULL, 0) != hipSuccess) {
+        dida_letkf_group_destroy(g);
+        return set_err(DIDA_ERR_HIP, "group: instantiate");
+    }
+    *out = g;
+    return DIDA_OK;
+}
+
+int dida_letkf_group_launch(dida_letkf_group *g, void *stream)
+{
+    if (!g || !g->exec) return set_err(DIDA_ERR_ARG, "no group graph");
+    HIP_TRY(hipGraphLaunch(g->exec, (hipStream_t)stream));
+    return DIDA_OK;
+}
+
 int dida_letkf_graph_launch(dida_letkf_ctx *c, void *stream)
 {
     if (!c || !c->gexec) return set_err(DIDA_ERR_ARG, "no captured graph");

@@ -29,7 +29,8 @@ EXPORTED = ["dida_letkf_params_default", "letkf_core_", "letkf_core_batch_",
             "dida_letkf_set_grid", "dida_letkf_sweep", "dida_letkf_graph_capture",
             "dida_letkf_graph_launch", "dida_letkf_stats_get", "dida_letkf_debug_arrays",
             "dida_letkf_post", "dida_letkf_last_error", "dida_letkf_set_timing",
-            "dida_letkf_stage_ms", "dida_letkf_phase_cycles"]
+            "dida_letkf_stage_ms", "dida_letkf_phase_cycles", "dida_letkf_group_capture",
+            "dida_letkf_group_launch", "dida_letkf_group_destroy"]
 PHASES = ["gram", "warm", "jacobi", "transform", "update"]
 STAGES = ["bin", "select", "transform", "update"]
 
@@ -102,6 +103,11 @@ def lib():
         L.dida_letkf_set_timing.argtypes = [vp, C.c_int32]
         L.dida_letkf_stage_ms.argtypes = [vp, C.POINTER(C.c_double)]
         L.dida_letkf_phase_cycles.argtypes = [vp, C.POINTER(C.c_double)]
+        L.dida_letkf_group_capture.argtypes = [C.POINTER(vp), C.c_int32, C.POINTER(vp),
+                                               C.POINTER(ObsC), C.POINTER(vp), C.POINTER(vp),
+                                               C.POINTER(vp), C.POINTER(vp), vp]
+        L.dida_letkf_group_launch.argtypes = [vp, vp]
+        L.dida_letkf_group_destroy.argtypes = [vp]
         L.letkf_core_.argtypes = [vp] * 10
         L.letkf_core_.restype = None
         L.letkf_core_batch_.argtypes = [vp] * 13
@@ -184,6 +190,37 @@ class DeviceObs:
     def c(self) -> ObsC:
         return ObsC(int(self.lat.numel()), 0, self.lat.data_ptr(), self.lon.data_ptr(),
                     self.yb.data_ptr(), self.dep.data_ptr(), self.rinv.data_ptr())
+
+
+class Group:
+    """One HIP graph with a sweep of each of several engines (domains), run
+    on parallel graph branches (dida_letkf_group_*)."""
+
+    def __init__(self, engines, obs, xb, axb, xa, axa, stream):
+        n = len(engines)
+        vp = C.c_void_p
+        arr = lambda xs: (vp * n)(*[x.data_ptr() for x in xs])
+        self._keep = [engines, obs, xb, axb, xa, axa]
+        obs_c = (ObsC * n)(*[o.c() for o in obs])
+        h = vp()
+        check(lib().dida_letkf_group_capture(C.byref(h), n, (vp * n)(*[e.h.value for e in engines]),
+                                             obs_c, arr(xb), arr(axb), arr(xa), arr(axa),
+                                             _stream_ptr(stream)), "group_capture")
+        self.h = h
+
+    def launch(self, stream=None) -> int:
+        return check(lib().dida_letkf_group_launch(self.h, _stream_ptr(stream)), "group_launch")
+
+    def close(self):
+        if self.h:
+            lib().dida_letkf_group_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Engine:
@@ -363,6 +400,34 @@ def synth_generate(cfg: SynthCfg, device="cuda", fp32_state=False, stream=None) 
 
 
 # Workloads of the benchmark configurations.
+def synth_region(data: dict, cfg: SynthCfg, j0: int, ny: int, i0: int, nx: int) -> dict:
+    """A regional domain cut from a global synthetic workload: the ny x nx
+    grid columns from latitude index j0 and longitude index i0 (no wrap), and
+    the obs inside the box the columns span.  Columns are numbered
+    jl + ny * il as in the global layout."""
+    import torch
+    dev = data["xb"].device
+    if not (0 <= j0 and j0 + ny <= cfg.nlat and 0 <= i0 and i0 + nx <= cfg.nlon):
+        raise ValueError("region outside the grid")
+    il = torch.arange(i0, i0 + nx, device=dev)
+    jl = torch.arange(j0, j0 + ny, device=dev)
+    cols = (jl[None, :] + cfg.nlat * il[:, None]).reshape(-1)
+    dlat, dlon = 180.0 / cfg.nlat, 360.0 / cfg.nlon
+    lat_lo = -90.0 + j0 * dlat
+    lat_hi = lat_lo + ny * dlat
+    lon_lo = i0 * dlon
+    lon_hi = lon_lo + nx * dlon
+    inside = ((data["lat"] >= lat_lo) & (data["lat"] < lat_hi) & (data["lon"] >= lon_lo) &
+              (data["lon"] < lon_hi))
+    sel = torch.nonzero(inside).reshape(-1)
+    return {"grid_lat": data["grid_lat"][j0:j0 + ny].contiguous(),
+            "grid_lon": data["grid_lon"][i0:i0 + nx].contiguous(),
+            "xb": data["xb"][cols].contiguous(), "axb": data["axb"][cols].contiguous(),
+            "lat": data["lat"][sel].contiguous(), "lon": data["lon"][sel].contiguous(),
+            "yb": data["yb"][sel].contiguous(), "dep": data["dep"][sel].contiguous(),
+            "rinv": data["rinv"][sel].contiguous()}
+
+
 def config_synth(i: int, scale: float = 1.0) -> tuple[SynthCfg, dict]:
     """Synthetic config i (0..4) and its analysis parameters.  `scale` < 1
     shrinks the grid and obs count proportionally (test sizes)."""

@@ -167,6 +167,23 @@ int dida_letkf_graph_capture(dida_letkf_ctx *ctx, const dida_letkf_obs *obs, con
                              const double *axb, double *xa, double *axa, void *stream);
 int dida_letkf_graph_launch(dida_letkf_ctx *ctx, void *stream);
 
+/* Several analysis domains in one HIP graph (the batching form's regional
+ * case: independent domains of different sizes and ensemble sizes, each with
+ * its own context).  The graph forks from `stream` into one branch per
+ * context, runs one sweep of each with the given inputs and outputs, and
+ * joins back, so the domains run concurrently; dida_letkf_group_launch
+ * replays it.  Every context must have run one eager sweep first (it sizes
+ * the selection lists).  Contexts must not be used elsewhere while their
+ * group graph runs.  No counterpart in the reference, which runs domains as
+ * separate jobs. */
+typedef struct dida_letkf_group dida_letkf_group;
+int dida_letkf_group_capture(dida_letkf_group **grp, int32_t n, dida_letkf_ctx *const *ctx,
+                             const dida_letkf_obs *obs, const double *const *xb,
+                             const double *const *axb, double *const *xa,
+                             double *const *axa, void *stream);
+int dida_letkf_group_launch(dida_letkf_group *grp, void *stream);
+int dida_letkf_group_destroy(dida_letkf_group *grp);
+
 /* Timing of eager sweeps.  `on` bit 0 (DIDA_TIMING_STAGES): HIP events on
  * the sweep's stream around each stage (the reference's GPTL regions
  * "Generate Yb matrix", "Lektf computation core", "Convert xb to xa",
