@@ -108,6 +108,8 @@ struct dida_letkf_ctx {
     const float *grid_lat, *grid_lon;
     dk_bin_geom geom;
     int end_bit;
+    int32_t nbins_max;   /* bin arrays are sized for the finest bins tried */
+    double bin_deg;
     double theta_deg, cos_far, cos_near;
     /* binning */
     uint32_t *key, *key_sorted;
@@ -131,6 +133,7 @@ struct dida_letkf_ctx {
     int32_t *info;
     /* batched form: chain queue */
     int32_t chain_len, chains_per_lon, nchain, nblocks;
+    int32_t nchain_max;  /* chain arrays are sized for chains of one column */
     uint32_t *chain_key, *chain_key_sorted;
     int32_t *chain_id, *chain_order, *queue_head;
     void *chain_tmp;
@@ -182,6 +185,43 @@ int dida_letkf_destroy(dida_letkf_ctx *c)
         }                                                                                \
     } while (0)
 
+/* Obs bins of bd degrees (clamped to [0.25, 15]): nband bands of 180 / nband
+ * and nlonbin longitude bins of 360 / nlonbin tile the sphere exactly. */
+static dk_bin_geom bin_geom(double bd, double obs_polar)
+{
+    if (bd < 0.25) bd = 0.25;
+    if (bd > 15.0) bd = 15.0;
+    dk_bin_geom g;
+    memset(&g, 0, sizeof g);
+    g.obs_polar = obs_polar;
+    g.nband = (int32_t)ceil(180.0 / bd);
+    g.nlonbin = (int32_t)ceil(360.0 / bd);
+    g.dlat = 180.0 / g.nband;
+    g.dlon = 360.0 / g.nlonbin;
+    g.nbins = g.nband * g.nlonbin;
+    return g;
+}
+
+/* Bin size and chain length of the sweeps that follow (within the sizes the
+ * context was created for). */
+static int set_layout(dida_letkf_ctx *c, double bin_deg, int32_t chain_len)
+{
+    dk_bin_geom g = bin_geom(bin_deg, c->prm.obs_polar);
+    if (g.nbins > c->nbins_max) return set_err(DIDA_ERR_ARG, "bins finer than allocated");
+    c->geom = g;
+    c->bin_deg = bin_deg;
+    c->end_bit = 1;
+    while ((1u << c->end_bit) <= (unsigned)g.nbins) ++c->end_bit;
+    if (c->prm.form == DIDA_FORM_BATCHED) {
+        if (chain_len < 1) chain_len = 1;
+        if (chain_len > c->prm.nlat) chain_len = c->prm.nlat;
+        c->chain_len = chain_len;
+        c->chains_per_lon = (c->prm.nlat + chain_len - 1) / chain_len;
+        c->nchain = c->chains_per_lon * c->prm.nlon;
+    }
+    return DIDA_OK;
+}
+
 int dida_letkf_create(dida_letkf_ctx **out, const dida_letkf_params *prm, int32_t nobs_max,
                       int64_t sel_capacity)
 {
@@ -205,31 +245,33 @@ int dida_letkf_create(dida_letkf_ctx **out, const dida_letkf_params *prm, int32_
     c->cos_far = (theta_rad * (1.0 + 1e-7) < pi) ? cos(theta_rad * (1.0 + 1e-7)) : -2.0;
     /* candidates closer than theta (1 - 1e-7) have gc > 0 (r < 2) for sure */
     c->cos_near = (theta_rad * (1.0 - 1e-7) < pi) ? cos(theta_rad * (1.0 - 1e-7)) : -1.0;
-    double bd = prm->bin_deg;
-    if (!(bd > 0.0)) {
-        bd = c->theta_deg / 4.0;
-        if (bd < 0.25) bd = 0.25;
-        if (bd > 15.0) bd = 15.0;
+    /* bins: the given size, or theta / 4; arrays sized for the finest size
+     * the autotuner tries (theta / 8) */
+    const double bd = prm->bin_deg > 0.0 ? prm->bin_deg : c->theta_deg / 4.0;
+    {
+        const dk_bin_geom g0 = bin_geom(bd, prm->obs_polar);
+        const dk_bin_geom g1 = bin_geom(c->theta_deg / 8.0, prm->obs_polar);
+        c->nbins_max = g0.nbins > g1.nbins ? g0.nbins : g1.nbins;
     }
-    c->geom.obs_polar = prm->obs_polar;
-    c->geom.nband = (int32_t)ceil(180.0 / bd);
-    c->geom.nlonbin = (int32_t)ceil(360.0 / bd);
-    c->geom.dlat = 180.0 / c->geom.nband;
-    c->geom.dlon = 360.0 / c->geom.nlonbin;
-    c->geom.nbins = c->geom.nband * c->geom.nlonbin;
-    c->end_bit = 1;
-    while ((1u << c->end_bit) <= (unsigned)c->geom.nbins) ++c->end_bit;
+    c->nchain_max = (int32_t)c->ncol;
+    rc = set_layout(c, bd, prm->chain_len > 0 ? prm->chain_len : 8);
+    if (rc) {
+        free(c);
+        return rc;
+    }
+    int end_bit_max = 1;
+    while ((1u << end_bit_max) <= (unsigned)c->nbins_max) ++end_bit_max;
 
     const size_t no = (size_t)(nobs_max > 0 ? nobs_max : 1);
     DMALLOC(c->key, no * sizeof(uint32_t));
     DMALLOC(c->key_sorted, no * sizeof(uint32_t));
     DMALLOC(c->iota, no * sizeof(int32_t));
     DMALLOC(c->perm, no * sizeof(int32_t));
-    DMALLOC(c->bin_count, (size_t)(c->geom.nbins + 1) * sizeof(int32_t));
-    DMALLOC(c->bin_start, (size_t)(c->geom.nbins + 2) * sizeof(int32_t));
+    DMALLOC(c->bin_count, (size_t)(c->nbins_max + 1) * sizeof(int32_t));
+    DMALLOC(c->bin_start, (size_t)(c->nbins_max + 2) * sizeof(int32_t));
     c->sort_tmp_bytes = 0;
     if (dk_sort_pairs_u32(NULL, &c->sort_tmp_bytes, c->key, c->key_sorted, c->iota, c->perm,
-                          (int32_t)no, c->end_bit, NULL)) {
+                          (int32_t)no, end_bit_max, NULL)) {
         dida_letkf_destroy(c);
         return set_err(DIDA_ERR_HIP, "sort size query failed");
     }
@@ -259,18 +301,14 @@ int dida_letkf_create(dida_letkf_ctx **out, const dida_letkf_params *prm, int32_
         c->chunk = (int32_t)(c->ncol < 32768 ? c->ncol : 32768);
         DMALLOC(c->trans, (size_t)c->chunk * k * k * sizeof(double));
     } else {
-        c->chain_len = prm->chain_len > 0 ? prm->chain_len : 8;
-        if (c->chain_len > prm->nlat) c->chain_len = prm->nlat;
-        c->chains_per_lon = (prm->nlat + c->chain_len - 1) / c->chain_len;
-        c->nchain = c->chains_per_lon * prm->nlon;
-        DMALLOC(c->chain_key, (size_t)c->nchain * sizeof(uint32_t));
-        DMALLOC(c->chain_key_sorted, (size_t)c->nchain * sizeof(uint32_t));
-        DMALLOC(c->chain_id, (size_t)c->nchain * sizeof(int32_t));
-        DMALLOC(c->chain_order, (size_t)c->nchain * sizeof(int32_t));
+        DMALLOC(c->chain_key, (size_t)c->nchain_max * sizeof(uint32_t));
+        DMALLOC(c->chain_key_sorted, (size_t)c->nchain_max * sizeof(uint32_t));
+        DMALLOC(c->chain_id, (size_t)c->nchain_max * sizeof(int32_t));
+        DMALLOC(c->chain_order, (size_t)c->nchain_max * sizeof(int32_t));
         DMALLOC(c->queue_head, 4 * sizeof(int32_t));
         DMALLOC(c->ymax, no * sizeof(double));
         if (dk_sort_pairs_u32(NULL, &c->chain_tmp_bytes, c->chain_key, c->chain_key_sorted,
-                              c->chain_id, c->chain_order, c->nchain, 32, NULL)) {
+                              c->chain_id, c->chain_order, c->nchain_max, 32, NULL)) {
             dida_letkf_destroy(c);
             return set_err(DIDA_ERR_HIP, "chain sort size query failed");
         }
@@ -542,6 +580,88 @@ int dida_letkf_graph_capture(dida_letkf_ctx *c, const dida_letkf_obs *o, const d
     c->graph = g;
     HIP_TRY(hipGraphInstantiate(&c->gexec, g, NULL, NULL, 0));
     c->last_nobs = o->nobs;
+    return DIDA_OK;
+}
+
+static int timed_sweep(dida_letkf_ctx *c, const dida_letkf_obs *o, const double *xb,
+                       const double *axb, double *xa, double *axa, hipStream_t s,
+                       hipEvent_t e0, hipEvent_t e1, double *ms)
+{
+    *ms = 1e300;
+    for (int rep = 0; rep < 2; ++rep) {
+        HIP_TRY(hipEventRecord(e0, s));
+        int rc = dida_letkf_sweep(c, o, xb, axb, xa, axa, s);
+        if (rc < 0) return rc;
+        HIP_TRY(hipEventRecord(e1, s));
+        HIP_TRY(hipEventSynchronize(e1));
+        float t = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+        if (t < *ms) *ms = t;
+    }
+    return DIDA_OK;
+}
+
+int dida_letkf_autotune(dida_letkf_ctx *c, const dida_letkf_obs *o, const double *xb,
+                        const double *axb, double *xa, double *axa, void *stream,
+                        dida_letkf_tuning *out)
+{
+    if (!out) return set_err(DIDA_ERR_ARG, "null tuning output");
+    memset(out, 0, sizeof *out);
+    int rc = check_sweep_args(c, o, xb, axb, xa, axa);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    rc = dida_letkf_sweep(c, o, xb, axb, xa, axa, stream); /* sizes the lists */
+    if (rc < 0) return rc;
+    dida_letkf_stats st;
+    rc = dida_letkf_stats_get(c, &st);
+    if (rc < 0) return rc;
+    const int timing = c->timing;
+    c->timing = 0;
+    hipEvent_t e0 = NULL, e1 = NULL;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        c->timing = timing;
+        if (e0) (void)hipEventDestroy(e0);
+        return set_err(DIDA_ERR_HIP, "autotune: events");
+    }
+    const double bd0 = c->bin_deg;
+    const int32_t cl0 = c->chain_len;
+    double best_bd = bd0, best_ms = 0.0;
+    int32_t best_cl = cl0;
+    rc = timed_sweep(c, o, xb, axb, xa, axa, s, e0, e1, &best_ms);
+    out->ms_initial = best_ms;
+    out->ncand = 1;
+    const double bins[3] = {c->theta_deg / 2.0, c->theta_deg / 4.0, c->theta_deg / 8.0};
+    const int nbin = st.nobsl_capped > 0 ? 1 : 3;
+    const int32_t chains[5] = {2, 4, 8, 16, 32};
+    const int nchn = c->prm.form == DIDA_FORM_BATCHED ? 5 : 1;
+    for (int ib = 0; ib < nbin && !rc; ++ib) {
+        const double bd = nbin == 1 ? bd0 : bins[ib];
+        for (int ic = 0; ic < nchn && !rc; ++ic) {
+            const int32_t cl = nchn == 1 ? cl0 : chains[ic];
+            if (bd == bd0 && cl == cl0) continue;
+            rc = set_layout(c, bd, cl);
+            if (rc) break;
+            double ms = 0.0;
+            rc = timed_sweep(c, o, xb, axb, xa, axa, s, e0, e1, &ms);
+            out->ncand++;
+            if (!rc && ms < best_ms) {
+                best_ms = ms;
+                best_bd = bd;
+                best_cl = cl;
+            }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->timing = timing;
+    int rc2 = set_layout(c, best_bd, best_cl);
+    if (!rc) rc = rc2;
+    if (rc) return rc;
+    rc = dida_letkf_sweep(c, o, xb, axb, xa, axa, stream);
+    if (rc < 0) return rc;
+    out->chain_len = c->prm.form == DIDA_FORM_BATCHED ? c->chain_len : 0;
+    out->bin_deg = c->bin_deg;
+    out->ms_best = best_ms;
     return DIDA_OK;
 }
 

@@ -30,7 +30,7 @@ EXPORTED = ["dida_letkf_params_default", "letkf_core_", "letkf_core_batch_",
             "dida_letkf_graph_launch", "dida_letkf_stats_get", "dida_letkf_debug_arrays",
             "dida_letkf_post", "dida_letkf_last_error", "dida_letkf_set_timing",
             "dida_letkf_stage_ms", "dida_letkf_phase_cycles", "dida_letkf_group_capture",
-            "dida_letkf_group_launch", "dida_letkf_group_destroy"]
+            "dida_letkf_group_launch", "dida_letkf_group_destroy", "dida_letkf_autotune"]
 PHASES = ["gram", "warm", "jacobi", "transform", "update"]
 STAGES = ["bin", "select", "transform", "update"]
 
@@ -44,6 +44,11 @@ class Params(C.Structure):
                 ("jacobi_max_sweeps", C.c_int32), ("jacobi_tol", C.c_double),
                 ("bin_deg", C.c_double), ("chain_len", C.c_int32), ("warm_start", C.c_int32),
                 ("jacobi_predict", C.c_int32)]
+
+
+class TuningC(C.Structure):
+    _fields_ = [("chain_len", C.c_int32), ("ncand", C.c_int32), ("bin_deg", C.c_double),
+                ("ms_initial", C.c_double), ("ms_best", C.c_double)]
 
 
 class ObsC(C.Structure):
@@ -107,6 +112,8 @@ def lib():
                                                C.POINTER(ObsC), C.POINTER(vp), C.POINTER(vp),
                                                C.POINTER(vp), C.POINTER(vp), vp]
         L.dida_letkf_group_launch.argtypes = [vp, vp]
+        L.dida_letkf_autotune.argtypes = [vp, C.POINTER(ObsC), vp, vp, vp, vp, vp,
+                                          C.POINTER(TuningC)]
         L.dida_letkf_group_destroy.argtypes = [vp]
         L.letkf_core_.argtypes = [vp] * 10
         L.letkf_core_.restype = None
@@ -259,6 +266,13 @@ class Engine:
         return check(lib().dida_letkf_graph_capture(self.h, C.byref(oc), _ptr(xb), _ptr(axb),
                                                     _ptr(xa), _ptr(axa), _stream_ptr(stream)),
                      "graph_capture")
+
+    def autotune(self, obs: DeviceObs, xb, axb, xa, axa, stream=None) -> dict:
+        """dida_letkf_autotune: keep the fastest layout for this workload."""
+        oc, t = obs.c(), TuningC()
+        check(lib().dida_letkf_autotune(self.h, C.byref(oc), _ptr(xb), _ptr(axb), _ptr(xa),
+                                        _ptr(axa), _stream_ptr(stream), C.byref(t)), "autotune")
+        return {f: getattr(t, f) for f, _ in TuningC._fields_}
 
     def graph_launch(self, stream=None) -> int:
         return check(lib().dida_letkf_graph_launch(self.h, _stream_ptr(stream)), "graph_launch")

@@ -167,6 +167,28 @@ int dida_letkf_graph_capture(dida_letkf_ctx *ctx, const dida_letkf_obs *obs, con
                              const double *axb, double *xa, double *axa, void *stream);
 int dida_letkf_graph_launch(dida_letkf_ctx *ctx, void *stream);
 
+/* Autotuner.  Times sweeps of the caller's workload (inputs as for
+ * dida_letkf_sweep) over candidate layouts and keeps the fastest for the
+ * sweeps that follow: the length of the warm-start chains of the batched form
+ * (2, 4, 8, 16, 32 adjacent columns per work item: longer chains warm-start
+ * more eigensolves, shorter ones balance the cost-sorted queue better) and,
+ * when no column reaches MAX_OBS, the obs bin size (theta/2, theta/4,
+ * theta/8, theta = support radius: coarser bins scan more candidates, finer
+ * ones visit more bins).  With MAX_OBS binding the bin size stays, because
+ * the bin order decides which obs a capped column keeps.  Each candidate is
+ * timed twice (HIP events, minimum); xa/axa hold the result of a sweep with
+ * the kept layout on return.  Synchronises. */
+typedef struct dida_letkf_tuning {
+    int32_t chain_len;          /* kept (batched form; 0 for the plain form) */
+    int32_t ncand;              /* layouts timed */
+    double bin_deg;             /* kept */
+    double ms_initial;          /* sweep time with the layout on entry */
+    double ms_best;             /* sweep time with the kept layout */
+} dida_letkf_tuning;
+int dida_letkf_autotune(dida_letkf_ctx *ctx, const dida_letkf_obs *obs, const double *xb,
+                        const double *axb, double *xa, double *axa, void *stream,
+                        dida_letkf_tuning *out);
+
 /* Several analysis domains in one HIP graph (the batching form's regional
  * case: independent domains of different sizes and ensemble sizes, each with
  * its own context).  The graph forks from `stream` into one branch per
