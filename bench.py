@@ -47,15 +47,17 @@ def make_engine(dl, cfg, prm, form, gram, nobs):
     return dl.Engine(p, nobs_max=nobs), p
 
 
-def time_form(dl, torch, data, cfg, prm, form, gram, steps, warmup):
-    """Warm up, then time `steps` sweeps with HIP events around each stage on
-    the sweep's stream (the batched kernel's launch is the "transform" stage),
-    then one untimed sweep with the in-kernel phase clocks."""
+def time_form(dl, torch, data, cfg, prm, form, gram, steps, warmup, tune=False):
+    """Optionally autotune the layout (untimed setup), warm up, then time
+    `steps` sweeps with HIP events around each stage on the sweep's stream
+    (the batched kernel's launch is the "transform" stage), then one untimed
+    sweep with the in-kernel phase clocks."""
     eng, p = make_engine(dl, cfg, prm, form, gram, cfg.nobs)
     eng.set_grid(data["grid_lat"], data["grid_lon"])
     obs = dl.DeviceObs(data["lat"], data["lon"], data["yb"], data["dep"], data["rinv"])
     xa = torch.empty_like(data["xb"])
     axa = torch.empty_like(data["axb"])
+    tuning = eng.autotune(obs, data["xb"], data["axb"], xa, axa) if tune else None
     for _ in range(max(warmup, 1)):
         eng.sweep(obs, data["xb"], data["axb"], xa, axa)
     st = eng.stats()
@@ -80,7 +82,7 @@ def time_form(dl, torch, data, cfg, prm, form, gram, steps, warmup):
     tot = sum(phases.values())
     phase_frac = {k: v / tot for k, v in phases.items()} if tot > 0 else {}
     return {"engine": eng, "params": p, "xa": xa, "axa": axa, "ms": ms, "stats": st,
-            "stages": stages, "phase_frac": phase_frac}
+            "stages": stages, "phase_frac": phase_frac, "tuning": tuning}
 
 
 def rel_errors(ref, r, data, cfg):
@@ -226,6 +228,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=25.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="", help="comma list of forms to time")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuner (batched forms keep the default layout)")
     ap.add_argument("--no-plain", action="store_true",
                     help="skip the plain FP64 form (profiling runs)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -268,7 +272,9 @@ def main():
     for name, (form, gram, extra) in forms.items():
         log(f"timing {name}")
         r = time_form(dl, torch, data, cfg, dict(prm, **extra), form, gram, args.steps,
-                      args.warmup)
+                      args.warmup, tune=form == dl.FORM_BATCHED and not args.no_tune)
+        if r["tuning"]:
+            log(f"  autotuned layout {r['tuning']}")
         log(f"  {r['ms']:.1f} ms/step, stages {r['stages']}")
         log(f"  mean sweeps {r['stats']['sweeps_sum'] / max(1, r['stats']['nanal']):.2f}, "
             f"phase fractions {r['phase_frac']}")
@@ -303,6 +309,7 @@ def main():
                            "jacobi_max_sweeps_used": rr["stats"]["max_sweeps_used"],
                            "jacobi_mean_sweeps": rr["stats"]["sweeps_sum"] / max(1, rr["stats"]["nanal"]),
                            "kernel_phase_fraction": rr["phase_frac"],
+                           "autotuned_layout": rr["tuning"],
                            "stage_ms": rr["stages"],
                            "update_GBps": update_bytes(cfg.k, n, ncol) / (rr["stages"]["update"] / 1e3) / 1e9
                            if rr["stages"]["update"] > 0 else None,

@@ -632,8 +632,8 @@ int dida_letkf_autotune(dida_letkf_ctx *c, const dida_letkf_obs *o, const double
     out->ncand = 1;
     const double bins[3] = {c->theta_deg / 2.0, c->theta_deg / 4.0, c->theta_deg / 8.0};
     const int nbin = st.nobsl_capped > 0 ? 1 : 3;
-    const int32_t chains[5] = {2, 4, 8, 16, 32};
-    const int nchn = c->prm.form == DIDA_FORM_BATCHED ? 5 : 1;
+    const int32_t chains[6] = {2, 4, 8, 16, 32, 64};
+    const int nchn = c->prm.form == DIDA_FORM_BATCHED ? 6 : 1;
     for (int ib = 0; ib < nbin && !rc; ++ib) {
         const double bd = nbin == 1 ? bd0 : bins[ib];
         for (int ic = 0; ic < nchn && !rc; ++ic) {

@@ -170,7 +170,7 @@ int dida_letkf_graph_launch(dida_letkf_ctx *ctx, void *stream);
 /* Autotuner.  Times sweeps of the caller's workload (inputs as for
  * dida_letkf_sweep) over candidate layouts and keeps the fastest for the
  * sweeps that follow: the length of the warm-start chains of the batched form
- * (2, 4, 8, 16, 32 adjacent columns per work item: longer chains warm-start
+ * (2, 4, 8, 16, 32, 64 adjacent columns per work item: longer chains warm-start
  * more eigensolves, shorter ones balance the cost-sorted queue better) and,
  * when no column reaches MAX_OBS, the obs bin size (theta/2, theta/4,
  * theta/8, theta = support radius: coarser bins scan more candidates, finer
