@@ -135,9 +135,8 @@ __device__ __forceinline__ double bperm_d(int addr, double v) {
   return __hiloint2double(hi, lo);
 }
 
-// 1 / sqrt(x) and 1 / x for positive normal x: the hardware estimates
-// (relative error about 2^-23) refined by two Newton steps, y (3 - x y^2) / 2
-// and y (2 - x y), to a few ulp.
+// 1 / sqrt(x) for positive normal x: the hardware estimate (relative error
+// about 2^-23) refined by two Newton steps, y (3 - x y^2) / 2, to a few ulp.
 __device__ __forceinline__ double rsq_nr(double x) {
   double y = __builtin_amdgcn_rsq(x);
   const double hx = 0.5 * x;
@@ -145,11 +144,15 @@ __device__ __forceinline__ double rsq_nr(double x) {
   y = y * fma(-hx * y, y, 1.5);
   return y;
 }
-__device__ __forceinline__ double rcp_nr(double x) {
-  double y = __builtin_amdgcn_rcp(x);
-  y = fma(y, fma(-x, y, 1.0), y);
-  y = fma(y, fma(-x, y, 1.0), y);
-  return y;
+// One Newton step each (relative error about 2^-45): enough for a rotation
+// angle, whose error only delays convergence.
+__device__ __forceinline__ double rsq_nr1(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  return y * fma(-0.5 * x * y, y, 1.5);
+}
+__device__ __forceinline__ double rcp_nr1(double x) {
+  const double y = __builtin_amdgcn_rcp(x);
+  return fma(y, fma(-x, y, 1.0), y);
 }
 
 // Metadata of one 64-obs block of an analysis, lane l holding obs base + l:
@@ -439,15 +442,13 @@ struct Wave {
 
   template <int P, int Q>
   __device__ __forceinline__ double dot() const {
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    double s0 = 0.0, s1 = 0.0;
 #pragma unroll
-    for (int q = 0; q < NQ; q += 4) {
+    for (int q = 0; q < NQ; q += 2) {
       s0 = fma(X[P][q], X[Q][q], s0);
       s1 = fma(X[P][q + 1], X[Q][q + 1], s1);
-      s2 = fma(X[P][q + 2], X[Q][q + 2], s2);
-      s3 = fma(X[P][q + 3], X[Q][q + 3], s3);
     }
-    return (s0 + s1) + (s2 + s3);
+    return s0 + s1;
   }
 
   // Rotate columns P, Q given gs = X[P] . X[Q] (summed over the wave).  A
@@ -466,8 +467,8 @@ struct Wave {
     // t = sign(d e) |e| / (|d| + sqrt(d^2 + e^2)), d = beta - alpha, e = 2 gamma
     const double d = be - al, e = 2.0 * gm;
     const double r2 = fma(d, d, e * e);
-    const double den = fabs(d) + r2 * rsq_nr(r2);
-    double t = fabs(e) * rcp_nr(den);
+    const double den = fabs(d) + r2 * rsq_nr1(r2);
+    double t = fabs(e) * rcp_nr1(den);
     if ((d < 0.0) != (e < 0.0) && d != 0.0) t = -t;
     t = need ? t : 0.0;
     const double u = fma(t, t, 1.0);
@@ -505,8 +506,8 @@ struct Wave {
     rotated |= need;
     const double d = be - al, e = 2.0 * gm;
     const double r2 = fma(d, d, e * e);
-    const double den = fabs(d) + r2 * rsq_nr(r2);
-    t = fabs(e) * rcp_nr(den);
+    const double den = fabs(d) + r2 * rsq_nr1(r2);
+    t = fabs(e) * rcp_nr1(den);
     if ((d < 0.0) != (e < 0.0) && d != 0.0) t = -t;
     t = need ? t : 0.0;
     const double u = fma(t, t, 1.0);
@@ -779,14 +780,12 @@ __device__ void wave_column(const dk_batched_args& a, int64_t col, bool warm, WS
   const int nrow = a.nrow;
   const double* xbc = a.xb + col * (int64_t)nrow * k;
   double* xac = a.xa + col * (int64_t)nrow * k;
-  for (int R0 = 0; R0 < nrow; R0 += 16) {
+  // Xb' of rows R0 + c (clamped to the last row, in bounds), members of the
+  // B-operand pattern; the next block's loads are issued before a block
+  // computes
+  auto load_xb = [&](int R0, double (&xl)[NJ][4]) {
     const int xr = R0 + c;
-    const bool rok = xr < nrow;
-    double4_t D[NJ];
-#pragma unroll
-    for (int E = 0; E < NJ; ++E) D[E] = double4_t{0.0, 0.0, 0.0, 0.0};
-    double xl[NJ][4];
-    const double* xrow = xbc + (int64_t)(rok ? xr : nrow - 1) * k;
+    const double* xrow = xbc + (int64_t)(xr < nrow ? xr : nrow - 1) * k;
 #pragma unroll
     for (int I = 0; I < NJ; ++I)
 #pragma unroll
@@ -794,6 +793,17 @@ __device__ void wave_column(const dk_batched_args& a, int64_t col, bool warm, WS
         const int i = rowB(I, r, g);
         xl[I][r] = xrow[i < k ? i : k - 1];
       }
+  };
+  double xl[NJ][4];
+  load_xb(0, xl);
+  for (int R0 = 0; R0 < nrow; R0 += 16) {
+    const int xr = R0 + c;
+    const bool rok = xr < nrow;
+    double4_t D[NJ];
+#pragma unroll
+    for (int E = 0; E < NJ; ++E) D[E] = double4_t{0.0, 0.0, 0.0, 0.0};
+    double xn[NJ][4];
+    load_xb(R0 + 16, xn);
     __builtin_amdgcn_sched_barrier(0);
     double ssb = 0.0;
 #pragma unroll
@@ -842,6 +852,10 @@ __device__ void wave_column(const dk_batched_args& a, int64_t col, bool warm, WS
         }
       if (g == 0) a.axa[col * nrow + xr] = mean;
     }
+#pragma unroll
+    for (int I = 0; I < NJ; ++I)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) xl[I][r] = xn[I][r];
   }
   clk.mark(DK_PHASE_UPDATE);
   (void)NQ;
