@@ -171,6 +171,37 @@ def measured_traffic(head, config, scale):
     return e["hbm_bytes_per_launch"] if e else None
 
 
+def time_obs_operator(dl, torch, data, cfg, reps=5):
+    """H(Xb) + centring for one record per synthetic obs position (one level,
+    the temperature variable, sigma levels), timed with HIP events.  Bytes
+    per scalar obs and member: 4 corners of surface pressure and 2 x 4 corners
+    of the variable, 8 B each (algorithmic; corners shared by neighbouring
+    obs hit in cache)."""
+    nz, k = cfg.nz, cfg.k
+    dev = data["xb"].device
+    rng = torch.Generator(device="cpu").manual_seed(7)
+    nrec = cfg.nobs
+    ph = (100.0 + 900.0 * torch.rand(nrec, 1, generator=rng, dtype=torch.float64)).to(dev)
+    yo = torch.zeros(nrec, 1, 1, dtype=torch.float64, device=dev)
+    vc = {"type": 0, "c1": torch.linspace(0.0, 1.0, nz + 1, dtype=torch.float64, device=dev),
+          "ptop": 10.0}
+    vs, vl = [0, 1, 1 + nz, 1 + 2 * nz], [1, nz, nz, nz]
+    args = (data["grid_lat"], data["grid_lon"], data["xb"], data["axb"], 0, vc, vs, vl, [3],
+            data["lat"], data["lon"], ph, yo)
+    dl.obs_operator(*args)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        dl.obs_operator(*args)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    nbytes = nrec * k * 12 * 8.0
+    return {"records": nrec, "levels": 1, "variables": 1, "ms": ms,
+            "algorithmic_GBps": nbytes / (ms / 1e3) / 1e9}
+
+
 def copy_probe(torch, nbytes=1 << 30, reps=3):
     """A plain device copy of known size (the FETCH_SIZE / WRITE_SIZE scale)."""
     src = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda").fill_(1.0)
@@ -315,6 +346,9 @@ def main():
                            if rr["stages"]["update"] > 0 else None,
                            "selection_GBps": selection_bytes(cfg.k, cfg.nobs, rr["stats"]["nobsl_sum"])
                            / ((rr["stages"]["bin"] + rr["stages"]["select"]) / 1e3) / 1e9}
+    log("timing the observation operator")
+    obsop = time_obs_operator(dl, torch, data, cfg)
+    log(f"  {obsop}")
     cpu = None
     if not args.no_cpu:
         log("timing the CPU oracle on a column sample")
@@ -335,6 +369,7 @@ def main():
                    "rel_increment_error_vs_fp64": r["err"] or None,
                    "speedup_vs_plain_fp64": plain_ms / r["ms"] if plain_ms else None},
         "roofline": roofline, "cpu_baseline": cpu, "forms": out_forms,
+        "obs_operator": obsop,
     }
     print(json.dumps(line), flush=True)
 

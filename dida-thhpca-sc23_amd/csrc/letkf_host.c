@@ -1037,3 +1037,72 @@ void letkf_core_batch_(const int32_t *batch, const int32_t *batch_large_matrix,
     if (rc) fprintf(stderr, "letkf_core_batch_: %s\n", dida_letkf_last_error());
     if (pflag && *pflag) printf("letkf_core_batch_: batch %d done\n", *batch);
 }
+
+/* ------------------------------------------------------------------------ */
+int dida_letkf_obs_operator(int32_t k, int32_t nrow, int32_t nlat, int32_t nlon,
+                            const float *grid_lat, const float *grid_lon, int32_t periodic_lon,
+                            const double *xb, const double *axb, int32_t ps_row,
+                            const dida_vert_coord *vc, int32_t nvar, const int32_t *var_start,
+                            const int32_t *var_lev, const dida_obs_records *rec, double *lat,
+                            double *lon, double *yb, double *dep, void *stream)
+{
+    if (!vc || !rec || !var_start || !var_lev || !grid_lat || !grid_lon || !xb || !axb)
+        return set_err(DIDA_ERR_ARG, "obs operator: null argument");
+    if (k < 1 || k > 128) return set_err(DIDA_ERR_UNSUPPORTED, "obs operator: k = %d (1..128)", k);
+    if (nrow < 1 || nlat < 1 || nlon < 1) return set_err(DIDA_ERR_ARG, "obs operator: empty grid");
+    if (nvar < 1 || nvar > DK_OBSOP_MAXVAR || rec->onv < 0 || rec->onv > DK_OBSOP_MAXVAR)
+        return set_err(DIDA_ERR_UNSUPPORTED, "obs operator: at most %d variables",
+                       DK_OBSOP_MAXVAR);
+    if (rec->nrec < 0 || rec->onz < 0) return set_err(DIDA_ERR_ARG, "obs operator: negative size");
+    if (ps_row < 0 || ps_row >= nrow) return set_err(DIDA_ERR_ARG, "obs operator: ps_row");
+    if (vc->type != DIDA_VCOORD_SIGMA && vc->type != DIDA_VCOORD_HYBRID)
+        return set_err(DIDA_ERR_ARG, "obs operator: vertical coordinate type");
+    if (vc->nz < 0 || !vc->c1 || (vc->type == DIDA_VCOORD_HYBRID && !vc->c2))
+        return set_err(DIDA_ERR_ARG, "obs operator: vertical coordinate");
+    dk_obsop_args a;
+    memset(&a, 0, sizeof a);
+    for (int v = 0; v < nvar; ++v) {
+        if (var_start[v] < 0 || var_lev[v] < 1 || var_start[v] + var_lev[v] > nrow ||
+            (var_lev[v] > 1 && var_lev[v] != vc->nz))
+            return set_err(DIDA_ERR_ARG, "obs operator: variable %d rows", v);
+        a.vstart[v] = var_start[v];
+        a.vlev[v] = var_lev[v];
+    }
+    for (int v = 0; v < rec->onv; ++v) {
+        if (!rec->vid || rec->vid[v] < 0 || rec->vid[v] >= nvar)
+            return set_err(DIDA_ERR_ARG, "obs operator: obs variable %d", v);
+        a.vid[v] = rec->vid[v];
+    }
+    if (rec->nrec == 0 || rec->onz == 0 || rec->onv == 0) return DIDA_OK;
+    if (!rec->lat || !rec->lon || !rec->ph || !rec->yo || !lat || !lon || !yb || !dep)
+        return set_err(DIDA_ERR_ARG, "obs operator: null record or output array");
+    a.k = k;
+    a.nrow = nrow;
+    a.nlat = nlat;
+    a.nlon = nlon;
+    a.periodic = periodic_lon ? 1 : 0;
+    a.ps_row = ps_row;
+    a.nz = vc->nz;
+    a.vtype = vc->type;
+    a.onz = rec->onz;
+    a.onv = rec->onv;
+    a.nrec = rec->nrec;
+    a.grid_lat = grid_lat;
+    a.grid_lon = grid_lon;
+    a.xb = xb;
+    a.axb = axb;
+    a.c1 = vc->c1;
+    a.c2 = vc->c2 ? vc->c2 : vc->c1;
+    a.ptop = vc->ptop;
+    a.p0 = vc->p0;
+    a.rec_lat = rec->lat;
+    a.rec_lon = rec->lon;
+    a.rec_ph = rec->ph;
+    a.yo = rec->yo;
+    a.lat = lat;
+    a.lon = lon;
+    a.yb = yb;
+    a.dep = dep;
+    HIP_TRY(dk_obs_operator(&a, stream));
+    return DIDA_OK;
+}

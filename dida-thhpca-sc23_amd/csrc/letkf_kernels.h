@@ -176,6 +176,24 @@ int dk_letkf_wave(const dk_batched_args *a, int nblocks, void *stream);
 int dk_obs_ymax(int32_t nobs, int32_t k, const double *yb, const float *rinv, double *ymax,
                 void *stream);
 
+/* Observation operator H(Xb) and centring, one wavefront per record. */
+#define DK_OBSOP_MAXVAR 32
+typedef struct dk_obsop_args {
+    int32_t k, nrow, nlat, nlon;
+    int32_t periodic, ps_row, nz, vtype;   /* vtype 0 sigma, 1 hybrid */
+    int32_t onz, onv;
+    int64_t nrec;
+    const float *grid_lat, *grid_lon;
+    const double *xb, *axb;
+    const double *c1, *c2;                 /* nz + 1 half levels */
+    double ptop, p0;
+    int32_t vstart[DK_OBSOP_MAXVAR], vlev[DK_OBSOP_MAXVAR], vid[DK_OBSOP_MAXVAR];
+    const double *rec_lat, *rec_lon, *rec_ph, *yo;
+    double *lat, *lon, *yb, *dep;
+} dk_obsop_args;
+
+int dk_obs_operator(const dk_obsop_args *a, void *stream);
+
 /* Post-analysis blend of run_da_parallel (RTPP-style gamma blend inside
  * |lat| <= polar_lat, background outside) and spreads. */
 typedef struct dk_post_args {

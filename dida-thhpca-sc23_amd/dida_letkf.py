@@ -30,7 +30,8 @@ EXPORTED = ["dida_letkf_params_default", "letkf_core_", "letkf_core_batch_",
             "dida_letkf_graph_launch", "dida_letkf_stats_get", "dida_letkf_debug_arrays",
             "dida_letkf_post", "dida_letkf_last_error", "dida_letkf_set_timing",
             "dida_letkf_stage_ms", "dida_letkf_phase_cycles", "dida_letkf_group_capture",
-            "dida_letkf_group_launch", "dida_letkf_group_destroy", "dida_letkf_autotune"]
+            "dida_letkf_group_launch", "dida_letkf_group_destroy", "dida_letkf_autotune",
+            "dida_letkf_obs_operator"]
 PHASES = ["gram", "warm", "jacobi", "transform", "update"]
 STAGES = ["bin", "select", "transform", "update"]
 
@@ -49,6 +50,17 @@ class Params(C.Structure):
 class TuningC(C.Structure):
     _fields_ = [("chain_len", C.c_int32), ("ncand", C.c_int32), ("bin_deg", C.c_double),
                 ("ms_initial", C.c_double), ("ms_best", C.c_double)]
+
+
+class VertCoordC(C.Structure):
+    _fields_ = [("type", C.c_int32), ("nz", C.c_int32), ("c1", C.c_void_p), ("c2", C.c_void_p),
+                ("ptop", C.c_double), ("p0", C.c_double)]
+
+
+class ObsRecordsC(C.Structure):
+    _fields_ = [("nrec", C.c_int32), ("onz", C.c_int32), ("onv", C.c_int32), ("pad_", C.c_int32),
+                ("vid", C.c_void_p), ("lat", C.c_void_p), ("lon", C.c_void_p),
+                ("ph", C.c_void_p), ("yo", C.c_void_p)]
 
 
 class ObsC(C.Structure):
@@ -112,6 +124,10 @@ def lib():
                                                C.POINTER(ObsC), C.POINTER(vp), C.POINTER(vp),
                                                C.POINTER(vp), C.POINTER(vp), vp]
         L.dida_letkf_group_launch.argtypes = [vp, vp]
+        L.dida_letkf_obs_operator.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, vp, vp,
+                                              C.c_int32, vp, vp, C.c_int32,
+                                              C.POINTER(VertCoordC), C.c_int32, vp, vp,
+                                              C.POINTER(ObsRecordsC), vp, vp, vp, vp, vp]
         L.dida_letkf_autotune.argtypes = [vp, C.POINTER(ObsC), vp, vp, vp, vp, vp,
                                           C.POINTER(TuningC)]
         L.dida_letkf_group_destroy.argtypes = [vp]
@@ -368,6 +384,40 @@ def post(k, nrow, nlat, nlon, grid_lat, polar_lat, gamma1, gamma2, xb, axb, xa, 
                                        float(gamma1), float(gamma2), _ptr(xb), _ptr(axb),
                                        _ptr(xa), _ptr(axa), _ptr(xa_out), _ptr(spread_b),
                                        _ptr(spread_a), _stream_ptr(stream)), "post")
+
+
+def obs_operator(grid_lat, grid_lon, xb, axb, ps_row, vcoord, var_start, var_lev, vid,
+                 rec_lat, rec_lon, rec_ph, yo, periodic_lon=True, stream=None):
+    """dida_letkf_obs_operator on device tensors.  xb (ncol, nrow, k), axb
+    (ncol, nrow); vcoord: dict(type, c1, c2, ptop, p0) with c1/c2 device
+    float64 tensors; rec_ph (nrec, onz); yo (nrec, onz, onv).  Returns
+    DeviceObs-ready arrays (lat, lon, yb (nobs, k), dep)."""
+    import torch
+    ncol, nrow, k = xb.shape
+    nrec, onz = rec_ph.shape
+    onv = len(vid)
+    ns = nrec * onz * onv
+    dev = xb.device
+    lat = torch.empty(ns, dtype=torch.float64, device=dev)
+    lon = torch.empty(ns, dtype=torch.float64, device=dev)
+    yb = torch.empty(ns, k, dtype=torch.float64, device=dev)
+    dep = torch.empty(ns, dtype=torch.float64, device=dev)
+    c1 = vcoord["c1"]
+    c2 = vcoord.get("c2", c1)
+    vc = VertCoordC(int(vcoord["type"]), int(c1.numel()) - 1, c1.data_ptr(), c2.data_ptr(),
+                    float(vcoord["ptop"]), float(vcoord.get("p0", 0.0)))
+    vid_h = np.ascontiguousarray(vid, np.int32)
+    vs = np.ascontiguousarray(var_start, np.int32)
+    vl = np.ascontiguousarray(var_lev, np.int32)
+    rc = ObsRecordsC(nrec, onz, onv, 0, vid_h.ctypes.data, rec_lat.data_ptr(), rec_lon.data_ptr(),
+                     rec_ph.data_ptr(), yo.data_ptr())
+    check(lib().dida_letkf_obs_operator(k, nrow, grid_lat.numel(), grid_lon.numel(),
+                                        _ptr(grid_lat), _ptr(grid_lon), int(bool(periodic_lon)),
+                                        _ptr(xb), _ptr(axb), int(ps_row), C.byref(vc), len(vs),
+                                        vs.ctypes.data, vl.ctypes.data, C.byref(rc), _ptr(lat),
+                                        _ptr(lon), _ptr(yb), _ptr(dep), _stream_ptr(stream)),
+          "obs_operator")
+    return lat, lon, yb, dep
 
 
 # ----------------------------------------------------------------------------

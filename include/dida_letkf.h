@@ -242,6 +242,53 @@ int dida_letkf_debug_arrays(dida_letkf_ctx *ctx, const int32_t **nobsl, const in
                             const int32_t **info);
 
 /* ------------------------------------------------------------------------ */
+/* Observation operator and centring: the step before the analysis, which
+ * produces its obs arrays (letkf_obs.F90:120-178 with find_obs_grid :187-243;
+ * obs_interpolation_3d_2 and ps2ph_3d_interp, obs_make.F90:233-381).
+ *
+ * Model levels: half-level pressure of level j = 0..nz from the member's
+ * surface pressure ps (vert_coord_mod.F90:547-563):
+ *   DIDA_VCOORD_SIGMA:  c1[j] (ps - ptop) + ptop
+ *   DIDA_VCOORD_HYBRID: c1[j] (p0 - ptop) + c2[j] (ps - ptop) + ptop
+ * Full levels are the means of adjacent half levels. */
+#define DIDA_VCOORD_SIGMA 0
+#define DIDA_VCOORD_HYBRID 1
+typedef struct dida_vert_coord {
+    int32_t type;
+    int32_t nz;                 /* full levels */
+    const double *c1, *c2;      /* device [nz + 1] (c2: hybrid only) */
+    double ptop, p0;            /* Pa */
+} dida_vert_coord;
+
+/* Observation records (read_obs_mod.F90:15-27 obs_list): nrec records at
+ * (lat, lon) with onz levels of pressure ph[r * onz + l] (Pa, ascending
+ * within a record, as the reference's level search assumes) and onv
+ * variables vid[v] (indices into the state variables); values
+ * yo[(r * onz + l) * onv + v].  Device pointers except vid (host). */
+typedef struct dida_obs_records {
+    int32_t nrec, onz, onv, pad_;
+    const int32_t *vid;
+    const double *lat, *lon, *ph, *yo;
+} dida_obs_records;
+
+/* H(Xb) for every member and the centring: outputs the analysis's scalar obs
+ * s = (r * onz + l) * onv + v: lat[s], lon[s], yb[s * k + e] = H(x_e) -
+ * mean_e H(x_e) (pro_yb), dep[s] = yo[s] - mean_e H(x_e) (dis_yo); R^-1 of
+ * the same scalars is the caller's.  State as for dida_letkf_sweep (Xb' and
+ * xbar_b, device); state variable v occupies rows var_start[v] ..
+ * var_start[v] + var_lev[v] - 1 (var_lev 1: a 2-D variable; host arrays,
+ * nvar <= 32, onv <= 32); surface pressure at row ps_row.  Grid brackets as
+ * find_obs_grid (first grid point >= the obs); outside the first / last grid
+ * latitude the nearest row; periodic_lon wraps the longitude.  k <= 128.
+ * Enqueued on `stream`. */
+int dida_letkf_obs_operator(int32_t k, int32_t nrow, int32_t nlat, int32_t nlon,
+                            const float *grid_lat, const float *grid_lon, int32_t periodic_lon,
+                            const double *xb, const double *axb, int32_t ps_row,
+                            const dida_vert_coord *vc, int32_t nvar, const int32_t *var_start,
+                            const int32_t *var_lev, const dida_obs_records *rec, double *lat,
+                            double *lon, double *yb, double *dep, void *stream);
+
+/* ------------------------------------------------------------------------ */
 /* Post-analysis (run_da_parallel, module_da_parallel.F90:216-259): inside
  * |grid_lat| <= polar_lat, xa_out = gamma1 (xbar_b + Xb') + gamma2 (xbar_a +
  * Xa'); outside, the background; spread_b/a = sqrt(sum_e (.)^2 / k).

@@ -606,3 +606,126 @@ void orc_post(int mode, int k, int nrow, int nlat, int nlon,
             }
         }
 }
+
+/* ------------------------------------------------------------------------ */
+/* Observation operator (letkf_obs.F90:120-178, 187-243; obs_make.F90:233-381;
+ * vert_coord_mod.F90:547-563).  Operation order follows the reference. */
+
+/* find_obs_grid for one axis: i1 = first index with g[i1] >= x; returns the
+ * weight of i0 (the point before).  Differences of REAL(4) grid values are
+ * formed in single precision, as the reference's expression is. */
+static double bracket(const float *g, int n, int periodic, double x, int *i0, int *i1)
+{
+    int j = 0;
+    while (j < n && (double)g[j] < x) ++j;
+    if (j == 0) { /* at or before the first point: that point alone */
+        *i0 = *i1 = 0;
+        return 0.0;
+    }
+    if (j == n) {
+        if (!periodic) { /* past the last point: that point alone */
+            *i0 = *i1 = n - 1;
+            return 1.0;
+        }
+        const float gw = g[0] + 360.0f; /* the wrapped first point */
+        if (!((double)gw >= x)) {
+            *i0 = *i1 = 0;
+            return 0.0;
+        }
+        const double d = (double)(float)(gw - g[n - 1]);
+        *i0 = n - 1;
+        *i1 = 0;
+        return (d - x + (double)g[n - 1]) / d;
+    }
+    const double d = (double)(float)(g[j] - g[j - 1]);
+    *i0 = j - 1;
+    *i1 = j;
+    return (d - x + (double)g[j - 1]) / d;
+}
+
+static double ph_half(const orc_vcoord *vc, int j, double ps)
+{
+    if (vc->type == 0) return vc->c1[j] * (ps - vc->ptop) + vc->ptop;
+    return vc->c1[j] * (vc->p0 - vc->ptop) + vc->c2[j] * (ps - vc->ptop) + vc->ptop;
+}
+
+void orc_obs_operator(int k, int nrow, int nlat, int nlon, const float *grid_lat,
+                      const float *grid_lon, int periodic_lon, const double *xb,
+                      const double *axb, int ps_row, const orc_vcoord *vc,
+                      const int32_t *var_start, const int32_t *var_lev, int nrec, int onz,
+                      int onv, const int32_t *vid, const double *rec_lat,
+                      const double *rec_lon, const double *rec_ph, const double *yo,
+                      double *lat, double *lon, double *yb, double *dep)
+{
+    const int nz = vc->nz;
+    double *ph = (double *)malloc(sizeof(double) * (size_t)(nz + 2));
+    int *pos = (int *)malloc(sizeof(int) * (size_t)(onz > 0 ? onz : 1));
+    double *h = (double *)malloc(sizeof(double) * (size_t)k);
+    for (int r = 0; r < nrec; ++r) {
+        int a0, a1, b0, b1;
+        const double t1 = bracket(grid_lat, nlat, 0, rec_lat[r], &a0, &a1);
+        const double t2 = bracket(grid_lon, nlon, periodic_lon, rec_lon[r], &b0, &b1);
+        const size_t c00 = (size_t)a0 + (size_t)nlat * b0, c10 = (size_t)a1 + (size_t)nlat * b0;
+        const size_t c01 = (size_t)a0 + (size_t)nlat * b1, c11 = (size_t)a1 + (size_t)nlat * b1;
+#define X(col, row, e) (xb[((col) * (size_t)nrow + (size_t)(row)) * k + (e)] + \
+                        axb[(col) * (size_t)nrow + (size_t)(row)])
+#define HORIZ(row, e) (((X(c00, row, e)) * t1 + (X(c10, row, e)) * (1 - t1)) * t2 + \
+                       ((X(c01, row, e)) * t1 + (X(c11, row, e)) * (1 - t1)) * (1 - t2))
+        for (int l = 0; l < onz; ++l)
+            for (int v = 0; v < onv; ++v) {
+                const size_t sidx = ((size_t)r * onz + l) * onv + v;
+                lat[sidx] = rec_lat[r];
+                lon[sidx] = rec_lon[r];
+            }
+        for (int e = 0; e < k; ++e) {
+            /* ps2ph_3d_interp */
+            const double r1 = X(c00, ps_row, e) * t1 + X(c10, ps_row, e) * (1 - t1);
+            const double r2 = X(c01, ps_row, e) * t1 + X(c11, ps_row, e) * (1 - t1);
+            const double ps = r1 * t2 + r2 * (1 - t2);
+            for (int j = 0; j < nz; ++j)
+                ph[j] = 0.5 * (ph_half(vc, j, ps) + ph_half(vc, j + 1, ps));
+            if (nz > 0) ph[nz] = ph[nz - 1];
+            /* level search, carried over the record's levels */
+            int j = 0;
+            for (int l = 0; l < onz; ++l) {
+                while (j < nz && rec_ph[(size_t)r * onz + l] > ph[j]) ++j;
+                pos[l] = j; /* the reference's ph_pos (1-based) */
+            }
+            for (int l = 0; l < onz; ++l)
+                for (int v = 0; v < onv; ++v) {
+                    const int var = vid[v];
+                    const size_t sidx = ((size_t)r * onz + l) * onv + v;
+                    double val;
+                    if (var_lev[var] > 1) {
+                        const int p = pos[l];
+                        if (p >= 1 && p < nz) {
+                            const int tl = var_start[var] + p - 1;
+                            const double po = rec_ph[(size_t)r * onz + l];
+                            const double t3 = (log(ph[p]) - log(po)) / (log(ph[p]) - log(ph[p - 1]));
+                            val = HORIZ(tl, e) * t3 + HORIZ(tl + 1, e) * (1 - t3);
+                        } else {
+                            const int tl = p == 0 ? var_start[var] : var_start[var] + nz - 1;
+                            val = HORIZ(tl, e);
+                        }
+                    } else {
+                        val = HORIZ(var_start[var], e);
+                    }
+                    yb[sidx * k + e] = val;
+                }
+        }
+#undef HORIZ
+#undef X
+    }
+    /* centring (letkf_obs.F90:156-178) */
+    const int ns = nrec * onz * onv;
+    for (int s2 = 0; s2 < ns; ++s2) {
+        double s = 0.0;
+        for (int e = 0; e < k; ++e) s = s + yb[(size_t)s2 * k + e];
+        s = s / k;
+        dep[s2] = yo[s2] - s;
+        for (int e = 0; e < k; ++e) yb[(size_t)s2 * k + e] = yb[(size_t)s2 * k + e] - s;
+    }
+    free(h);
+    free(pos);
+    free(ph);
+}

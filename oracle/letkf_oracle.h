@@ -14,6 +14,9 @@
  *   orc_sweep         code/x86/TradDA/DA/LETKF/src/letkf_compute.F90:12-126
  *   orc_centre_obs    code/x86/TradDA/DA/LETKF/src/letkf_obs.F90:156-178
  *   orc_post          code/x86/TradDA/DA/EnDNN/src/module_da_parallel.F90:216-259
+ *   orc_obs_operator  code/x86/TradDA/DA/LETKF/src/letkf_obs.F90:120-178, 187-243
+ *                     (find_obs_grid), obs_make.F90:233-381 (obs_interpolation_3d_2,
+ *                     ps2ph_3d_interp), DA/utils/vert_coord_mod.F90:547-563
  * and keeps the reference's numeric quirks: single-precision literals in the
  * distance and Gaspari-Cohn code, REAL(4) R^-1 and grid coordinates, strict
  * ">" thresholds and MAX_OBS truncation in scan order.
@@ -146,6 +149,42 @@ int orc_sweep_compact(const orc_params *prm, const float *grid_lat, const float 
 /* Observation centring: ybar = mean_e hxb; dep = yo - ybar; yb = hxb - ybar. */
 void orc_centre_obs(int k, int nobs, const double *hxb, const double *yo,
                     double *yb, double *dep);
+
+/* Vertical coordinate of the model levels: half-level pressure of level
+ * j = 0..nz (vert_coord_mod.F90:547-563):
+ *   sigma  (type 0): c1[j] (ps - ptop) + ptop
+ *   hybrid (type 1): c1[j] (p0 - ptop) + c2[j] (ps - ptop) + ptop */
+typedef struct orc_vcoord {
+    int32_t type, nz;
+    const double *c1, *c2;
+    double ptop, p0;
+} orc_vcoord;
+
+/* Observation operator H(Xb) and centring.  State xb[(col * nrow + row) * k
+ * + e] (perturbations) and axb[col * nrow + row] (mean), col = ilat + nlat
+ * ilon, grid_lat/grid_lon ascending (REAL(4)); ps at row ps_row; variable
+ * v of the state starts at row var_start[v] with var_lev[v] levels (1: a
+ * 2-D variable).  Records r = 0..nrec-1 at (rec_lat, rec_lon) with onz
+ * levels of pressure rec_ph[r * onz + l] (Pa) and onv variables vid[]; the
+ * outputs are scalar obs s = (r * onz + l) * onv + v, in the order the
+ * reference's selection appends them: lat/lon, yb[s * k + e] = H(x_e) -
+ * mean_e, dep[s] = yo[s] - mean_e.  Per member: bilinear surface pressure,
+ * full-level pressures (mean of half levels, level nz repeated), the level
+ * search of obs_interpolation_3d_2 (carried over the record's levels in
+ * order), log-pressure interpolation between the bracketing levels (the
+ * nearest level outside them), bilinear in the horizontal.  Grid brackets:
+ * the first grid latitude (longitude) >= the obs; an obs at or before the
+ * first one takes that point alone (weight 0 on the missing neighbour, as
+ * the reference's halo point gets); past the last latitude the last row
+ * alone (the reference has no point there on a global grid); with
+ * periodic_lon the longitude after the last wraps to the first + 360. */
+void orc_obs_operator(int k, int nrow, int nlat, int nlon, const float *grid_lat,
+                      const float *grid_lon, int periodic_lon, const double *xb,
+                      const double *axb, int ps_row, const orc_vcoord *vc,
+                      const int32_t *var_start, const int32_t *var_lev, int nrec, int onz,
+                      int onv, const int32_t *vid, const double *rec_lat,
+                      const double *rec_lon, const double *rec_ph, const double *yo,
+                      double *lat, double *lon, double *yb, double *dep);
 
 /* Post-analysis blend and spreads (run_da_parallel):
  *   mode 0: xa_out = gamma1 (xb + axb) + gamma2 (xa + axa) inside

@@ -27,6 +27,11 @@ class OrcObs(C.Structure):
                 ("yb", C.c_void_p), ("dep", C.c_void_p), ("rinv", C.c_void_p)]
 
 
+class OrcVcoord(C.Structure):
+    _fields_ = [("type", C.c_int32), ("nz", C.c_int32), ("c1", C.c_void_p), ("c2", C.c_void_p),
+                ("ptop", C.c_double), ("p0", C.c_double)]
+
+
 class OrcParams(C.Structure):
     _fields_ = [("k", C.c_int32), ("nrow", C.c_int32), ("nlat", C.c_int32),
                 ("nlon", C.c_int32), ("max_obs", C.c_int32), ("gc_filter", C.c_int32),
@@ -91,6 +96,11 @@ def lib():
                                         _ip, C.c_int]
         L.orc_centre_obs.restype = None
         L.orc_centre_obs.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp]
+        L.orc_obs_operator.restype = None
+        L.orc_obs_operator.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _fp, _fp, C.c_int,
+                                       _dp, _dp, C.c_int, C.POINTER(OrcVcoord), _ip, _ip,
+                                       C.c_int, C.c_int, C.c_int, _ip, _dp, _dp, _dp, _dp,
+                                       _dp, _dp, _dp, _dp]
         L.orc_post.restype = None
         L.orc_post.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _fp, C.c_double,
                                C.c_double, C.c_double, C.c_double, _dp, _dp, _dp, _dp, _dp,
@@ -289,3 +299,35 @@ def post(mode, grid_lat, nlat, nlon, polar_lat, gamma1, gamma2, alpha, xb, axb, 
                    np.ascontiguousarray(xa, np.float64).reshape(-1),
                    np.ascontiguousarray(axa, np.float64).reshape(-1), out, sb, sa)
     return out.reshape(xb.shape), sb.reshape(ncol, nrow), sa.reshape(ncol, nrow)
+
+
+def obs_operator(grid_lat, grid_lon, xb, axb, ps_row, vcoord, var_start, var_lev, vid,
+                 rec_lat, rec_lon, rec_ph, yo, periodic_lon=True):
+    """letkf_obs.F90:120-178 / obs_make.F90:233-381 on host arrays.
+    xb: (ncol, nrow, k) perturbations, axb: (ncol, nrow); vcoord: dict with
+    type (0 sigma, 1 hybrid), c1, c2 (nz + 1 half levels), ptop, p0;
+    rec_ph: (nrec, onz); yo: (nrec, onz, onv).  Returns (lat, lon, yb, dep)
+    per scalar obs in (record, level, variable) order, yb (nobs, k)."""
+    xb = np.ascontiguousarray(xb, np.float64)
+    ncol, nrow, k = xb.shape
+    glat = np.ascontiguousarray(grid_lat, np.float32)
+    glon = np.ascontiguousarray(grid_lon, np.float32)
+    c1 = np.ascontiguousarray(vcoord["c1"], np.float64)
+    c2 = np.ascontiguousarray(vcoord.get("c2", vcoord["c1"]), np.float64)
+    vc = OrcVcoord(int(vcoord["type"]), len(c1) - 1, c1.ctypes.data, c2.ctypes.data,
+                   float(vcoord["ptop"]), float(vcoord.get("p0", 0.0)))
+    rec_ph = np.ascontiguousarray(rec_ph, np.float64)
+    nrec, onz = rec_ph.shape
+    vid = np.ascontiguousarray(vid, np.int32)
+    onv = len(vid)
+    ns = nrec * onz * onv
+    lat, lon, dep = np.zeros(ns), np.zeros(ns), np.zeros(ns)
+    yb = np.zeros(ns * k)
+    lib().orc_obs_operator(k, nrow, len(glat), len(glon), glat, glon, int(bool(periodic_lon)),
+                           xb.reshape(-1), np.ascontiguousarray(axb, np.float64).reshape(-1),
+                           int(ps_row), C.byref(vc), np.ascontiguousarray(var_start, np.int32),
+                           np.ascontiguousarray(var_lev, np.int32), nrec, onz, onv, vid,
+                           np.ascontiguousarray(rec_lat, np.float64),
+                           np.ascontiguousarray(rec_lon, np.float64), rec_ph.reshape(-1),
+                           np.ascontiguousarray(yo, np.float64).reshape(-1), lat, lon, yb, dep)
+    return lat, lon, yb.reshape(ns, k), dep
