@@ -729,3 +729,90 @@ void orc_obs_operator(int k, int nrow, int nlat, int nlon, const float *grid_lat
     free(pos);
     free(ph);
 }
+
+/* letkf_compute_nooverlap_vertical (letkf_compute.F90:358-526) with cube_n = 1. */
+int orc_sweep_vertical(const orc_params *prm, const float *grid_lat, const float *grid_lon,
+                       const orc_obs *obs, const double *obs_ph, const double *xb,
+                       const double *axb, double *xa, double *axa, const orc_bins *bins,
+                       const orc_vcoord *vc, int ps_row, int nvar, const int32_t *var_start,
+                       const int32_t *var_lev, double vert_dist, int cube_nz,
+                       int gc_filter_vertical, int nthreads)
+{
+    const int k = prm->k, n = prm->nrow, nz = vc->nz;
+    const int64_t ncol = (int64_t)prm->nlat * prm->nlon;
+    int nfail = 0;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+#pragma omp parallel reduction(+ : nfail)
+    {
+        int32_t *idx = (int32_t *)malloc(sizeof(int32_t) * (size_t)prm->max_obs);
+        double *rl = (double *)malloc(sizeof(double) * (size_t)prm->max_obs);
+        double *rlv = (double *)malloc(sizeof(double) * (size_t)prm->max_obs);
+        double *olnp = (double *)malloc(sizeof(double) * (size_t)prm->max_obs);
+        double *hx = (double *)malloc(sizeof(double) * (size_t)prm->max_obs * k);
+        double *dp = (double *)malloc(sizeof(double) * (size_t)prm->max_obs);
+        float *ri = (float *)malloc(sizeof(float) * (size_t)prm->max_obs);
+        double *tr = (double *)malloc(sizeof(double) * (size_t)k * k);
+        double *wb = (double *)malloc(sizeof(double) * (size_t)k);
+        double *alnp = (double *)malloc(sizeof(double) * (size_t)(nz > 0 ? nz : 1));
+#pragma omp for schedule(dynamic, 4)
+        for (int64_t col = 0; col < ncol; ++col) {
+            int ilat = (int)(col % prm->nlat), ilon = (int)(col / prm->nlat);
+            double clat = (double)grid_lat[ilat], clon = (double)grid_lon[ilon];
+            int p = bins ? orc_select_column_binned(prm, obs, bins->bin_start, bins->nband,
+                                                    bins->nlonbin, bins->bin_deg, clat,
+                                                    clon, idx, rl)
+                         : orc_select_column(prm, obs, clat, clon, idx, rl);
+            for (int o = 0; o < p; ++o) {
+                memcpy(hx + (size_t)o * k, obs->yb + (size_t)idx[o] * k, sizeof(double) * k);
+                dp[o] = obs->dep[idx[o]];
+                ri[o] = obs->rinv[idx[o]];
+                olnp[o] = log(obs_ph[idx[o]]);
+            }
+            /* ps2alnp from the column's mean surface pressure */
+            const double aps = axb[(size_t)col * n + ps_row];
+            for (int i = 0; i < nz; ++i) {
+                alnp[i] = 0.5 * (ph_half(vc, i, aps) + ph_half(vc, i + 1, aps));
+                alnp[i] = log(alnp[i]);
+            }
+            for (int sk = 0; sk < nz; sk += cube_nz) {
+                int ek = sk + cube_nz - 1;
+                if (ek > nz - 1) ek = nz - 1;
+                const int ck = (sk + 1 + ek + 1) / 2 - 1;
+                for (int o = 0; o < p; ++o) {
+                    const double r = fabs(alnp[ck] - olnp[o]) / vert_dist;
+                    rlv[o] = rl[o] * (gc_filter_vertical ? orc_schr2(r) : orc_schr(r));
+                }
+                if (orc_letkf_core(k, p, hx, ri, dp, rlv, tr, wb, prm->inflation) < 0) ++nfail;
+                for (int v = 0; v < nvar; ++v) {
+                    int r0, r1;
+                    if (var_lev[v] == 1) {
+                        if (ek != nz - 1) continue;
+                        r0 = r1 = var_start[v];
+                    } else {
+                        r0 = var_start[v] + sk;
+                        r1 = var_start[v] + ek;
+                    }
+                    for (int row = r0; row <= r1; ++row)
+                        orc_xb_2_xa(k, 1, xb + ((size_t)col * n + row) * k,
+                                    axb + (size_t)col * n + row, tr,
+                                    xa + ((size_t)col * n + row) * k, axa + (size_t)col * n + row);
+                }
+            }
+        }
+        free(alnp);
+        free(wb);
+        free(tr);
+        free(ri);
+        free(dp);
+        free(hx);
+        free(olnp);
+        free(rlv);
+        free(rl);
+        free(idx);
+    }
+    return nfail;
+}

@@ -14,6 +14,10 @@
  *   orc_sweep         code/x86/TradDA/DA/LETKF/src/letkf_compute.F90:12-126
  *   orc_centre_obs    code/x86/TradDA/DA/LETKF/src/letkf_obs.F90:156-178
  *   orc_post          code/x86/TradDA/DA/EnDNN/src/module_da_parallel.F90:216-259
+ *   orc_sweep_vertical code/x86/TradDA/DA/LETKF/src/letkf_compute.F90:358-526
+ *                     (letkf_compute_nooverlap_vertical), letkf_yb.F90:137-243
+ *                     (gen_yb_dis_vertical, vertical_localization),
+ *                     obs_make.F90:418-437 (ps2alnp)
  *   orc_obs_operator  code/x86/TradDA/DA/LETKF/src/letkf_obs.F90:120-178, 187-243
  *                     (find_obs_grid), obs_make.F90:233-381 (obs_interpolation_3d_2,
  *                     ps2ph_3d_interp), DA/utils/vert_coord_mod.F90:547-563
@@ -146,10 +150,6 @@ int orc_sweep_compact(const orc_params *prm, const float *grid_lat, const float 
                       int32_t ncols, const double *xb, const double *axb, double *xa,
                       double *axa, int32_t *nobsl_out, int nthreads);
 
-/* Observation centring: ybar = mean_e hxb; dep = yo - ybar; yb = hxb - ybar. */
-void orc_centre_obs(int k, int nobs, const double *hxb, const double *yo,
-                    double *yb, double *dep);
-
 /* Vertical coordinate of the model levels: half-level pressure of level
  * j = 0..nz (vert_coord_mod.F90:547-563):
  *   sigma  (type 0): c1[j] (ps - ptop) + ptop
@@ -159,6 +159,29 @@ typedef struct orc_vcoord {
     const double *c1, *c2;
     double ptop, p0;
 } orc_vcoord;
+
+/* Vertical localisation (cube_n = 1): for every column, the horizontal
+ * selection of orc_sweep, then for each vertical cube of cube_nz levels
+ * (sk..ek, the last one shorter) an analysis with rloc multiplied by
+ * schr2 (gc_filter_vertical) or schr of |ln p_c - ln p_obs| / vert_dist,
+ * where p_c is the full-level pressure of level (sk + ek) / 2 (1-based) from
+ * the column's mean surface pressure (axb at ps_row; levels from vc), ln p_obs
+ * = log(obs_ph).  Each cube's transform updates rows sk..ek of every 3-D
+ * variable and, with the last cube, the 2-D variables; the rows are then
+ * averaged and re-centred (xa, axa as orc_sweep).  The reference leaves the
+ * bottom half level of ps2alnp unset (obs_make.F90:429-434); it is computed
+ * here from vc.  bins as orc_sweep.  Returns non-converged eigensolves. */
+int orc_sweep_vertical(const orc_params *prm, const float *grid_lat, const float *grid_lon,
+                       const orc_obs *obs, const double *obs_ph, const double *xb,
+                       const double *axb, double *xa, double *axa, const orc_bins *bins,
+                       const orc_vcoord *vc, int ps_row, int nvar, const int32_t *var_start,
+                       const int32_t *var_lev, double vert_dist, int cube_nz,
+                       int gc_filter_vertical, int nthreads);
+
+/* Observation centring: ybar = mean_e hxb; dep = yo - ybar; yb = hxb - ybar. */
+void orc_centre_obs(int k, int nobs, const double *hxb, const double *yo,
+                    double *yb, double *dep);
+
 
 /* Observation operator H(Xb) and centring.  State xb[(col * nrow + row) * k
  * + e] (perturbations) and axb[col * nrow + row] (mean), col = ilat + nlat

@@ -96,6 +96,11 @@ def lib():
                                         _ip, C.c_int]
         L.orc_centre_obs.restype = None
         L.orc_centre_obs.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp]
+        L.orc_sweep_vertical.restype = C.c_int
+        L.orc_sweep_vertical.argtypes = [C.POINTER(OrcParams), _fp, _fp, C.POINTER(OrcObs), _dp,
+                                         _dp, _dp, _dp, _dp, C.c_void_p, C.POINTER(OrcVcoord),
+                                         C.c_int, C.c_int, _ip, _ip, C.c_double, C.c_int,
+                                         C.c_int, C.c_int]
         L.orc_obs_operator.restype = None
         L.orc_obs_operator.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _fp, _fp, C.c_int,
                                        _dp, _dp, C.c_int, C.POINTER(OrcVcoord), _ip, _ip,
@@ -331,3 +336,34 @@ def obs_operator(grid_lat, grid_lon, xb, axb, ps_row, vcoord, var_start, var_lev
                            np.ascontiguousarray(rec_lon, np.float64), rec_ph.reshape(-1),
                            np.ascontiguousarray(yo, np.float64).reshape(-1), lat, lon, yb, dep)
     return lat, lon, yb.reshape(ns, k), dep
+
+
+def sweep_vertical(prm: OrcParams, grid_lat, grid_lon, obs: Obs, obs_ph, xb, axb, vcoord,
+                   ps_row, var_start, var_lev, vert_dist, cube_nz, gc_filter_vertical=True,
+                   bins=None, nthreads=0):
+    """letkf_compute_nooverlap_vertical (cube_n = 1).  obs_ph: obs pressure
+    in the order of obs; vcoord as obs_operator.  Returns dict(xa, axa, nfail)."""
+    k, n = prm.k, prm.nrow
+    ncol = prm.nlat * prm.nlon
+    xa = np.zeros(ncol * n * k)
+    axa = np.zeros(ncol * n)
+    c1 = np.ascontiguousarray(vcoord["c1"], np.float64)
+    c2 = np.ascontiguousarray(vcoord.get("c2", vcoord["c1"]), np.float64)
+    vc = OrcVcoord(int(vcoord["type"]), len(c1) - 1, c1.ctypes.data, c2.ctypes.data,
+                   float(vcoord["ptop"]), float(vcoord.get("p0", 0.0)))
+    bins_c = None
+    if bins is not None:
+        bin_start, nband, nlonbin, bin_deg = bins
+        bs = np.ascontiguousarray(bin_start, dtype=np.int32)
+        bins_c = OrcBins(bs.ctypes.data, nband, nlonbin, float(bin_deg))
+    vs = np.ascontiguousarray(var_start, np.int32)
+    vl = np.ascontiguousarray(var_lev, np.int32)
+    nfail = lib().orc_sweep_vertical(
+        C.byref(prm), np.ascontiguousarray(grid_lat, np.float32),
+        np.ascontiguousarray(grid_lon, np.float32), C.byref(obs.c),
+        np.ascontiguousarray(obs_ph, np.float64), np.ascontiguousarray(xb, np.float64).reshape(-1),
+        np.ascontiguousarray(axb, np.float64).reshape(-1), xa, axa,
+        None if bins_c is None else C.cast(C.pointer(bins_c), C.c_void_p), C.byref(vc),
+        int(ps_row), len(vs), vs, vl, float(vert_dist), int(cube_nz), int(bool(gc_filter_vertical)),
+        int(nthreads))
+    return {"xa": xa.reshape(ncol, n, k), "axa": axa.reshape(ncol, n), "nfail": nfail}
