@@ -59,6 +59,9 @@ void dida_letkf_params_default(dida_letkf_params *p)
     p->chain_len = 0;
     p->warm_start = 1;
     p->jacobi_predict = -1;
+    p->vert_dist = 0.0;
+    p->cube_nz = 1;
+    p->gc_filter_vertical = 1;
 }
 
 static double jacobi_tol_of(const dida_letkf_params *p)
@@ -95,6 +98,12 @@ static int check_params(const dida_letkf_params *p)
         return set_err(DIDA_ERR_UNSUPPORTED, "gram precision %d is not available", p->gram);
     if (p->form == DIDA_FORM_PLAIN && p->gram != DIDA_GRAM_FP64)
         return set_err(DIDA_ERR_UNSUPPORTED, "the plain form computes the Gram in FP64 only");
+    if (p->vert_dist > 0.0) {
+        if (p->form != DIDA_FORM_BATCHED)
+            return set_err(DIDA_ERR_UNSUPPORTED,
+                           "vertical localisation runs in the batched form only");
+        if (p->cube_nz < 1) return set_err(DIDA_ERR_ARG, "cube_nz must be >= 1");
+    }
     return DIDA_OK;
 }
 
@@ -139,6 +148,12 @@ struct dida_letkf_ctx {
     void *chain_tmp;
     size_t chain_tmp_bytes;
     double *ymax;
+    /* vertical localisation: analyses per column, binned obs ln p, cube
+     * centre ln p [ncol * ncube], rows per cube (CSR over cubes) */
+    int32_t ncube, ps_row;
+    dida_vert_coord vc;
+    double *obs_lnp, *cube_lnp;
+    int32_t *cube_row_off, *cube_rows;
     unsigned long long *phase_clk; /* [DK_NPHASE], allocated when timing is on */
     /* per-stage HIP event timing (eager sweeps only) */
     int timing;
@@ -170,7 +185,8 @@ int dida_letkf_destroy(dida_letkf_ctx *c)
                     c->dep_b, c->rinv_b,
                     c->nobsl, c->offset, c->scan_tmp, c->csr_idx, c->csr_w, c->flags,
                     c->trans, c->info, c->chain_key, c->chain_key_sorted, c->chain_id,
-                    c->chain_order, c->queue_head, c->chain_tmp, c->ymax, c->phase_clk};
+                    c->chain_order, c->queue_head, c->chain_tmp, c->ymax, c->phase_clk,
+                    c->obs_lnp, c->cube_lnp, c->cube_row_off, c->cube_rows};
     for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; ++i) dfree(ptrs[i]);
     free(c);
     return DIDA_OK;
@@ -319,6 +335,7 @@ int dida_letkf_create(dida_letkf_ctx **out, const dida_letkf_params *prm, int32_
             ncu <= 0)
             ncu = 256;
         c->nblocks = ncu * dk_wave_blocks_per_cu(k, prm->gram);
+        if (prm->vert_dist > 0.0) DMALLOC(c->obs_lnp, no * sizeof(double));
     }
     *out = c;
     return DIDA_OK;
@@ -330,6 +347,90 @@ int dida_letkf_set_grid(dida_letkf_ctx *c, const float *grid_lat, const float *g
     c->grid_lat = grid_lat;
     c->grid_lon = grid_lon;
     return DIDA_OK;
+}
+
+int dida_letkf_set_levels(dida_letkf_ctx *c, const dida_vert_coord *vc, int32_t ps_row,
+                          int32_t nvar, const int32_t *var_start, const int32_t *var_lev)
+{
+    if (!c || !vc || !var_start || !var_lev) return set_err(DIDA_ERR_ARG, "null argument");
+    if (!(c->prm.vert_dist > 0.0))
+        return set_err(DIDA_ERR_ARG, "set_levels: vertical localisation is off (vert_dist <= 0)");
+    const int32_t nrow = c->prm.nrow, nz = vc->nz;
+    if (vc->type != DIDA_VCOORD_SIGMA && vc->type != DIDA_VCOORD_HYBRID)
+        return set_err(DIDA_ERR_ARG, "set_levels: vertical coordinate type");
+    if (nz < 1 || !vc->c1 || (vc->type == DIDA_VCOORD_HYBRID && !vc->c2))
+        return set_err(DIDA_ERR_ARG, "set_levels: vertical coordinate");
+    if (ps_row < 0 || ps_row >= nrow) return set_err(DIDA_ERR_ARG, "set_levels: ps_row");
+    if (nvar < 1) return set_err(DIDA_ERR_ARG, "set_levels: no variables");
+    const int32_t cnz = c->prm.cube_nz;
+    const int32_t ncube = (nz + cnz - 1) / cnz;
+    int32_t *seen = (int32_t *)calloc((size_t)nrow, sizeof(int32_t));
+    int32_t *off = (int32_t *)malloc((size_t)(ncube + 1) * sizeof(int32_t));
+    int32_t *rows = (int32_t *)malloc((size_t)nrow * sizeof(int32_t));
+    int rc = DIDA_OK;
+    if (!seen || !off || !rows) rc = set_err(DIDA_ERR_NOMEM, "set_levels: host tables");
+    for (int v = 0; v < nvar && !rc; ++v) {
+        if (var_start[v] < 0 || (var_lev[v] != 1 && var_lev[v] != nz) ||
+            var_start[v] + var_lev[v] > nrow) {
+            rc = set_err(DIDA_ERR_ARG, "set_levels: variable %d rows (levels must be 1 or nz)", v);
+            break;
+        }
+        for (int l = 0; l < var_lev[v]; ++l) seen[var_start[v] + l]++;
+    }
+    for (int r = 0; r < nrow && !rc; ++r)
+        if (seen[r] != 1) rc = set_err(DIDA_ERR_ARG, "set_levels: row %d is in %d variables", r, seen[r]);
+    /* cube q: levels sk..ek of every 3-D variable, then the 2-D variables
+     * with the last cube (letkf_compute.F90:438-520) */
+    int32_t nr = 0;
+    for (int q = 0; q < ncube && !rc; ++q) {
+        const int sk = q * cnz, ek = sk + cnz - 1 < nz - 1 ? sk + cnz - 1 : nz - 1;
+        off[q] = nr;
+        for (int v = 0; v < nvar; ++v) {
+            if (var_lev[v] == 1) {
+                if (q == ncube - 1) rows[nr++] = var_start[v];
+            } else {
+                for (int l = sk; l <= ek; ++l) rows[nr++] = var_start[v] + l;
+            }
+        }
+    }
+    if (!rc) off[ncube] = nr;
+    if (!rc) {
+        dfree(c->cube_row_off);
+        dfree(c->cube_rows);
+        dfree(c->cube_lnp);
+        c->cube_row_off = NULL;
+        c->cube_rows = NULL;
+        c->cube_lnp = NULL;
+        c->ncube = 0;
+        if (hipMalloc((void **)&c->cube_row_off, (size_t)(ncube + 1) * sizeof(int32_t)) ||
+            hipMalloc((void **)&c->cube_rows, (size_t)nrow * sizeof(int32_t)) ||
+            hipMalloc((void **)&c->cube_lnp, (size_t)c->ncol * ncube * sizeof(double)))
+            rc = set_err(DIDA_ERR_NOMEM, "set_levels: device tables");
+        else if (hipMemcpy(c->cube_row_off, off, (size_t)(ncube + 1) * sizeof(int32_t),
+                           hipMemcpyHostToDevice) ||
+                 hipMemcpy(c->cube_rows, rows, (size_t)nrow * sizeof(int32_t),
+                           hipMemcpyHostToDevice))
+            rc = set_err(DIDA_ERR_HIP, "set_levels: copy");
+    }
+    if (!rc) {
+        /* info holds one entry per analysis */
+        int32_t *info = NULL;
+        if (hipMalloc((void **)&info, (size_t)c->ncol * ncube * sizeof(int32_t)))
+            rc = set_err(DIDA_ERR_NOMEM, "set_levels: info");
+        else {
+            dfree(c->info);
+            c->info = info;
+        }
+    }
+    if (!rc) {
+        c->vc = *vc;
+        c->ps_row = ps_row;
+        c->ncube = ncube;
+    }
+    free(seen);
+    free(off);
+    free(rows);
+    return rc;
 }
 
 /* Record a timing mark closing `stage` (-1: the start mark). */
@@ -361,6 +462,7 @@ static int sweep_select_count(dida_letkf_ctx *c, const dida_letkf_obs *o, hipStr
     HIP_TRY(dk_bin_scan(c->geom.nbins, c->bin_count, c->bin_start, s));
     HIP_TRY(dk_obs_gather(o->nobs, k, c->perm, o->lat, o->lon, o->yb, o->dep, o->rinv, c->cx,
                           c->sx, c->yr, c->ux, c->uy, c->uz, c->yb_b, c->dep_b, c->rinv_b, s));
+    if (c->ncube > 0) HIP_TRY(dk_obs_lnp(o->nobs, c->perm, o->ph, c->obs_lnp, s));
     tmark(c, DIDA_STAGE_BIN, s);
     memset(sa, 0, sizeof *sa);
     sa->nlat = c->prm.nlat;
@@ -449,6 +551,31 @@ static int sweep_batched(dida_letkf_ctx *c, const double *xb, const double *axb,
         HIP_TRY(hipMemsetAsync(c->phase_clk, 0, DK_NPHASE * sizeof(unsigned long long), s));
         ba.phase_clk = c->phase_clk;
     }
+    if (c->ncube > 0) {
+        dk_cube_lnp_args la;
+        memset(&la, 0, sizeof la);
+        la.nrow = c->prm.nrow;
+        la.ps_row = c->ps_row;
+        la.nz = c->vc.nz;
+        la.vtype = c->vc.type;
+        la.ncube = c->ncube;
+        la.cube_nz = c->prm.cube_nz;
+        la.ncol = c->ncol;
+        la.c1 = c->vc.c1;
+        la.c2 = c->vc.c2 ? c->vc.c2 : c->vc.c1;
+        la.ptop = c->vc.ptop;
+        la.p0 = c->vc.p0;
+        la.axb = axb;
+        la.cube_lnp = c->cube_lnp;
+        HIP_TRY(dk_cube_lnp(&la, s));
+        ba.ncube = c->ncube;
+        ba.gc_vert = c->prm.gc_filter_vertical ? 1 : 0;
+        ba.vert_dist = c->prm.vert_dist;
+        ba.obs_lnp = c->obs_lnp;
+        ba.cube_lnp = c->cube_lnp;
+        ba.cube_row_off = c->cube_row_off;
+        ba.cube_rows = c->cube_rows;
+    }
     if (c->prm.gram == DIDA_GRAM_FP16X3)
         HIP_TRY(dk_obs_ymax(c->last_nobs_pending, ba.k, c->yb_b, c->rinv_b, c->ymax, s));
     HIP_TRY(dk_chain_keys(&ba, c->chain_key, c->chain_id, s));
@@ -527,6 +654,12 @@ static int check_sweep_args(dida_letkf_ctx *c, const dida_letkf_obs *o, const do
         return set_err(DIDA_ERR_ARG, "nobs = %d outside [0, %d]", o->nobs, c->nobs_max);
     if (o->nobs > 0 && (!o->lat || !o->lon || !o->yb || !o->dep || !o->rinv))
         return set_err(DIDA_ERR_ARG, "null obs array");
+    if (c->prm.vert_dist > 0.0) {
+        if (c->ncube <= 0)
+            return set_err(DIDA_ERR_ARG, "vertical localisation: levels not set (dida_letkf_set_levels)");
+        if (o->nobs > 0 && !o->ph)
+            return set_err(DIDA_ERR_ARG, "vertical localisation: obs ph is null");
+    }
     return DIDA_OK;
 }
 
@@ -827,9 +960,10 @@ int dida_letkf_stats_get(dida_letkf_ctx *c, dida_letkf_stats *st)
     if (!c || !st) return set_err(DIDA_ERR_ARG, "null argument");
     memset(st, 0, sizeof *st);
     HIP_TRY(hipStreamSynchronize(c->last_stream));
-    size_t n = (size_t)c->ncol;
+    const size_t n = (size_t)c->ncol;
+    const size_t nq = c->ncube > 0 ? (size_t)c->ncube : 1; /* analyses per column */
     int32_t *nb = (int32_t *)malloc(n * sizeof(int32_t));
-    int32_t *inf = (int32_t *)malloc(n * sizeof(int32_t));
+    int32_t *inf = (int32_t *)malloc(n * nq * sizeof(int32_t));
     int32_t flags[4];
     if (!nb || !inf) {
         free(nb);
@@ -838,15 +972,16 @@ int dida_letkf_stats_get(dida_letkf_ctx *c, dida_letkf_stats *st)
     }
     int rc = DIDA_OK;
     if (hipMemcpy(nb, c->nobsl, n * sizeof(int32_t), hipMemcpyDeviceToHost) ||
-        hipMemcpy(inf, c->info, n * sizeof(int32_t), hipMemcpyDeviceToHost) ||
+        hipMemcpy(inf, c->info, n * nq * sizeof(int32_t), hipMemcpyDeviceToHost) ||
         hipMemcpy(flags, c->flags, sizeof flags, hipMemcpyDeviceToHost)) {
         rc = set_err(DIDA_ERR_HIP, "stats copy failed");
     } else {
-        st->nanal = (int64_t)n;
-        for (size_t i = 0; i < n; ++i) {
-            st->nobsl_sum += nb[i];
-            if (nb[i] > st->nobsl_max) st->nobsl_max = nb[i];
-            if (nb[i] >= c->prm.max_obs) st->nobsl_capped++;
+        st->nanal = (int64_t)(n * nq);
+        for (size_t i = 0; i < n * nq; ++i) {
+            const int32_t p = nb[i / nq];
+            st->nobsl_sum += p;
+            if (p > st->nobsl_max) st->nobsl_max = p;
+            if (p >= c->prm.max_obs) st->nobsl_capped++;
             if (inf[i] < 0) st->nonconverged++;
             else {
                 st->sweeps_sum += inf[i];
@@ -1044,7 +1179,7 @@ int dida_letkf_obs_operator(int32_t k, int32_t nrow, int32_t nlat, int32_t nlon,
                             const double *xb, const double *axb, int32_t ps_row,
                             const dida_vert_coord *vc, int32_t nvar, const int32_t *var_start,
                             const int32_t *var_lev, const dida_obs_records *rec, double *lat,
-                            double *lon, double *yb, double *dep, void *stream)
+                            double *lon, double *yb, double *dep, double *ph, void *stream)
 {
     if (!vc || !rec || !var_start || !var_lev || !grid_lat || !grid_lon || !xb || !axb)
         return set_err(DIDA_ERR_ARG, "obs operator: null argument");
@@ -1103,6 +1238,7 @@ int dida_letkf_obs_operator(int32_t k, int32_t nrow, int32_t nlat, int32_t nlon,
     a.lon = lon;
     a.yb = yb;
     a.dep = dep;
+    a.ph = ph;
     HIP_TRY(dk_obs_operator(&a, stream));
     return DIDA_OK;
 }

@@ -156,6 +156,15 @@ typedef struct dk_batched_args {
     const int32_t *queue;   /* chain ids in processing order (NULL: identity) */
     int32_t *queue_head;    /* work counter, zero before the launch */
     unsigned long long *phase_clk; /* optional [DK_NPHASE] device clock sums */
+    /* vertical localisation (ncube > 0): one analysis per (column, cube),
+     * analysis id col * ncube + cube; rloc scaled by gc_weight(|cube_lnp -
+     * obs_lnp| / vert_dist, gc_vert); cube q updates rows
+     * cube_rows[cube_row_off[q] .. cube_row_off[q + 1]) */
+    int32_t ncube, gc_vert;
+    double vert_dist;
+    const double *obs_lnp;      /* [binned obs] ln p */
+    const double *cube_lnp;     /* [col * ncube + q] ln p of the cube's centre level */
+    const int32_t *cube_row_off, *cube_rows;
 } dk_batched_args;
 
 /* Phases of the batched kernel timed by phase_clk (wave 0 of each block). */
@@ -176,6 +185,23 @@ int dk_letkf_wave(const dk_batched_args *a, int nblocks, void *stream);
 int dk_obs_ymax(int32_t nobs, int32_t k, const double *yb, const float *rinv, double *ymax,
                 void *stream);
 
+/* Vertical localisation inputs (letkf_compute.F90:424-447, obs_make.F90:418-437):
+ * cube_lnp[col * ncube + q] = ln of the full-level pressure of level
+ * (sk + ek) / 2 (1-based) of cube q = levels sk..ek, from the column's mean
+ * surface pressure axb[col * nrow + ps_row] and the half levels c1/c2. */
+typedef struct dk_cube_lnp_args {
+    int32_t nrow, ps_row, nz, vtype;
+    int32_t ncube, cube_nz;
+    int64_t ncol;
+    const double *c1, *c2;
+    double ptop, p0;
+    const double *axb;
+    double *cube_lnp;
+} dk_cube_lnp_args;
+int dk_cube_lnp(const dk_cube_lnp_args *a, void *stream);
+/* lnp[i] = log(ph[perm[i]]) (obs pressures in bin order). */
+int dk_obs_lnp(int32_t nobs, const int32_t *perm, const double *ph, double *lnp, void *stream);
+
 /* Observation operator H(Xb) and centring, one wavefront per record. */
 #define DK_OBSOP_MAXVAR 32
 typedef struct dk_obsop_args {
@@ -190,6 +216,7 @@ typedef struct dk_obsop_args {
     int32_t vstart[DK_OBSOP_MAXVAR], vlev[DK_OBSOP_MAXVAR], vid[DK_OBSOP_MAXVAR];
     const double *rec_lat, *rec_lon, *rec_ph, *yo;
     double *lat, *lon, *yb, *dep;
+    double *ph;                            /* optional: scalar obs pressure */
 } dk_obsop_args;
 
 int dk_obs_operator(const dk_obsop_args *a, void *stream);

@@ -61,14 +61,18 @@ __device__ __forceinline__ double bracket(const float* g, int n, bool periodic, 
   return (d - x + (double)g[j - 1]) / d;
 }
 
-__device__ __forceinline__ double ph_half(const dk_obsop_args& a, int j, double ps) {
+// half-level pressure j (0-based, top first) over surface pressure ps
+// (vert_coord_mod.F90:547-563); A holds vtype, c1, c2, ptop, p0
+template <class A>
+__device__ __forceinline__ double ph_half(const A& a, int j, double ps) {
 #pragma clang fp contract(off)
   if (a.vtype == 0) return a.c1[j] * (ps - a.ptop) + a.ptop;
   return a.c1[j] * (a.p0 - a.ptop) + a.c2[j] * (ps - a.ptop) + a.ptop;
 }
 
 // full-level pressure j (0-based) of a member with surface pressure ps
-__device__ __forceinline__ double ph_full(const dk_obsop_args& a, int j, double ps) {
+template <class A>
+__device__ __forceinline__ double ph_full(const A& a, int j, double ps) {
 #pragma clang fp contract(off)
   return 0.5 * (ph_half(a, j, ps) + ph_half(a, j + 1, ps));
 }
@@ -89,6 +93,7 @@ __global__ __launch_bounds__(256) void k_obs_operator(dk_obsop_args a) {
   for (int q = lane; q < onz * onv; q += 64) {
     a.lat[s0 + q] = rlat;
     a.lon[s0 + q] = rlon;
+    if (a.ph) a.ph[s0 + q] = a.rec_ph[r * onz + q / onv];
   }
   // member e of row `row` at a corner column, and the bilinear value
   auto X = [&](int64_t col, int row, int e) {
@@ -156,7 +161,44 @@ __global__ __launch_bounds__(256) void k_obs_operator(dk_obsop_args a) {
   }
 }
 
+// ps2alnp (obs_make.F90:418-437) at the centre level of every cube: one
+// thread per (column, cube).  The reference reads the bottom half level
+// ph_lev(nz + 1) without setting it; here it is the surface pressure of the
+// coordinate table, as for every other half level.
+__global__ void k_cube_lnp(dk_cube_lnp_args a) {
+#pragma clang fp contract(off)
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.ncol * a.ncube) return;
+  const int64_t col = t / a.ncube;
+  const int q = (int)(t - col * a.ncube);
+  const int sk = q * a.cube_nz;
+  const int ek = min(sk + a.cube_nz - 1, a.nz - 1);
+  const int ck = (sk + 1 + ek + 1) / 2 - 1;
+  const double ps = a.axb[col * a.nrow + a.ps_row];
+  a.cube_lnp[t] = log(ph_full(a, ck, ps));
+}
+
+__global__ void k_obs_lnp(int32_t nobs, const int32_t* __restrict__ perm,
+                          const double* __restrict__ ph, double* __restrict__ lnp) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nobs) lnp[i] = log(ph[perm[i]]);
+}
+
 }  // namespace
+
+extern "C" int dk_cube_lnp(const dk_cube_lnp_args* a, void* stream) {
+  const int64_t n = a->ncol * a->ncube;
+  if (n <= 0) return 0;
+  k_cube_lnp<<<(unsigned)((n + 255) / 256), 256, 0, (hipStream_t)stream>>>(*a);
+  return (int)hipGetLastError();
+}
+
+extern "C" int dk_obs_lnp(int32_t nobs, const int32_t* perm, const double* ph, double* lnp,
+                          void* stream) {
+  if (nobs <= 0) return 0;
+  k_obs_lnp<<<(nobs + 255) / 256, 256, 0, (hipStream_t)stream>>>(nobs, perm, ph, lnp);
+  return (int)hipGetLastError();
+}
 
 extern "C" int dk_obs_operator(const dk_obsop_args* a, void* stream) {
   if (a->nrec <= 0) return 0;

@@ -163,7 +163,7 @@ __device__ __forceinline__ double rcp_nr1(double x) {
 struct ObsMeta {
   int gi;
   bool ok;
-  double rl, ri, dp;
+  double rl, ri, dp, lp;
   __device__ __forceinline__ void load_index(const dk_batched_args& a, int64_t off, int p,
                                              int base) {
     const int o = base + (int)threadIdx.x;
@@ -172,18 +172,26 @@ struct ObsMeta {
     gi = a.idx[off + oc];
     rl = a.rloc[off + oc];
   }
+  template <bool VERT>
   __device__ __forceinline__ void load_obs(const dk_batched_args& a) {
     ri = (double)a.rinv[gi];
     dp = a.dep[gi];
+    if constexpr (VERT) lp = a.obs_lnp[gi];
   }
   template <int KP>
   __device__ __forceinline__ void store_index(WSmem<KP>& sm, int buf) const {
     sm.ogi[buf][threadIdx.x] = gi;
   }
-  template <int KP>
-  __device__ __forceinline__ void store_weights(WSmem<KP>& sm, int buf) const {
+  // sqrt(R^-1 rloc) and R^-1 rloc d; with vertical localisation rloc is
+  // scaled by the weight of |ln p_cube - ln p_obs| / vert_dist
+  // (letkf_yb.F90:223-243)
+  template <int KP, bool VERT>
+  __device__ __forceinline__ void store_weights(WSmem<KP>& sm, int buf, const dk_batched_args& a,
+                                                double clnp) const {
     const int l = threadIdx.x;
-    const double r2 = ok ? ri * rl : 0.0;
+    double rv = rl;
+    if constexpr (VERT) rv = rl * gc_weight(fabs(clnp - lp) / a.vert_dist, a.gc_vert);
+    const double r2 = ok ? ri * rv : 0.0;
     const double rr = sqrt(r2);
     sm.orr[buf][l] = rr;
     sm.ow[buf][l] = rr * (rr * dp);
@@ -281,7 +289,9 @@ struct Wave {
   // two 32-obs halves.  Rows past k read row k - 1 (in bounds) and are masked
   // once at the end (a row of Yhat only reaches the row and column of C it
   // indexes).
-  __device__ void gram(const dk_batched_args& a, int64_t off, int p, WSmem<KP>& sm) {
+  template <bool VERT>
+  __device__ void gram(const dk_batched_args& a, int64_t off, int p, double clnp,
+                       WSmem<KP>& sm) {
     const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
     const int k = a.k;
     constexpr bool kF64 = GRAM == DK_GRAM_FP64;
@@ -294,7 +304,10 @@ struct Wave {
       double mx = 0.0;
       for (int o = lane; o < p; o += 64) {
         const int gi = a.idx[off + o];
-        mx = fmax(mx, a.ymax[gi] * sqrt(a.rloc[off + o]));
+        double rv = a.rloc[off + o];
+        if constexpr (VERT)
+          rv *= gc_weight(fabs(clnp - a.obs_lnp[gi]) / a.vert_dist, a.gc_vert);
+        mx = fmax(mx, a.ymax[gi] * sqrt(rv));
       }
       for (int s = 32; s >= 1; s >>= 1) mx = fmax(mx, __shfl_xor(mx, s, kWave));
       if (mx > 0.0) {
@@ -323,9 +336,9 @@ struct Wave {
       // staged in the other LDS buffer during the current block
       ObsMeta m;
       m.load_index(a, off, p, 0);
-      m.load_obs(a);
+      m.load_obs<VERT>(a);
       m.store_index<KP>(sm, 0);
-      m.store_weights<KP>(sm, 0);
+      m.store_weights<KP, VERT>(sm, 0, a, clnp);
       __syncthreads();
       double rawA[8][NJ], rawB[8][NJ];
       load_rows(a, sm, 0, 0, rawA);
@@ -336,13 +349,13 @@ struct Wave {
         load_rows(a, sm, buf, 1, rawB);
         __builtin_amdgcn_sched_barrier(0);
         gram_half(sm, buf, 0, rawA, scale, bacc, accd, acch);
-        n.load_obs(a);
+        n.load_obs<VERT>(a);
         n.store_index<KP>(sm, buf ^ 1);
         __syncthreads();
         load_rows(a, sm, buf ^ 1, 0, rawA);
         __builtin_amdgcn_sched_barrier(0);
         if (base + 32 < p) gram_half(sm, buf, 1, rawB, scale, bacc, accd, acch);
-        n.store_weights<KP>(sm, buf ^ 1);
+        n.store_weights<KP, VERT>(sm, buf ^ 1, a, clnp);
         __syncthreads();
       }
     }
@@ -687,9 +700,9 @@ struct Wave {
   }
 };
 
-template <int KP, int GRAM>
-__device__ void wave_column(const dk_batched_args& a, int64_t col, bool warm, WSmem<KP>& sm,
-                            Wave<KP, GRAM>& w) {
+template <int KP, int GRAM, bool VERT>
+__device__ void wave_column(const dk_batched_args& a, int64_t col, int cube, bool warm,
+                            WSmem<KP>& sm, Wave<KP, GRAM>& w) {
   constexpr int NJ = Wave<KP, GRAM>::NJ;
   constexpr int NQ = Wave<KP, GRAM>::NQ;
   constexpr int LDV = WSmem<KP>::LDV;
@@ -700,8 +713,11 @@ __device__ void wave_column(const dk_batched_args& a, int64_t col, bool warm, WS
   int p = a.nobsl[col];
   p = p < 0 ? 0 : (p > a.max_p ? a.max_p : p);
   if (off < 0 || off + p > a.capacity) p = 0;
+  // analysis id; the cube's centre ln p with vertical localisation
+  const int64_t aid = VERT ? col * a.ncube + cube : col;
+  const double clnp = VERT ? a.cube_lnp[aid] : 0.0;
 
-  w.gram(a, off, p, sm);
+  w.template gram<VERT>(a, off, p, clnp, sm);
   const double km1 = (double)(k - 1);
   w.shift(k, km1 / a.rho);
   clk.mark(DK_PHASE_GRAM);
@@ -765,27 +781,32 @@ __device__ void wave_column(const dk_batched_args& a, int64_t col, bool warm, WS
     for (int I2 = 0; I2 < NJ; ++I2)
 #pragma unroll
       for (int r = 0; r < 4; ++r) T[I][I2][r] += wb[I][r];
-  if (a.info && lane == 0) a.info[col] = conv ? sweeps : -1;
+  if (a.info && lane == 0) a.info[aid] = conv ? sweeps : -1;
   if (a.wbar_out && c == 0)
 #pragma unroll
     for (int I = 0; I < NJ; ++I)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int i = rowB(I, r, g);
-        if (i < k) a.wbar_out[col * k + i] = wb[I][r];
+        if (i < k) a.wbar_out[aid * k + i] = wb[I][r];
       }
   clk.mark(DK_PHASE_TRANSFORM);
 
-  // update, 16 state rows at a time: D_E[4r' + g][c] = Xa(E*16 + 4r' + g, R0 + c)
+  // update, 16 state rows at a time: D_E[4r' + g][c] = Xa(E*16 + 4r' + g, row
+  // R0 + c of the analysis); an analysis updates all nrow rows of its column,
+  // or with vertical localisation the cube's list of rows
   const int nrow = a.nrow;
   const double* xbc = a.xb + col * (int64_t)nrow * k;
   double* xac = a.xa + col * (int64_t)nrow * k;
+  const int32_t* rmap = VERT ? a.cube_rows + a.cube_row_off[cube] : nullptr;
+  const int nr = VERT ? a.cube_row_off[cube + 1] - a.cube_row_off[cube] : nrow;
+  auto phys = [&](int r) { return VERT ? rmap[r] : r; };
   // Xb' of rows R0 + c (clamped to the last row, in bounds), members of the
   // B-operand pattern; the next block's loads are issued before a block
   // computes
   auto load_xb = [&](int R0, double (&xl)[NJ][4]) {
     const int xr = R0 + c;
-    const double* xrow = xbc + (int64_t)(xr < nrow ? xr : nrow - 1) * k;
+    const double* xrow = xbc + (int64_t)phys(xr < nr ? xr : nr - 1) * k;
 #pragma unroll
     for (int I = 0; I < NJ; ++I)
 #pragma unroll
@@ -795,10 +816,11 @@ __device__ void wave_column(const dk_batched_args& a, int64_t col, bool warm, WS
       }
   };
   double xl[NJ][4];
-  load_xb(0, xl);
-  for (int R0 = 0; R0 < nrow; R0 += 16) {
+  if (nr > 0) load_xb(0, xl);
+  for (int R0 = 0; R0 < nr; R0 += 16) {
     const int xr = R0 + c;
-    const bool rok = xr < nrow;
+    const bool rok = xr < nr;
+    const int pr = phys(rok ? xr : nr - 1);
     double4_t D[NJ];
 #pragma unroll
     for (int E = 0; E < NJ; ++E) D[E] = double4_t{0.0, 0.0, 0.0, 0.0};
@@ -816,7 +838,7 @@ __device__ void wave_column(const dk_batched_args& a, int64_t col, bool warm, WS
 #pragma unroll
         for (int E = 0; E < NJ; ++E) D[E] = mfma_f64_16x16x4(T[I][E][r], xv, D[E]);
       }
-    const double xbar = a.axb[col * nrow + (rok ? xr : nrow - 1)];
+    const double xbar = a.axb[col * nrow + pr];
     double s = 0.0;
 #pragma unroll
     for (int E = 0; E < NJ; ++E)
@@ -848,9 +870,9 @@ __device__ void wave_column(const dk_batched_args& a, int64_t col, bool warm, WS
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int e = rowB(E, r, g);
-          if (e < k) xac[(int64_t)xr * k + e] = D[E][r] * fac;
+          if (e < k) xac[(int64_t)pr * k + e] = D[E][r] * fac;
         }
-      if (g == 0) a.axa[col * nrow + xr] = mean;
+      if (g == 0) a.axa[col * nrow + pr] = mean;
     }
 #pragma unroll
     for (int I = 0; I < NJ; ++I)
@@ -861,7 +883,7 @@ __device__ void wave_column(const dk_batched_args& a, int64_t col, bool warm, WS
   (void)NQ;
 }
 
-template <int KP, int GRAM>
+template <int KP, int GRAM, bool VERT>
 __global__ __launch_bounds__(64, 1) void k_letkf_wave(dk_batched_args a) {
   __shared__ WSmem<KP> sm;
   Wave<KP, GRAM> w;
@@ -877,14 +899,25 @@ __global__ __launch_bounds__(64, 1) void k_letkf_wave(dk_batched_args a) {
     const int lat1 = min(lat0 + a.chain_len, a.nlat);
     for (int ilat = lat0; ilat < lat1; ++ilat) {
       const int64_t col = (int64_t)ilon * a.nlat + ilat;
-      wave_column<KP, GRAM>(a, col, a.warm_start && ilat > lat0, sm, w);
+      if constexpr (VERT) {
+        // the cubes of a column in alternating order, so that every analysis
+        // warm-starts from the adjacent cube of its column or the same cube
+        // of the previous column
+        const bool down = ((ilat - lat0) & 1) != 0;
+        for (int t = 0; t < a.ncube; ++t)
+          wave_column<KP, GRAM, true>(a, col, down ? a.ncube - 1 - t : t,
+                                      a.warm_start && (ilat > lat0 || t > 0), sm, w);
+      } else {
+        wave_column<KP, GRAM, false>(a, col, 0, a.warm_start && ilat > lat0, sm, w);
+      }
     }
   }
 }
 
 // Sort key of chain c: 2^32 - 1 - cost, so that an ascending radix sort puts
 // the most expensive chains first.  Cost of a column ~ its local obs count
-// (Gram) + 10 k (eigensolve, transform and update).
+// (Gram) + 10 k (eigensolve, transform and update), times the number of
+// vertical cubes.
 __global__ void k_chain_keys(dk_batched_args a, uint32_t* key, int32_t* id) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= a.nchain) return;
@@ -894,20 +927,23 @@ __global__ void k_chain_keys(dk_batched_args a, uint32_t* key, int32_t* id) {
   double cost = 0.0;
   for (int ilat = lat0; ilat < lat1; ++ilat)
     cost += (double)a.nobsl[(int64_t)ilon * a.nlat + ilat] + 10.0 * a.k;
+  if (a.ncube > 0) cost *= a.ncube;
   key[c] = 0xFFFFFFFFu - (uint32_t)fmin(cost, 4.0e9);
   id[c] = c;
 }
 
 template <int KP, int GRAM>
 int wlaunch(const dk_batched_args* a, int nblocks, hipStream_t s) {
-  k_letkf_wave<KP, GRAM><<<nblocks, 64, 0, s>>>(*a);
+  if (a->ncube > 0) k_letkf_wave<KP, GRAM, true><<<nblocks, 64, 0, s>>>(*a);
+  else k_letkf_wave<KP, GRAM, false><<<nblocks, 64, 0, s>>>(*a);
   return (int)hipGetLastError();
 }
 
 template <int KP, int GRAM>
 int wocc() {
   int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_letkf_wave<KP, GRAM>, 64, 0)) return 1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_letkf_wave<KP, GRAM, false>, 64, 0))
+    return 1;
   return nb > 0 ? nb : 1;
 }
 

@@ -31,7 +31,7 @@ EXPORTED = ["dida_letkf_params_default", "letkf_core_", "letkf_core_batch_",
             "dida_letkf_post", "dida_letkf_last_error", "dida_letkf_set_timing",
             "dida_letkf_stage_ms", "dida_letkf_phase_cycles", "dida_letkf_group_capture",
             "dida_letkf_group_launch", "dida_letkf_group_destroy", "dida_letkf_autotune",
-            "dida_letkf_obs_operator"]
+            "dida_letkf_obs_operator", "dida_letkf_set_levels"]
 PHASES = ["gram", "warm", "jacobi", "transform", "update"]
 STAGES = ["bin", "select", "transform", "update"]
 
@@ -44,7 +44,8 @@ class Params(C.Structure):
                 ("rtps_alpha", C.c_double), ("gram", C.c_int32),
                 ("jacobi_max_sweeps", C.c_int32), ("jacobi_tol", C.c_double),
                 ("bin_deg", C.c_double), ("chain_len", C.c_int32), ("warm_start", C.c_int32),
-                ("jacobi_predict", C.c_int32)]
+                ("jacobi_predict", C.c_int32), ("vert_dist", C.c_double),
+                ("cube_nz", C.c_int32), ("gc_filter_vertical", C.c_int32)]
 
 
 class TuningC(C.Structure):
@@ -66,7 +67,7 @@ class ObsRecordsC(C.Structure):
 class ObsC(C.Structure):
     _fields_ = [("nobs", C.c_int32), ("pad_", C.c_int32), ("lat", C.c_void_p),
                 ("lon", C.c_void_p), ("yb", C.c_void_p), ("dep", C.c_void_p),
-                ("rinv", C.c_void_p)]
+                ("rinv", C.c_void_p), ("ph", C.c_void_p)]
 
 
 class StatsC(C.Structure):
@@ -127,7 +128,9 @@ def lib():
         L.dida_letkf_obs_operator.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, vp, vp,
                                               C.c_int32, vp, vp, C.c_int32,
                                               C.POINTER(VertCoordC), C.c_int32, vp, vp,
-                                              C.POINTER(ObsRecordsC), vp, vp, vp, vp, vp]
+                                              C.POINTER(ObsRecordsC), vp, vp, vp, vp, vp, vp]
+        L.dida_letkf_set_levels.argtypes = [vp, C.POINTER(VertCoordC), C.c_int32, C.c_int32,
+                                            vp, vp]
         L.dida_letkf_autotune.argtypes = [vp, C.POINTER(ObsC), vp, vp, vp, vp, vp,
                                           C.POINTER(TuningC)]
         L.dida_letkf_group_destroy.argtypes = [vp]
@@ -209,10 +212,12 @@ class DeviceObs:
     yb: object      # (nobs, k) float64
     dep: object
     rinv: object    # float32
+    ph: object = None   # float64 obs pressure (vertical localisation only)
 
     def c(self) -> ObsC:
         return ObsC(int(self.lat.numel()), 0, self.lat.data_ptr(), self.lon.data_ptr(),
-                    self.yb.data_ptr(), self.dep.data_ptr(), self.rinv.data_ptr())
+                    self.yb.data_ptr(), self.dep.data_ptr(), self.rinv.data_ptr(),
+                    None if self.ph is None else self.ph.data_ptr())
 
 
 class Group:
@@ -271,6 +276,19 @@ class Engine:
     def set_grid(self, grid_lat, grid_lon):
         self._keep = [grid_lat, grid_lon]
         check(lib().dida_letkf_set_grid(self.h, _ptr(grid_lat), _ptr(grid_lon)), "set_grid")
+
+    def set_levels(self, vcoord, ps_row, var_start, var_lev):
+        """dida_letkf_set_levels (vertical localisation): vcoord dict(type, c1,
+        c2, ptop, p0) with c1/c2 device float64 tensors of nz + 1 half levels."""
+        c1 = vcoord["c1"]
+        c2 = vcoord.get("c2", c1)
+        self._keep_levels = (c1, c2)
+        vc = VertCoordC(int(vcoord["type"]), int(c1.numel()) - 1, c1.data_ptr(), c2.data_ptr(),
+                        float(vcoord["ptop"]), float(vcoord.get("p0", 0.0)))
+        vs = np.ascontiguousarray(var_start, np.int32)
+        vl = np.ascontiguousarray(var_lev, np.int32)
+        check(lib().dida_letkf_set_levels(self.h, C.byref(vc), int(ps_row), len(vs),
+                                          vs.ctypes.data, vl.ctypes.data), "set_levels")
 
     def sweep(self, obs: DeviceObs, xb, axb, xa, axa, stream=None) -> int:
         oc = obs.c()
@@ -387,11 +405,12 @@ def post(k, nrow, nlat, nlon, grid_lat, polar_lat, gamma1, gamma2, xb, axb, xa, 
 
 
 def obs_operator(grid_lat, grid_lon, xb, axb, ps_row, vcoord, var_start, var_lev, vid,
-                 rec_lat, rec_lon, rec_ph, yo, periodic_lon=True, stream=None):
+                 rec_lat, rec_lon, rec_ph, yo, periodic_lon=True, stream=None, with_ph=False):
     """dida_letkf_obs_operator on device tensors.  xb (ncol, nrow, k), axb
     (ncol, nrow); vcoord: dict(type, c1, c2, ptop, p0) with c1/c2 device
     float64 tensors; rec_ph (nrec, onz); yo (nrec, onz, onv).  Returns
-    DeviceObs-ready arrays (lat, lon, yb (nobs, k), dep)."""
+    DeviceObs-ready arrays (lat, lon, yb (nobs, k), dep), and the scalar obs
+    pressures after them with with_ph."""
     import torch
     ncol, nrow, k = xb.shape
     nrec, onz = rec_ph.shape
@@ -402,6 +421,7 @@ def obs_operator(grid_lat, grid_lon, xb, axb, ps_row, vcoord, var_start, var_lev
     lon = torch.empty(ns, dtype=torch.float64, device=dev)
     yb = torch.empty(ns, k, dtype=torch.float64, device=dev)
     dep = torch.empty(ns, dtype=torch.float64, device=dev)
+    ph = torch.empty(ns, dtype=torch.float64, device=dev) if with_ph else None
     c1 = vcoord["c1"]
     c2 = vcoord.get("c2", c1)
     vc = VertCoordC(int(vcoord["type"]), int(c1.numel()) - 1, c1.data_ptr(), c2.data_ptr(),
@@ -415,9 +435,10 @@ def obs_operator(grid_lat, grid_lon, xb, axb, ps_row, vcoord, var_start, var_lev
                                         _ptr(grid_lat), _ptr(grid_lon), int(bool(periodic_lon)),
                                         _ptr(xb), _ptr(axb), int(ps_row), C.byref(vc), len(vs),
                                         vs.ctypes.data, vl.ctypes.data, C.byref(rc), _ptr(lat),
-                                        _ptr(lon), _ptr(yb), _ptr(dep), _stream_ptr(stream)),
+                                        _ptr(lon), _ptr(yb), _ptr(dep),
+                                        None if ph is None else _ptr(ph), _stream_ptr(stream)),
           "obs_operator")
-    return lat, lon, yb, dep
+    return (lat, lon, yb, dep, ph) if with_ph else (lat, lon, yb, dep)
 
 
 # ----------------------------------------------------------------------------

@@ -73,6 +73,14 @@ typedef struct dida_letkf_params {
                                    Gram, 0 with the FP64 Gram, whose 1e-10 bar
                                    the prediction can miss on cold starts with
                                    clustered eigenvalues */
+    /* Vertical localisation (letkf_compute_nooverlap_vertical,
+     * letkf_compute.F90:358-526; letkf.F90:228 picks it when vert_dist > 0):
+     * one analysis per (column, cube of cube_nz levels), rloc scaled by
+     * schr2 (gc_filter_vertical 1) or schr (0) of |ln p_cube - ln p_obs| /
+     * vert_dist.  Batched form only; needs dida_letkf_set_levels and obs ph. */
+    double vert_dist;           /* da_vert_distance, in ln p; <= 0: off (default) */
+    int32_t cube_nz;            /* da_vertical_interval, levels per cube (default 1) */
+    int32_t gc_filter_vertical; /* da_gc_enable_vertical (default 1) */
 } dida_letkf_params;
 
 /* Observations, device pointers, one entry per scalar observation (the
@@ -85,6 +93,8 @@ typedef struct dida_letkf_obs {
     const double *yb;           /* [nobs * k] H(Xb) - mean, ensemble fastest (pro_yb) */
     const double *dep;          /* [nobs] yo - mean H(Xb) (dis_yo) */
     const float *rinv;          /* [nobs] R^-1, REAL(4) as in the reference */
+    const double *ph;           /* [nobs] obs pressure (Pa), read only with vertical
+                                   localisation (obs_list ph, read_obs_mod.F90:15-27) */
 } dida_letkf_obs;
 
 /* Diagnostics of the last sweep (dida_letkf_stats synchronises). */
@@ -273,7 +283,8 @@ typedef struct dida_obs_records {
 
 /* H(Xb) for every member and the centring: outputs the analysis's scalar obs
  * s = (r * onz + l) * onv + v: lat[s], lon[s], yb[s * k + e] = H(x_e) -
- * mean_e H(x_e) (pro_yb), dep[s] = yo[s] - mean_e H(x_e) (dis_yo); R^-1 of
+ * mean_e H(x_e) (pro_yb), dep[s] = yo[s] - mean_e H(x_e) (dis_yo), and, if ph is not NULL,
+ * ph[s] = the record's pressure at level l (for vertical localisation); R^-1 of
  * the same scalars is the caller's.  State as for dida_letkf_sweep (Xb' and
  * xbar_b, device); state variable v occupies rows var_start[v] ..
  * var_start[v] + var_lev[v] - 1 (var_lev 1: a 2-D variable; host arrays,
@@ -286,7 +297,21 @@ int dida_letkf_obs_operator(int32_t k, int32_t nrow, int32_t nlat, int32_t nlon,
                             const double *xb, const double *axb, int32_t ps_row,
                             const dida_vert_coord *vc, int32_t nvar, const int32_t *var_start,
                             const int32_t *var_lev, const dida_obs_records *rec, double *lat,
-                            double *lon, double *yb, double *dep, void *stream);
+                            double *lon, double *yb, double *dep, double *ph, void *stream);
+
+/* Model levels for vertical localisation (params vert_dist > 0): the
+ * coordinate table (device arrays, kept by reference), the surface-pressure
+ * row and the variable layout (host arrays, copied): variable v occupies rows
+ * var_start[v] .. var_start[v] + var_lev[v] - 1, var_lev[v] = 1 (2-D) or
+ * vc->nz (3-D), every row of the column exactly once (var_info.F90:22).
+ * Cube q = levels sk..ek (sk = q cube_nz) updates rows var_start + sk..ek of
+ * every 3-D variable; the last cube also updates the 2-D variables
+ * (letkf_compute.F90:438-520).  The cube's ln p is that of full level
+ * (sk + ek) / 2 (1-based) over the column's mean surface pressure
+ * (ps2alnp, obs_make.F90:418-437, whose unset bottom half level ph_lev(nz+1)
+ * is here taken from the table).  Call before the first sweep. */
+int dida_letkf_set_levels(dida_letkf_ctx *ctx, const dida_vert_coord *vc, int32_t ps_row,
+                          int32_t nvar, const int32_t *var_start, const int32_t *var_lev);
 
 /* ------------------------------------------------------------------------ */
 /* Post-analysis (run_da_parallel, module_da_parallel.F90:216-259): inside

@@ -56,3 +56,46 @@ def test_params_default_and_errors(dl):
                             gram=dl.GRAM_BF16X3)
     rc = dl.lib().dida_letkf_create(ctypes.byref(h), ctypes.byref(bad), 10, 0)
     assert rc == -5 and b"batched" in dl.lib().dida_letkf_last_error()
+
+
+def test_struct_layouts_match_header(dl, tmp_path):
+    """The ctypes mirrors have the size and field offsets of the C structs."""
+    import shutil
+    import subprocess
+    import pytest
+    if not shutil.which("gcc"):
+        pytest.skip("gcc not available")
+    pairs = {"dida_letkf_params": dl.Params, "dida_letkf_obs": dl.ObsC,
+             "dida_letkf_stats": dl.StatsC, "dida_letkf_tuning": dl.TuningC,
+             "dida_vert_coord": dl.VertCoordC, "dida_obs_records": dl.ObsRecordsC}
+    lines = ['#include <stdio.h>', '#include "dida_letkf.h"', "int main(void) {"]
+    for cname, py in pairs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o",
+                    str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    for line in out.splitlines():
+        cname, field, val = line.split()
+        py = pairs[cname]
+        want = ctypes.sizeof(py) if field == "size" else getattr(py, field).offset
+        assert int(val) == want, (cname, field, int(val), want)
+
+
+def test_vertical_params(dl):
+    p = dl.default_params()
+    assert p.vert_dist == 0.0 and p.cube_nz == 1 and p.gc_filter_vertical == 1
+    h = ctypes.c_void_p()
+    # vertical localisation: batched form only, cube_nz >= 1
+    bad = dl.default_params(nanals=8, nrow=3, nlat=2, nlon=2, vert_dist=1.0)
+    rc = dl.lib().dida_letkf_create(ctypes.byref(h), ctypes.byref(bad), 10, 0)
+    assert rc == -5 and b"batched" in dl.lib().dida_letkf_last_error()
+    bad = dl.default_params(nanals=8, nrow=3, nlat=2, nlon=2, vert_dist=1.0, cube_nz=0,
+                            form=dl.FORM_BATCHED)
+    rc = dl.lib().dida_letkf_create(ctypes.byref(h), ctypes.byref(bad), 10, 0)
+    assert rc < 0 and b"cube_nz" in dl.lib().dida_letkf_last_error()
