@@ -202,6 +202,58 @@ def time_obs_operator(dl, torch, data, cfg, reps=5):
             "algorithmic_GBps": nbytes / (ms / 1e3) / 1e9}
 
 
+def time_vertical(dl, torch, data, cfg, prm, layout, steps=2, cube_nz=1, vert_dist=0.5):
+    """Vertical localisation on the same workload: one analysis per (column,
+    cube of cube_nz levels), weights scaled by Gaspari-Cohn in ln p with
+    support 2 vert_dist; obs pressures uniform in 100-1000 hPa, sigma levels
+    over the synthetic surface pressure (hPa).  The split-BF16 batched form
+    is timed (HIP events around each sweep) and compared with the FP64
+    batched form's output (per-variable increment error)."""
+    nz = cfg.nz
+    dev = data["xb"].device
+    rng = torch.Generator(device="cpu").manual_seed(11)
+    ph = (100.0 + 900.0 * torch.rand(cfg.nobs, generator=rng, dtype=torch.float64)).to(dev)
+    vc = {"type": 0, "c1": torch.linspace(0.0, 1.0, nz + 1, dtype=torch.float64, device=dev),
+          "ptop": 10.0}
+    vs, vl = [0, 1, 1 + nz, 1 + 2 * nz], [1, nz, nz, nz]
+    obs = dl.DeviceObs(data["lat"], data["lon"], data["yb"], data["dep"], data["rinv"], ph)
+    extra = dict(vert_dist=vert_dist, cube_nz=cube_nz, gc_filter_vertical=1)
+    if layout:
+        extra.update(chain_len=int(layout["chain_len"]), bin_deg=float(layout["bin_deg"]))
+    outs, res = {}, {}
+    for name, gram, nstep in (("batched_bf16x3", dl.GRAM_BF16X3, steps),
+                              ("batched_fp64", dl.GRAM_FP64, 1)):
+        eng, p = make_engine(dl, cfg, dict(prm, **extra), dl.FORM_BATCHED, gram, cfg.nobs)
+        eng.set_grid(data["grid_lat"], data["grid_lon"])
+        eng.set_levels(vc, 0, vs, vl)
+        xa, axa = torch.empty_like(data["xb"]), torch.empty_like(data["axb"])
+        eng.sweep(obs, data["xb"], data["axb"], xa, axa)
+        st = eng.stats()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(nstep):
+            eng.sweep(obs, data["xb"], data["axb"], xa, axa)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / nstep
+        eng.close()
+        outs[name] = (xa, axa)
+        res[name] = {"ms_per_sweep": ms, "analyses": st["nanal"],
+                     "analyses_per_s": st["nanal"] / (ms / 1e3),
+                     "columns_per_s": cfg.nlat * cfg.nlon / (ms / 1e3),
+                     "jacobi_mean_sweeps": st["sweeps_sum"] / max(1, st["nanal"]),
+                     "nonconverged": st["nonconverged"]}
+        log(f"  vertical {name}: {res[name]}")
+    ref = outs["batched_fp64"]
+    r = {"xa": outs["batched_bf16x3"][0], "axa": outs["batched_bf16x3"][1]}
+    res["batched_bf16x3"]["rel_err_vs_batched_fp64"] = rel_errors(ref, r, data, cfg)
+    del outs, r, ref
+    torch.cuda.empty_cache()
+    return {"vert_dist": vert_dist, "cube_nz": cube_nz, "cubes_per_column": -(-nz // cube_nz),
+            "forms": res}
+
+
 def copy_probe(torch, nbytes=1 << 30, reps=3):
     """A plain device copy of known size (the FETCH_SIZE / WRITE_SIZE scale)."""
     src = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda").fill_(1.0)
@@ -263,6 +315,8 @@ def main():
                     help="skip the autotuner (batched forms keep the default layout)")
     ap.add_argument("--no-plain", action="store_true",
                     help="skip the plain FP64 form (profiling runs)")
+    ap.add_argument("--no-vertical", action="store_true",
+                    help="skip the vertical-localisation timing")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device copy (counter scale for profiles)")
     args = ap.parse_args()
@@ -346,6 +400,11 @@ def main():
                            if rr["stages"]["update"] > 0 else None,
                            "selection_GBps": selection_bytes(cfg.k, cfg.nobs, rr["stats"]["nobsl_sum"])
                            / ((rr["stages"]["bin"] + rr["stages"]["select"]) / 1e3) / 1e9}
+    vertical = None
+    if not args.no_vertical:
+        log("timing vertical localisation")
+        vertical = time_vertical(dl, torch, data, cfg, prm,
+                                 res.get("batched_bf16x3", {}).get("tuning"))
     log("timing the observation operator")
     obsop = time_obs_operator(dl, torch, data, cfg)
     log(f"  {obsop}")
@@ -369,7 +428,7 @@ def main():
                    "rel_increment_error_vs_fp64": r["err"] or None,
                    "speedup_vs_plain_fp64": plain_ms / r["ms"] if plain_ms else None},
         "roofline": roofline, "cpu_baseline": cpu, "forms": out_forms,
-        "obs_operator": obsop,
+        "obs_operator": obsop, "vertical_localisation": vertical,
     }
     print(json.dumps(line), flush=True)
 

@@ -79,8 +79,9 @@ typedef struct dida_letkf_params {
      * schr2 (gc_filter_vertical 1) or schr (0) of |ln p_cube - ln p_obs| /
      * vert_dist.  Batched form only; needs dida_letkf_set_levels and obs ph. */
     double vert_dist;           /* da_vert_distance, in ln p; <= 0: off (default) */
-    int32_t cube_nz;            /* da_vertical_interval, levels per cube (default 1) */
-    int32_t gc_filter_vertical; /* da_gc_enable_vertical (default 1) */
+    int32_t cube_nz;            /* da_vertical_interval, levels per cube (default 1,
+                                   as namelists.input_100km_32mem:69) */
+    int32_t gc_filter_vertical; /* da_gc_enable_vertical (default 1, as :67) */
 } dida_letkf_params;
 
 /* Observations, device pointers, one entry per scalar observation (the
