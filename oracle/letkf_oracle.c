@@ -500,6 +500,117 @@ int orc_sweep(const orc_params *prm, const float *grid_lat,
     return nfail;
 }
 
+int orc_sweep_cubes(const orc_params *prm, const float *grid_lat, const float *grid_lon,
+                    const orc_obs *obs, const double *xb, const double *axb, double *xa,
+                    double *axa, const orc_bins *bins, int cube_n, int overlap,
+                    double *trans_out, int nthreads)
+{
+    const int k = prm->k, n = prm->nrow, nlat = prm->nlat, nlon = prm->nlon;
+    const int step = overlap > 0 ? cube_n - overlap : cube_n;
+    if (cube_n < 1 || overlap < 0 || step < 1) return -1;
+    const int nalat = (nlat + step - 1) / step, nalon = (nlon + step - 1) / step;
+    const int64_t nanal = (int64_t)nalat * nalon;
+    double *trans = trans_out ? trans_out
+                              : (double *)malloc(sizeof(double) * (size_t)nanal * k * k);
+    if (!trans) return -1;
+    int nfail = 0;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+    /* one analysis per cube (gen_yb_dis + letkf_core) */
+#pragma omp parallel reduction(+ : nfail)
+    {
+        int32_t *idx = (int32_t *)malloc(sizeof(int32_t) * (size_t)prm->max_obs);
+        double *rl = (double *)malloc(sizeof(double) * (size_t)prm->max_obs);
+        double *hx = (double *)malloc(sizeof(double) * (size_t)prm->max_obs * k);
+        double *dp = (double *)malloc(sizeof(double) * (size_t)prm->max_obs);
+        float *ri = (float *)malloc(sizeof(float) * (size_t)prm->max_obs);
+        double *wb = (double *)malloc(sizeof(double) * (size_t)k);
+#pragma omp for schedule(dynamic, 4)
+        for (int64_t a = 0; a < nanal; ++a) {
+            const int aj = (int)(a % nalat), ai = (int)(a / nalat);
+            const int sj = aj * step, si = ai * step;
+            const int ej = sj + cube_n - 1 < nlat - 1 ? sj + cube_n - 1 : nlat - 1;
+            const int ei = si + cube_n - 1 < nlon - 1 ? si + cube_n - 1 : nlon - 1;
+            /* 1-based (s + e) / 2, back to 0-based */
+            const int cj = (sj + 1 + ej + 1) / 2 - 1, ci = (si + 1 + ei + 1) / 2 - 1;
+            const double clat = (double)grid_lat[cj], clon = (double)grid_lon[ci];
+            int p = bins ? orc_select_column_binned(prm, obs, bins->bin_start, bins->nband,
+                                                    bins->nlonbin, bins->bin_deg, clat,
+                                                    clon, idx, rl)
+                         : orc_select_column(prm, obs, clat, clon, idx, rl);
+            for (int o = 0; o < p; ++o) {
+                memcpy(hx + (size_t)o * k, obs->yb + (size_t)idx[o] * k, sizeof(double) * k);
+                dp[o] = obs->dep[idx[o]];
+                ri[o] = obs->rinv[idx[o]];
+            }
+            if (orc_letkf_core(k, p, hx, ri, dp, rl, trans + (size_t)a * k * k, wb,
+                               prm->inflation) < 0)
+                ++nfail;
+        }
+        free(wb);
+        free(ri);
+        free(dp);
+        free(hx);
+        free(rl);
+        free(idx);
+    }
+    /* the update of every column from the cubes covering it */
+#pragma omp parallel
+    {
+        double *dist = (double *)malloc(sizeof(double) * (size_t)k);
+#pragma omp for schedule(static)
+        for (int64_t col = 0; col < (int64_t)nlat * nlon; ++col) {
+            const int ilat = (int)(col % nlat), ilon = (int)(col / nlat);
+            const double *xbc = xb + (size_t)col * n * k;
+            const double *axbc = axb + (size_t)col * n;
+            double *xac = xa + (size_t)col * n * k;
+            double *axac = axa + (size_t)col * n;
+            if (overlap == 0) {
+                const int64_t a = ilat / step + (int64_t)nalat * (ilon / step);
+                orc_xb_2_xa(k, n, xbc, axbc, trans + (size_t)a * k * k, xac, axac);
+                continue;
+            }
+            memset(xac, 0, sizeof(double) * (size_t)n * k);
+            memset(axac, 0, sizeof(double) * (size_t)n);
+            int count = 0;
+            for (int ai = 0; ai < nalon; ++ai) {
+                const int si = ai * step;
+                const int ei = si + cube_n - 1 < nlon - 1 ? si + cube_n - 1 : nlon - 1;
+                if (ilon < si || ilon > ei) continue;
+                for (int aj = 0; aj < nalat; ++aj) {
+                    const int sj = aj * step;
+                    const int ej = sj + cube_n - 1 < nlat - 1 ? sj + cube_n - 1 : nlat - 1;
+                    if (ilat < sj || ilat > ej) continue;
+                    const double *tr = trans + ((size_t)aj + (size_t)nalat * ai) * k * k;
+                    for (int v = 0; v < n; ++v) {
+                        double mean = 0.0;
+                        for (int e = 0; e < k; ++e) {
+                            double d = 0.0;
+                            for (int m = 0; m < k; ++m) d += xbc[(size_t)v * k + m] * tr[m + (size_t)e * k];
+                            dist[e] = axbc[v] + d;
+                            mean += dist[e];
+                        }
+                        mean /= k;
+                        axac[v] += mean;
+                        for (int e = 0; e < k; ++e) xac[(size_t)v * k + e] += dist[e] - mean;
+                    }
+                    ++count;
+                }
+            }
+            for (int v = 0; v < n; ++v) {
+                axac[v] /= count;
+                for (int e = 0; e < k; ++e) xac[(size_t)v * k + e] /= count;
+            }
+        }
+        free(dist);
+    }
+    if (!trans_out) free(trans);
+    return nfail;
+}
+
 int orc_sweep_compact(const orc_params *prm, const float *grid_lat, const float *grid_lon,
                       const orc_obs *obs, const orc_bins *bins, const int32_t *cols,
                       int32_t ncols, const double *xb, const double *axb, double *xa,

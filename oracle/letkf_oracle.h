@@ -18,6 +18,9 @@
  *                     (letkf_compute_nooverlap_vertical), letkf_yb.F90:137-243
  *                     (gen_yb_dis_vertical, vertical_localization),
  *                     obs_make.F90:418-437 (ps2alnp)
+ *   orc_sweep_cubes   code/x86/TradDA/DA/LETKF/src/letkf_compute.F90:12-126 with
+ *                     cube_n > 1, and :128-228 (letkf_compute, overlapping cubes)
+ *                     with xb_add2_xa :304-354 and the cube_count average
  *   orc_obs_operator  code/x86/TradDA/DA/LETKF/src/letkf_obs.F90:120-178, 187-243
  *                     (find_obs_grid), obs_make.F90:233-381 (obs_interpolation_3d_2,
  *                     ps2ph_3d_interp), DA/utils/vert_coord_mod.F90:547-563
@@ -141,6 +144,23 @@ int orc_sweep(const orc_params *prm, const float *grid_lat,
               const double *axb, double *xa, double *axa, const int32_t *cols,
               int32_t ncols, int32_t *nobsl_out, double *trans_out,
               double *wbar_out, const orc_bins *bins, int nthreads);
+
+/* Horizontal cubes: the grid is tiled by cubes of cube_n x cube_n columns
+ * whose starts advance by step = cube_n - overlap (cube_n when overlap = 0)
+ * from the first column in each direction, clipped at the last one.  One
+ * analysis per cube, with the selection centred on column
+ * ((sj + ej) / 2, (si + ei) / 2) (1-based integer division, gen_yb_dis,
+ * letkf_yb.F90:19-20).  overlap = 0: the cube's transform updates each of its
+ * columns (xb_2_xa).  overlap > 0: each column accumulates, over the cubes
+ * covering it in the reference's loop order (longitude starts outer,
+ * latitude starts inner), dist_e = xbar_b + sum_m Xb'_m trans(m, e), its
+ * member mean and dist_e - mean (xb_add2_xa), and is divided by the number of
+ * covering cubes.  trans_out[a * k * k] (optional) holds the transform of
+ * analysis a = aj + nalat ai.  Returns non-converged eigensolves. */
+int orc_sweep_cubes(const orc_params *prm, const float *grid_lat, const float *grid_lon,
+                    const orc_obs *obs, const double *xb, const double *axb, double *xa,
+                    double *axa, const orc_bins *bins, int cube_n, int overlap,
+                    double *trans_out, int nthreads);
 
 /* The sweep restricted to the listed columns with compact state arrays:
  * xb/xa [ncols * nrow * k] and axb/axa [ncols * nrow] hold column cols[w] at

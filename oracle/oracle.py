@@ -94,6 +94,10 @@ def lib():
         L.orc_sweep_compact.argtypes = [C.POINTER(OrcParams), _fp, _fp, C.POINTER(OrcObs),
                                         C.POINTER(OrcBins), _ip, C.c_int32, _dp, _dp, _dp, _dp,
                                         _ip, C.c_int]
+        L.orc_sweep_cubes.restype = C.c_int
+        L.orc_sweep_cubes.argtypes = [C.POINTER(OrcParams), _fp, _fp, C.POINTER(OrcObs), _dp,
+                                      _dp, _dp, _dp, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
+                                      C.c_int]
         L.orc_centre_obs.restype = None
         L.orc_centre_obs.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp]
         L.orc_sweep_vertical.restype = C.c_int
@@ -336,6 +340,40 @@ def obs_operator(grid_lat, grid_lon, xb, axb, ps_row, vcoord, var_start, var_lev
                            np.ascontiguousarray(rec_lon, np.float64), rec_ph.reshape(-1),
                            np.ascontiguousarray(yo, np.float64).reshape(-1), lat, lon, yb, dep)
     return lat, lon, yb.reshape(ns, k), dep
+
+
+def sweep_cubes(prm: OrcParams, grid_lat, grid_lon, obs: Obs, xb, axb, cube_n, overlap,
+                bins=None, nthreads=0, want_trans=False):
+    """letkf_compute_nooverlap with cube_n x cube_n cubes (overlap = 0) or
+    letkf_compute with overlapping cubes averaged per column (overlap > 0).
+    Returns dict(xa, axa, nfail[, trans (nanal, k, k) as trans[a][i][e]])."""
+    k, n = prm.k, prm.nrow
+    ncol = prm.nlat * prm.nlon
+    step = cube_n - overlap if overlap > 0 else cube_n
+    if cube_n < 1 or overlap < 0 or step < 1:
+        raise ValueError("cube_n / overlap out of range")
+    nanal = -(-prm.nlat // step) * -(-prm.nlon // step)
+    xa = np.zeros(ncol * n * k)
+    axa = np.zeros(ncol * n)
+    trans = np.zeros(nanal * k * k) if want_trans else None
+    bins_c = None
+    if bins is not None:
+        bin_start, nband, nlonbin, bin_deg = bins
+        bs = np.ascontiguousarray(bin_start, dtype=np.int32)
+        bins_c = OrcBins(bs.ctypes.data, nband, nlonbin, float(bin_deg))
+    nfail = lib().orc_sweep_cubes(
+        C.byref(prm), np.ascontiguousarray(grid_lat, np.float32),
+        np.ascontiguousarray(grid_lon, np.float32), C.byref(obs.c),
+        np.ascontiguousarray(xb, np.float64).reshape(-1),
+        np.ascontiguousarray(axb, np.float64).reshape(-1), xa, axa,
+        None if bins_c is None else C.cast(C.pointer(bins_c), C.c_void_p), int(cube_n),
+        int(overlap), None if trans is None else trans.ctypes.data, int(nthreads))
+    if nfail < 0:
+        raise ValueError("cube_n / overlap out of range")
+    out = {"xa": xa.reshape(ncol, n, k), "axa": axa.reshape(ncol, n), "nfail": nfail}
+    if want_trans:
+        out["trans"] = trans.reshape(nanal, k, k).transpose(0, 2, 1).copy()
+    return out
 
 
 def sweep_vertical(prm: OrcParams, grid_lat, grid_lon, obs: Obs, obs_ph, xb, axb, vcoord,
