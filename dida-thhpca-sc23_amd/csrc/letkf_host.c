@@ -62,6 +62,15 @@ void dida_letkf_params_default(dida_letkf_params *p)
     p->vert_dist = 0.0;
     p->cube_nz = 1;
     p->gc_filter_vertical = 1;
+    p->cube_n = 1;
+    p->overlap = 0;
+}
+
+/* Starts of horizontal cubes advance by cube_n (no overlap) or cube_n -
+ * overlap (letkf_compute.F90:55,61 and :180,188). */
+static int cube_step(const dida_letkf_params *p)
+{
+    return p->overlap > 0 ? p->cube_n - p->overlap : p->cube_n;
 }
 
 static double jacobi_tol_of(const dida_letkf_params *p)
@@ -98,7 +107,13 @@ static int check_params(const dida_letkf_params *p)
         return set_err(DIDA_ERR_UNSUPPORTED, "gram precision %d is not available", p->gram);
     if (p->form == DIDA_FORM_PLAIN && p->gram != DIDA_GRAM_FP64)
         return set_err(DIDA_ERR_UNSUPPORTED, "the plain form computes the Gram in FP64 only");
+    if (p->cube_n < 1 || p->overlap < 0 || cube_step(p) < 1)
+        return set_err(DIDA_ERR_ARG, "cube_n = %d, overlap = %d: need cube_n >= 1 and "
+                       "0 <= overlap < cube_n", p->cube_n, p->overlap);
     if (p->vert_dist > 0.0) {
+        if (p->cube_n != 1)
+            return set_err(DIDA_ERR_UNSUPPORTED,
+                           "vertical localisation runs with cube_n = 1 only");
         if (p->form != DIDA_FORM_BATCHED)
             return set_err(DIDA_ERR_UNSUPPORTED,
                            "vertical localisation runs in the batched form only");
@@ -114,6 +129,12 @@ struct dida_letkf_ctx {
     dida_letkf_params prm;
     int32_t nobs_max;
     int64_t ncol;
+    /* analysis grid: one selection list per column (cube_n = 1) or per
+     * horizontal cube, nsel = nalat * nalon; cube centres and transforms */
+    int32_t nalat, nalon, step;
+    int64_t nsel;
+    float *ca_lat, *ca_lon;
+    double *trans_all;
     const float *grid_lat, *grid_lon;
     dk_bin_geom geom;
     int end_bit;
@@ -186,7 +207,8 @@ int dida_letkf_destroy(dida_letkf_ctx *c)
                     c->nobsl, c->offset, c->scan_tmp, c->csr_idx, c->csr_w, c->flags,
                     c->trans, c->info, c->chain_key, c->chain_key_sorted, c->chain_id,
                     c->chain_order, c->queue_head, c->chain_tmp, c->ymax, c->phase_clk,
-                    c->obs_lnp, c->cube_lnp, c->cube_row_off, c->cube_rows};
+                    c->obs_lnp, c->cube_lnp, c->cube_row_off, c->cube_rows, c->ca_lat,
+                    c->ca_lon, c->trans_all};
     for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; ++i) dfree(ptrs[i]);
     free(c);
     return DIDA_OK;
@@ -230,10 +252,10 @@ static int set_layout(dida_letkf_ctx *c, double bin_deg, int32_t chain_len)
     while ((1u << c->end_bit) <= (unsigned)g.nbins) ++c->end_bit;
     if (c->prm.form == DIDA_FORM_BATCHED) {
         if (chain_len < 1) chain_len = 1;
-        if (chain_len > c->prm.nlat) chain_len = c->prm.nlat;
+        if (chain_len > c->nalat) chain_len = c->nalat;
         c->chain_len = chain_len;
-        c->chains_per_lon = (c->prm.nlat + chain_len - 1) / chain_len;
-        c->nchain = c->chains_per_lon * c->prm.nlon;
+        c->chains_per_lon = (c->nalat + chain_len - 1) / chain_len;
+        c->nchain = c->chains_per_lon * c->nalon;
     }
     return DIDA_OK;
 }
@@ -251,6 +273,10 @@ int dida_letkf_create(dida_letkf_ctx **out, const dida_letkf_params *prm, int32_
     c->prm = *prm;
     c->nobs_max = nobs_max;
     c->ncol = (int64_t)prm->nlat * prm->nlon;
+    c->step = cube_step(prm);
+    c->nalat = (prm->nlat + c->step - 1) / c->step;
+    c->nalon = (prm->nlon + c->step - 1) / c->step;
+    c->nsel = (int64_t)c->nalat * c->nalon;
     const int k = prm->nanals;
 
     /* support radius in degrees, widened so that no kept obs falls outside
@@ -269,7 +295,7 @@ int dida_letkf_create(dida_letkf_ctx **out, const dida_letkf_params *prm, int32_
         const dk_bin_geom g1 = bin_geom(c->theta_deg / 8.0, prm->obs_polar);
         c->nbins_max = g0.nbins > g1.nbins ? g0.nbins : g1.nbins;
     }
-    c->nchain_max = (int32_t)c->ncol;
+    c->nchain_max = (int32_t)c->nsel;
     rc = set_layout(c, bd, prm->chain_len > 0 ? prm->chain_len : 8);
     if (rc) {
         free(c);
@@ -302,9 +328,9 @@ int dida_letkf_create(dida_letkf_ctx **out, const dida_letkf_params *prm, int32_
     DMALLOC(c->dep_b, no * sizeof(double));
     DMALLOC(c->rinv_b, no * sizeof(float));
 
-    DMALLOC(c->nobsl, (size_t)c->ncol * sizeof(int32_t));
-    DMALLOC(c->offset, (size_t)(c->ncol + 1) * sizeof(int64_t));
-    dk_scan_offsets(NULL, &c->scan_tmp_bytes, NULL, NULL, (int32_t)c->ncol, NULL);
+    DMALLOC(c->nobsl, (size_t)c->nsel * sizeof(int32_t));
+    DMALLOC(c->offset, (size_t)(c->nsel + 1) * sizeof(int64_t));
+    dk_scan_offsets(NULL, &c->scan_tmp_bytes, NULL, NULL, (int32_t)c->nsel, NULL);
     DMALLOC(c->scan_tmp, c->scan_tmp_bytes);
     DMALLOC(c->flags, 4 * sizeof(int32_t));
     c->capacity = sel_capacity;
@@ -312,10 +338,15 @@ int dida_letkf_create(dida_letkf_ctx **out, const dida_letkf_params *prm, int32_
         DMALLOC(c->csr_idx, (size_t)sel_capacity * sizeof(int32_t));
         DMALLOC(c->csr_w, (size_t)sel_capacity * sizeof(double));
     }
-    DMALLOC(c->info, (size_t)c->ncol * sizeof(int32_t));
+    DMALLOC(c->info, (size_t)c->nsel * sizeof(int32_t));
+    if (prm->cube_n > 1) {
+        DMALLOC(c->ca_lat, (size_t)c->nalat * sizeof(float));
+        DMALLOC(c->ca_lon, (size_t)c->nalon * sizeof(float));
+        DMALLOC(c->trans_all, (size_t)c->nsel * k * k * sizeof(double));
+    }
     if (prm->form == DIDA_FORM_PLAIN) {
-        c->chunk = (int32_t)(c->ncol < 32768 ? c->ncol : 32768);
-        DMALLOC(c->trans, (size_t)c->chunk * k * k * sizeof(double));
+        c->chunk = (int32_t)(c->nsel < 32768 ? c->nsel : 32768);
+        if (prm->cube_n == 1) DMALLOC(c->trans, (size_t)c->chunk * k * k * sizeof(double));
     } else {
         DMALLOC(c->chain_key, (size_t)c->nchain_max * sizeof(uint32_t));
         DMALLOC(c->chain_key_sorted, (size_t)c->nchain_max * sizeof(uint32_t));
@@ -465,10 +496,17 @@ static int sweep_select_count(dida_letkf_ctx *c, const dida_letkf_obs *o, hipStr
     if (c->ncube > 0) HIP_TRY(dk_obs_lnp(o->nobs, c->perm, o->ph, c->obs_lnp, s));
     tmark(c, DIDA_STAGE_BIN, s);
     memset(sa, 0, sizeof *sa);
-    sa->nlat = c->prm.nlat;
-    sa->nlon = c->prm.nlon;
+    sa->nlat = c->nalat;
+    sa->nlon = c->nalon;
     sa->grid_lat = c->grid_lat;
     sa->grid_lon = c->grid_lon;
+    if (c->prm.cube_n > 1) {
+        /* one selection per horizontal cube, at its centre column */
+        HIP_TRY(dk_cube_centres(c->prm.nlat, c->prm.nlon, c->prm.cube_n, c->step, c->grid_lat,
+                                c->grid_lon, c->ca_lat, c->ca_lon, s));
+        sa->grid_lat = c->ca_lat;
+        sa->grid_lon = c->ca_lon;
+    }
     sa->geom = c->geom;
     sa->obs.cx = c->cx;
     sa->obs.sx = c->sx;
@@ -491,7 +529,7 @@ static int sweep_select_count(dida_letkf_ctx *c, const dida_letkf_obs *o, hipStr
     sa->overflow = c->flags;
     HIP_TRY(dk_select_count(sa, s));
     size_t tb = c->scan_tmp_bytes;
-    HIP_TRY(dk_scan_offsets(c->scan_tmp, &tb, c->nobsl, c->offset, (int32_t)c->ncol, s));
+    HIP_TRY(dk_scan_offsets(c->scan_tmp, &tb, c->nobsl, c->offset, (int32_t)c->nsel, s));
     return DIDA_OK;
 }
 
@@ -499,13 +537,41 @@ static int ensure_capacity(dida_letkf_ctx *c, hipStream_t s)
 {
     if (c->capacity > 0) return DIDA_OK;
     int64_t total = 0;
-    HIP_TRY(hipMemcpyAsync(&total, c->offset + c->ncol, sizeof total, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(&total, c->offset + c->nsel, sizeof total, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     int64_t cap = total + total / 8 + 1024;
     if (hipMalloc((void **)&c->csr_idx, (size_t)cap * sizeof(int32_t)) != hipSuccess ||
         hipMalloc((void **)&c->csr_w, (size_t)cap * sizeof(double)) != hipSuccess)
         return set_err(DIDA_ERR_NOMEM, "selection workspace of %lld entries", (long long)cap);
     c->capacity = cap;
+    return DIDA_OK;
+}
+
+/* Horizontal cubes: every column from the mean transform of its covering
+ * cubes (trans_all). */
+static int cube_update(dida_letkf_ctx *c, const double *xb, const double *axb, double *xa,
+                       double *axa, hipStream_t s)
+{
+    dk_update_args ua;
+    memset(&ua, 0, sizeof ua);
+    ua.k = c->prm.nanals;
+    ua.nrow = c->prm.nrow;
+    ua.ncol = (int32_t)c->ncol;
+    ua.col0 = 0;
+    ua.trans = c->trans_all;
+    ua.xb = xb;
+    ua.axb = axb;
+    ua.xa = xa;
+    ua.axa = axa;
+    ua.rtps = c->prm.rtps;
+    ua.rtps_alpha = c->prm.rtps_alpha;
+    ua.nlat = c->prm.nlat;
+    ua.nlon = c->prm.nlon;
+    ua.cube_n = c->prm.cube_n;
+    ua.step = c->step;
+    ua.nalat = c->nalat;
+    ua.nalon = c->nalon;
+    HIP_TRY(dk_update_f64(&ua, s));
     return DIDA_OK;
 }
 
@@ -517,8 +583,9 @@ static int sweep_batched(dida_letkf_ctx *c, const double *xb, const double *axb,
     memset(&ba, 0, sizeof ba);
     ba.k = c->prm.nanals;
     ba.nrow = c->prm.nrow;
-    ba.nlat = c->prm.nlat;
-    ba.nlon = c->prm.nlon;
+    ba.nlat = c->nalat;
+    ba.nlon = c->nalon;
+    ba.trans_out = c->prm.cube_n > 1 ? c->trans_all : NULL;
     ba.nchain = c->nchain;
     ba.chains_per_lon = c->chains_per_lon;
     ba.chain_len = c->chain_len;
@@ -586,6 +653,10 @@ static int sweep_batched(dida_letkf_ctx *c, const double *xb, const double *axb,
     tmark(c, DIDA_STAGE_SELECT, s);
     HIP_TRY(dk_letkf_wave(&ba, c->nblocks, s));
     tmark(c, DIDA_STAGE_TRANSFORM, s);
+    if (c->prm.cube_n > 1) {
+        HIP_TRY(cube_update(c, xb, axb, xa, axa, s));
+        tmark(c, DIDA_STAGE_UPDATE, s);
+    }
     return DIDA_OK;
 }
 
@@ -631,6 +702,19 @@ static int sweep_rest(dida_letkf_ctx *c, dk_select_args *sa, const double *xb,
     ua.axa = axa;
     ua.rtps = c->prm.rtps;
     ua.rtps_alpha = c->prm.rtps_alpha;
+    if (c->prm.cube_n > 1) {
+        for (int64_t c0 = 0; c0 < c->nsel; c0 += c->chunk) {
+            int32_t n = (int32_t)((c->nsel - c0) < c->chunk ? (c->nsel - c0) : c->chunk);
+            ca.aid0 = (int32_t)c0;
+            ca.nanal = n;
+            ca.trans = c->trans_all + c0 * k * k;
+            HIP_TRY(dk_letkf_core_f64(&ca, s));
+        }
+        tmark(c, DIDA_STAGE_TRANSFORM, s);
+        HIP_TRY(cube_update(c, xb, axb, xa, axa, s));
+        tmark(c, DIDA_STAGE_UPDATE, s);
+        return DIDA_OK;
+    }
     for (int64_t c0 = 0; c0 < c->ncol; c0 += c->chunk) {
         int32_t n = (int32_t)((c->ncol - c0) < c->chunk ? (c->ncol - c0) : c->chunk);
         ca.aid0 = (int32_t)c0;
@@ -960,8 +1044,8 @@ int dida_letkf_stats_get(dida_letkf_ctx *c, dida_letkf_stats *st)
     if (!c || !st) return set_err(DIDA_ERR_ARG, "null argument");
     memset(st, 0, sizeof *st);
     HIP_TRY(hipStreamSynchronize(c->last_stream));
-    const size_t n = (size_t)c->ncol;
-    const size_t nq = c->ncube > 0 ? (size_t)c->ncube : 1; /* analyses per column */
+    const size_t n = (size_t)c->nsel;
+    const size_t nq = c->ncube > 0 ? (size_t)c->ncube : 1; /* analyses per selection */
     int32_t *nb = (int32_t *)malloc(n * sizeof(int32_t));
     int32_t *inf = (int32_t *)malloc(n * nq * sizeof(int32_t));
     int32_t flags[4];

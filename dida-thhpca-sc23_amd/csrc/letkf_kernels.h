@@ -122,9 +122,21 @@ typedef struct dk_update_args {
     int32_t rtps;           /* 1: relax to prior spread with alpha */
     int32_t pad_;
     double rtps_alpha;
+    /* horizontal cubes (cube_n > 1): column col = ilat + nlat ilon takes the
+     * mean of the transforms trans[a * k * k] of the cubes covering it, a =
+     * aj + nalat ai over cube starts aj * step (latitude), ai * step
+     * (longitude), each cube cube_n columns wide (clipped at the last) */
+    int32_t nlat, nlon, cube_n, step, nalat, nalon;
 } dk_update_args;
 
 int dk_update_f64(const dk_update_args *a, void *stream);
+
+/* Centres of horizontal cubes: ca_lat[aj] = grid_lat[c(aj)], c(aj) the
+ * 0-based lower middle of columns aj * step .. min(aj * step + cube_n, nlat) - 1
+ * (gen_yb_dis, letkf_yb.F90:19-20); likewise ca_lon. */
+int dk_cube_centres(int32_t nlat, int32_t nlon, int32_t cube_n, int32_t step,
+                    const float *grid_lat, const float *grid_lon, float *ca_lat, float *ca_lon,
+                    void *stream);
 
 /* Precision of the Gram in the batched form (values of DIDA_GRAM_*). */
 #define DK_GRAM_FP64 0
@@ -165,6 +177,9 @@ typedef struct dk_batched_args {
     const double *obs_lnp;      /* [binned obs] ln p */
     const double *cube_lnp;     /* [col * ncube + q] ln p of the cube's centre level */
     const int32_t *cube_row_off, *cube_rows;
+    /* optional: write trans(i, e) = W + wbar 1^T of analysis col at
+     * trans_out[col * k * k + i + e k] instead of updating the state */
+    double *trans_out;
 } dk_batched_args;
 
 /* Phases of the batched kernel timed by phase_clk (wave 0 of each block). */

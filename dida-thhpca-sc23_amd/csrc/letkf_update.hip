@@ -48,10 +48,32 @@ __global__ __launch_bounds__(kThreads) void k_update_f64(dk_update_args u) {
   const int c = blockIdx.x;
   const int64_t col = (int64_t)u.col0 + c;
   const int k = u.k, nrow = u.nrow;
-  const double* T = u.trans + (int64_t)c * k * k;
-  for (int t = threadIdx.x; t < k * k; t += kThreads) {
-    const int i = t % k, e = t / k;
-    sT[i * LDT + e] = T[t];
+  if (u.cube_n > 1) {
+    // mean transform of the cubes covering the column, summed in the
+    // reference's cube order (longitude starts outer, latitude inner);
+    // averaging the transforms equals averaging the cubes' re-centred
+    // analyses (xb_add2_xa / cube_count), which are linear in trans
+    const int ilat = (int)(col % u.nlat), ilon = (int)(col / u.nlat);
+    const int st = u.step, cn = u.cube_n;
+    const int aj0 = ilat - cn + 1 > 0 ? (ilat - cn + 1 + st - 1) / st : 0;
+    const int aj1 = min(ilat / st, u.nalat - 1);
+    const int ai0 = ilon - cn + 1 > 0 ? (ilon - cn + 1 + st - 1) / st : 0;
+    const int ai1 = min(ilon / st, u.nalon - 1);
+    const double cnt = (double)((aj1 - aj0 + 1) * (ai1 - ai0 + 1));
+    for (int t = threadIdx.x; t < k * k; t += kThreads) {
+      double sum = 0.0;
+      for (int ai = ai0; ai <= ai1; ++ai)
+        for (int aj = aj0; aj <= aj1; ++aj)
+          sum += u.trans[((int64_t)aj + (int64_t)u.nalat * ai) * k * k + t];
+      const int i = t % k, e = t / k;
+      sT[i * LDT + e] = sum / cnt;
+    }
+  } else {
+    const double* T = u.trans + (int64_t)c * k * k;
+    for (int t = threadIdx.x; t < k * k; t += kThreads) {
+      const int i = t % k, e = t / k;
+      sT[i * LDT + e] = T[t];
+    }
   }
   const int e = threadIdx.x % KP, slot = threadIdx.x / KP;
   const double rk = 1.0 / (double)k;
@@ -78,6 +100,20 @@ __global__ __launch_bounds__(kThreads) void k_update_f64(dk_update_args u) {
     }
     if (ok) u.xa[cr * k + e] = xa;
     if (rowok && e == 0) u.axa[cr] = mean;
+  }
+}
+
+__global__ void k_cube_centres(int32_t nlat, int32_t nlon, int32_t cube_n, int32_t step,
+                               const float* grid_lat, const float* grid_lon, float* ca_lat,
+                               float* ca_lon) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nalat = (nlat + step - 1) / step, nalon = (nlon + step - 1) / step;
+  if (t < nalat) {
+    const int s = t * step, e = min(s + cube_n - 1, nlat - 1);
+    ca_lat[t] = grid_lat[(s + e) / 2];
+  } else if (t < nalat + nalon) {
+    const int s = (t - nalat) * step, e = min(s + cube_n - 1, nlon - 1);
+    ca_lon[t - nalat] = grid_lon[(s + e) / 2];
   }
 }
 
@@ -119,6 +155,17 @@ extern "C" int dk_update_f64(const dk_update_args* u, void* stream) {
   else if (k <= 32) k_update_f64<32><<<u->ncol, kThreads, 0, s>>>(*u);
   else if (k <= 64) k_update_f64<64><<<u->ncol, kThreads, 0, s>>>(*u);
   else k_update_f64<128><<<u->ncol, kThreads, 0, s>>>(*u);
+  return (int)hipGetLastError();
+}
+
+extern "C" int dk_cube_centres(int32_t nlat, int32_t nlon, int32_t cube_n, int32_t step,
+                               const float* grid_lat, const float* grid_lon, float* ca_lat,
+                               float* ca_lon, void* stream) {
+  if (nlat < 1 || nlon < 1 || cube_n < 1 || step < 1) return (int)hipErrorInvalidValue;
+  const int n = (nlat + step - 1) / step + (nlon + step - 1) / step;
+  k_cube_centres<<<(n + 255) / 256, 256, 0, (hipStream_t)stream>>>(nlat, nlon, cube_n, step,
+                                                                   grid_lat, grid_lon, ca_lat,
+                                                                   ca_lon);
   return (int)hipGetLastError();
 }
 

@@ -160,6 +160,10 @@ __device__ __forceinline__ double rcp_nr1(double x) {
 // index of obs 0 and zero weights, so every load is unconditional and in
 // bounds (callers only use blocks with base < p, p > 0).  The loads are split
 // in two so the dependent ones can be issued once the indices have arrived.
+constexpr int kModeColumn = 0;
+constexpr int kModeVertical = 1;
+constexpr int kModeTransOut = 2;
+
 struct ObsMeta {
   int gi;
   bool ok;
@@ -700,12 +704,16 @@ struct Wave {
   }
 };
 
-template <int KP, int GRAM, bool VERT>
+// MODE: kModeColumn (one analysis per column, fused update), kModeVertical
+// (one per column and level cube, fused update of the cube's rows) or
+// kModeTransOut (one per horizontal cube, transform written to HBM)
+template <int KP, int GRAM, int MODE>
 __device__ void wave_column(const dk_batched_args& a, int64_t col, int cube, bool warm,
                             WSmem<KP>& sm, Wave<KP, GRAM>& w) {
   constexpr int NJ = Wave<KP, GRAM>::NJ;
   constexpr int NQ = Wave<KP, GRAM>::NQ;
   constexpr int LDV = WSmem<KP>::LDV;
+  constexpr bool VERT = MODE == kModeVertical;
   const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
   const int k = a.k;
   PhaseClock clk(a.phase_clk);
@@ -791,6 +799,21 @@ __device__ void wave_column(const dk_batched_args& a, int64_t col, int cube, boo
         if (i < k) a.wbar_out[aid * k + i] = wb[I][r];
       }
   clk.mark(DK_PHASE_TRANSFORM);
+  if constexpr (MODE == kModeTransOut) {
+    // horizontal cubes: the transform goes to HBM; a per-column kernel
+    // applies the mean of the covering cubes' transforms
+    double* tr = a.trans_out + aid * k * k;
+#pragma unroll
+    for (int I = 0; I < NJ; ++I)
+#pragma unroll
+      for (int I2 = 0; I2 < NJ; ++I2)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = rowB(I, r, g), e = I2 * 16 + c;
+          if (i < k && e < k) tr[i + (int64_t)e * k] = T[I][I2][r];
+        }
+    return;
+  }
 
   // update, 16 state rows at a time: D_E[4r' + g][c] = Xa(E*16 + 4r' + g, row
   // R0 + c of the analysis); an analysis updates all nrow rows of its column,
@@ -883,7 +906,7 @@ __device__ void wave_column(const dk_batched_args& a, int64_t col, int cube, boo
   (void)NQ;
 }
 
-template <int KP, int GRAM, bool VERT>
+template <int KP, int GRAM, int MODE>
 __global__ __launch_bounds__(64, 1) void k_letkf_wave(dk_batched_args a) {
   __shared__ WSmem<KP> sm;
   Wave<KP, GRAM> w;
@@ -899,16 +922,16 @@ __global__ __launch_bounds__(64, 1) void k_letkf_wave(dk_batched_args a) {
     const int lat1 = min(lat0 + a.chain_len, a.nlat);
     for (int ilat = lat0; ilat < lat1; ++ilat) {
       const int64_t col = (int64_t)ilon * a.nlat + ilat;
-      if constexpr (VERT) {
+      if constexpr (MODE == kModeVertical) {
         // the cubes of a column in alternating order, so that every analysis
         // warm-starts from the adjacent cube of its column or the same cube
         // of the previous column
         const bool down = ((ilat - lat0) & 1) != 0;
         for (int t = 0; t < a.ncube; ++t)
-          wave_column<KP, GRAM, true>(a, col, down ? a.ncube - 1 - t : t,
+          wave_column<KP, GRAM, MODE>(a, col, down ? a.ncube - 1 - t : t,
                                       a.warm_start && (ilat > lat0 || t > 0), sm, w);
       } else {
-        wave_column<KP, GRAM, false>(a, col, 0, a.warm_start && ilat > lat0, sm, w);
+        wave_column<KP, GRAM, MODE>(a, col, 0, a.warm_start && ilat > lat0, sm, w);
       }
     }
   }
@@ -934,15 +957,17 @@ __global__ void k_chain_keys(dk_batched_args a, uint32_t* key, int32_t* id) {
 
 template <int KP, int GRAM>
 int wlaunch(const dk_batched_args* a, int nblocks, hipStream_t s) {
-  if (a->ncube > 0) k_letkf_wave<KP, GRAM, true><<<nblocks, 64, 0, s>>>(*a);
-  else k_letkf_wave<KP, GRAM, false><<<nblocks, 64, 0, s>>>(*a);
+  if (a->ncube > 0) k_letkf_wave<KP, GRAM, kModeVertical><<<nblocks, 64, 0, s>>>(*a);
+  else if (a->trans_out) k_letkf_wave<KP, GRAM, kModeTransOut><<<nblocks, 64, 0, s>>>(*a);
+  else k_letkf_wave<KP, GRAM, kModeColumn><<<nblocks, 64, 0, s>>>(*a);
   return (int)hipGetLastError();
 }
 
 template <int KP, int GRAM>
 int wocc() {
   int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_letkf_wave<KP, GRAM, false>, 64, 0))
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_letkf_wave<KP, GRAM, kModeColumn>, 64,
+                                                   0))
     return 1;
   return nb > 0 ? nb : 1;
 }

@@ -45,7 +45,8 @@ class Params(C.Structure):
                 ("jacobi_max_sweeps", C.c_int32), ("jacobi_tol", C.c_double),
                 ("bin_deg", C.c_double), ("chain_len", C.c_int32), ("warm_start", C.c_int32),
                 ("jacobi_predict", C.c_int32), ("vert_dist", C.c_double),
-                ("cube_nz", C.c_int32), ("gc_filter_vertical", C.c_int32)]
+                ("cube_nz", C.c_int32), ("gc_filter_vertical", C.c_int32),
+                ("cube_n", C.c_int32), ("overlap", C.c_int32)]
 
 
 class TuningC(C.Structure):

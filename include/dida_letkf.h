@@ -82,6 +82,17 @@ typedef struct dida_letkf_params {
     int32_t cube_nz;            /* da_vertical_interval, levels per cube (default 1,
                                    as namelists.input_100km_32mem:69) */
     int32_t gc_filter_vertical; /* da_gc_enable_vertical (default 1, as :67) */
+    /* Horizontal cubes (letkf_compute_nooverlap with cube_n > 1,
+     * letkf_compute.F90:12-126; letkf_compute for overlap > 0, :128-228 with
+     * xb_add2_xa :304-354): cubes of cube_n x cube_n columns whose starts
+     * advance by cube_n - overlap (cube_n without overlap) from the first
+     * column, clipped at the last; one analysis per cube, selected around the
+     * cube's lower-middle column (gen_yb_dis, letkf_yb.F90:19-20); every
+     * column takes the mean of the analyses of the cubes covering it.  Both
+     * forms; cube_n = 1 with vertical localisation. */
+    int32_t cube_n;             /* da_block_interval (default 1, as
+                                   namelists.input_100km_32mem:68) */
+    int32_t overlap;            /* da_block_overlap, 0 <= overlap < cube_n (default 0) */
 } dida_letkf_params;
 
 /* Observations, device pointers, one entry per scalar observation (the

@@ -99,3 +99,17 @@ def test_vertical_params(dl):
                             form=dl.FORM_BATCHED)
     rc = dl.lib().dida_letkf_create(ctypes.byref(h), ctypes.byref(bad), 10, 0)
     assert rc < 0 and b"cube_nz" in dl.lib().dida_letkf_last_error()
+
+
+def test_cube_params(dl):
+    p = dl.default_params()
+    assert p.cube_n == 1 and p.overlap == 0
+    h = ctypes.c_void_p()
+    for cube_n, overlap in ((0, 0), (2, 2), (3, -1)):
+        bad = dl.default_params(nanals=8, nrow=3, nlat=4, nlon=4, cube_n=cube_n, overlap=overlap)
+        rc = dl.lib().dida_letkf_create(ctypes.byref(h), ctypes.byref(bad), 10, 0)
+        assert rc < 0 and b"cube_n" in dl.lib().dida_letkf_last_error()
+    bad = dl.default_params(nanals=8, nrow=3, nlat=4, nlon=4, cube_n=2, vert_dist=1.0,
+                            form=dl.FORM_BATCHED)
+    rc = dl.lib().dida_letkf_create(ctypes.byref(h), ctypes.byref(bad), 10, 0)
+    assert rc == -5 and b"cube_n = 1" in dl.lib().dida_letkf_last_error()
