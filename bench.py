@@ -254,6 +254,59 @@ def time_vertical(dl, torch, data, cfg, prm, layout, steps=2, cube_nz=1, vert_di
             "forms": res}
 
 
+def time_cubes(dl, torch, data, cfg, prm, layout, steps=3,
+               modes=((4, 2), (2, 0))):
+    """Horizontal cubes on the same workload: one analysis per cube of
+    cube_n x cube_n columns, overlapping cubes averaged per column (the
+    reference's namelist defaults are cube_n 4, overlap 2).  The split-BF16
+    batched form is timed (HIP events) and compared with the FP64 batched
+    form in the same mode (per-variable increment error)."""
+    out = []
+    obs = dl.DeviceObs(data["lat"], data["lon"], data["yb"], data["dep"], data["rinv"])
+    for cube_n, overlap in modes:
+        extra = dict(cube_n=cube_n, overlap=overlap)
+        if layout:
+            extra["bin_deg"] = float(layout["bin_deg"])
+        outs, res = {}, {}
+        for name, gram, nstep in (("batched_bf16x3", dl.GRAM_BF16X3, steps),
+                                  ("batched_fp64", dl.GRAM_FP64, 1)):
+            eng, p = make_engine(dl, cfg, dict(prm, **extra), dl.FORM_BATCHED, gram, cfg.nobs)
+            eng.set_grid(data["grid_lat"], data["grid_lon"])
+            xa, axa = torch.empty_like(data["xb"]), torch.empty_like(data["axb"])
+            tuning = eng.autotune(obs, data["xb"], data["axb"], xa, axa)
+            st = eng.stats()
+            eng.set_timing(True)
+            stage_sum = {}
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            e0.record()
+            for _ in range(nstep):
+                eng.sweep(obs, data["xb"], data["axb"], xa, axa)
+                for key, v in eng.stage_ms().items():
+                    stage_sum[key] = stage_sum.get(key, 0.0) + v
+            e1.record()
+            torch.cuda.synchronize()
+            eng.set_timing(False)
+            ms = e0.elapsed_time(e1) / nstep
+            eng.close()
+            outs[name] = (xa, axa)
+            res[name] = {"ms_per_sweep": ms, "analyses": st["nanal"],
+                         "analyses_per_s": st["nanal"] / (ms / 1e3),
+                         "columns_per_s": cfg.nlat * cfg.nlon / (ms / 1e3),
+                         "stage_ms": {k: v / nstep for k, v in stage_sum.items()},
+                         "autotuned_layout": tuning,
+                         "jacobi_mean_sweeps": st["sweeps_sum"] / max(1, st["nanal"]),
+                         "nonconverged": st["nonconverged"]}
+            log(f"  cubes {cube_n}/{overlap} {name}: {res[name]}")
+        r = {"xa": outs["batched_bf16x3"][0], "axa": outs["batched_bf16x3"][1]}
+        res["batched_bf16x3"]["rel_err_vs_batched_fp64"] = rel_errors(outs["batched_fp64"], r,
+                                                                      data, cfg)
+        del outs, r
+        torch.cuda.empty_cache()
+        out.append({"cube_n": cube_n, "overlap": overlap, "forms": res})
+    return out
+
+
 def copy_probe(torch, nbytes=1 << 30, reps=3):
     """A plain device copy of known size (the FETCH_SIZE / WRITE_SIZE scale)."""
     src = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda").fill_(1.0)
@@ -317,6 +370,8 @@ def main():
                     help="skip the plain FP64 form (profiling runs)")
     ap.add_argument("--no-vertical", action="store_true",
                     help="skip the vertical-localisation timing")
+    ap.add_argument("--no-cubes", action="store_true",
+                    help="skip the horizontal-cube timing")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device copy (counter scale for profiles)")
     args = ap.parse_args()
@@ -405,6 +460,11 @@ def main():
         log("timing vertical localisation")
         vertical = time_vertical(dl, torch, data, cfg, prm,
                                  res.get("batched_bf16x3", {}).get("tuning"))
+    cubes = None
+    if not args.no_cubes:
+        log("timing horizontal cubes")
+        cubes = time_cubes(dl, torch, data, cfg, prm,
+                           res.get("batched_bf16x3", {}).get("tuning"))
     log("timing the observation operator")
     obsop = time_obs_operator(dl, torch, data, cfg)
     log(f"  {obsop}")
@@ -428,7 +488,7 @@ def main():
                    "rel_increment_error_vs_fp64": r["err"] or None,
                    "speedup_vs_plain_fp64": plain_ms / r["ms"] if plain_ms else None},
         "roofline": roofline, "cpu_baseline": cpu, "forms": out_forms,
-        "obs_operator": obsop, "vertical_localisation": vertical,
+        "obs_operator": obsop, "vertical_localisation": vertical, "horizontal_cubes": cubes,
     }
     print(json.dumps(line), flush=True)
 

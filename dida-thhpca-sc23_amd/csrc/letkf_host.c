@@ -571,7 +571,8 @@ static int cube_update(dida_letkf_ctx *c, const double *xb, const double *axb, d
     ua.step = c->step;
     ua.nalat = c->nalat;
     ua.nalon = c->nalon;
-    HIP_TRY(dk_update_f64(&ua, s));
+    if (ua.k <= 64) HIP_TRY(dk_update_cols(&ua, s));
+    else HIP_TRY(dk_update_f64(&ua, s));
     return DIDA_OK;
 }
 
@@ -723,7 +724,8 @@ static int sweep_rest(dida_letkf_ctx *c, dk_select_args *sa, const double *xb,
         tmark(c, DIDA_STAGE_TRANSFORM, s);
         ua.col0 = (int32_t)c0;
         ua.ncol = n;
-        HIP_TRY(dk_update_f64(&ua, s));
+        if (k <= 64) HIP_TRY(dk_update_cols(&ua, s));
+        else HIP_TRY(dk_update_f64(&ua, s));
         tmark(c, DIDA_STAGE_UPDATE, s);
     }
     return DIDA_OK;

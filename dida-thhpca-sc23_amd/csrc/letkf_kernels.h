@@ -130,6 +130,8 @@ typedef struct dk_update_args {
 } dk_update_args;
 
 int dk_update_f64(const dk_update_args *a, void *stream);
+/* The same update with FP64 MFMA, one wavefront per column (k <= 64). */
+int dk_update_cols(const dk_update_args *a, void *stream);
 
 /* Centres of horizontal cubes: ca_lat[aj] = grid_lat[c(aj)], c(aj) the
  * 0-based lower middle of columns aj * step .. min(aj * step + cube_n, nlat) - 1

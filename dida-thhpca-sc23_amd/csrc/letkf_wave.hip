@@ -704,6 +704,102 @@ struct Wave {
   }
 };
 
+// Xa' (re-centred, optionally RTPS-relaxed) and xbar_a of nr rows of column
+// col from the transform tiles T (T[I][E][r] at lane (g, c) = trans(I*16 + 4r
+// + g, E*16 + c), zero past k), 16 rows at a time: D_E[4r' + g][c] =
+// Xa(E*16 + 4r' + g, row R0 + c); rows rmap[0 .. nr) with MAP, else 0 .. nr.
+template <int KP, bool MAP>
+__device__ __forceinline__ void update_rows(int k, int nrow, const double* __restrict__ xb,
+                                            const double* __restrict__ axb, double* xa,
+                                            double* axa, bool rtps, double rtps_alpha,
+                                            int64_t col,
+                                            const double4_t (&T)[KP / 16][KP / 16],
+                                            const int32_t* rmap, int nr) {
+  constexpr int NJ = KP / 16;
+  const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
+  const double* xbc = xb + col * (int64_t)nrow * k;
+  double* xac = xa + col * (int64_t)nrow * k;
+  auto phys = [&](int r) { return MAP ? rmap[r] : r; };
+  // Xb' of rows R0 + c (clamped to the last row, in bounds), members of the
+  // B-operand pattern; the next block's loads are issued before a block
+  // computes
+  auto load_xb = [&](int R0, double (&xl)[NJ][4]) {
+    const int xr = R0 + c;
+    const double* xrow = xbc + (int64_t)phys(xr < nr ? xr : nr - 1) * k;
+#pragma unroll
+    for (int I = 0; I < NJ; ++I)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = rowB(I, r, g);
+        xl[I][r] = xrow[i < k ? i : k - 1];
+      }
+  };
+  double xl[NJ][4];
+  if (nr > 0) load_xb(0, xl);
+  for (int R0 = 0; R0 < nr; R0 += 16) {
+    const int xr = R0 + c;
+    const bool rok = xr < nr;
+    const int pr = phys(rok ? xr : nr - 1);
+    double4_t D[NJ];
+#pragma unroll
+    for (int E = 0; E < NJ; ++E) D[E] = double4_t{0.0, 0.0, 0.0, 0.0};
+    double xn[NJ][4];
+    load_xb(R0 + 16, xn);
+    __builtin_amdgcn_sched_barrier(0);
+    double ssb = 0.0;
+#pragma unroll
+    for (int I = 0; I < NJ; ++I)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = rowB(I, r, g);
+        const double xv = (rok && i < k) ? xl[I][r] : 0.0;
+        ssb += xv * xv;
+#pragma unroll
+        for (int E = 0; E < NJ; ++E) D[E] = mfma_f64_16x16x4(T[I][E][r], xv, D[E]);
+      }
+    const double xbar = axb[col * nrow + pr];
+    double s = 0.0;
+#pragma unroll
+    for (int E = 0; E < NJ; ++E)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int e = rowB(E, r, g);
+        D[E][r] = e < k ? D[E][r] + xbar : 0.0;
+        s += D[E][r];
+      }
+    const double mean = sum_g(s) / (double)k;
+    double ssa = 0.0;
+#pragma unroll
+    for (int E = 0; E < NJ; ++E)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int e = rowB(E, r, g);
+        D[E][r] = e < k ? D[E][r] - mean : 0.0;
+        ssa += D[E][r] * D[E][r];
+      }
+    double fac = 1.0;
+    if (rtps) {
+      ssb = sum_g(ssb);
+      ssa = sum_g(ssa);
+      if (ssa > 0.0) fac = rtps_alpha * sqrt(ssb) / sqrt(ssa) + 1.0 - rtps_alpha;
+    }
+    if (rok) {
+#pragma unroll
+      for (int E = 0; E < NJ; ++E)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int e = rowB(E, r, g);
+          if (e < k) xac[(int64_t)pr * k + e] = D[E][r] * fac;
+        }
+      if (g == 0) axa[col * nrow + pr] = mean;
+    }
+#pragma unroll
+    for (int I = 0; I < NJ; ++I)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) xl[I][r] = xn[I][r];
+  }
+}
+
 // MODE: kModeColumn (one analysis per column, fused update), kModeVertical
 // (one per column and level cube, fused update of the cube's rows) or
 // kModeTransOut (one per horizontal cube, transform written to HBM)
@@ -815,93 +911,12 @@ __device__ void wave_column(const dk_batched_args& a, int64_t col, int cube, boo
     return;
   }
 
-  // update, 16 state rows at a time: D_E[4r' + g][c] = Xa(E*16 + 4r' + g, row
-  // R0 + c of the analysis); an analysis updates all nrow rows of its column,
-  // or with vertical localisation the cube's list of rows
-  const int nrow = a.nrow;
-  const double* xbc = a.xb + col * (int64_t)nrow * k;
-  double* xac = a.xa + col * (int64_t)nrow * k;
+  // update: all nrow rows of the column, or with vertical localisation the
+  // cube's list of rows
   const int32_t* rmap = VERT ? a.cube_rows + a.cube_row_off[cube] : nullptr;
-  const int nr = VERT ? a.cube_row_off[cube + 1] - a.cube_row_off[cube] : nrow;
-  auto phys = [&](int r) { return VERT ? rmap[r] : r; };
-  // Xb' of rows R0 + c (clamped to the last row, in bounds), members of the
-  // B-operand pattern; the next block's loads are issued before a block
-  // computes
-  auto load_xb = [&](int R0, double (&xl)[NJ][4]) {
-    const int xr = R0 + c;
-    const double* xrow = xbc + (int64_t)phys(xr < nr ? xr : nr - 1) * k;
-#pragma unroll
-    for (int I = 0; I < NJ; ++I)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int i = rowB(I, r, g);
-        xl[I][r] = xrow[i < k ? i : k - 1];
-      }
-  };
-  double xl[NJ][4];
-  if (nr > 0) load_xb(0, xl);
-  for (int R0 = 0; R0 < nr; R0 += 16) {
-    const int xr = R0 + c;
-    const bool rok = xr < nr;
-    const int pr = phys(rok ? xr : nr - 1);
-    double4_t D[NJ];
-#pragma unroll
-    for (int E = 0; E < NJ; ++E) D[E] = double4_t{0.0, 0.0, 0.0, 0.0};
-    double xn[NJ][4];
-    load_xb(R0 + 16, xn);
-    __builtin_amdgcn_sched_barrier(0);
-    double ssb = 0.0;
-#pragma unroll
-    for (int I = 0; I < NJ; ++I)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int i = rowB(I, r, g);
-        const double xv = (rok && i < k) ? xl[I][r] : 0.0;
-        ssb += xv * xv;
-#pragma unroll
-        for (int E = 0; E < NJ; ++E) D[E] = mfma_f64_16x16x4(T[I][E][r], xv, D[E]);
-      }
-    const double xbar = a.axb[col * nrow + pr];
-    double s = 0.0;
-#pragma unroll
-    for (int E = 0; E < NJ; ++E)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int e = rowB(E, r, g);
-        D[E][r] = e < k ? D[E][r] + xbar : 0.0;
-        s += D[E][r];
-      }
-    const double mean = sum_g(s) / (double)k;
-    double ssa = 0.0;
-#pragma unroll
-    for (int E = 0; E < NJ; ++E)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int e = rowB(E, r, g);
-        D[E][r] = e < k ? D[E][r] - mean : 0.0;
-        ssa += D[E][r] * D[E][r];
-      }
-    double fac = 1.0;
-    if (a.rtps) {
-      ssb = sum_g(ssb);
-      ssa = sum_g(ssa);
-      if (ssa > 0.0) fac = a.rtps_alpha * sqrt(ssb) / sqrt(ssa) + 1.0 - a.rtps_alpha;
-    }
-    if (rok) {
-#pragma unroll
-      for (int E = 0; E < NJ; ++E)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int e = rowB(E, r, g);
-          if (e < k) xac[(int64_t)pr * k + e] = D[E][r] * fac;
-        }
-      if (g == 0) a.axa[col * nrow + pr] = mean;
-    }
-#pragma unroll
-    for (int I = 0; I < NJ; ++I)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) xl[I][r] = xn[I][r];
-  }
+  const int nr = VERT ? a.cube_row_off[cube + 1] - a.cube_row_off[cube] : a.nrow;
+  update_rows<KP, VERT>(k, a.nrow, a.xb, a.axb, a.xa, a.axa, a.rtps != 0, a.rtps_alpha, col, T,
+                        rmap, nr);
   clk.mark(DK_PHASE_UPDATE);
   (void)NQ;
 }
@@ -955,6 +970,66 @@ __global__ void k_chain_keys(dk_batched_args a, uint32_t* key, int32_t* id) {
   id[c] = c;
 }
 
+// Update of every column from a stored transform: the column's own (trans +
+// (col - col0) k^2) or, with horizontal cubes, the mean of the covering
+// cubes' transforms (summed in the reference's cube order: longitude starts
+// outer, latitude inner; averaging transforms equals averaging the cubes'
+// re-centred analyses, which are linear in trans).  One wave per column,
+// FP64 MFMA as the fused kernel's update.
+template <int KP>
+__global__ __launch_bounds__(64) void k_update_cols(dk_update_args u) {
+  constexpr int NJ = KP / 16;
+  const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
+  const int64_t col = (int64_t)u.col0 + blockIdx.x;
+  const int k = u.k;
+  double4_t T[NJ][NJ];
+  if (u.cube_n > 1) {
+    const int ilat = (int)(col % u.nlat), ilon = (int)(col / u.nlat);
+    const int st = u.step, cn = u.cube_n;
+    const int aj0 = ilat - cn + 1 > 0 ? (ilat - cn + 1 + st - 1) / st : 0;
+    const int aj1 = min(ilat / st, u.nalat - 1);
+    const int ai0 = ilon - cn + 1 > 0 ? (ilon - cn + 1 + st - 1) / st : 0;
+    const int ai1 = min(ilon / st, u.nalon - 1);
+    const double cnt = (double)((aj1 - aj0 + 1) * (ai1 - ai0 + 1));
+#pragma unroll
+    for (int I = 0; I < NJ; ++I)
+#pragma unroll
+      for (int E = 0; E < NJ; ++E) T[I][E] = double4_t{0.0, 0.0, 0.0, 0.0};
+    for (int ai = ai0; ai <= ai1; ++ai)
+      for (int aj = aj0; aj <= aj1; ++aj) {
+        const double* tr = u.trans + ((int64_t)aj + (int64_t)u.nalat * ai) * k * k;
+#pragma unroll
+        for (int I = 0; I < NJ; ++I)
+#pragma unroll
+          for (int E = 0; E < NJ; ++E)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const int i = rowB(I, r, g), e = E * 16 + c;
+              T[I][E][r] += (i < k && e < k) ? tr[i + (int64_t)e * k] : 0.0;
+            }
+      }
+#pragma unroll
+    for (int I = 0; I < NJ; ++I)
+#pragma unroll
+      for (int E = 0; E < NJ; ++E)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) T[I][E][r] /= cnt;
+  } else {
+    const double* tr = u.trans + (int64_t)blockIdx.x * k * k;
+#pragma unroll
+    for (int I = 0; I < NJ; ++I)
+#pragma unroll
+      for (int E = 0; E < NJ; ++E)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = rowB(I, r, g), e = E * 16 + c;
+          T[I][E][r] = (i < k && e < k) ? tr[i + (int64_t)e * k] : 0.0;
+        }
+  }
+  update_rows<KP, false>(k, u.nrow, u.xb, u.axb, u.xa, u.axa, u.rtps != 0, u.rtps_alpha, col, T,
+                         nullptr, u.nrow);
+}
+
 template <int KP, int GRAM>
 int wlaunch(const dk_batched_args* a, int nblocks, hipStream_t s) {
   if (a->ncube > 0) k_letkf_wave<KP, GRAM, kModeVertical><<<nblocks, 64, 0, s>>>(*a);
@@ -999,6 +1074,15 @@ int wocc_dispatch(int gram) {
 extern "C" int dk_wave_blocks_per_cu(int k, int gram) {
   if (k <= 32) return wocc_dispatch<32>(gram);
   return wocc_dispatch<64>(gram);
+}
+
+extern "C" int dk_update_cols(const dk_update_args* u, void* stream) {
+  if (u->ncol <= 0) return 0;
+  if (u->k < 2 || u->k > 64) return (int)hipErrorInvalidValue;
+  hipStream_t s = (hipStream_t)stream;
+  if (u->k <= 32) k_update_cols<32><<<u->ncol, 64, 0, s>>>(*u);
+  else k_update_cols<64><<<u->ncol, 64, 0, s>>>(*u);
+  return (int)hipGetLastError();
 }
 
 extern "C" int dk_chain_keys(const dk_batched_args* a, uint32_t* key, int32_t* id, void* stream) {
