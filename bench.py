@@ -307,6 +307,39 @@ def time_cubes(dl, torch, data, cfg, prm, layout, steps=3,
     return out
 
 
+def time_struct(dl, torch, data, cfg, reps=3):
+    """Coupler layout conversions for one 3-D variable of the workload (all
+    k members, nz levels, the whole grid): the coupler's member-fastest block
+    into the state and back, timed with HIP events.  Bytes = read + write of
+    the block (8 B per value); the mean (_a) form for the same variable."""
+    nz, k = cfg.nz, cfg.k
+    nlat, nlon = cfg.nlat, cfg.nlon
+    n_max = 1 + 3 * nz
+    xb = data["xb"].view(nlon, nlat, n_max, k)
+    axb = data["axb"].view(nlon, nlat, n_max)
+    atm = torch.empty(nz, nlat, nlon, k, dtype=torch.float64, device=xb.device)
+    atm_a = torch.empty(nz, nlat, nlon, dtype=torch.float64, device=xb.device)
+    out = {}
+    calls = {"endnn2atm": lambda: dl.struct_endnn2atm(1, 2, xb, atm),
+             "atm2endnn": lambda: dl.struct_atm2endnn(1, 2, atm, xb),
+             "endnn2atm_a": lambda: dl.struct_endnn2atm_a(2, axb, atm_a),
+             "atm2endnn_a": lambda: dl.struct_atm2endnn_a(2, atm_a, axb)}
+    for name, fn in calls.items():
+        fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        nbytes = 2.0 * 8 * nz * nlat * nlon * (1 if name.endswith("_a") else k)
+        out[name] = {"ms": ms, "GBps": nbytes / (ms / 1e3) / 1e9, "bytes": nbytes}
+    del atm, atm_a
+    return out
+
+
 def copy_probe(torch, nbytes=1 << 30, reps=3):
     """A plain device copy of known size (the FETCH_SIZE / WRITE_SIZE scale)."""
     src = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda").fill_(1.0)
@@ -465,6 +498,9 @@ def main():
         log("timing horizontal cubes")
         cubes = time_cubes(dl, torch, data, cfg, prm,
                            res.get("batched_bf16x3", {}).get("tuning"))
+    log("timing the coupler layout conversions")
+    struct = time_struct(dl, torch, data, cfg)
+    log(f"  {struct}")
     log("timing the observation operator")
     obsop = time_obs_operator(dl, torch, data, cfg)
     log(f"  {obsop}")
@@ -489,6 +525,7 @@ def main():
                    "speedup_vs_plain_fp64": plain_ms / r["ms"] if plain_ms else None},
         "roofline": roofline, "cpu_baseline": cpu, "forms": out_forms,
         "obs_operator": obsop, "vertical_localisation": vertical, "horizontal_cubes": cubes,
+        "coupler_layout": struct,
     }
     print(json.dumps(line), flush=True)
 

@@ -1328,3 +1328,58 @@ int dida_letkf_obs_operator(int32_t k, int32_t nrow, int32_t nlat, int32_t nlon,
     HIP_TRY(dk_obs_operator(&a, stream));
     return DIDA_OK;
 }
+
+static int check_struct(int32_t ens, int32_t n, int32_t nx, int32_t ny, int32_t ens_max,
+                        int32_t n_max, int32_t ens_start, int32_t n_start, const void *a,
+                        const void *b)
+{
+    if (!a || !b) return set_err(DIDA_ERR_ARG, "struct: null array");
+    if (ens < 0 || n < 0 || nx < 0 || ny < 0) return set_err(DIDA_ERR_ARG, "struct: negative size");
+    if (ens_start < 1 || ens_start + ens - 1 > ens_max || n_start < 1 || n_start + n - 1 > n_max)
+        return set_err(DIDA_ERR_ARG,
+                       "struct: block (members %d..%d, rows %d..%d) outside (%d, %d)", ens_start,
+                       ens_start + ens - 1, n_start, n_start + n - 1, ens_max, n_max);
+    return DIDA_OK;
+}
+
+int dida_struct_atm2endnn(int32_t ens, int32_t n, int32_t nx, int32_t ny, int32_t ens_max,
+                          int32_t n_max, int32_t ens_start, int32_t n_start,
+                          const double *xb_atm, double *xb, void *stream)
+{
+    int rc = check_struct(ens, n, nx, ny, ens_max, n_max, ens_start, n_start, xb_atm, xb);
+    if (rc) return rc;
+    HIP_TRY(dk_struct_ens(1, ens, n, nx, ny, ens_max, n_max, ens_start - 1, n_start - 1, xb_atm,
+                          xb, stream));
+    return DIDA_OK;
+}
+
+int dida_struct_endnn2atm(int32_t ens, int32_t n, int32_t nx, int32_t ny, int32_t ens_max,
+                          int32_t n_max, int32_t ens_start, int32_t n_start, const double *xb,
+                          double *xb_atm, void *stream)
+{
+    int rc = check_struct(ens, n, nx, ny, ens_max, n_max, ens_start, n_start, xb, xb_atm);
+    if (rc) return rc;
+    HIP_TRY(dk_struct_ens(0, ens, n, nx, ny, ens_max, n_max, ens_start - 1, n_start - 1, xb,
+                          xb_atm, stream));
+    return DIDA_OK;
+}
+
+int dida_struct_atm2endnn_a(int32_t n, int32_t nx, int32_t ny, int32_t n_max, int32_t n_start,
+                            const double *axb_atm, double *axb, void *stream)
+{
+    int rc = check_struct(1, n, nx, ny, 1, n_max, 1, n_start, axb_atm, axb);
+    if (rc) return rc;
+    if (nx > 65535) return set_err(DIDA_ERR_UNSUPPORTED, "struct: nx > 65535");
+    HIP_TRY(dk_struct_mean(1, n, nx, ny, n_max, n_start - 1, axb_atm, axb, stream));
+    return DIDA_OK;
+}
+
+int dida_struct_endnn2atm_a(int32_t n, int32_t nx, int32_t ny, int32_t n_max, int32_t n_start,
+                            const double *axb, double *axb_atm, void *stream)
+{
+    int rc = check_struct(1, n, nx, ny, 1, n_max, 1, n_start, axb, axb_atm);
+    if (rc) return rc;
+    if (nx > 65535) return set_err(DIDA_ERR_UNSUPPORTED, "struct: nx > 65535");
+    HIP_TRY(dk_struct_mean(0, n, nx, ny, n_max, n_start - 1, axb, axb_atm, stream));
+    return DIDA_OK;
+}

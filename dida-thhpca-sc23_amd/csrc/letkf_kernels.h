@@ -250,6 +250,14 @@ typedef struct dk_post_args {
 
 int dk_post(const dk_post_args *a, void *stream);
 
+/* Coupler layout conversions (0-based starts): to_state 1 copies the
+ * coupler block into the state, 0 the state block out. */
+int dk_struct_ens(int to_state, int32_t ens, int32_t n, int32_t nx, int32_t ny, int32_t ens_max,
+                  int32_t n_max, int32_t e0, int32_t n0, const double *src, double *dst,
+                  void *stream);
+int dk_struct_mean(int to_state, int32_t n, int32_t nx, int32_t ny, int32_t n_max, int32_t n0,
+                   const double *src, double *dst, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

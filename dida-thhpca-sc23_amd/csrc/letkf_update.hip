@@ -144,7 +144,92 @@ __global__ void k_post(dk_post_args a) {
   a.spread_a[t] = sqrt(sa / k);
 }
 
+// Coupler block xb_atm[((nn nx + i) ny + j) ens + e] <-> state
+// xb[((j nx + i) n_max + n0 + nn) ens_max + e0 + e], one thread per element in
+// coupler order: coupler-side accesses are fully coalesced, state-side ones
+// contiguous over the ens members of a (row, column).
+template <bool TO_STATE>
+__global__ void k_struct_ens(int32_t ens, int32_t n, int32_t nx, int32_t ny, int32_t ens_max,
+                             int32_t n_max, int32_t e0, int32_t n0, const double* __restrict__ src,
+                             double* __restrict__ dst) {
+  const int64_t total = (int64_t)ens * n * nx * ny;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int e = (int)(t % ens);
+    int64_t r = t / ens;
+    const int j = (int)(r % ny);
+    r /= ny;
+    const int i = (int)(r % nx);
+    const int nn = (int)(r / nx);
+    const int64_t s = (((int64_t)j * nx + i) * n_max + n0 + nn) * ens_max + e0 + e;
+    if (TO_STATE) dst[s] = src[t];
+    else dst[t] = src[s];
+  }
+}
+
+// Means: axb_atm[(nn nx + i) ny + j] <-> axb[(j nx + i) n_max + n0 + nn], a
+// transpose of (nn, j) per latitude i through a 32 x 33 LDS tile so that
+// both sides are read and written along their contiguous index.
+template <bool TO_STATE>
+__global__ __launch_bounds__(256) void k_struct_mean(int32_t n, int32_t nx, int32_t ny,
+                                                     int32_t n_max, int32_t n0,
+                                                     const double* __restrict__ src,
+                                                     double* __restrict__ dst) {
+  __shared__ double tile[32][33];
+  const int j0 = blockIdx.x * 32, nn0 = blockIdx.y * 32, i = blockIdx.z;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  auto atm = [&](int nn, int j) { return ((int64_t)nn * nx + i) * ny + j; };
+  auto st = [&](int nn, int j) { return ((int64_t)j * nx + i) * n_max + n0 + nn; };
+  if (TO_STATE) {
+    for (int y = ty; y < 32; y += 8) {
+      const int nn = nn0 + y, j = j0 + tx;
+      if (nn < n && j < ny) tile[y][tx] = src[atm(nn, j)];
+    }
+    __syncthreads();
+    for (int y = ty; y < 32; y += 8) {
+      const int j = j0 + y, nn = nn0 + tx;
+      if (nn < n && j < ny) dst[st(nn, j)] = tile[tx][y];
+    }
+  } else {
+    for (int y = ty; y < 32; y += 8) {
+      const int j = j0 + y, nn = nn0 + tx;
+      if (nn < n && j < ny) tile[tx][y] = src[st(nn, j)];
+    }
+    __syncthreads();
+    for (int y = ty; y < 32; y += 8) {
+      const int nn = nn0 + y, j = j0 + tx;
+      if (nn < n && j < ny) dst[atm(nn, j)] = tile[y][tx];
+    }
+  }
+}
+
 }  // namespace
+
+extern "C" int dk_struct_ens(int to_state, int32_t ens, int32_t n, int32_t nx, int32_t ny,
+                             int32_t ens_max, int32_t n_max, int32_t e0, int32_t n0,
+                             const double* src, double* dst, void* stream) {
+  const int64_t total = (int64_t)ens * n * nx * ny;
+  if (total <= 0) return 0;
+  const int64_t nb = (total + 255) / 256;
+  const unsigned blocks = (unsigned)(nb < 65536 * 8 ? nb : 65536 * 8);
+  hipStream_t s = (hipStream_t)stream;
+  if (to_state)
+    k_struct_ens<true><<<blocks, 256, 0, s>>>(ens, n, nx, ny, ens_max, n_max, e0, n0, src, dst);
+  else
+    k_struct_ens<false><<<blocks, 256, 0, s>>>(ens, n, nx, ny, ens_max, n_max, e0, n0, src, dst);
+  return (int)hipGetLastError();
+}
+
+extern "C" int dk_struct_mean(int to_state, int32_t n, int32_t nx, int32_t ny, int32_t n_max,
+                              int32_t n0, const double* src, double* dst, void* stream) {
+  if ((int64_t)n * nx * ny <= 0) return 0;
+  if (nx > 65535) return (int)hipErrorInvalidValue;
+  const dim3 grid((unsigned)((ny + 31) / 32), (unsigned)((n + 31) / 32), (unsigned)nx);
+  hipStream_t s = (hipStream_t)stream;
+  if (to_state) k_struct_mean<true><<<grid, 256, 0, s>>>(n, nx, ny, n_max, n0, src, dst);
+  else k_struct_mean<false><<<grid, 256, 0, s>>>(n, nx, ny, n_max, n0, src, dst);
+  return (int)hipGetLastError();
+}
 
 extern "C" int dk_update_f64(const dk_update_args* u, void* stream) {
   if (u->ncol <= 0) return 0;

@@ -31,7 +31,8 @@ EXPORTED = ["dida_letkf_params_default", "letkf_core_", "letkf_core_batch_",
             "dida_letkf_post", "dida_letkf_last_error", "dida_letkf_set_timing",
             "dida_letkf_stage_ms", "dida_letkf_phase_cycles", "dida_letkf_group_capture",
             "dida_letkf_group_launch", "dida_letkf_group_destroy", "dida_letkf_autotune",
-            "dida_letkf_obs_operator", "dida_letkf_set_levels"]
+            "dida_letkf_obs_operator", "dida_letkf_set_levels", "dida_struct_atm2endnn",
+            "dida_struct_endnn2atm", "dida_struct_atm2endnn_a", "dida_struct_endnn2atm_a"]
 PHASES = ["gram", "warm", "jacobi", "transform", "update"]
 STAGES = ["bin", "select", "transform", "update"]
 
@@ -132,6 +133,10 @@ def lib():
                                               C.POINTER(ObsRecordsC), vp, vp, vp, vp, vp, vp]
         L.dida_letkf_set_levels.argtypes = [vp, C.POINTER(VertCoordC), C.c_int32, C.c_int32,
                                             vp, vp]
+        for nm in ("dida_struct_atm2endnn", "dida_struct_endnn2atm"):
+            getattr(L, nm).argtypes = [C.c_int32] * 8 + [vp, vp, vp]
+        for nm in ("dida_struct_atm2endnn_a", "dida_struct_endnn2atm_a"):
+            getattr(L, nm).argtypes = [C.c_int32] * 5 + [vp, vp, vp]
         L.dida_letkf_autotune.argtypes = [vp, C.POINTER(ObsC), vp, vp, vp, vp, vp,
                                           C.POINTER(TuningC)]
         L.dida_letkf_group_destroy.argtypes = [vp]
@@ -403,6 +408,46 @@ def post(k, nrow, nlat, nlon, grid_lat, polar_lat, gamma1, gamma2, xb, axb, xa, 
                                        float(gamma1), float(gamma2), _ptr(xb), _ptr(axb),
                                        _ptr(xa), _ptr(axa), _ptr(xa_out), _ptr(spread_b),
                                        _ptr(spread_a), _stream_ptr(stream)), "post")
+
+
+def struct_atm2endnn(ens_start, n_start, xb_atm, xb, stream=None) -> int:
+    """dida_struct_atm2endnn on device tensors: xb_atm (n, nx, ny, ens) (the
+    coupler's member-fastest block), xb (ny, nx, n_max, ens_max) updated in
+    place at 1-based member / row starts."""
+    n, nx, ny, ens = xb_atm.shape
+    ny2, nx2, n_max, ens_max = xb.shape
+    if (ny2, nx2) != (ny, nx):
+        raise ValueError("grid shapes differ")
+    return check(lib().dida_struct_atm2endnn(ens, n, nx, ny, ens_max, n_max, int(ens_start),
+                                             int(n_start), _ptr(xb_atm), _ptr(xb),
+                                             _stream_ptr(stream)), "struct_atm2endnn")
+
+
+def struct_endnn2atm(ens_start, n_start, xb, xb_atm, stream=None) -> int:
+    """dida_struct_endnn2atm: the block of xb (ny, nx, n_max, ens_max) into
+    xb_atm (n, nx, ny, ens)."""
+    n, nx, ny, ens = xb_atm.shape
+    ny2, nx2, n_max, ens_max = xb.shape
+    if (ny2, nx2) != (ny, nx):
+        raise ValueError("grid shapes differ")
+    return check(lib().dida_struct_endnn2atm(ens, n, nx, ny, ens_max, n_max, int(ens_start),
+                                             int(n_start), _ptr(xb), _ptr(xb_atm),
+                                             _stream_ptr(stream)), "struct_endnn2atm")
+
+
+def struct_atm2endnn_a(n_start, axb_atm, axb, stream=None) -> int:
+    """Means: axb_atm (n, nx, ny) into axb (ny, nx, n_max)."""
+    n, nx, ny = axb_atm.shape
+    return check(lib().dida_struct_atm2endnn_a(n, nx, ny, axb.shape[2], int(n_start),
+                                               _ptr(axb_atm), _ptr(axb), _stream_ptr(stream)),
+                 "struct_atm2endnn_a")
+
+
+def struct_endnn2atm_a(n_start, axb, axb_atm, stream=None) -> int:
+    n, nx, ny = axb_atm.shape
+    return check(lib().dida_struct_endnn2atm_a(n, nx, ny, axb.shape[2], int(n_start),
+                                               _ptr(axb), _ptr(axb_atm), _stream_ptr(stream)),
+                 "struct_endnn2atm_a")
 
 
 def obs_operator(grid_lat, grid_lon, xb, axb, ps_row, vcoord, var_start, var_lev, vid,

@@ -14,6 +14,8 @@
  *                      (gen_yb_dis :72, letkf_core :97, xb_2_xa :111)
  *   dida_letkf_post    the post-analysis blend and spreads of run_da_parallel,
  *                      code/x86/TradDA/DA/EnDNN/src/module_da_parallel.F90:216-259
+ *   dida_struct_*      struct_atm2endnn / struct_endnn2atm (and _a),
+ *                      code/x86/TradDA/DA/EnDNN/src/data_trans.F90:2-57
  */
 #ifndef DIDA_LETKF_H
 #define DIDA_LETKF_H
@@ -334,6 +336,27 @@ int dida_letkf_post(int32_t k, int32_t nrow, int32_t nlat, int32_t nlon, const f
                     double polar_lat, double gamma1, double gamma2, const double *xb,
                     const double *axb, const double *xa, const double *axa, double *xa_out,
                     double *spread_b, double *spread_a, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Coupler layout conversions (data_trans.F90:2-57, called per member group
+ * and variable at module_da_coupler.F90:378 and :405).  The coupler's
+ * member-fastest block xb_atm(ens, ny, nx, n) -- ny longitudes, nx latitudes,
+ * n levels of one variable, as unpacked from the MCT attribute vector -- is
+ * copied into, or out of, the analysis state xb(ens_max, n_max, nx, ny) at
+ * members ens_start .. ens_start + ens - 1 and rows n_start .. n_start + n - 1
+ * (1-based, as the reference passes them); state entries outside the block are
+ * left as they are.  The _a forms move the means axb_atm(ny, nx, n) <->
+ * axb(n_max, nx, ny).  Device pointers, enqueued on `stream`. */
+int dida_struct_atm2endnn(int32_t ens, int32_t n, int32_t nx, int32_t ny, int32_t ens_max,
+                          int32_t n_max, int32_t ens_start, int32_t n_start,
+                          const double *xb_atm, double *xb, void *stream);
+int dida_struct_endnn2atm(int32_t ens, int32_t n, int32_t nx, int32_t ny, int32_t ens_max,
+                          int32_t n_max, int32_t ens_start, int32_t n_start, const double *xb,
+                          double *xb_atm, void *stream);
+int dida_struct_atm2endnn_a(int32_t n, int32_t nx, int32_t ny, int32_t n_max, int32_t n_start,
+                            const double *axb_atm, double *axb, void *stream);
+int dida_struct_endnn2atm_a(int32_t n, int32_t nx, int32_t ny, int32_t n_max, int32_t n_start,
+                            const double *axb, double *axb_atm, void *stream);
 
 /* Last error message of this thread (static storage). */
 const char *dida_letkf_last_error(void);

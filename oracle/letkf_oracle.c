@@ -927,3 +927,62 @@ int orc_sweep_vertical(const orc_params *prm, const float *grid_lat, const float
     }
     return nfail;
 }
+
+/* data_trans.F90:2-20 (struct_atm2endnn): xb2(ens_start:, nn, i, j) =
+ * xb(1:ens, j, i, nn - n_start + 1), Fortran column-major. */
+void orc_struct_atm2endnn(int ens, int n, int nx, int ny, int ens_max, int n_max, int ens_start,
+                          int n_start, const double *xb_atm, double *xb)
+{
+    for (int nn = n_start; nn <= n_start + n - 1; ++nn)
+        for (int i = 1; i <= nx; ++i)
+            for (int j = 1; j <= ny; ++j)
+                for (int e = 1; e <= ens; ++e) {
+                    const size_t src = (size_t)(e - 1) +
+                                       (size_t)ens * ((j - 1) + (size_t)ny * ((i - 1) +
+                                       (size_t)nx * (nn - n_start)));
+                    const size_t dst = (size_t)(ens_start + e - 2) +
+                                       (size_t)ens_max * ((nn - 1) + (size_t)n_max * ((i - 1) +
+                                       (size_t)nx * (j - 1)));
+                    xb[dst] = xb_atm[src];
+                }
+}
+
+/* data_trans.F90:22-38 (struct_endnn2atm). */
+void orc_struct_endnn2atm(int ens, int n, int nx, int ny, int ens_max, int n_max, int ens_start,
+                          int n_start, const double *xb, double *xb_atm)
+{
+    for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i)
+            for (int nn = 1; nn <= n; ++nn)
+                for (int e = 1; e <= ens; ++e) {
+                    const size_t dst = (size_t)(e - 1) +
+                                       (size_t)ens * ((j - 1) + (size_t)ny * ((i - 1) +
+                                       (size_t)nx * (nn - 1)));
+                    const size_t src = (size_t)(ens_start + e - 2) +
+                                       (size_t)ens_max * ((nn + n_start - 2) + (size_t)n_max *
+                                       ((i - 1) + (size_t)nx * (j - 1)));
+                    xb_atm[dst] = xb[src];
+                }
+}
+
+/* data_trans.F90:40-50 (struct_atm2endnn_a). */
+void orc_struct_atm2endnn_a(int n, int nx, int ny, int n_max, int n_start, const double *axb_atm,
+                            double *axb)
+{
+    for (int nn = n_start; nn <= n_start + n - 1; ++nn)
+        for (int i = 1; i <= nx; ++i)
+            for (int j = 1; j <= ny; ++j)
+                axb[(size_t)(nn - 1) + (size_t)n_max * ((i - 1) + (size_t)nx * (j - 1))] =
+                    axb_atm[(size_t)(j - 1) + (size_t)ny * ((i - 1) + (size_t)nx * (nn - n_start))];
+}
+
+/* data_trans.F90:52-57 (struct_endnn2atm_a). */
+void orc_struct_endnn2atm_a(int n, int nx, int ny, int n_max, int n_start, const double *axb,
+                            double *axb_atm)
+{
+    for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i)
+            for (int nn = 1; nn <= n; ++nn)
+                axb_atm[(size_t)(j - 1) + (size_t)ny * ((i - 1) + (size_t)nx * (nn - 1))] =
+                    axb[(size_t)(nn + n_start - 2) + (size_t)n_max * ((i - 1) + (size_t)nx * (j - 1))];
+}

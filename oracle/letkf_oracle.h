@@ -21,6 +21,7 @@
  *   orc_sweep_cubes   code/x86/TradDA/DA/LETKF/src/letkf_compute.F90:12-126 with
  *                     cube_n > 1, and :128-228 (letkf_compute, overlapping cubes)
  *                     with xb_add2_xa :304-354 and the cube_count average
+ *   orc_struct_*      code/x86/TradDA/DA/EnDNN/src/data_trans.F90:2-57
  *   orc_obs_operator  code/x86/TradDA/DA/LETKF/src/letkf_obs.F90:120-178, 187-243
  *                     (find_obs_grid), obs_make.F90:233-381 (obs_interpolation_3d_2,
  *                     ps2ph_3d_interp), DA/utils/vert_coord_mod.F90:547-563
@@ -244,4 +245,17 @@ void orc_post(int mode, int k, int nrow, int nlat, int nlon,
 #ifdef __cplusplus
 }
 #endif
+/* Coupler layout conversions (data_trans.F90:2-57), Fortran loops restated
+ * with 1-based starts: xb_atm(ens, ny, nx, n) <-> xb(ens_max, n_max, nx, ny)
+ * at members ens_start.., rows n_start..; means axb_atm(ny, nx, n) <->
+ * axb(n_max, nx, ny). */
+void orc_struct_atm2endnn(int ens, int n, int nx, int ny, int ens_max, int n_max, int ens_start,
+                          int n_start, const double *xb_atm, double *xb);
+void orc_struct_endnn2atm(int ens, int n, int nx, int ny, int ens_max, int n_max, int ens_start,
+                          int n_start, const double *xb, double *xb_atm);
+void orc_struct_atm2endnn_a(int n, int nx, int ny, int n_max, int n_start, const double *axb_atm,
+                            double *axb);
+void orc_struct_endnn2atm_a(int n, int nx, int ny, int n_max, int n_start, const double *axb,
+                            double *axb_atm);
+
 #endif

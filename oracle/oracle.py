@@ -98,6 +98,12 @@ def lib():
         L.orc_sweep_cubes.argtypes = [C.POINTER(OrcParams), _fp, _fp, C.POINTER(OrcObs), _dp,
                                       _dp, _dp, _dp, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
                                       C.c_int]
+        for nm in ("orc_struct_atm2endnn", "orc_struct_endnn2atm"):
+            getattr(L, nm).restype = None
+            getattr(L, nm).argtypes = [C.c_int] * 8 + [_dp, _dp]
+        for nm in ("orc_struct_atm2endnn_a", "orc_struct_endnn2atm_a"):
+            getattr(L, nm).restype = None
+            getattr(L, nm).argtypes = [C.c_int] * 5 + [_dp, _dp]
         L.orc_centre_obs.restype = None
         L.orc_centre_obs.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp]
         L.orc_sweep_vertical.restype = C.c_int
@@ -405,3 +411,42 @@ def sweep_vertical(prm: OrcParams, grid_lat, grid_lon, obs: Obs, obs_ph, xb, axb
         int(ps_row), len(vs), vs, vl, float(vert_dist), int(cube_nz), int(bool(gc_filter_vertical)),
         int(nthreads))
     return {"xa": xa.reshape(ncol, n, k), "axa": axa.reshape(ncol, n), "nfail": nfail}
+
+
+def struct_atm2endnn(ens_start, n_start, xb_atm, xb):
+    """data_trans.F90 struct_atm2endnn.  xb_atm: C array (n, nx, ny, ens)
+    (= Fortran xb(ens, ny, nx, n)); xb: C array (ny, nx, n_max, ens_max),
+    updated in place (1-based starts)."""
+    n, nx, ny, ens = xb_atm.shape
+    ny2, nx2, n_max, ens_max = xb.shape
+    assert (ny2, nx2) == (ny, nx) and xb.flags.c_contiguous and xb.dtype == np.float64
+    lib().orc_struct_atm2endnn(ens, n, nx, ny, ens_max, n_max, ens_start, n_start,
+                               np.ascontiguousarray(xb_atm, np.float64).reshape(-1),
+                               xb.reshape(-1))
+
+
+def struct_endnn2atm(ens, n, ens_start, n_start, xb):
+    """struct_endnn2atm: returns the (n, nx, ny, ens) block of xb (C array
+    (ny, nx, n_max, ens_max))."""
+    ny, nx, n_max, ens_max = xb.shape
+    out = np.zeros(n * nx * ny * ens)
+    lib().orc_struct_endnn2atm(ens, n, nx, ny, ens_max, n_max, ens_start, n_start,
+                               np.ascontiguousarray(xb, np.float64).reshape(-1), out)
+    return out.reshape(n, nx, ny, ens)
+
+
+def struct_atm2endnn_a(n_start, axb_atm, axb):
+    """struct_atm2endnn_a: axb_atm C (n, nx, ny) into axb C (ny, nx, n_max)."""
+    n, nx, ny = axb_atm.shape
+    assert axb.flags.c_contiguous and axb.dtype == np.float64
+    lib().orc_struct_atm2endnn_a(n, nx, ny, axb.shape[2], n_start,
+                                 np.ascontiguousarray(axb_atm, np.float64).reshape(-1),
+                                 axb.reshape(-1))
+
+
+def struct_endnn2atm_a(n, n_start, axb):
+    ny, nx, n_max = axb.shape
+    out = np.zeros(n * nx * ny)
+    lib().orc_struct_endnn2atm_a(n, nx, ny, n_max, n_start,
+                                 np.ascontiguousarray(axb, np.float64).reshape(-1), out)
+    return out.reshape(n, nx, ny)
