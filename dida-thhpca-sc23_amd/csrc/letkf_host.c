@@ -175,6 +175,12 @@ struct dida_letkf_ctx {
     dida_vert_coord vc;
     double *obs_lnp, *cube_lnp;
     int32_t *cube_row_off, *cube_rows;
+    /* selection lists ordered by obs ln p (capacity entries each) */
+    float *vkey_in, *vkey_out;
+    int32_t *vpos_in, *vpos_out, *vidx;
+    double *vw;
+    void *vsort_tmp;
+    size_t vsort_tmp_bytes;
     unsigned long long *phase_clk; /* [DK_NPHASE], allocated when timing is on */
     /* per-stage HIP event timing (eager sweeps only) */
     int timing;
@@ -208,7 +214,8 @@ int dida_letkf_destroy(dida_letkf_ctx *c)
                     c->trans, c->info, c->chain_key, c->chain_key_sorted, c->chain_id,
                     c->chain_order, c->queue_head, c->chain_tmp, c->ymax, c->phase_clk,
                     c->obs_lnp, c->cube_lnp, c->cube_row_off, c->cube_rows, c->ca_lat,
-                    c->ca_lon, c->trans_all};
+                    c->ca_lon, c->trans_all, c->vkey_in, c->vkey_out, c->vpos_in,
+                    c->vpos_out, c->vidx, c->vw, c->vsort_tmp};
     for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; ++i) dfree(ptrs[i]);
     free(c);
     return DIDA_OK;
@@ -380,6 +387,28 @@ int dida_letkf_set_grid(dida_letkf_ctx *c, const float *grid_lat, const float *g
     return DIDA_OK;
 }
 
+/* Buffers that order the selection lists by obs ln p (vertical
+ * localisation), once the selection capacity is known. */
+static int alloc_vertical_sort(dida_letkf_ctx *c)
+{
+    if (c->prm.vert_dist <= 0.0 || c->capacity <= 0 || c->vkey_in) return DIDA_OK;
+    const size_t cap = (size_t)c->capacity;
+    size_t tb = 0;
+    if (dk_seg_sort_lnp(NULL, &tb, c->nsel, c->capacity, c->offset, NULL, NULL, NULL, NULL, NULL,
+                        NULL, NULL, NULL, NULL, NULL))
+        return set_err(DIDA_ERR_HIP, "segmented sort size query failed");
+    if (hipMalloc((void **)&c->vkey_in, cap * sizeof(float)) ||
+        hipMalloc((void **)&c->vkey_out, cap * sizeof(float)) ||
+        hipMalloc((void **)&c->vpos_in, cap * sizeof(int32_t)) ||
+        hipMalloc((void **)&c->vpos_out, cap * sizeof(int32_t)) ||
+        hipMalloc((void **)&c->vidx, cap * sizeof(int32_t)) ||
+        hipMalloc((void **)&c->vw, cap * sizeof(double)) ||
+        hipMalloc(&c->vsort_tmp, tb ? tb : 16))
+        return set_err(DIDA_ERR_NOMEM, "vertical ordering buffers of %zu entries", cap);
+    c->vsort_tmp_bytes = tb;
+    return DIDA_OK;
+}
+
 int dida_letkf_set_levels(dida_letkf_ctx *c, const dida_vert_coord *vc, int32_t ps_row,
                           int32_t nvar, const int32_t *var_start, const int32_t *var_lev)
 {
@@ -457,6 +486,7 @@ int dida_letkf_set_levels(dida_letkf_ctx *c, const dida_vert_coord *vc, int32_t 
         c->vc = *vc;
         c->ps_row = ps_row;
         c->ncube = ncube;
+        rc = alloc_vertical_sort(c);
     }
     free(seen);
     free(off);
@@ -544,7 +574,7 @@ static int ensure_capacity(dida_letkf_ctx *c, hipStream_t s)
         hipMalloc((void **)&c->csr_w, (size_t)cap * sizeof(double)) != hipSuccess)
         return set_err(DIDA_ERR_NOMEM, "selection workspace of %lld entries", (long long)cap);
     c->capacity = cap;
-    return DIDA_OK;
+    return alloc_vertical_sort(c);
 }
 
 /* Horizontal cubes: every column from the mean transform of its covering
@@ -643,6 +673,15 @@ static int sweep_batched(dida_letkf_ctx *c, const double *xb, const double *axb,
         ba.cube_lnp = c->cube_lnp;
         ba.cube_row_off = c->cube_row_off;
         ba.cube_rows = c->cube_rows;
+        if (c->vkey_in) {
+            size_t tb = c->vsort_tmp_bytes;
+            HIP_TRY(dk_seg_sort_lnp(c->vsort_tmp, &tb, c->nsel, c->capacity, c->offset,
+                                    c->csr_idx, c->csr_w, c->obs_lnp, c->vkey_in, c->vkey_out,
+                                    c->vpos_in, c->vpos_out, c->vidx, c->vw, s));
+            ba.idx = c->vidx;
+            ba.rloc = c->vw;
+            ba.sel_key = c->vkey_out;
+        }
     }
     if (c->prm.gram == DIDA_GRAM_FP16X3)
         HIP_TRY(dk_obs_ymax(c->last_nobs_pending, ba.k, c->yb_b, c->rinv_b, c->ymax, s));

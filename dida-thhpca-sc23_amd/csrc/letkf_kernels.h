@@ -177,6 +177,8 @@ typedef struct dk_batched_args {
     int32_t ncube, gc_vert;
     double vert_dist;
     const double *obs_lnp;      /* [binned obs] ln p */
+    const float *sel_key;       /* per selection entry: (float) ln p, ascending within
+                                   each column (idx / rloc reordered alike) */
     const double *cube_lnp;     /* [col * ncube + q] ln p of the cube's centre level */
     const int32_t *cube_row_off, *cube_rows;
     /* optional: write trans(i, e) = W + wbar 1^T of analysis col at
@@ -216,6 +218,14 @@ typedef struct dk_cube_lnp_args {
     double *cube_lnp;
 } dk_cube_lnp_args;
 int dk_cube_lnp(const dk_cube_lnp_args *a, void *stream);
+/* Vertical localisation: every column's selection list [offset[c],
+ * offset[c+1]) reordered by ascending obs ln p (stable): key_out = (float)
+ * lnp of the entries, idx2 / w2 = the reordered idx / w.  tmp == NULL queries
+ * the rocPRIM work space for `capacity` entries. */
+int dk_seg_sort_lnp(void *tmp, size_t *tmp_bytes, int64_t nseg, int64_t capacity,
+                    const int64_t *offset, const int32_t *idx, const double *w,
+                    const double *lnp, float *key_in, float *key_out, int32_t *pos_in,
+                    int32_t *pos_out, int32_t *idx2, double *w2, void *stream);
 /* lnp[i] = log(ph[perm[i]]) (obs pressures in bin order). */
 int dk_obs_lnp(int32_t nobs, const int32_t *perm, const double *ph, double *lnp, void *stream);
 

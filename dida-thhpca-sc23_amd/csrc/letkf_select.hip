@@ -11,6 +11,7 @@
 #include "letkf_kernels.h"
 
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_segmented_radix_sort.hpp>
 
 using namespace dida;
 
@@ -372,5 +373,73 @@ extern "C" int dk_scan_offsets(void* tmp, size_t* tmp_bytes, const int32_t* nobs
   k_scan_block_sums<<<nb, kScanBlock, 0, s>>>(nobsl, n, bsum);
   k_scan_sums<<<1, kScanBlock, 0, s>>>(bsum, nb);
   k_scan_apply<<<nb, kScanBlock, 0, s>>>(nobsl, n, bsum, offset);
+  return (int)hipGetLastError();
+}
+
+// Vertical localisation: each column's selection list ordered by obs ln p.
+// Keys (float ln p) and in-segment positions, one block per column.
+__global__ void k_seg_keys(int64_t nseg, const int64_t* __restrict__ offset,
+                           const int32_t* __restrict__ idx, const double* __restrict__ lnp,
+                           float* __restrict__ key, int32_t* __restrict__ pos) {
+  const int64_t col = blockIdx.x;
+  if (col >= nseg) return;
+  const int64_t o0 = offset[col], o1 = offset[col + 1];
+  for (int64_t j = o0 + threadIdx.x; j < o1; j += blockDim.x) {
+    key[j] = (float)lnp[idx[j]];
+    pos[j] = (int32_t)(j - o0);
+  }
+}
+
+__global__ void k_seg_gather(int64_t nseg, const int64_t* __restrict__ offset,
+                             const int32_t* __restrict__ pos, const int32_t* __restrict__ idx,
+                             const double* __restrict__ w, int32_t* __restrict__ idx2,
+                             double* __restrict__ w2) {
+  const int64_t col = blockIdx.x;
+  if (col >= nseg) return;
+  const int64_t o0 = offset[col], o1 = offset[col + 1];
+  for (int64_t j = o0 + threadIdx.x; j < o1; j += blockDim.x) {
+    const int64_t src = o0 + pos[j];
+    idx2[j] = idx[src];
+    w2[j] = w[src];
+  }
+}
+
+// Segment bounds clamped to the workspace (lists past it are not sorted;
+// their analyses see no obs and the sweep reports the overflow).
+__global__ void k_clamp_offsets(int64_t n, const int64_t* __restrict__ offset, int64_t capacity,
+                                int64_t* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i <= n) out[i] = offset[i] < capacity ? offset[i] : capacity;
+}
+
+extern "C" int dk_seg_sort_lnp(void* tmp, size_t* tmp_bytes, int64_t nseg, int64_t capacity,
+                               const int64_t* offset, const int32_t* idx, const double* w,
+                               const double* lnp, float* key_in, float* key_out,
+                               int32_t* pos_in, int32_t* pos_out, int32_t* idx2, double* w2,
+                               void* stream) {
+  // work space: rocPRIM's, then the clamped offsets (nseg + 1, 256-aligned)
+  const size_t off_bytes = ((size_t)(nseg + 1) * sizeof(int64_t) + 255) & ~(size_t)255;
+  hipStream_t s = (hipStream_t)stream;
+  if (tmp == nullptr) {
+    size_t rb = 0;
+    const int64_t* null_off = nullptr;
+    int e = (int)rocprim::segmented_radix_sort_pairs(
+        nullptr, rb, key_in, key_out, pos_in, pos_out, (unsigned int)capacity,
+        (unsigned int)nseg, null_off, null_off, 0, 32, s, false);
+    *tmp_bytes = off_bytes + ((rb + 255) & ~(size_t)255);
+    return e;
+  }
+  if (nseg <= 0) return 0;
+  int64_t* coff = (int64_t*)tmp;
+  void* rtmp = (char*)tmp + off_bytes;
+  size_t rb = *tmp_bytes - off_bytes;
+  k_clamp_offsets<<<(unsigned)((nseg + 1 + 255) / 256), 256, 0, s>>>(nseg, offset, capacity, coff);
+  k_seg_keys<<<(unsigned)nseg, 256, 0, s>>>(nseg, coff, idx, lnp, key_in, pos_in);
+  int e = (int)rocprim::segmented_radix_sort_pairs(rtmp, rb, key_in, key_out, pos_in, pos_out,
+                                                   (unsigned int)capacity, (unsigned int)nseg,
+                                                   (const int64_t*)coff, (const int64_t*)coff + 1,
+                                                   0, 32, s, false);
+  if (e) return e;
+  k_seg_gather<<<(unsigned)nseg, 256, 0, s>>>(nseg, coff, pos_out, idx, w, idx2, w2);
   return (int)hipGetLastError();
 }

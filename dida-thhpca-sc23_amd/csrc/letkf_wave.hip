@@ -820,8 +820,31 @@ __device__ void wave_column(const dk_batched_args& a, int64_t col, int cube, boo
   // analysis id; the cube's centre ln p with vertical localisation
   const int64_t aid = VERT ? col * a.ncube + cube : col;
   const double clnp = VERT ? a.cube_lnp[aid] : 0.0;
+  int lo = 0, hi = p;
+  if (VERT && a.sel_key) {
+    // the column's list is ordered by ln p: only [lo, hi) can lie within
+    // 2 vert_dist of the cube's centre (float keys, so a margin of 1e-5 in
+    // ln p on each side); the rest carry a vertical weight of exactly 0
+    const double half = 2.0 * a.vert_dist * (1.0 + 1e-12) + 1e-5;
+    const double klo = clnp - half, khi = clnp + half;
+    const float* key = a.sel_key + off;
+    int L = 0, R = p;
+    while (L < R) {
+      const int m = (L + R) >> 1;
+      if ((double)key[m] < klo) L = m + 1;
+      else R = m;
+    }
+    lo = L;
+    R = p;
+    while (L < R) {
+      const int m = (L + R) >> 1;
+      if ((double)key[m] <= khi) L = m + 1;
+      else R = m;
+    }
+    hi = L;
+  }
 
-  w.template gram<VERT>(a, off, p, clnp, sm);
+  w.template gram<VERT>(a, off + lo, hi - lo, clnp, sm);
   const double km1 = (double)(k - 1);
   w.shift(k, km1 / a.rho);
   clk.mark(DK_PHASE_GRAM);
