@@ -163,6 +163,7 @@ struct dida_letkf_ctx {
     int32_t *info;
     /* batched form: chain queue */
     int32_t chain_len, chains_per_lon, nchain, nblocks;
+    int32_t ngroup;      /* 0: one cost-sorted queue; DK_XCD_GROUPS: stripes per XCD */
     int32_t nchain_max;  /* chain arrays are sized for chains of one column */
     uint32_t *chain_key, *chain_key_sorted;
     int32_t *chain_id, *chain_order, *queue_head;
@@ -247,9 +248,9 @@ static dk_bin_geom bin_geom(double bd, double obs_polar)
     return g;
 }
 
-/* Bin size and chain length of the sweeps that follow (within the sizes the
- * context was created for). */
-static int set_layout(dida_letkf_ctx *c, double bin_deg, int32_t chain_len)
+/* Bin size, chain length and queue order of the sweeps that follow (within
+ * the sizes the context was created for). */
+static int set_layout(dida_letkf_ctx *c, double bin_deg, int32_t chain_len, int32_t tiles)
 {
     dk_bin_geom g = bin_geom(bin_deg, c->prm.obs_polar);
     if (g.nbins > c->nbins_max) return set_err(DIDA_ERR_ARG, "bins finer than allocated");
@@ -263,6 +264,7 @@ static int set_layout(dida_letkf_ctx *c, double bin_deg, int32_t chain_len)
         c->chain_len = chain_len;
         c->chains_per_lon = (c->nalat + chain_len - 1) / chain_len;
         c->nchain = c->chains_per_lon * c->nalon;
+        c->ngroup = tiles ? DK_XCD_GROUPS : 0;
     }
     return DIDA_OK;
 }
@@ -303,7 +305,7 @@ int dida_letkf_create(dida_letkf_ctx **out, const dida_letkf_params *prm, int32_
         c->nbins_max = g0.nbins > g1.nbins ? g0.nbins : g1.nbins;
     }
     c->nchain_max = (int32_t)c->nsel;
-    rc = set_layout(c, bd, prm->chain_len > 0 ? prm->chain_len : 8);
+    rc = set_layout(c, bd, prm->chain_len > 0 ? prm->chain_len : 8, prm->xcd_tiles);
     if (rc) {
         free(c);
         return rc;
@@ -359,7 +361,7 @@ int dida_letkf_create(dida_letkf_ctx **out, const dida_letkf_params *prm, int32_
         DMALLOC(c->chain_key_sorted, (size_t)c->nchain_max * sizeof(uint32_t));
         DMALLOC(c->chain_id, (size_t)c->nchain_max * sizeof(int32_t));
         DMALLOC(c->chain_order, (size_t)c->nchain_max * sizeof(int32_t));
-        DMALLOC(c->queue_head, 4 * sizeof(int32_t));
+        DMALLOC(c->queue_head, 16 * sizeof(int32_t));
         DMALLOC(c->ymax, no * sizeof(double));
         if (dk_sort_pairs_u32(NULL, &c->chain_tmp_bytes, c->chain_key, c->chain_key_sorted,
                               c->chain_id, c->chain_order, c->nchain_max, 32, NULL)) {
@@ -685,11 +687,16 @@ static int sweep_batched(dida_letkf_ctx *c, const double *xb, const double *axb,
     }
     if (c->prm.gram == DIDA_GRAM_FP16X3)
         HIP_TRY(dk_obs_ymax(c->last_nobs_pending, ba.k, c->yb_b, c->rinv_b, c->ymax, s));
-    HIP_TRY(dk_chain_keys(&ba, c->chain_key, c->chain_id, s));
-    size_t tb = c->chain_tmp_bytes;
-    HIP_TRY(dk_sort_pairs_u32(c->chain_tmp, &tb, c->chain_key, c->chain_key_sorted, c->chain_id,
-                              c->chain_order, c->nchain, 32, s));
-    HIP_TRY(dk_fill_i32(c->queue_head, 0, 1, s));
+    ba.ngroup = c->ngroup;
+    if (c->ngroup > 0) {
+        HIP_TRY(dk_chain_groups(&ba, c->chain_order, s));
+    } else {
+        HIP_TRY(dk_chain_keys(&ba, c->chain_key, c->chain_id, s));
+        size_t tb = c->chain_tmp_bytes;
+        HIP_TRY(dk_sort_pairs_u32(c->chain_tmp, &tb, c->chain_key, c->chain_key_sorted,
+                                  c->chain_id, c->chain_order, c->nchain, 32, s));
+    }
+    HIP_TRY(dk_fill_i32(c->queue_head, 0, 16, s));
     tmark(c, DIDA_STAGE_SELECT, s);
     HIP_TRY(dk_letkf_wave(&ba, c->nblocks, s));
     tmark(c, DIDA_STAGE_TRANSFORM, s);
@@ -882,9 +889,9 @@ int dida_letkf_autotune(dida_letkf_ctx *c, const dida_letkf_obs *o, const double
         return set_err(DIDA_ERR_HIP, "autotune: events");
     }
     const double bd0 = c->bin_deg;
-    const int32_t cl0 = c->chain_len;
+    const int32_t cl0 = c->chain_len, tl0 = c->ngroup > 0;
     double best_bd = bd0, best_ms = 0.0;
-    int32_t best_cl = cl0;
+    int32_t best_cl = cl0, best_tl = tl0;
     rc = timed_sweep(c, o, xb, axb, xa, axa, s, e0, e1, &best_ms);
     out->ms_initial = best_ms;
     out->ncand = 1;
@@ -892,27 +899,32 @@ int dida_letkf_autotune(dida_letkf_ctx *c, const dida_letkf_obs *o, const double
     const int nbin = st.nobsl_capped > 0 ? 1 : 3;
     const int32_t chains[6] = {2, 4, 8, 16, 32, 64};
     const int nchn = c->prm.form == DIDA_FORM_BATCHED ? 6 : 1;
+    const int ntl = c->prm.form == DIDA_FORM_BATCHED ? 2 : 1;
     for (int ib = 0; ib < nbin && !rc; ++ib) {
         const double bd = nbin == 1 ? bd0 : bins[ib];
         for (int ic = 0; ic < nchn && !rc; ++ic) {
             const int32_t cl = nchn == 1 ? cl0 : chains[ic];
-            if (bd == bd0 && cl == cl0) continue;
-            rc = set_layout(c, bd, cl);
-            if (rc) break;
-            double ms = 0.0;
-            rc = timed_sweep(c, o, xb, axb, xa, axa, s, e0, e1, &ms);
-            out->ncand++;
-            if (!rc && ms < best_ms) {
-                best_ms = ms;
-                best_bd = bd;
-                best_cl = cl;
+            for (int it = 0; it < ntl && !rc; ++it) {
+                const int32_t tl = ntl == 1 ? tl0 : it;
+                if (bd == bd0 && cl == cl0 && tl == tl0) continue;
+                rc = set_layout(c, bd, cl, tl);
+                if (rc) break;
+                double ms = 0.0;
+                rc = timed_sweep(c, o, xb, axb, xa, axa, s, e0, e1, &ms);
+                out->ncand++;
+                if (!rc && ms < best_ms) {
+                    best_ms = ms;
+                    best_bd = bd;
+                    best_cl = cl;
+                    best_tl = tl;
+                }
             }
         }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     c->timing = timing;
-    int rc2 = set_layout(c, best_bd, best_cl);
+    int rc2 = set_layout(c, best_bd, best_cl, best_tl);
     if (!rc) rc = rc2;
     if (rc) return rc;
     rc = dida_letkf_sweep(c, o, xb, axb, xa, axa, stream);
@@ -920,6 +932,7 @@ int dida_letkf_autotune(dida_letkf_ctx *c, const dida_letkf_obs *o, const double
     out->chain_len = c->prm.form == DIDA_FORM_BATCHED ? c->chain_len : 0;
     out->bin_deg = c->bin_deg;
     out->ms_best = best_ms;
+    out->xcd_tiles = c->prm.form == DIDA_FORM_BATCHED ? c->ngroup > 0 : 0;
     return DIDA_OK;
 }
 

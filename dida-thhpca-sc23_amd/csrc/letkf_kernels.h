@@ -184,7 +184,16 @@ typedef struct dk_batched_args {
     /* optional: write trans(i, e) = W + wbar 1^T of analysis col at
      * trans_out[col * k * k + i + e k] instead of updating the state */
     double *trans_out;
+    /* ngroup > 0: the queue holds ngroup groups of chains (longitude stripe
+     * g = columns [g nlon / ngroup, (g + 1) nlon / ngroup), chains in
+     * latitude-band-major order), group g starting at stripe_lo(g) *
+     * chains_per_lon; workgroup b drains group b % ngroup first (one group per
+     * XCD, so neighbouring analyses share an L2), then the others;
+     * queue_head holds ngroup counters */
+    int32_t ngroup, pad3_;
 } dk_batched_args;
+
+#define DK_XCD_GROUPS 8
 
 /* Phases of the batched kernel timed by phase_clk (wave 0 of each block). */
 #define DK_PHASE_GRAM 0
@@ -196,6 +205,8 @@ typedef struct dk_batched_args {
 
 /* Sort keys of the chains: 2^32-1 - cost (ascending sort = largest first). */
 int dk_chain_keys(const dk_batched_args *a, uint32_t *key, int32_t *id, void *stream);
+/* Grouped queue (a->ngroup > 0): order[position] = chain, no sort needed. */
+int dk_chain_groups(const dk_batched_args *a, int32_t *order, void *stream);
 /* Batched kernel (one 64-thread wavefront per analysis): resident
  * workgroups per CU for (k, gram), and the persistent launch. */
 int dk_wave_blocks_per_cu(int k, int gram);

@@ -944,17 +944,36 @@ __device__ void wave_column(const dk_batched_args& a, int64_t col, int cube, boo
   (void)NQ;
 }
 
+// First longitude of stripe g of n groups (groups of whole longitudes).
+__device__ __forceinline__ int stripe_lo(int g, int n, int nlon) {
+  return (int)(((int64_t)g * nlon) / n);
+}
+
 template <int KP, int GRAM, int MODE>
 __global__ __launch_bounds__(64, 1) void k_letkf_wave(dk_batched_args a) {
   __shared__ WSmem<KP> sm;
   Wave<KP, GRAM> w;
   const int lane = threadIdx.x;
+  // one queue (ngroup 0) or ngroup group queues, own group first
+  const int ng = a.ngroup > 0 ? a.ngroup : 1;
+  const int g0 = a.ngroup > 0 ? (int)(blockIdx.x % (unsigned)ng) : 0;
+  int s = 0, gq = g0, beg = 0, cnt = a.nchain;
+  if (a.ngroup > 0) {
+    beg = stripe_lo(gq, ng, a.nlon) * a.chains_per_lon;
+    cnt = stripe_lo(gq + 1, ng, a.nlon) * a.chains_per_lon - beg;
+  }
   for (;;) {
     int item = 0;
-    if (lane == 0) item = atomicAdd(a.queue_head, 1);
+    if (lane == 0) item = atomicAdd(a.queue_head + gq, 1);
     item = __shfl(item, 0, kWave);
-    if (item >= a.nchain) break;
-    const int chain = a.queue ? a.queue[item] : item;
+    if (item >= cnt) {
+      if (++s >= ng) break;
+      gq = (g0 + s) % ng;
+      beg = stripe_lo(gq, ng, a.nlon) * a.chains_per_lon;
+      cnt = stripe_lo(gq + 1, ng, a.nlon) * a.chains_per_lon - beg;
+      continue;
+    }
+    const int chain = a.queue ? a.queue[beg + item] : beg + item;
     const int ilon = chain / a.chains_per_lon;
     const int lat0 = (chain % a.chains_per_lon) * a.chain_len;
     const int lat1 = min(lat0 + a.chain_len, a.nlat);
@@ -1053,6 +1072,19 @@ __global__ __launch_bounds__(64) void k_update_cols(dk_update_args u) {
                          nullptr, u.nrow);
 }
 
+// Grouped queue: chain (ilon, latchain) of stripe g goes to position
+// stripe_lo(g) cpl + latchain * width(g) + (ilon - stripe_lo(g)).
+__global__ void k_chain_groups(dk_batched_args a, int32_t* order) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.nchain) return;
+  const int cpl = a.chains_per_lon, ng = a.ngroup;
+  const int ilon = c / cpl, lc = c % cpl;
+  int g = 0;
+  while (g + 1 < ng && stripe_lo(g + 1, ng, a.nlon) <= ilon) ++g;
+  const int l0 = stripe_lo(g, ng, a.nlon), wdt = stripe_lo(g + 1, ng, a.nlon) - l0;
+  order[l0 * cpl + lc * wdt + (ilon - l0)] = c;
+}
+
 template <int KP, int GRAM>
 int wlaunch(const dk_batched_args* a, int nblocks, hipStream_t s) {
   if (a->ncube > 0) k_letkf_wave<KP, GRAM, kModeVertical><<<nblocks, 64, 0, s>>>(*a);
@@ -1105,6 +1137,12 @@ extern "C" int dk_update_cols(const dk_update_args* u, void* stream) {
   hipStream_t s = (hipStream_t)stream;
   if (u->k <= 32) k_update_cols<32><<<u->ncol, 64, 0, s>>>(*u);
   else k_update_cols<64><<<u->ncol, 64, 0, s>>>(*u);
+  return (int)hipGetLastError();
+}
+
+extern "C" int dk_chain_groups(const dk_batched_args* a, int32_t* order, void* stream) {
+  if (a->nchain <= 0 || a->ngroup <= 0) return 0;
+  k_chain_groups<<<(a->nchain + 255) / 256, 256, 0, (hipStream_t)stream>>>(*a, order);
   return (int)hipGetLastError();
 }
 

@@ -47,12 +47,14 @@ class Params(C.Structure):
                 ("bin_deg", C.c_double), ("chain_len", C.c_int32), ("warm_start", C.c_int32),
                 ("jacobi_predict", C.c_int32), ("vert_dist", C.c_double),
                 ("cube_nz", C.c_int32), ("gc_filter_vertical", C.c_int32),
-                ("cube_n", C.c_int32), ("overlap", C.c_int32)]
+                ("cube_n", C.c_int32), ("overlap", C.c_int32), ("xcd_tiles", C.c_int32),
+                ("pad_", C.c_int32)]
 
 
 class TuningC(C.Structure):
     _fields_ = [("chain_len", C.c_int32), ("ncand", C.c_int32), ("bin_deg", C.c_double),
-                ("ms_initial", C.c_double), ("ms_best", C.c_double)]
+                ("ms_initial", C.c_double), ("ms_best", C.c_double), ("xcd_tiles", C.c_int32),
+                ("pad_", C.c_int32)]
 
 
 class VertCoordC(C.Structure):
@@ -312,7 +314,7 @@ class Engine:
         oc, t = obs.c(), TuningC()
         check(lib().dida_letkf_autotune(self.h, C.byref(oc), _ptr(xb), _ptr(axb), _ptr(xa),
                                         _ptr(axa), _stream_ptr(stream), C.byref(t)), "autotune")
-        return {f: getattr(t, f) for f, _ in TuningC._fields_}
+        return {f: getattr(t, f) for f, _ in TuningC._fields_ if f != "pad_"}
 
     def graph_launch(self, stream=None) -> int:
         return check(lib().dida_letkf_graph_launch(self.h, _stream_ptr(stream)), "graph_launch")

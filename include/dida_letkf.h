@@ -95,6 +95,11 @@ typedef struct dida_letkf_params {
     int32_t cube_n;             /* da_block_interval (default 1, as
                                    namelists.input_100km_32mem:68) */
     int32_t overlap;            /* da_block_overlap, 0 <= overlap < cube_n (default 0) */
+    int32_t xcd_tiles;          /* batched form: 1 drains the chains by longitude stripes,
+                                   one stripe per accelerator die (XCD) first, so that
+                                   concurrent analyses of a die share its L2; 0 (default)
+                                   one queue, most expensive chains first */
+    int32_t pad_;
 } dida_letkf_params;
 
 /* Observations, device pointers, one entry per scalar observation (the
@@ -208,6 +213,8 @@ typedef struct dida_letkf_tuning {
     double bin_deg;             /* kept */
     double ms_initial;          /* sweep time with the layout on entry */
     double ms_best;             /* sweep time with the kept layout */
+    int32_t xcd_tiles;          /* kept queue order (batched form) */
+    int32_t pad_;
 } dida_letkf_tuning;
 int dida_letkf_autotune(dida_letkf_ctx *ctx, const dida_letkf_obs *obs, const double *xb,
                         const double *axb, double *xa, double *axa, void *stream,
