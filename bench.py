@@ -118,7 +118,7 @@ def selection_bytes(k, nobs, nobsl_sum):
     return nobs * (8.0 * k + 32.0) + 12.0 * nobsl_sum
 
 
-def kernel_roofline(cfg, st, head, kernel_ms, config, scale):
+def kernel_roofline(cfg, st, head, kernel_ms, config, scale, layout=None):
     """Roofline of the dominant kernel (the batched form's fused kernel, or
     the plain form's transform kernel) from algorithmic flops per sweep:
     Gram 2k^2 p (three half-precision MFMA passes in the split-x3 forms),
@@ -146,7 +146,7 @@ def kernel_roofline(cfg, st, head, kernel_ms, config, scale):
     # update's 2kn f64 + 2n f64 per column
     gathered = p_sum * (8.0 * k + 24.0)
     upd = ncol * (2.0 * k * n * 8 + 2.0 * n * 8) if fused else 0.0
-    traffic = measured_traffic(head, config, scale)
+    traffic = measured_traffic(head, config, scale, layout)
     return {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
             "frac": achieved / peak if achieved else None, "traffic": traffic,
             "kernel": "k_letkf_wave (fused Gram, eigensolve, transform, update)" if fused
@@ -158,16 +158,18 @@ def kernel_roofline(cfg, st, head, kernel_ms, config, scale):
             if kernel_ms > 0 else None}
 
 
-def measured_traffic(head, config, scale):
+def measured_traffic(head, config, scale, layout=None):
     """HBM bytes per launch of the dominant kernel from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled by a copy kernel's
-    reading), for this exact workload; None when not measured."""
+    reading), for this exact workload and kept layout (chain length, queue
+    order); None when not measured."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
-    if not os.path.exists(path):
+    if not os.path.exists(path) or not layout:
         return None
     with open(path) as f:
         table = json.load(f)
-    e = table.get(f"{head}/config{config}/scale{scale:g}")
+    e = table.get(f"{head}/config{config}/scale{scale:g}/chain{int(layout['chain_len'])}"
+                  f"/tiles{int(layout.get('xcd_tiles', 0))}")
     return e["hbm_bytes_per_launch"] if e else None
 
 
@@ -472,7 +474,7 @@ def main():
     value = ncol / (ms / 1e3)
 
     roofline = kernel_roofline(cfg, st, head, r["stages"]["transform"], args.config,
-                               args.scale)
+                               args.scale, r["tuning"])
 
     out_forms = {}
     for name, rr in res.items():
