@@ -147,6 +147,7 @@ struct dida_letkf_ctx {
     void *sort_tmp;
     size_t sort_tmp_bytes;
     double *cx, *sx, *yr, *ux, *uy, *uz, *yb_b, *dep_b;
+    int32_t ldy;         /* row stride of yb_b: k (plain), the padded k (batched) */
     float *rinv_b;
     /* selection */
     int32_t *nobsl;
@@ -333,7 +334,8 @@ int dida_letkf_create(dida_letkf_ctx **out, const dida_letkf_params *prm, int32_
     DMALLOC(c->ux, no * sizeof(double));
     DMALLOC(c->uy, no * sizeof(double));
     DMALLOC(c->uz, no * sizeof(double));
-    DMALLOC(c->yb_b, no * (size_t)k * sizeof(double));
+    c->ldy = prm->form == DIDA_FORM_BATCHED ? (k <= 32 ? 32 : 64) : k;
+    DMALLOC(c->yb_b, no * (size_t)c->ldy * sizeof(double));
     DMALLOC(c->dep_b, no * sizeof(double));
     DMALLOC(c->rinv_b, no * sizeof(float));
 
@@ -523,8 +525,9 @@ static int sweep_select_count(dida_letkf_ctx *c, const dida_letkf_obs *o, hipStr
                                   o->nobs, c->end_bit, s));
     }
     HIP_TRY(dk_bin_scan(c->geom.nbins, c->bin_count, c->bin_start, s));
-    HIP_TRY(dk_obs_gather(o->nobs, k, c->perm, o->lat, o->lon, o->yb, o->dep, o->rinv, c->cx,
-                          c->sx, c->yr, c->ux, c->uy, c->uz, c->yb_b, c->dep_b, c->rinv_b, s));
+    HIP_TRY(dk_obs_gather(o->nobs, k, c->ldy, c->perm, o->lat, o->lon, o->yb, o->dep, o->rinv,
+                          c->cx, c->sx, c->yr, c->ux, c->uy, c->uz, c->yb_b, c->dep_b,
+                          c->rinv_b, s));
     if (c->ncube > 0) HIP_TRY(dk_obs_lnp(o->nobs, c->perm, o->ph, c->obs_lnp, s));
     tmark(c, DIDA_STAGE_BIN, s);
     memset(sa, 0, sizeof *sa);
@@ -637,6 +640,7 @@ static int sweep_batched(dida_letkf_ctx *c, const double *xb, const double *axb,
     ba.idx = c->csr_idx;
     ba.rloc = c->csr_w;
     ba.yb = c->yb_b;
+    ba.ldy = c->ldy;
     ba.rinv = c->rinv_b;
     ba.dep = c->dep_b;
     ba.ymax = c->ymax;
@@ -686,7 +690,7 @@ static int sweep_batched(dida_letkf_ctx *c, const double *xb, const double *axb,
         }
     }
     if (c->prm.gram == DIDA_GRAM_FP16X3)
-        HIP_TRY(dk_obs_ymax(c->last_nobs_pending, ba.k, c->yb_b, c->rinv_b, c->ymax, s));
+        HIP_TRY(dk_obs_ymax(c->last_nobs_pending, ba.k, c->ldy, c->yb_b, c->rinv_b, c->ymax, s));
     ba.ngroup = c->ngroup;
     if (c->ngroup > 0) {
         HIP_TRY(dk_chain_groups(&ba, c->chain_order, s));

@@ -69,7 +69,8 @@ int dk_sort_pairs_u32(void *tmp, size_t *tmp_bytes, const uint32_t *kin, uint32_
 /* Exclusive scan of nbins + 1 counts into bin_start[nbins + 2] (one block). */
 int dk_bin_scan(int32_t nbins, const int32_t *count, int32_t *bin_start, void *stream);
 /* Gather obs into binned order and precompute the distance factors. */
-int dk_obs_gather(int32_t nobs, int32_t k, const int32_t *perm, const double *lat,
+/* yb rows are copied into rows of ldy >= k values, zero past k. */
+int dk_obs_gather(int32_t nobs, int32_t k, int32_t ldy, const int32_t *perm, const double *lat,
                   const double *lon, const double *yb, const double *dep, const float *rinv,
                   double *cx, double *sx, double *yr, double *ux, double *uy, double *uz,
                   double *yb_b, double *dep_b, float *rinv_b, void *stream);
@@ -152,14 +153,14 @@ typedef struct dk_batched_args {
     int32_t k, nrow, nlat, nlon;
     int32_t nchain, chains_per_lon, chain_len, gram;
     int32_t warm_start, rtps, max_sweeps, max_p;
-    int32_t predict, pad_;
+    int32_t predict, ldy;   /* ldy: row stride of yb, the kernel's padded k (32 or 64) */
     double rtps_alpha, rho, tol;
     int64_t capacity;
     const int64_t *offset;
     const int32_t *nobsl;
     const int32_t *idx;     /* CSR binned obs index */
     const double *rloc;
-    const double *yb;       /* binned obs, k per obs */
+    const double *yb;       /* binned obs rows of ldy values, zero past k */
     const float *rinv;
     const double *dep;
     const double *ymax;     /* per binned obs: max |yb| sqrt(R^-1) (FP16 scaling) */
@@ -212,8 +213,8 @@ int dk_chain_groups(const dk_batched_args *a, int32_t *order, void *stream);
 int dk_wave_blocks_per_cu(int k, int gram);
 int dk_letkf_wave(const dk_batched_args *a, int nblocks, void *stream);
 /* ymax[i] = max_e |yb[i k + e]| sqrt(rinv[i]). */
-int dk_obs_ymax(int32_t nobs, int32_t k, const double *yb, const float *rinv, double *ymax,
-                void *stream);
+int dk_obs_ymax(int32_t nobs, int32_t k, int32_t ldy, const double *yb, const float *rinv,
+                double *ymax, void *stream);
 
 /* Vertical localisation inputs (letkf_compute.F90:424-447, obs_make.F90:418-437):
  * cube_lnp[col * ncube + q] = ln of the full-level pressure of level

@@ -100,22 +100,24 @@ __global__ void k_obs_gather(int32_t nobs, const int32_t* __restrict__ perm,
   rinv_b[i] = rinv[s];
 }
 
-__global__ void k_obs_gather_yb(int32_t nobs, int32_t k, const int32_t* __restrict__ perm,
-                                const double* __restrict__ yb, double* __restrict__ yb_b) {
+// Binned rows of ldy >= k values, zero past k.
+__global__ void k_obs_gather_yb(int32_t nobs, int32_t k, int32_t ldy,
+                                const int32_t* __restrict__ perm, const double* __restrict__ yb,
+                                double* __restrict__ yb_b) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int64_t total = (int64_t)nobs * k;
+  int64_t total = (int64_t)nobs * ldy;
   if (t >= total) return;
-  int64_t i = t / k;
-  int e = (int)(t - i * k);
-  yb_b[t] = yb[(int64_t)perm[i] * k + e];
+  int64_t i = t / ldy;
+  int e = (int)(t - i * ldy);
+  yb_b[t] = e < k ? yb[(int64_t)perm[i] * k + e] : 0.0;
 }
 
-__global__ void k_obs_ymax(int32_t nobs, int32_t k, const double* __restrict__ yb,
+__global__ void k_obs_ymax(int32_t nobs, int32_t k, int32_t ldy, const double* __restrict__ yb,
                            const float* __restrict__ rinv, double* __restrict__ ymax) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nobs) return;
   double m = 0.0;
-  for (int e = 0; e < k; ++e) m = fmax(m, fabs(yb[(int64_t)i * k + e]));
+  for (int e = 0; e < k; ++e) m = fmax(m, fabs(yb[(int64_t)i * ldy + e]));
   ymax[i] = m * sqrt((double)rinv[i]);
 }
 
@@ -258,7 +260,8 @@ extern "C" int dk_bin_scan(int32_t nbins, const int32_t* count, int32_t* bin_sta
   return (int)hipGetLastError();
 }
 
-extern "C" int dk_obs_gather(int32_t nobs, int32_t k, const int32_t* perm, const double* lat,
+extern "C" int dk_obs_gather(int32_t nobs, int32_t k, int32_t ldy, const int32_t* perm,
+                             const double* lat,
                              const double* lon, const double* yb, const double* dep,
                              const float* rinv, double* cx, double* sx, double* yr,
                              double* ux, double* uy, double* uz, double* yb_b, double* dep_b,
@@ -267,15 +270,15 @@ extern "C" int dk_obs_gather(int32_t nobs, int32_t k, const int32_t* perm, const
   hipStream_t s = (hipStream_t)stream;
   k_obs_gather<<<(nobs + 255) / 256, 256, 0, s>>>(nobs, perm, lat, lon, dep, rinv, cx, sx, yr,
                                                   ux, uy, uz, dep_b, rinv_b);
-  int64_t total = (int64_t)nobs * k;
-  k_obs_gather_yb<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(nobs, k, perm, yb, yb_b);
+  int64_t total = (int64_t)nobs * ldy;
+  k_obs_gather_yb<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(nobs, k, ldy, perm, yb, yb_b);
   return (int)hipGetLastError();
 }
 
-extern "C" int dk_obs_ymax(int32_t nobs, int32_t k, const double* yb, const float* rinv,
-                            double* ymax, void* stream) {
+extern "C" int dk_obs_ymax(int32_t nobs, int32_t k, int32_t ldy, const double* yb,
+                           const float* rinv, double* ymax, void* stream) {
   if (nobs <= 0) return 0;
-  k_obs_ymax<<<(nobs + 255) / 256, 256, 0, (hipStream_t)stream>>>(nobs, k, yb, rinv, ymax);
+  k_obs_ymax<<<(nobs + 255) / 256, 256, 0, (hipStream_t)stream>>>(nobs, k, ldy, yb, rinv, ymax);
   return (int)hipGetLastError();
 }
 

@@ -218,19 +218,17 @@ struct Wave {
     return h * 32 + (GRAM == DK_GRAM_FP64 ? 4 * e + g : 8 * g + e);
   }
 
-  // Yb rows I*16 + c (row k - 1 past k) of the 8 obs of half h in buffer buf
+  // Yb rows I*16 + c of the 8 obs of half h in buffer buf (binned rows are
+  // KP values apart and zero past k, so every row is in bounds and the I
+  // offsets are constants of one address per obs)
   __device__ __forceinline__ void load_rows(const dk_batched_args& a, const WSmem<KP>& sm,
                                             int buf, int h, double (&raw)[8][NJ]) const {
     const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
-    const int k = a.k;
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
-      const double* y = a.yb + (int64_t)sm.ogi[buf][half_obs(h, e, g)] * k;
+      const double* y = a.yb + (int64_t)sm.ogi[buf][half_obs(h, e, g)] * KP + c;
 #pragma unroll
-      for (int I = 0; I < NJ; ++I) {
-        const int row = I * 16 + c;
-        raw[e][I] = y[row < k ? row : k - 1];
-      }
+      for (int I = 0; I < NJ; ++I) raw[e][I] = y[I * 16];
     }
   }
 
@@ -1155,6 +1153,7 @@ extern "C" int dk_chain_keys(const dk_batched_args* a, uint32_t* key, int32_t* i
 extern "C" int dk_letkf_wave(const dk_batched_args* a, int nblocks, void* stream) {
   if (a->nchain <= 0) return 0;
   if (a->k < 2 || a->k > 64) return (int)hipErrorInvalidValue;
+  if (a->ldy != (a->k <= 32 ? 32 : 64)) return (int)hipErrorInvalidValue;
   hipStream_t s = (hipStream_t)stream;
   if (a->k <= 32) return wdispatch<32>(a, nblocks, s);
   return wdispatch<64>(a, nblocks, s);
