@@ -30,6 +30,8 @@ CASES = {
     "plain_fp64_cube4_overlap3_rtps": ("plain", "fp64", 4, 3, {"rtps": 1, "rtps_alpha": 0.7},
                                        1e-10),
     "batched_bf16x3_cube3_overlap1": ("batched", "bf16x3", 3, 1, {}, 1e-4),
+    # k > 64: the plain form's update on the VALU kernel, trans in LDS
+    "plain_fp64_k100_cube3_overlap1": ("plain", "fp64", 3, 1, {"k": 100}, 1e-10),
 }
 
 
@@ -37,8 +39,10 @@ CASES = {
 def test_cube_sweep_matches_oracle(cuda, dl, oracle, case):
     torch = cuda
     form, gram, cube_n, overlap, extra, tol = CASES[case]
+    extra = dict(extra)
+    k = extra.pop("k", 24)
     print(f"cube sweep case {case}", flush=True)
-    cfg = dl.synth_cfg(37, 71, 3, 24, 3000, seed=0xD1DA0300 + len(case), cluster_frac=0.5,
+    cfg = dl.synth_cfg(37, 71, 3, k, 3000, seed=0xD1DA0300 + len(case), cluster_frac=0.5,
                        nclusters=3, cluster_radius_km=900.0, rscale=(0.5, 2.0))
     data = dl.synth_generate(cfg)
     n = 1 + 3 * cfg.nz
