@@ -35,77 +35,108 @@ struct CoreSmem {
   int flag[4];
 };
 
-// KP = 128 Gram: wave w accumulates lower tiles t = w, w + 4, ... (I >= J)
-// over all obs; 4 obs a step per lane group, loads issued before the MFMAs.
-template <int KP>
-__device__ __forceinline__ void gram_f64_tiles(const dk_core_args& a, int64_t off, int p,
-                                               CoreSmem<KP>& sm) {
+// KP = 128 Gram: wave W owns the lower tiles of tile rows W and NT-1-W
+// (9 tiles each, fixed at compile time); every wave reads every obs.  Obs go
+// in stages of U steps of 4 (lane group q4 takes obs 4u + q4 of a step): the
+// rows and weights of stage s + 1 are in flight while stage s computes, the
+// indices of stage s + 2 behind them.  Wave 0 also forms b.
+template <int KP, int W>
+__device__ __forceinline__ void gram_f64_rows(const dk_core_args& a, int64_t off, int p,
+                                              CoreSmem<KP>& sm) {
   constexpr int NT = KP / 16;
-  constexpr int NTILE = NT * (NT + 1) / 2;
-  constexpr int NW = (NTILE + 3) / 4;  // tiles per wave
+  constexpr int IA = W, IB = NT - 1 - W;  // IA < IB
+  constexpr int NF = IB + 1;              // tile rows this wave reads
+  constexpr int U = 6;
   constexpr int LDA = CoreSmem<KP>::LDA;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int r16 = lane & 15, q4 = lane >> 4;
+  const int lane = threadIdx.x & 63, r16 = lane & 15, q4 = lane >> 4;
   const int k = a.k;
-  int ti[NW], tj[NW];
+  double4_t accA[IA + 1], accB[IB + 1];
 #pragma unroll
-  for (int w = 0; w < NW; ++w) {
-    int t = wave + 4 * w, I = 0;
-    if (t >= NTILE) t = NTILE - 1;  // duplicate of a real tile, not stored
-    while ((I + 1) * (I + 2) / 2 <= t) ++I;
-    ti[w] = I;
-    tj[w] = t - I * (I + 1) / 2;
-  }
-  double4_t acc[NW];
+  for (int J = 0; J <= IA; ++J) accA[J] = double4_t{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-  for (int w = 0; w < NW; ++w) acc[w] = double4_t{0.0, 0.0, 0.0, 0.0};
+  for (int J = 0; J <= IB; ++J) accB[J] = double4_t{0.0, 0.0, 0.0, 0.0};
   double bacc[NT];
 #pragma unroll
   for (int T = 0; T < NT; ++T) bacc[T] = 0.0;
-  for (int o0 = 0; o0 < p; o0 += 4) {
-    const int o = o0 + q4;
-    const bool ok = o < p;
-    const int oc = ok ? o : 0;
-    const int64_t gi = a.idx ? (int64_t)a.idx[off + oc] : off + oc;
-    __builtin_amdgcn_sched_barrier(0);
-    const double* y = a.yb + gi * k;
-    double f[NT];
+  const int nst = (p + 4 * U - 1) / (4 * U);
+  int gn[U];
+  auto load_idx = [&](int st) {
 #pragma unroll
-    for (int T = 0; T < NT; ++T) {
-      const int row = T * 16 + r16;
-      f[T] = y[row < k ? row : k - 1];
+    for (int u = 0; u < U; ++u) {
+      const int o = st * 4 * U + 4 * u + q4;
+      const int oc = o < p ? o : 0;
+      gn[u] = a.idx ? a.idx[off + oc] : (int)(off + oc);
     }
-    const double rr = ok ? sqrt((double)a.rinv[gi] * a.rloc[off + oc]) : 0.0;
-    const double sd = rr * a.dep[gi];
-    __builtin_amdgcn_sched_barrier(0);
+  };
+  // rows T*16 + r16 (clamped to k - 1, masked later) and sqrt(R^-1 rloc),
+  // its product with d, of stage st
+  auto load_rows = [&](int st, double (&f)[U][NF], double (&rr)[U], double (&sd)[U]) {
 #pragma unroll
-    for (int T = 0; T < NT; ++T) f[T] = (T * 16 + r16 < k) ? f[T] * rr : 0.0;
+    for (int u = 0; u < U; ++u) {
+      const int o = st * 4 * U + 4 * u + q4;
+      const bool ok = o < p;
+      const int64_t gi = gn[u];
+      const double* y = a.yb + gi * k;
 #pragma unroll
-    for (int w = 0; w < NW; ++w) {
-      double fa = 0.0, fb = 0.0;
-#pragma unroll
-      for (int T = 0; T < NT; ++T) {
-        fa = ti[w] == T ? f[T] : fa;
-        fb = tj[w] == T ? f[T] : fb;
+      for (int T = 0; T < NF; ++T) {
+        const int row = T * 16 + r16;
+        f[u][T] = y[row < k ? row : k - 1];
       }
-      acc[w] = mfma_f64_16x16x4(fa, fb, acc[w]);
+      const double w = ok ? (double)a.rinv[gi] * a.rloc[off + (ok ? o : 0)] : 0.0;
+      rr[u] = w;
+      sd[u] = a.dep[gi];
     }
-    if (wave == 0)
+  };
+  auto compute = [&](double (&f)[U][NF], const double (&rr)[U], const double (&sd)[U]) {
 #pragma unroll
-      for (int T = 0; T < NT; ++T) bacc[T] += f[T] * sd;
+    for (int u = 0; u < U; ++u) {
+      const double r = sqrt(rr[u]), d = r * sd[u];
+#pragma unroll
+      for (int T = 0; T < NF; ++T) f[u][T] = (T * 16 + r16 < k) ? f[u][T] * r : 0.0;
+#pragma unroll
+      for (int J = 0; J <= IA; ++J) accA[J] = mfma_f64_16x16x4(f[u][IA], f[u][J], accA[J]);
+#pragma unroll
+      for (int J = 0; J <= IB; ++J) accB[J] = mfma_f64_16x16x4(f[u][IB], f[u][J], accB[J]);
+      if constexpr (W == 0)
+#pragma unroll
+        for (int T = 0; T < NT; ++T) bacc[T] += f[u][T] * d;
+    }
+  };
+  if (nst > 0) {
+    double fA[U][NF], fB[U][NF], rA[U], rB[U], sA[U], sB[U];
+    load_idx(0);
+    load_rows(0, fA, rA, sA);
+    if (nst > 1) load_idx(1);
+    for (int st = 0; st < nst; st += 2) {
+      if (st + 1 < nst) load_rows(st + 1, fB, rB, sB);
+      if (st + 2 < nst) load_idx(st + 2);
+      __builtin_amdgcn_sched_barrier(0);
+      compute(fA, rA, sA);
+      if (st + 1 >= nst) break;
+      if (st + 2 < nst) load_rows(st + 2, fA, rA, sA);
+      if (st + 3 < nst) load_idx(st + 3);
+      __builtin_amdgcn_sched_barrier(0);
+      compute(fB, rB, sB);
+    }
   }
+  // D[4r + q4][r16] of tile (I, J) = C(I*16 + 4r + q4, J*16 + r16)
 #pragma unroll
-  for (int w = 0; w < NW; ++w) {
-    if (wave + 4 * w >= NTILE) continue;
-    const int I = ti[w], J = tj[w];
+  for (int J = 0; J <= IA; ++J)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int i = I * 16 + 4 * r + q4, j = J * 16 + r16;
-      sm.A[i + j * LDA] = acc[w][r];
-      if (I != J) sm.A[j + i * LDA] = acc[w][r];
+      const int i = IA * 16 + 4 * r + q4, j = J * 16 + r16;
+      sm.A[i + j * LDA] = accA[J][r];
+      if (IA != J) sm.A[j + i * LDA] = accA[J][r];
     }
-  }
-  if (wave == 0) {
+#pragma unroll
+  for (int J = 0; J <= IB; ++J)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = IB * 16 + 4 * r + q4, j = J * 16 + r16;
+      sm.A[i + j * LDA] = accB[J][r];
+      if (IB != J) sm.A[j + i * LDA] = accB[J][r];
+    }
+  if constexpr (W == 0) {
 #pragma unroll
     for (int T = 0; T < NT; ++T) {
       bacc[T] += shfl_xor_d(bacc[T], 16);
@@ -114,6 +145,18 @@ __device__ __forceinline__ void gram_f64_tiles(const dk_core_args& a, int64_t of
     if (q4 == 0)
 #pragma unroll
       for (int T = 0; T < NT; ++T) sm.b[T * 16 + r16] = bacc[T];
+  }
+}
+
+template <int KP>
+__device__ __forceinline__ void gram_f64_tiles(const dk_core_args& a, int64_t off, int p,
+                                               CoreSmem<KP>& sm) {
+  static_assert(KP / 16 == 8, "four waves own two tile rows each");
+  switch (threadIdx.x >> 6) {
+    case 0: gram_f64_rows<KP, 0>(a, off, p, sm); break;
+    case 1: gram_f64_rows<KP, 1>(a, off, p, sm); break;
+    case 2: gram_f64_rows<KP, 2>(a, off, p, sm); break;
+    default: gram_f64_rows<KP, 3>(a, off, p, sm); break;
   }
   __syncthreads();
 }
