@@ -111,9 +111,10 @@ static int check_params(const dida_letkf_params *p)
         return set_err(DIDA_ERR_ARG, "cube_n = %d, overlap = %d: need cube_n >= 1 and "
                        "0 <= overlap < cube_n", p->cube_n, p->overlap);
     if (p->vert_dist > 0.0) {
-        if (p->cube_n != 1)
+        if (p->overlap != 0)
             return set_err(DIDA_ERR_UNSUPPORTED,
-                           "vertical localisation runs with cube_n = 1 only");
+                           "vertical localisation runs with overlap = 0 only (the reference's "
+                           "overlapping-cube sweep has none)");
         if (p->form != DIDA_FORM_BATCHED)
             return set_err(DIDA_ERR_UNSUPPORTED,
                            "vertical localisation runs in the batched form only");
@@ -353,7 +354,8 @@ int dida_letkf_create(dida_letkf_ctx **out, const dida_letkf_params *prm, int32_
     if (prm->cube_n > 1) {
         DMALLOC(c->ca_lat, (size_t)c->nalat * sizeof(float));
         DMALLOC(c->ca_lon, (size_t)c->nalon * sizeof(float));
-        DMALLOC(c->trans_all, (size_t)c->nsel * k * k * sizeof(double));
+        if (!(prm->vert_dist > 0.0))  /* vertical cubes update in the fused kernel */
+            DMALLOC(c->trans_all, (size_t)c->nsel * k * k * sizeof(double));
     }
     if (prm->form == DIDA_FORM_PLAIN) {
         c->chunk = (int32_t)(c->nsel < 32768 ? c->nsel : 32768);
@@ -468,7 +470,7 @@ int dida_letkf_set_levels(dida_letkf_ctx *c, const dida_vert_coord *vc, int32_t 
         c->ncube = 0;
         if (hipMalloc((void **)&c->cube_row_off, (size_t)(ncube + 1) * sizeof(int32_t)) ||
             hipMalloc((void **)&c->cube_rows, (size_t)nrow * sizeof(int32_t)) ||
-            hipMalloc((void **)&c->cube_lnp, (size_t)c->ncol * ncube * sizeof(double)))
+            hipMalloc((void **)&c->cube_lnp, (size_t)c->nsel * ncube * sizeof(double)))
             rc = set_err(DIDA_ERR_NOMEM, "set_levels: device tables");
         else if (hipMemcpy(c->cube_row_off, off, (size_t)(ncube + 1) * sizeof(int32_t),
                            hipMemcpyHostToDevice) ||
@@ -479,7 +481,7 @@ int dida_letkf_set_levels(dida_letkf_ctx *c, const dida_vert_coord *vc, int32_t 
     if (!rc) {
         /* info holds one entry per analysis */
         int32_t *info = NULL;
-        if (hipMalloc((void **)&info, (size_t)c->ncol * ncube * sizeof(int32_t)))
+        if (hipMalloc((void **)&info, (size_t)c->nsel * ncube * sizeof(int32_t)))
             rc = set_err(DIDA_ERR_NOMEM, "set_levels: info");
         else {
             dfree(c->info);
@@ -664,7 +666,11 @@ static int sweep_batched(dida_letkf_ctx *c, const double *xb, const double *axb,
         la.vtype = c->vc.type;
         la.ncube = c->ncube;
         la.cube_nz = c->prm.cube_nz;
-        la.ncol = c->ncol;
+        la.ncol = c->nsel;
+        la.cube_n = c->prm.cube_n;
+        la.nalat = c->nalat;
+        la.grid_nlat = c->prm.nlat;
+        la.grid_nlon = c->prm.nlon;
         la.c1 = c->vc.c1;
         la.c2 = c->vc.c2 ? c->vc.c2 : c->vc.c1;
         la.ptop = c->vc.ptop;
@@ -679,6 +685,10 @@ static int sweep_batched(dida_letkf_ctx *c, const double *xb, const double *axb,
         ba.cube_lnp = c->cube_lnp;
         ba.cube_row_off = c->cube_row_off;
         ba.cube_rows = c->cube_rows;
+        ba.cube_n = c->prm.cube_n;
+        ba.grid_nlat = c->prm.nlat;
+        ba.grid_nlon = c->prm.nlon;
+        ba.trans_out = NULL;
         if (c->vkey_in) {
             size_t tb = c->vsort_tmp_bytes;
             HIP_TRY(dk_seg_sort_lnp(c->vsort_tmp, &tb, c->nsel, c->capacity, c->offset,
@@ -704,7 +714,7 @@ static int sweep_batched(dida_letkf_ctx *c, const double *xb, const double *axb,
     tmark(c, DIDA_STAGE_SELECT, s);
     HIP_TRY(dk_letkf_wave(&ba, c->nblocks, s));
     tmark(c, DIDA_STAGE_TRANSFORM, s);
-    if (c->prm.cube_n > 1) {
+    if (c->prm.cube_n > 1 && c->ncube == 0) {
         HIP_TRY(cube_update(c, xb, axb, xa, axa, s));
         tmark(c, DIDA_STAGE_UPDATE, s);
     }

@@ -185,6 +185,10 @@ typedef struct dk_batched_args {
     /* optional: write trans(i, e) = W + wbar 1^T of analysis col at
      * trans_out[col * k * k + i + e k] instead of updating the state */
     double *trans_out;
+    /* vertical localisation with horizontal cubes (cube_n > 1): analysis
+     * aj + nlat ai updates the columns of cube aj, ai of the state grid
+     * grid_nlat x grid_nlon (cubes start every cube_n columns) */
+    int32_t cube_n, grid_nlat, grid_nlon, pad4_;
     /* ngroup > 0: the queue holds ngroup groups of chains (longitude stripe
      * g = columns [g nlon / ngroup, (g + 1) nlon / ngroup), chains in
      * latitude-band-major order), group g starting at stripe_lo(g) *
@@ -223,7 +227,11 @@ int dk_obs_ymax(int32_t nobs, int32_t k, int32_t ldy, const double *yb, const fl
 typedef struct dk_cube_lnp_args {
     int32_t nrow, ps_row, nz, vtype;
     int32_t ncube, cube_nz;
-    int64_t ncol;
+    int64_t ncol;               /* horizontal analyses (columns, or cubes of columns) */
+    /* horizontal cubes of cube_n x cube_n columns (analysis aj + nalat ai):
+     * the surface pressure is the 1-based centre column's (cube_n 1: the
+     * column's own) */
+    int32_t cube_n, nalat, grid_nlat, grid_nlon;
     const double *c1, *c2;
     double ptop, p0;
     const double *axb;

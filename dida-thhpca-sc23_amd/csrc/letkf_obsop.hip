@@ -169,11 +169,19 @@ __global__ void k_cube_lnp(dk_cube_lnp_args a) {
 #pragma clang fp contract(off)
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= a.ncol * a.ncube) return;
-  const int64_t col = t / a.ncube;
-  const int q = (int)(t - col * a.ncube);
+  const int64_t an = t / a.ncube;
+  const int q = (int)(t - an * a.ncube);
   const int sk = q * a.cube_nz;
   const int ek = min(sk + a.cube_nz - 1, a.nz - 1);
   const int ck = (sk + 1 + ek + 1) / 2 - 1;
+  int64_t col = an;
+  if (a.cube_n > 1) {
+    const int aj = (int)(an % a.nalat), ai = (int)(an / a.nalat);
+    const int sj = aj * a.cube_n, si = ai * a.cube_n;
+    const int ej = min(sj + a.cube_n - 1, a.grid_nlat - 1);
+    const int ei = min(si + a.cube_n - 1, a.grid_nlon - 1);
+    col = (int64_t)((sj + ej) / 2) + (int64_t)a.grid_nlat * ((si + ei) / 2);
+  }
   const double ps = a.axb[col * a.nrow + a.ps_row];
   a.cube_lnp[t] = log(ph_full(a, ck, ps));
 }

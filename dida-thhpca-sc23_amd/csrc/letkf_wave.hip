@@ -936,8 +936,20 @@ __device__ void wave_column(const dk_batched_args& a, int64_t col, int cube, boo
   // cube's list of rows
   const int32_t* rmap = VERT ? a.cube_rows + a.cube_row_off[cube] : nullptr;
   const int nr = VERT ? a.cube_row_off[cube + 1] - a.cube_row_off[cube] : a.nrow;
-  update_rows<KP, VERT>(k, a.nrow, a.xb, a.axb, a.xa, a.axa, a.rtps != 0, a.rtps_alpha, col, T,
-                        rmap, nr);
+  if (VERT && a.cube_n > 1) {
+    // horizontal cube of columns: every column takes the cube's transform
+    const int aj = (int)(col % a.nlat), ai = (int)(col / a.nlat);
+    const int sj = aj * a.cube_n, si = ai * a.cube_n;
+    const int ej = min(sj + a.cube_n - 1, a.grid_nlat - 1);
+    const int ei = min(si + a.cube_n - 1, a.grid_nlon - 1);
+    for (int ilon = si; ilon <= ei; ++ilon)
+      for (int ilat = sj; ilat <= ej; ++ilat)
+        update_rows<KP, VERT>(k, a.nrow, a.xb, a.axb, a.xa, a.axa, a.rtps != 0, a.rtps_alpha,
+                              (int64_t)ilat + (int64_t)a.grid_nlat * ilon, T, rmap, nr);
+  } else {
+    update_rows<KP, VERT>(k, a.nrow, a.xb, a.axb, a.xa, a.axa, a.rtps != 0, a.rtps_alpha, col,
+                          T, rmap, nr);
+  }
   clk.mark(DK_PHASE_UPDATE);
   (void)NQ;
 }

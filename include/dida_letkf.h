@@ -91,7 +91,9 @@ typedef struct dida_letkf_params {
      * column, clipped at the last; one analysis per cube, selected around the
      * cube's lower-middle column (gen_yb_dis, letkf_yb.F90:19-20); every
      * column takes the mean of the analyses of the cubes covering it.  Both
-     * forms; cube_n = 1 with vertical localisation. */
+     * forms; with vertical localisation overlap = 0 (as the reference's
+     * vertical sweep, letkf_compute.F90:408-425: one analysis per horizontal
+     * and vertical cube, levels over the centre column's surface pressure). */
     int32_t cube_n;             /* da_block_interval (default 1, as
                                    namelists.input_100km_32mem:68) */
     int32_t overlap;            /* da_block_overlap, 0 <= overlap < cube_n (default 0) */

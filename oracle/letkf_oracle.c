@@ -841,16 +841,18 @@ void orc_obs_operator(int k, int nrow, int nlat, int nlon, const float *grid_lat
     free(ph);
 }
 
-/* letkf_compute_nooverlap_vertical (letkf_compute.F90:358-526) with cube_n = 1. */
+/* letkf_compute_nooverlap_vertical (letkf_compute.F90:358-526). */
 int orc_sweep_vertical(const orc_params *prm, const float *grid_lat, const float *grid_lon,
                        const orc_obs *obs, const double *obs_ph, const double *xb,
                        const double *axb, double *xa, double *axa, const orc_bins *bins,
                        const orc_vcoord *vc, int ps_row, int nvar, const int32_t *var_start,
                        const int32_t *var_lev, double vert_dist, int cube_nz,
-                       int gc_filter_vertical, int nthreads)
+                       int gc_filter_vertical, int cube_n, int nthreads)
 {
-    const int k = prm->k, n = prm->nrow, nz = vc->nz;
-    const int64_t ncol = (int64_t)prm->nlat * prm->nlon;
+    const int k = prm->k, n = prm->nrow, nz = vc->nz, nlat = prm->nlat, nlon = prm->nlon;
+    if (cube_n < 1) cube_n = 1;
+    const int nalat = (nlat + cube_n - 1) / cube_n, nalon = (nlon + cube_n - 1) / cube_n;
+    const int64_t nanal = (int64_t)nalat * nalon;
     int nfail = 0;
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
@@ -870,9 +872,16 @@ int orc_sweep_vertical(const orc_params *prm, const float *grid_lat, const float
         double *wb = (double *)malloc(sizeof(double) * (size_t)k);
         double *alnp = (double *)malloc(sizeof(double) * (size_t)(nz > 0 ? nz : 1));
 #pragma omp for schedule(dynamic, 4)
-        for (int64_t col = 0; col < ncol; ++col) {
-            int ilat = (int)(col % prm->nlat), ilon = (int)(col / prm->nlat);
-            double clat = (double)grid_lat[ilat], clon = (double)grid_lon[ilon];
+        for (int64_t an = 0; an < nanal; ++an) {
+            /* horizontal cube: columns sj..ej x si..ei, centred on the
+             * 1-based (s + e) / 2 (letkf_compute.F90:409-425) */
+            const int aj = (int)(an % nalat), ai = (int)(an / nalat);
+            const int sj = aj * cube_n, si = ai * cube_n;
+            const int ej = sj + cube_n - 1 < nlat - 1 ? sj + cube_n - 1 : nlat - 1;
+            const int ei = si + cube_n - 1 < nlon - 1 ? si + cube_n - 1 : nlon - 1;
+            const int cj = (sj + 1 + ej + 1) / 2 - 1, ci = (si + 1 + ei + 1) / 2 - 1;
+            const int64_t ccol = cj + (int64_t)nlat * ci;
+            double clat = (double)grid_lat[cj], clon = (double)grid_lon[ci];
             int p = bins ? orc_select_column_binned(prm, obs, bins->bin_start, bins->nband,
                                                     bins->nlonbin, bins->bin_deg, clat,
                                                     clon, idx, rl)
@@ -883,8 +892,8 @@ int orc_sweep_vertical(const orc_params *prm, const float *grid_lat, const float
                 ri[o] = obs->rinv[idx[o]];
                 olnp[o] = log(obs_ph[idx[o]]);
             }
-            /* ps2alnp from the column's mean surface pressure */
-            const double aps = axb[(size_t)col * n + ps_row];
+            /* ps2alnp from the centre column's mean surface pressure */
+            const double aps = axb[(size_t)ccol * n + ps_row];
             for (int i = 0; i < nz; ++i) {
                 alnp[i] = 0.5 * (ph_half(vc, i, aps) + ph_half(vc, i + 1, aps));
                 alnp[i] = log(alnp[i]);
@@ -898,20 +907,25 @@ int orc_sweep_vertical(const orc_params *prm, const float *grid_lat, const float
                     rlv[o] = rl[o] * (gc_filter_vertical ? orc_schr2(r) : orc_schr(r));
                 }
                 if (orc_letkf_core(k, p, hx, ri, dp, rlv, tr, wb, prm->inflation) < 0) ++nfail;
-                for (int v = 0; v < nvar; ++v) {
-                    int r0, r1;
-                    if (var_lev[v] == 1) {
-                        if (ek != nz - 1) continue;
-                        r0 = r1 = var_start[v];
-                    } else {
-                        r0 = var_start[v] + sk;
-                        r1 = var_start[v] + ek;
+                for (int ilon = si; ilon <= ei; ++ilon)
+                    for (int ilat = sj; ilat <= ej; ++ilat) {
+                        const int64_t col = ilat + (int64_t)nlat * ilon;
+                        for (int v = 0; v < nvar; ++v) {
+                            int r0, r1;
+                            if (var_lev[v] == 1) {
+                                if (ek != nz - 1) continue;
+                                r0 = r1 = var_start[v];
+                            } else {
+                                r0 = var_start[v] + sk;
+                                r1 = var_start[v] + ek;
+                            }
+                            for (int row = r0; row <= r1; ++row)
+                                orc_xb_2_xa(k, 1, xb + ((size_t)col * n + row) * k,
+                                            axb + (size_t)col * n + row, tr,
+                                            xa + ((size_t)col * n + row) * k,
+                                            axa + (size_t)col * n + row);
+                        }
                     }
-                    for (int row = r0; row <= r1; ++row)
-                        orc_xb_2_xa(k, 1, xb + ((size_t)col * n + row) * k,
-                                    axb + (size_t)col * n + row, tr,
-                                    xa + ((size_t)col * n + row) * k, axa + (size_t)col * n + row);
-                }
             }
         }
         free(alnp);

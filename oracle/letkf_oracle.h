@@ -181,14 +181,17 @@ typedef struct orc_vcoord {
     double ptop, p0;
 } orc_vcoord;
 
-/* Vertical localisation (cube_n = 1): for every column, the horizontal
- * selection of orc_sweep, then for each vertical cube of cube_nz levels
+/* Vertical localisation: for every horizontal cube of cube_n x cube_n
+ * columns (cube_n = 1: every column; starts every cube_n columns, clipped,
+ * selection and surface pressure at the 1-based centre (s + e) / 2), the
+ * horizontal selection of orc_sweep, then for each vertical cube of cube_nz levels
  * (sk..ek, the last one shorter) an analysis with rloc multiplied by
  * schr2 (gc_filter_vertical) or schr of |ln p_c - ln p_obs| / vert_dist,
  * where p_c is the full-level pressure of level (sk + ek) / 2 (1-based) from
- * the column's mean surface pressure (axb at ps_row; levels from vc), ln p_obs
- * = log(obs_ph).  Each cube's transform updates rows sk..ek of every 3-D
- * variable and, with the last cube, the 2-D variables; the rows are then
+ * the centre column's mean surface pressure (axb at ps_row; levels from vc),
+ * ln p_obs = log(obs_ph).  Each cube's transform updates rows sk..ek of every
+ * 3-D variable and, with the last cube, the 2-D variables, in every column of
+ * the horizontal cube; the rows are then
  * averaged and re-centred (xa, axa as orc_sweep).  The reference leaves the
  * bottom half level of ps2alnp unset (obs_make.F90:429-434); it is computed
  * here from vc.  bins as orc_sweep.  Returns non-converged eigensolves. */
@@ -197,7 +200,7 @@ int orc_sweep_vertical(const orc_params *prm, const float *grid_lat, const float
                        const double *axb, double *xa, double *axa, const orc_bins *bins,
                        const orc_vcoord *vc, int ps_row, int nvar, const int32_t *var_start,
                        const int32_t *var_lev, double vert_dist, int cube_nz,
-                       int gc_filter_vertical, int nthreads);
+                       int gc_filter_vertical, int cube_n, int nthreads);
 
 /* Observation centring: ybar = mean_e hxb; dep = yo - ybar; yb = hxb - ybar. */
 void orc_centre_obs(int k, int nobs, const double *hxb, const double *yo,

@@ -110,7 +110,7 @@ def lib():
         L.orc_sweep_vertical.argtypes = [C.POINTER(OrcParams), _fp, _fp, C.POINTER(OrcObs), _dp,
                                          _dp, _dp, _dp, _dp, C.c_void_p, C.POINTER(OrcVcoord),
                                          C.c_int, C.c_int, _ip, _ip, C.c_double, C.c_int,
-                                         C.c_int, C.c_int]
+                                         C.c_int, C.c_int, C.c_int]
         L.orc_obs_operator.restype = None
         L.orc_obs_operator.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _fp, _fp, C.c_int,
                                        _dp, _dp, C.c_int, C.POINTER(OrcVcoord), _ip, _ip,
@@ -384,9 +384,10 @@ def sweep_cubes(prm: OrcParams, grid_lat, grid_lon, obs: Obs, xb, axb, cube_n, o
 
 def sweep_vertical(prm: OrcParams, grid_lat, grid_lon, obs: Obs, obs_ph, xb, axb, vcoord,
                    ps_row, var_start, var_lev, vert_dist, cube_nz, gc_filter_vertical=True,
-                   bins=None, nthreads=0):
-    """letkf_compute_nooverlap_vertical (cube_n = 1).  obs_ph: obs pressure
-    in the order of obs; vcoord as obs_operator.  Returns dict(xa, axa, nfail)."""
+                   bins=None, nthreads=0, cube_n=1):
+    """letkf_compute_nooverlap_vertical with cube_n x cube_n horizontal cubes.
+    obs_ph: obs pressure in the order of obs; vcoord as obs_operator.  Returns
+    dict(xa, axa, nfail)."""
     k, n = prm.k, prm.nrow
     ncol = prm.nlat * prm.nlon
     xa = np.zeros(ncol * n * k)
@@ -409,7 +410,7 @@ def sweep_vertical(prm: OrcParams, grid_lat, grid_lon, obs: Obs, obs_ph, xb, axb
         np.ascontiguousarray(axb, np.float64).reshape(-1), xa, axa,
         None if bins_c is None else C.cast(C.pointer(bins_c), C.c_void_p), C.byref(vc),
         int(ps_row), len(vs), vs, vl, float(vert_dist), int(cube_nz), int(bool(gc_filter_vertical)),
-        int(nthreads))
+        int(cube_n), int(nthreads))
     return {"xa": xa.reshape(ncol, n, k), "axa": axa.reshape(ncol, n), "nfail": nfail}
 
 

@@ -109,7 +109,7 @@ def test_cube_params(dl):
         bad = dl.default_params(nanals=8, nrow=3, nlat=4, nlon=4, cube_n=cube_n, overlap=overlap)
         rc = dl.lib().dida_letkf_create(ctypes.byref(h), ctypes.byref(bad), 10, 0)
         assert rc < 0 and b"cube_n" in dl.lib().dida_letkf_last_error()
-    bad = dl.default_params(nanals=8, nrow=3, nlat=4, nlon=4, cube_n=2, vert_dist=1.0,
-                            form=dl.FORM_BATCHED)
+    bad = dl.default_params(nanals=8, nrow=3, nlat=4, nlon=4, cube_n=3, overlap=1,
+                            vert_dist=1.0, form=dl.FORM_BATCHED)
     rc = dl.lib().dida_letkf_create(ctypes.byref(h), ctypes.byref(bad), 10, 0)
-    assert rc == -5 and b"cube_n = 1" in dl.lib().dida_letkf_last_error()
+    assert rc == -5 and b"overlap = 0" in dl.lib().dida_letkf_last_error()

@@ -59,10 +59,12 @@ def _oracle(oracle, h, ph, params, vc, vs, vl):
     return oracle.sweep_vertical(prm, h["grid_lat"], h["grid_lon"], o.permuted(perm), ph[perm],
                                  h["xb"], h["axb"], vc, 0, vs, vl, params.vert_dist,
                                  params.cube_nz, bool(params.gc_filter_vertical),
-                                 bins=(bin_start, nband, nlonbin, params.bin_deg))
+                                 bins=(bin_start, nband, nlonbin, params.bin_deg),
+                                 cube_n=params.cube_n)
 
 
-@pytest.mark.parametrize("case", ["fp64_gc_cube1", "fp64_boxcar_cube4_capped", "bf16x3_cube2"])
+@pytest.mark.parametrize("case", ["fp64_gc_cube1", "fp64_boxcar_cube4_capped", "bf16x3_cube2",
+                                  "fp64_gc_hcube3", "fp64_gc_hcube2_cube3"])
 def test_vertical_sweep_matches_oracle(cuda, dl, oracle, case):
     torch = cuda
     print(f"vertical sweep case {case}", flush=True)
@@ -72,15 +74,20 @@ def test_vertical_sweep_matches_oracle(cuda, dl, oracle, case):
     elif case == "fp64_boxcar_cube4_capped":
         over, tol = dict(vert_dist=0.7, cube_nz=4, gc_filter_vertical=0, gram=dl.GRAM_FP64,
                          max_obs=60), 1e-10
-    else:
+    elif case == "bf16x3_cube2":
         over, tol = dict(vert_dist=0.5, cube_nz=2, gram=dl.GRAM_BF16X3), 1e-4
+    elif case == "fp64_gc_hcube3":
+        over, tol = dict(vert_dist=0.5, cube_nz=1, cube_n=3, gram=dl.GRAM_FP64), 1e-10
+    else:
+        over, tol = dict(vert_dist=0.6, cube_nz=3, cube_n=2, gram=dl.GRAM_FP64), 1e-10
     params, eng, obs = _engine(dl, torch, cfg, data, ph, vc, vs, vl, **over)
     xa, axa = torch.empty_like(data["xb"]), torch.empty_like(data["axb"])
     eng.sweep(obs, data["xb"], data["axb"], xa, axa)
     st = eng.stats()
     ncube = -(-NZ // params.cube_nz)
+    nh = -(-cfg.nlat // params.cube_n) * -(-cfg.nlon // params.cube_n)
     assert st["status"] == 0 and st["nonconverged"] == 0, st
-    assert st["nanal"] == cfg.nlat * cfg.nlon * ncube
+    assert st["nanal"] == nh * ncube
     if params.max_obs < 10000:
         assert st["nobsl_capped"] > 0
     h = {k: v.cpu().numpy() for k, v in data.items()}

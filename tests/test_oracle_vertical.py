@@ -81,3 +81,43 @@ def test_every_row_written_once(oracle):
     # re-centred analysis perturbations on every row, finite everywhere
     assert np.allclose(v["xa"].sum(axis=2), 0.0, atol=1e-14 * np.abs(v["axa"]).max() * 8)
     assert np.isfinite(v["axa"]).all() and (v["axa"] != 0).all()
+
+
+@pytest.mark.parametrize("cube_n,cube_nz", [(2, 1), (3, 4), (4, NZ)])
+def test_horizontal_cubes_with_unit_vertical_weights_are_the_cube_sweep(oracle, cube_n, cube_nz):
+    # every vertical cube of a horizontal cube takes the same analysis, so the
+    # result is the horizontal-cube sweep without vertical localisation
+    glat, glon, xb, axb, o, ph = _case(3)
+    k, n = xb.shape[2], xb.shape[1]
+    obs = oracle.Obs(*o)
+    prm = oracle.params(k, n, len(glat), len(glon), 3000.0, 1.1)
+    v = oracle.sweep_vertical(prm, glat, glon, obs, ph, xb, axb, SIGMA, 0, VS, VL, 1.0e30,
+                              cube_nz, True, cube_n=cube_n)
+    c = oracle.sweep_cubes(prm, glat, glon, obs, xb, axb, cube_n, 0)
+    assert v["nfail"] == 0
+    assert np.allclose(v["xa"], c["xa"], atol=1e-11)
+    assert np.allclose(v["axa"], c["axa"], atol=1e-11)
+
+
+def test_horizontal_cube_levels_use_the_centre_column_pressure(oracle):
+    # cube_n = 3 on the 6 x 10 grid: the cube of columns (0..2) x (0..2) is
+    # centred on column (1, 1); its analyses are those of the one-column sweep
+    # at (1, 1), so column (1, 1) matches that sweep and any other column of
+    # the cube takes the same transforms (its rows differ only through xb)
+    glat, glon, xb, axb, o, ph = _case(4)
+    k, n = xb.shape[2], xb.shape[1]
+    obs = oracle.Obs(*o)
+    prm = oracle.params(k, n, len(glat), len(glon), 3000.0, 1.1)
+    v3 = oracle.sweep_vertical(prm, glat, glon, obs, ph, xb, axb, SIGMA, 0, VS, VL, 0.4, 2, True,
+                               cube_n=3)
+    v1 = oracle.sweep_vertical(prm, glat, glon, obs, ph, xb, axb, SIGMA, 0, VS, VL, 0.4, 2, True)
+    c = 1 + len(glat) * 1
+    assert np.allclose(v3["xa"][c], v1["xa"][c], atol=1e-12)
+    assert np.allclose(v3["axa"][c], v1["axa"][c], atol=1e-12)
+    # column (0, 0) with the centre's xb gives the centre's result
+    xb2, axb2 = xb.copy(), axb.copy()
+    xb2[0], axb2[0] = xb[c], axb[c]
+    v3b = oracle.sweep_vertical(prm, glat, glon, obs, ph, xb2, axb2, SIGMA, 0, VS, VL, 0.4, 2,
+                                True, cube_n=3)
+    assert np.allclose(v3b["xa"][0], v3b["xa"][c], atol=1e-12)
+    assert np.allclose(v3b["axa"][0], v3b["axa"][c], atol=1e-12)
