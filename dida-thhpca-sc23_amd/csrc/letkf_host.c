@@ -158,6 +158,7 @@ struct dida_letkf_ctx {
     int32_t *csr_idx;
     double *csr_w;
     int64_t capacity;
+    int32_t user_capacity; /* capacity given at create: plain sweeps run in chunks */
     int32_t *flags; /* [0] capacity overflow */
     /* transform (plain form, chunked) */
     int32_t chunk;
@@ -346,6 +347,7 @@ int dida_letkf_create(dida_letkf_ctx **out, const dida_letkf_params *prm, int32_
     DMALLOC(c->scan_tmp, c->scan_tmp_bytes);
     DMALLOC(c->flags, 4 * sizeof(int32_t));
     c->capacity = sel_capacity;
+    c->user_capacity = sel_capacity > 0;
     if (sel_capacity > 0) {
         DMALLOC(c->csr_idx, (size_t)sel_capacity * sizeof(int32_t));
         DMALLOC(c->csr_w, (size_t)sel_capacity * sizeof(double));
@@ -721,22 +723,19 @@ static int sweep_batched(dida_letkf_ctx *c, const double *xb, const double *axb,
     return DIDA_OK;
 }
 
-static int sweep_rest(dida_letkf_ctx *c, dk_select_args *sa, const double *xb,
-                      const double *axb, double *xa, double *axa, hipStream_t s)
+/* Plain form: transforms (and, for columns, updates) of analyses [a0, a1)
+ * whose selection entries sit at offset[] - base; `last` closes the sweep
+ * (horizontal cubes update every column once all transforms exist). */
+static int plain_transform(dida_letkf_ctx *c, int64_t a0, int64_t a1, int64_t base, int last,
+                           const double *xb, const double *axb, double *xa, double *axa,
+                           hipStream_t s)
 {
     const int k = c->prm.nanals;
-    sa->csr_idx = c->csr_idx;
-    sa->csr_w = c->csr_w;
-    sa->capacity = c->capacity;
-    HIP_TRY(dk_select_fill(sa, s));
-    tmark(c, DIDA_STAGE_SELECT, s);
-
-    if (c->prm.form == DIDA_FORM_BATCHED) return sweep_batched(c, xb, axb, xa, axa, s);
-
     dk_core_args ca;
     memset(&ca, 0, sizeof ca);
     ca.k = k;
     ca.offset = c->offset;
+    ca.base = base;
     ca.nobsl = c->nobsl;
     ca.idx = c->csr_idx;
     ca.rloc = c->csr_w;
@@ -764,20 +763,22 @@ static int sweep_rest(dida_letkf_ctx *c, dk_select_args *sa, const double *xb,
     ua.rtps = c->prm.rtps;
     ua.rtps_alpha = c->prm.rtps_alpha;
     if (c->prm.cube_n > 1) {
-        for (int64_t c0 = 0; c0 < c->nsel; c0 += c->chunk) {
-            int32_t n = (int32_t)((c->nsel - c0) < c->chunk ? (c->nsel - c0) : c->chunk);
+        for (int64_t c0 = a0; c0 < a1; c0 += c->chunk) {
+            int32_t n = (int32_t)((a1 - c0) < c->chunk ? (a1 - c0) : c->chunk);
             ca.aid0 = (int32_t)c0;
             ca.nanal = n;
             ca.trans = c->trans_all + c0 * k * k;
             HIP_TRY(dk_letkf_core_f64(&ca, s));
         }
         tmark(c, DIDA_STAGE_TRANSFORM, s);
-        HIP_TRY(cube_update(c, xb, axb, xa, axa, s));
-        tmark(c, DIDA_STAGE_UPDATE, s);
+        if (last) {
+            HIP_TRY(cube_update(c, xb, axb, xa, axa, s));
+            tmark(c, DIDA_STAGE_UPDATE, s);
+        }
         return DIDA_OK;
     }
-    for (int64_t c0 = 0; c0 < c->ncol; c0 += c->chunk) {
-        int32_t n = (int32_t)((c->ncol - c0) < c->chunk ? (c->ncol - c0) : c->chunk);
+    for (int64_t c0 = a0; c0 < a1; c0 += c->chunk) {
+        int32_t n = (int32_t)((a1 - c0) < c->chunk ? (a1 - c0) : c->chunk);
         ca.aid0 = (int32_t)c0;
         ca.nanal = n;
         HIP_TRY(dk_letkf_core_f64(&ca, s));
@@ -789,6 +790,68 @@ static int sweep_rest(dida_letkf_ctx *c, dk_select_args *sa, const double *xb,
         tmark(c, DIDA_STAGE_UPDATE, s);
     }
     return DIDA_OK;
+}
+
+/* Plain form with a workspace fixed at create: the analyses run in runs
+ * whose selection lists fit it (the host reads the offsets once per sweep;
+ * one run when everything fits). */
+static int sweep_plain_chunked(dida_letkf_ctx *c, dk_select_args *sa, const double *xb,
+                               const double *axb, double *xa, double *axa, hipStream_t s)
+{
+    const int64_t n = c->nsel, cap = c->capacity;
+    int64_t *off = (int64_t *)malloc((size_t)(n + 1) * sizeof(int64_t));
+    if (!off) return set_err(DIDA_ERR_NOMEM, "host offsets");
+    int rc = DIDA_OK;
+    if (hipMemcpyAsync(off, c->offset, (size_t)(n + 1) * sizeof(int64_t), hipMemcpyDeviceToHost,
+                       s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        rc = set_err(DIDA_ERR_HIP, "offsets copy failed");
+    sa->csr_idx = c->csr_idx;
+    sa->csr_w = c->csr_w;
+    sa->capacity = cap;
+    int64_t a0 = 0;
+    while (!rc && a0 < n) {
+        if (off[a0 + 1] - off[a0] > cap) {
+            rc = set_err(DIDA_ERR_CAPACITY, "analysis %lld alone needs %lld selection entries > %lld",
+                         (long long)a0, (long long)(off[a0 + 1] - off[a0]), (long long)cap);
+            break;
+        }
+        int64_t lo = a0 + 1, hi = n; /* largest a1 with off[a1] - off[a0] <= cap */
+        while (lo < hi) {
+            const int64_t mid = lo + (hi - lo + 1) / 2;
+            if (off[mid] - off[a0] <= cap) lo = mid;
+            else hi = mid - 1;
+        }
+        const int64_t a1 = lo;
+        sa->col0 = a0;
+        sa->ncols = a1 - a0;
+        sa->base = off[a0];
+        if (dk_select_fill(sa, s)) {
+            rc = set_err(DIDA_ERR_HIP, "selection fill failed");
+            break;
+        }
+        tmark(c, DIDA_STAGE_SELECT, s);
+        rc = plain_transform(c, a0, a1, off[a0], a1 == n, xb, axb, xa, axa, s);
+        a0 = a1;
+    }
+    sa->col0 = sa->ncols = sa->base = 0;
+    free(off);
+    return rc;
+}
+
+static int sweep_rest(dida_letkf_ctx *c, dk_select_args *sa, const double *xb,
+                      const double *axb, double *xa, double *axa, hipStream_t s)
+{
+    const int k = c->prm.nanals;
+    sa->csr_idx = c->csr_idx;
+    sa->csr_w = c->csr_w;
+    sa->capacity = c->capacity;
+    HIP_TRY(dk_select_fill(sa, s));
+    tmark(c, DIDA_STAGE_SELECT, s);
+
+    if (c->prm.form == DIDA_FORM_BATCHED) return sweep_batched(c, xb, axb, xa, axa, s);
+
+    return plain_transform(c, 0, c->nsel, 0, 1, xb, axb, xa, axa, s);
 }
 
 static int check_sweep_args(dida_letkf_ctx *c, const dida_letkf_obs *o, const double *xb,
@@ -820,7 +883,10 @@ int dida_letkf_sweep(dida_letkf_ctx *c, const dida_letkf_obs *o, const double *x
     if (rc) return rc;
     rc = ensure_capacity(c, s);
     if (rc) return rc;
-    rc = sweep_rest(c, &sa, xb, axb, xa, axa, s);
+    if (c->prm.form == DIDA_FORM_PLAIN && c->user_capacity)
+        rc = sweep_plain_chunked(c, &sa, xb, axb, xa, axa, s);
+    else
+        rc = sweep_rest(c, &sa, xb, axb, xa, axa, s);
     c->last_stream = s;
     c->last_nobs = o->nobs;
     return rc;

@@ -55,6 +55,9 @@ typedef struct dk_select_args {
     double *csr_w;       /* fill pass: rloc */
     int64_t capacity;    /* fill pass: entries available in csr_idx / csr_w */
     int32_t *overflow;   /* fill pass: set to 1 when a column does not fit */
+    /* fill pass over analyses [col0, col0 + ncols) only (ncols 0: all),
+     * writing entry offset[col] - base */
+    int64_t col0, ncols, base;
 } dk_select_args;
 
 /* Fill n int32 with v (a kernel, so that captured graphs hold no memset). */
@@ -103,6 +106,7 @@ typedef struct dk_core_args {
     double tol;             /* Jacobi: rotate while |g| > tol sqrt(a b) */
     int32_t max_sweeps;
     int32_t predict;        /* Jacobi: stop when the next sweep would rotate nothing */
+    int64_t base;           /* idx / rloc hold entries offset[id] - base */
     double *trans;          /* [(id - aid0) * k * k] trans(i, j) at i + j k */
     double *wbar;           /* [(id - aid0) * k] (optional) */
     int32_t *info;          /* [id]: sweeps, or -1 when not converged (optional) */

@@ -172,8 +172,8 @@ __device__ __forceinline__ void for_candidate_ranges(const dk_select_args& a, do
 template <bool kFill>
 __global__ __launch_bounds__(256) void k_select(dk_select_args a) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int64_t col = (int64_t)blockIdx.x * 4 + wave;
-  const int64_t ncol = (int64_t)a.nlat * a.nlon;
+  const int64_t ncol = a.ncols > 0 ? a.col0 + a.ncols : (int64_t)a.nlat * a.nlon;
+  const int64_t col = a.col0 + (int64_t)blockIdx.x * 4 + wave;
   if (col >= ncol) return;
   const int ilat = (int)(col % a.nlat), ilon = (int)(col / a.nlat);
   const double clat = (double)a.grid_lat[ilat];
@@ -186,7 +186,7 @@ __global__ __launch_bounds__(256) void k_select(dk_select_args a) {
   // around the support radius exactly as the reference formula would
   const double far_lo = a.cos_far - 1e-12, near_hi = a.cos_near + 1e-12;
   const int limit = kFill ? a.nobsl[col] : a.max_obs;
-  int64_t out = kFill ? a.offset[col] : 0;
+  int64_t out = kFill ? a.offset[col] - a.base : 0;
   if (kFill && out + limit > a.capacity && lane == 0) *a.overflow = 1;
   int cnt = 0;
   if (limit > 0) {
@@ -283,14 +283,14 @@ extern "C" int dk_obs_ymax(int32_t nobs, int32_t k, int32_t ldy, const double* y
 }
 
 extern "C" int dk_select_count(const dk_select_args* a, void* stream) {
-  int64_t ncol = (int64_t)a->nlat * a->nlon;
+  int64_t ncol = a->ncols > 0 ? a->ncols : (int64_t)a->nlat * a->nlon;
   if (ncol <= 0) return 0;
   k_select<false><<<(unsigned)((ncol + 3) / 4), 256, 0, (hipStream_t)stream>>>(*a);
   return (int)hipGetLastError();
 }
 
 extern "C" int dk_select_fill(const dk_select_args* a, void* stream) {
-  int64_t ncol = (int64_t)a->nlat * a->nlon;
+  int64_t ncol = a->ncols > 0 ? a->ncols : (int64_t)a->nlat * a->nlon;
   if (ncol <= 0) return 0;
   k_select<true><<<(unsigned)((ncol + 3) / 4), 256, 0, (hipStream_t)stream>>>(*a);
   return (int)hipGetLastError();

@@ -277,7 +277,7 @@ __global__ __launch_bounds__(kThreads) void k_letkf_core_f64(dk_core_args a) {
   const int aid = a.list ? a.list[blockIdx.x] : a.aid0 + (int)blockIdx.x;
   const int64_t slot = (int64_t)aid - a.aid0;
   const int k = a.k;
-  const int64_t off = a.offset[aid];
+  const int64_t off = a.offset[aid] - a.base;
   int p = a.nobsl[aid];
   p = p < 0 ? 0 : (p > a.max_p ? a.max_p : p);
   if (off < 0 || off + p > a.capacity) p = 0;

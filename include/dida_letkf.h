@@ -176,7 +176,11 @@ int dida_letkf_core_batch_dev(int32_t batch, int32_t max_obs, int32_t nanals,
  *
  * nobs_max bounds the observation count of later sweeps; sel_capacity is the
  * number of (column, obs) selection entries to reserve (0: sized by the first
- * sweep, which then synchronises once). */
+ * sweep, which then synchronises once).  With a reserved workspace the plain
+ * form runs each sweep in runs of consecutive analyses whose lists fit it (it
+ * reads the list offsets back once per sweep), so workloads whose selection
+ * lists exceed the device memory left next to the state still run; the
+ * batched form and captured graphs report DIDA_ERR_CAPACITY instead. */
 int dida_letkf_create(dida_letkf_ctx **ctx, const dida_letkf_params *prm, int32_t nobs_max,
                       int64_t sel_capacity);
 int dida_letkf_destroy(dida_letkf_ctx *ctx);
