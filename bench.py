@@ -40,11 +40,15 @@ def var_rows(nz):
             "T": list(range(1 + 2 * nz, 1 + 3 * nz))}
 
 
+SEL_CAPACITY = 0   # selection entries reserved per engine (0: sized by the first sweep)
+
+
 def make_engine(dl, cfg, prm, form, gram, nobs):
     n = 1 + 3 * cfg.nz
     p = dl.default_params(nanals=cfg.k, nrow=n, nlat=cfg.nlat, nlon=cfg.nlon, form=form,
                           gram=gram, **prm)
-    return dl.Engine(p, nobs_max=nobs), p
+    cap = SEL_CAPACITY if form == dl.FORM_PLAIN else 0
+    return dl.Engine(p, nobs_max=nobs, capacity=cap), p
 
 
 def time_form(dl, torch, data, cfg, prm, form, gram, steps, warmup, tune=False):
@@ -407,6 +411,11 @@ def main():
                     help="skip the vertical-localisation timing")
     ap.add_argument("--no-cubes", action="store_true",
                     help="skip the horizontal-cube timing")
+    ap.add_argument("--sel-gb", type=float, default=0.0,
+                    help="plain form: selection workspace in GB (12 B an entry); sweeps whose "
+                         "lists exceed it run in runs of analyses")
+    ap.add_argument("--no-struct", action="store_true",
+                    help="skip the coupler layout timing (it allocates one 3-D variable block)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device copy (counter scale for profiles)")
     args = ap.parse_args()
@@ -423,6 +432,8 @@ def main():
     from oracle import oracle as O
 
     torch.cuda.set_device(0)
+    global SEL_CAPACITY
+    SEL_CAPACITY = int(args.sel_gb * 1e9 / 12)
     if args.copy_probe:
         copy_probe(torch)
         return
@@ -500,9 +511,11 @@ def main():
         log("timing horizontal cubes")
         cubes = time_cubes(dl, torch, data, cfg, prm,
                            res.get("batched_bf16x3", {}).get("tuning"))
-    log("timing the coupler layout conversions")
-    struct = time_struct(dl, torch, data, cfg)
-    log(f"  {struct}")
+    struct = None
+    if not args.no_struct:
+        log("timing the coupler layout conversions")
+        struct = time_struct(dl, torch, data, cfg)
+        log(f"  {struct}")
     log("timing the observation operator")
     obsop = time_obs_operator(dl, torch, data, cfg)
     log(f"  {obsop}")
@@ -517,8 +530,13 @@ def main():
         "dtype": "f64" if head == "plain_fp64" else "f64+" + head.split("_")[1],
         "data": "synthetic",
         "config": {"workload": f"LETKF global {cfg.nlat}x{cfg.nlon} columns x {cfg.nz} levels "
-                               f"(n={n}), k={cfg.k}, {cfg.nobs} obs (70% in 6 clusters), "
-                               f"L={prm['local_dist']} km, RTPS 0.8, MAX_OBS 10000",
+                               f"(n={n}), k={cfg.k}, {cfg.nobs} obs "
+                               f"({cfg.cluster_frac:.0%} in {cfg.nclusters} clusters), "
+                               f"R x {cfg.rscale_min:g}-{cfg.rscale_max:g}, "
+                               f"L={prm['local_dist']} km, "
+                               + (f"RTPS {prm['rtps_alpha']:g}, " if prm.get("rtps") else
+                                  f"inflation {prm['inflation_factor']:g}, ")
+                               + "MAX_OBS 10000",
                    "form": head, "columns": ncol, "nobsl_sum": st["nobsl_sum"],
                    "nobsl_max": st["nobsl_max"], "nobsl_capped": st["nobsl_capped"],
                    "jacobi_max_sweeps_used": st["max_sweeps_used"],
