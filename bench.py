@@ -346,6 +346,84 @@ def time_struct(dl, torch, data, cfg, reps=3):
     return out
 
 
+# Regional-domain workload: twelve domains of 64 x 64 to 384 x 512 columns,
+# 32 levels, k in {32, 48, 64}, 0.5-5 obs per column (each domain its own
+# synthetic field), L = 150 km
+DOMAINS = [(64, 64, 32, 0.5), (96, 128, 48, 1.0), (128, 128, 64, 2.0), (128, 192, 32, 3.0),
+           (192, 192, 48, 5.0), (160, 256, 64, 0.8), (256, 256, 32, 1.5), (200, 320, 48, 4.0),
+           (256, 384, 64, 2.5), (320, 384, 32, 1.2), (384, 448, 48, 3.5), (384, 512, 64, 0.6)]
+
+
+def time_domains(dl, torch, steps=3):
+    """All twelve domains' sweeps recorded as one HIP graph with a parallel
+    branch per domain (dida_letkf_group_*), replayed `steps` times and timed
+    with HIP events on the launching stream; BF16x3 timed, the FP64 batched
+    form once for the per-variable increment error."""
+    nz = 32
+    data = []
+    for d, (ny, nx, k, dens) in enumerate(DOMAINS):
+        cfg = dl.synth_cfg(ny, nx, nz, k, int(dens * ny * nx), seed=0xD1DA0300 + d,
+                           rscale=(0.5, 3.0))
+        data.append((cfg, dl.synth_generate(cfg)))
+    ncol = sum(c.nlat * c.nlon for c, _ in data)
+    res, outs = {}, {}
+    for name, gram, nstep in (("batched_fp64", dl.GRAM_FP64, 1),
+                              ("batched_bf16x3", dl.GRAM_BF16X3, steps)):
+        engs, obs, xa, axa = [], [], [], []
+        for cfg, g in data:
+            p = dl.default_params(nanals=cfg.k, nrow=1 + 3 * nz, nlat=cfg.nlat, nlon=cfg.nlon,
+                                  local_dist=150.0, inflation_factor=1.05, form=dl.FORM_BATCHED,
+                                  gram=gram)
+            e = dl.Engine(p, nobs_max=cfg.nobs)
+            e.set_grid(g["grid_lat"], g["grid_lon"])
+            o = dl.DeviceObs(g["lat"], g["lon"], g["yb"], g["dep"], g["rinv"])
+            x, ax = torch.empty_like(g["xb"]), torch.empty_like(g["axb"])
+            e.sweep(o, g["xb"], g["axb"], x, ax)     # sizes the selection lists
+            engs.append(e)
+            obs.append(o)
+            xa.append(x)
+            axa.append(ax)
+        st = [e.stats() for e in engs]
+        s = torch.cuda.Stream()
+        with torch.cuda.stream(s):
+            grp = dl.Group(engs, obs, [g["xb"] for _, g in data], [g["axb"] for _, g in data],
+                           xa, axa, s)
+            grp.launch(s)
+        s.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        with torch.cuda.stream(s):
+            e0.record(s)
+            for _ in range(nstep):
+                grp.launch(s)
+            e1.record(s)
+        s.synchronize()
+        ms = e0.elapsed_time(e1) / nstep
+        grp.close()
+        for e in engs:
+            e.close()
+        outs[name] = (xa, axa)
+        res[name] = {"ms_per_graph": ms, "analyses_per_s": ncol / (ms / 1e3),
+                     "nobsl_mean": sum(t["nobsl_sum"] for t in st) / ncol,
+                     "nonconverged": sum(t["nonconverged"] for t in st)}
+        log(f"  domains {name}: {res[name]}")
+    err = {}
+    for var, rows in var_rows(nz).items():
+        num = den = 0.0
+        for i, (cfg, g) in enumerate(data):
+            ri = torch.tensor(rows, device=g["axb"].device)
+            inc_r = outs["batched_fp64"][1][i][:, ri] - g["axb"][:, ri]
+            inc_g = outs["batched_bf16x3"][1][i][:, ri] - g["axb"][:, ri]
+            num += float(torch.linalg.norm(inc_g - inc_r)) ** 2
+            den += float(torch.linalg.norm(inc_r)) ** 2
+        err[var] = (num / max(den, 1e-300)) ** 0.5
+    res["batched_bf16x3"]["rel_err_vs_batched_fp64"] = err
+    del outs, data
+    torch.cuda.empty_cache()
+    return {"domains": [{"ny": a, "nx": b, "k": k, "obs_per_column": d}
+                        for a, b, k, d in DOMAINS], "columns": ncol, "nz": nz,
+            "local_dist_km": 150.0, "forms": res}
+
+
 def copy_probe(torch, nbytes=1 << 30, reps=3):
     """A plain device copy of known size (the FETCH_SIZE / WRITE_SIZE scale)."""
     src = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda").fill_(1.0)
@@ -414,6 +492,8 @@ def main():
     ap.add_argument("--sel-gb", type=float, default=0.0,
                     help="plain form: selection workspace in GB (12 B an entry); sweeps whose "
                          "lists exceed it run in runs of analyses")
+    ap.add_argument("--no-domains", action="store_true",
+                    help="skip the twelve-domain group-graph timing")
     ap.add_argument("--no-struct", action="store_true",
                     help="skip the coupler layout timing (it allocates one 3-D variable block)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -516,6 +596,10 @@ def main():
         log("timing the coupler layout conversions")
         struct = time_struct(dl, torch, data, cfg)
         log(f"  {struct}")
+    domains = None
+    if not args.no_domains:
+        log("timing twelve regional domains in one graph")
+        domains = time_domains(dl, torch)
     log("timing the observation operator")
     obsop = time_obs_operator(dl, torch, data, cfg)
     log(f"  {obsop}")
@@ -545,7 +629,7 @@ def main():
                    "speedup_vs_plain_fp64": plain_ms / r["ms"] if plain_ms else None},
         "roofline": roofline, "cpu_baseline": cpu, "forms": out_forms,
         "obs_operator": obsop, "vertical_localisation": vertical, "horizontal_cubes": cubes,
-        "coupler_layout": struct,
+        "coupler_layout": struct, "regional_domains": domains,
     }
     print(json.dumps(line), flush=True)
 
