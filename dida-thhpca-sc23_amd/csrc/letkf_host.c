@@ -842,7 +842,6 @@ static int sweep_plain_chunked(dida_letkf_ctx *c, dk_select_args *sa, const doub
 static int sweep_rest(dida_letkf_ctx *c, dk_select_args *sa, const double *xb,
                       const double *axb, double *xa, double *axa, hipStream_t s)
 {
-    const int k = c->prm.nanals;
     sa->csr_idx = c->csr_idx;
     sa->csr_w = c->csr_w;
     sa->capacity = c->capacity;

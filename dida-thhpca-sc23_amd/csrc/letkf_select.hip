@@ -65,7 +65,7 @@ __global__ __launch_bounds__(1024) void k_bin_scan(int32_t nbins, const int32_t*
     s[threadIdx.x] = v;
     __syncthreads();
     for (int d = 1; d < 1024; d <<= 1) {
-      int t = threadIdx.x >= d ? s[threadIdx.x - d] : 0;
+      int t = (int)threadIdx.x >= d ? s[threadIdx.x - d] : 0;
       __syncthreads();
       s[threadIdx.x] += t;
       __syncthreads();
