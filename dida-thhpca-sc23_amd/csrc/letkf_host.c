@@ -462,7 +462,15 @@ int dida_letkf_set_levels(dida_letkf_ctx *c, const dida_vert_coord *vc, int32_t 
         }
     }
     if (!rc) off[ncube] = nr;
+    if (!rc && c->gexec) {
+        /* a captured sweep refers to the tables replaced below */
+        (void)hipGraphExecDestroy(c->gexec);
+        c->gexec = NULL;
+        if (c->graph) (void)hipGraphDestroy(c->graph);
+        c->graph = NULL;
+    }
     if (!rc) {
+        /* hipFree waits for the device work that may still read them */
         dfree(c->cube_row_off);
         dfree(c->cube_rows);
         dfree(c->cube_lnp);

@@ -336,7 +336,9 @@ int dida_letkf_obs_operator(int32_t k, int32_t nrow, int32_t nlat, int32_t nlon,
  * (letkf_compute.F90:438-520).  The cube's ln p is that of full level
  * (sk + ek) / 2 (1-based) over the column's mean surface pressure
  * (ps2alnp, obs_make.F90:418-437, whose unset bottom half level ph_lev(nz+1)
- * is here taken from the table).  Call before the first sweep. */
+ * is here taken from the table).  Call before the first sweep; calling it
+ * again replaces the tables and drops a captured graph of this context
+ * (capture again). */
 int dida_letkf_set_levels(dida_letkf_ctx *ctx, const dida_vert_coord *vc, int32_t ps_row,
                           int32_t nvar, const int32_t *var_start, const int32_t *var_lev);
 
