@@ -65,6 +65,7 @@ def time_form(dl, torch, data, cfg, prm, form, gram, steps, warmup, tune=False):
     for _ in range(max(warmup, 1)):
         eng.sweep(obs, data["xb"], data["axb"], xa, axa)
     st = eng.stats()
+    counts = eng.local_counts()
     eng.set_timing(True)
     stage_sum = {}
     torch.cuda.synchronize()
@@ -86,7 +87,22 @@ def time_form(dl, torch, data, cfg, prm, form, gram, steps, warmup, tune=False):
     tot = sum(phases.values())
     phase_frac = {k: v / tot for k, v in phases.items()} if tot > 0 else {}
     return {"engine": eng, "params": p, "xa": xa, "axa": axa, "ms": ms, "stats": st,
-            "stages": stages, "phase_frac": phase_frac, "tuning": tuning}
+            "stages": stages, "phase_frac": phase_frac, "tuning": tuning,
+            "histogram": count_histogram(counts)}
+
+
+def count_histogram(counts):
+    """Local obs counts of the analyses, binned (upper edges inclusive)."""
+    import numpy as np
+    edges = [0, 100, 250, 500, 1000, 1500, 2500, 5000, 9999, 10000]
+    out, lo = {}, -1
+    for e in edges:
+        out[f"{lo + 1}-{e}" if lo + 1 < e else f"{e}"] = int(((counts > lo) & (counts <= e)).sum())
+        lo = e
+    out[">10000"] = int((counts > 10000).sum())
+    return {"bins": out, "mean": float(counts.mean()) if counts.size else 0.0,
+            "median": float(np.median(counts)) if counts.size else 0.0,
+            "max": int(counts.max()) if counts.size else 0}
 
 
 def rel_errors(ref, r, data, cfg):
@@ -623,6 +639,7 @@ def main():
                                + "MAX_OBS 10000",
                    "form": head, "columns": ncol, "nobsl_sum": st["nobsl_sum"],
                    "nobsl_max": st["nobsl_max"], "nobsl_capped": st["nobsl_capped"],
+                   "nobsl_histogram": r["histogram"],
                    "jacobi_max_sweeps_used": st["max_sweeps_used"],
                    "nonconverged": st["nonconverged"],
                    "rel_increment_error_vs_fp64": r["err"] or None,

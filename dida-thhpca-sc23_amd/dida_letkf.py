@@ -341,6 +341,15 @@ class Engine:
         d["status"] = rc
         return d
 
+    def local_counts(self) -> np.ndarray:
+        """Local obs count of every analysis of the last sweep (host copy)."""
+        ptrs = [C.c_void_p() for _ in range(6)]
+        check(lib().dida_letkf_debug_arrays(self.h, *[C.byref(p) for p in ptrs]), "debug")
+        step = self.params.cube_n - self.params.overlap if self.params.overlap else \
+            self.params.cube_n
+        nsel = -(-self.params.nlat // step) * -(-self.params.nlon // step)
+        return d2h(ptrs[0].value, nsel, np.int32)
+
     def debug(self, nobs: int) -> dict:
         """Copies of the last sweep's selection arrays (host numpy)."""
         ptrs = [C.c_void_p() for _ in range(6)]
