@@ -89,9 +89,11 @@ static int check_params(const dida_letkf_params *p)
 {
     if (p->nanals < 2 || p->nanals > 128)
         return set_err(DIDA_ERR_UNSUPPORTED, "nanals = %d: supported range is 2..128", p->nanals);
-    if (p->form == DIDA_FORM_BATCHED && p->nanals > 64)
+    if (p->form == DIDA_FORM_BATCHED && p->nanals > 64 &&
+        (p->gram != DIDA_GRAM_FP64 || p->vert_dist > 0.0))
         return set_err(DIDA_ERR_UNSUPPORTED,
-                       "nanals = %d: the batched form supports 2..64 (plain form: 2..128)",
+                       "nanals = %d: the batched form takes k > 64 with the FP64 Gram and "
+                       "no vertical localisation only (split Grams and vertical: 2..64)",
                        p->nanals);
     if (p->nrow < 1 || p->nlat < 1 || p->nlon < 1)
         return set_err(DIDA_ERR_ARG, "grid %d x %d x %d rows is empty", p->nlat, p->nlon, p->nrow);
@@ -167,6 +169,9 @@ struct dida_letkf_ctx {
     /* batched form: chain queue */
     int32_t chain_len, chains_per_lon, nchain, nblocks;
     int32_t ngroup;      /* 0: one cost-sorted queue; DK_XCD_GROUPS: stripes per XCD */
+    int32_t chained;     /* batched form, k > 64: chained LDS analyses (vscr per CU) */
+    int32_t ncu;
+    double *vscr;
     int32_t nchain_max;  /* chain arrays are sized for chains of one column */
     uint32_t *chain_key, *chain_key_sorted;
     int32_t *chain_id, *chain_order, *queue_head;
@@ -219,7 +224,7 @@ int dida_letkf_destroy(dida_letkf_ctx *c)
                     c->chain_order, c->queue_head, c->chain_tmp, c->ymax, c->phase_clk,
                     c->obs_lnp, c->cube_lnp, c->cube_row_off, c->cube_rows, c->ca_lat,
                     c->ca_lon, c->trans_all, c->vkey_in, c->vkey_out, c->vpos_in,
-                    c->vpos_out, c->vidx, c->vw, c->vsort_tmp};
+                    c->vpos_out, c->vidx, c->vw, c->vsort_tmp, c->vscr};
     for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; ++i) dfree(ptrs[i]);
     free(c);
     return DIDA_OK;
@@ -336,7 +341,8 @@ int dida_letkf_create(dida_letkf_ctx **out, const dida_letkf_params *prm, int32_
     DMALLOC(c->ux, no * sizeof(double));
     DMALLOC(c->uy, no * sizeof(double));
     DMALLOC(c->uz, no * sizeof(double));
-    c->ldy = prm->form == DIDA_FORM_BATCHED ? (k <= 32 ? 32 : 64) : k;
+    c->chained = prm->form == DIDA_FORM_BATCHED && k > 64;
+    c->ldy = prm->form == DIDA_FORM_BATCHED && !c->chained ? (k <= 32 ? 32 : 64) : k;
     DMALLOC(c->yb_b, no * (size_t)c->ldy * sizeof(double));
     DMALLOC(c->dep_b, no * sizeof(double));
     DMALLOC(c->rinv_b, no * sizeof(float));
@@ -359,10 +365,11 @@ int dida_letkf_create(dida_letkf_ctx **out, const dida_letkf_params *prm, int32_
         if (!(prm->vert_dist > 0.0))  /* vertical cubes update in the fused kernel */
             DMALLOC(c->trans_all, (size_t)c->nsel * k * k * sizeof(double));
     }
-    if (prm->form == DIDA_FORM_PLAIN) {
+    if (prm->form == DIDA_FORM_PLAIN || c->chained) {
         c->chunk = (int32_t)(c->nsel < 32768 ? c->nsel : 32768);
         if (prm->cube_n == 1) DMALLOC(c->trans, (size_t)c->chunk * k * k * sizeof(double));
-    } else {
+    }
+    if (prm->form == DIDA_FORM_BATCHED) {
         DMALLOC(c->chain_key, (size_t)c->nchain_max * sizeof(uint32_t));
         DMALLOC(c->chain_key_sorted, (size_t)c->nchain_max * sizeof(uint32_t));
         DMALLOC(c->chain_id, (size_t)c->nchain_max * sizeof(int32_t));
@@ -381,7 +388,10 @@ int dida_letkf_create(dida_letkf_ctx **out, const dida_letkf_params *prm, int32_
             ncu <= 0)
             ncu = 256;
         c->nblocks = ncu * dk_wave_blocks_per_cu(k, prm->gram);
+        c->ncu = ncu;
         if (prm->vert_dist > 0.0) DMALLOC(c->obs_lnp, no * sizeof(double));
+        /* chained analyses: one workgroup per CU, 128 x 128 eigenvectors each */
+        if (c->chained) DMALLOC(c->vscr, (size_t)ncu * 128 * 128 * sizeof(double));
     }
     *out = c;
     return DIDA_OK;
@@ -731,6 +741,21 @@ static int sweep_batched(dida_letkf_ctx *c, const double *xb, const double *axb,
     return DIDA_OK;
 }
 
+/* Transforms of analyses [aid0, aid0 + nanal): one workgroup each, or, for
+ * the batched form with k > 64, chains of warm-started analyses. */
+static int core_launch(dida_letkf_ctx *c, dk_core_args *ca, hipStream_t s)
+{
+    if (!c->chained) return dk_letkf_core_f64(ca, s);
+    dk_chain_args ch;
+    ch.chain_len = c->chain_len;
+    ch.nchain = (ca->nanal + c->chain_len - 1) / c->chain_len;
+    ch.head = c->queue_head;
+    ch.vscr = c->vscr;
+    int e = dk_fill_i32(c->queue_head, 0, 1, s);
+    if (e) return e;
+    return dk_letkf_chain_f64(ca, &ch, c->ncu, s);
+}
+
 /* Plain form: transforms (and, for columns, updates) of analyses [a0, a1)
  * whose selection entries sit at offset[] - base; `last` closes the sweep
  * (horizontal cubes update every column once all transforms exist). */
@@ -776,7 +801,7 @@ static int plain_transform(dida_letkf_ctx *c, int64_t a0, int64_t a1, int64_t ba
             ca.aid0 = (int32_t)c0;
             ca.nanal = n;
             ca.trans = c->trans_all + c0 * k * k;
-            HIP_TRY(dk_letkf_core_f64(&ca, s));
+            HIP_TRY(core_launch(c, &ca, s));
         }
         tmark(c, DIDA_STAGE_TRANSFORM, s);
         if (last) {
@@ -789,7 +814,7 @@ static int plain_transform(dida_letkf_ctx *c, int64_t a0, int64_t a1, int64_t ba
         int32_t n = (int32_t)((a1 - c0) < c->chunk ? (a1 - c0) : c->chunk);
         ca.aid0 = (int32_t)c0;
         ca.nanal = n;
-        HIP_TRY(dk_letkf_core_f64(&ca, s));
+        HIP_TRY(core_launch(c, &ca, s));
         tmark(c, DIDA_STAGE_TRANSFORM, s);
         ua.col0 = (int32_t)c0;
         ua.ncol = n;
@@ -856,7 +881,8 @@ static int sweep_rest(dida_letkf_ctx *c, dk_select_args *sa, const double *xb,
     HIP_TRY(dk_select_fill(sa, s));
     tmark(c, DIDA_STAGE_SELECT, s);
 
-    if (c->prm.form == DIDA_FORM_BATCHED) return sweep_batched(c, xb, axb, xa, axa, s);
+    if (c->prm.form == DIDA_FORM_BATCHED && !c->chained)
+        return sweep_batched(c, xb, axb, xa, axa, s);
 
     return plain_transform(c, 0, c->nsel, 0, 1, xb, axb, xa, axa, s);
 }
@@ -890,7 +916,7 @@ int dida_letkf_sweep(dida_letkf_ctx *c, const dida_letkf_obs *o, const double *x
     if (rc) return rc;
     rc = ensure_capacity(c, s);
     if (rc) return rc;
-    if (c->prm.form == DIDA_FORM_PLAIN && c->user_capacity)
+    if ((c->prm.form == DIDA_FORM_PLAIN || c->chained) && c->user_capacity)
         rc = sweep_plain_chunked(c, &sa, xb, axb, xa, axa, s);
     else
         rc = sweep_rest(c, &sa, xb, axb, xa, axa, s);

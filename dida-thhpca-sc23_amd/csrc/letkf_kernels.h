@@ -114,6 +114,19 @@ typedef struct dk_core_args {
 
 int dk_letkf_core_f64(const dk_core_args *a, void *stream);
 
+/* Chained analyses (64 < k <= 128): chains of chain_len consecutive
+ * analyses of [aid0, aid0 + nanal), drained by nblocks persistent
+ * workgroups from *head (zero before the launch); each workgroup keeps the
+ * last eigenvectors in vscr + blockIdx * 128 * 128 and warm-starts the next
+ * analysis of its chain from them. */
+typedef struct dk_chain_args {
+    int32_t chain_len, nchain;
+    int32_t *head;
+    double *vscr;
+} dk_chain_args;
+int dk_letkf_chain_f64(const dk_core_args *a, const dk_chain_args *ch, int nblocks,
+                       void *stream);
+
 /* Update (xb_2_xa) for a set of columns, with optional RTPS. */
 typedef struct dk_update_args {
     int32_t k, nrow;

@@ -40,13 +40,12 @@ struct CoreSmem {
 // in stages of U steps of 4 (lane group q4 takes obs 4u + q4 of a step): the
 // rows and weights of stage s + 1 are in flight while stage s computes, the
 // indices of stage s + 2 behind them.  Wave 0 also forms b.
-template <int KP, int W>
+template <int KP, int W, int U>
 __device__ __forceinline__ void gram_f64_rows(const dk_core_args& a, int64_t off, int p,
                                               CoreSmem<KP>& sm) {
   constexpr int NT = KP / 16;
   constexpr int IA = W, IB = NT - 1 - W;  // IA < IB
   constexpr int NF = IB + 1;              // tile rows this wave reads
-  constexpr int U = 6;
   constexpr int LDA = CoreSmem<KP>::LDA;
   const int lane = threadIdx.x & 63, r16 = lane & 15, q4 = lane >> 4;
   const int k = a.k;
@@ -148,15 +147,15 @@ __device__ __forceinline__ void gram_f64_rows(const dk_core_args& a, int64_t off
   }
 }
 
-template <int KP>
+template <int KP, int U>
 __device__ __forceinline__ void gram_f64_tiles(const dk_core_args& a, int64_t off, int p,
                                                CoreSmem<KP>& sm) {
   static_assert(KP / 16 == 8, "four waves own two tile rows each");
   switch (threadIdx.x >> 6) {
-    case 0: gram_f64_rows<KP, 0>(a, off, p, sm); break;
-    case 1: gram_f64_rows<KP, 1>(a, off, p, sm); break;
-    case 2: gram_f64_rows<KP, 2>(a, off, p, sm); break;
-    default: gram_f64_rows<KP, 3>(a, off, p, sm); break;
+    case 0: gram_f64_rows<KP, 0, U>(a, off, p, sm); break;
+    case 1: gram_f64_rows<KP, 1, U>(a, off, p, sm); break;
+    case 2: gram_f64_rows<KP, 2, U>(a, off, p, sm); break;
+    default: gram_f64_rows<KP, 3, U>(a, off, p, sm); break;
   }
   __syncthreads();
 }
@@ -262,19 +261,22 @@ __device__ __forceinline__ void gram_f64_obs(const dk_core_args& a, int64_t off,
   }
 }
 
-template <int KP>
+// U: k = 128 Gram stage length (steps of 4 obs), traded against registers
+template <int KP, int U>
 __device__ __forceinline__ void gram_f64(const dk_core_args& a, int64_t off, int p,
                                          CoreSmem<KP>& sm) {
-  if constexpr (KP > 64) gram_f64_tiles<KP>(a, off, p, sm);
+  if constexpr (KP > 64) gram_f64_tiles<KP, U>(a, off, p, sm);
   else gram_f64_obs<KP>(a, off, p, sm);
 }
 
-template <int KP>
-__global__ __launch_bounds__(kThreads) void k_letkf_core_f64(dk_core_args a) {
+// One analysis by the workgroup.  vprev: eigenvectors of the previous
+// analysis of a chain (column-major, KP x KP, in HBM), from which the
+// eigensolve starts (A = M V_prev); vnext: where this analysis leaves its own.
+template <int KP, int U>
+__device__ void analysis_f64(const dk_core_args& a, int aid, CoreSmem<KP>& sm,
+                             const double* vprev, double* vnext) {
   constexpr int NT = KP / 16;
   constexpr int LDA = CoreSmem<KP>::LDA;
-  __shared__ CoreSmem<KP> sm;
-  const int aid = a.list ? a.list[blockIdx.x] : a.aid0 + (int)blockIdx.x;
   const int64_t slot = (int64_t)aid - a.aid0;
   const int k = a.k;
   const int64_t off = a.offset[aid] - a.base;
@@ -284,7 +286,7 @@ __global__ __launch_bounds__(kThreads) void k_letkf_core_f64(dk_core_args a) {
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63, r16 = lane & 15, q4 = lane >> 4;
 
-  gram_f64<KP>(a, off, p, sm);
+  gram_f64<KP, U>(a, off, p, sm);
   const double km1 = (double)(k - 1);
   const double a0 = km1 / a.rho;
   // M = C + (k-1)/rho I on the first k diagonal entries (letkf_core.F90:110-113
@@ -292,6 +294,38 @@ __global__ __launch_bounds__(kThreads) void k_letkf_core_f64(dk_core_args a) {
   // converged column is a non-null lambda_j v_j)
   for (int i = tid; i < k; i += kThreads) sm.A[i + i * LDA] += a0;
   __syncthreads();
+  if constexpr (KP == 128) {
+    if (vprev) {
+      // warm start A = M V_prev on MFMA: wave w owns the tiles of tile
+      // columns w and w + 4 (all 8 tile rows); V_prev is read coherently
+      // with L2 (another analysis of this workgroup wrote it)
+      double4_t d[NT][2];
+#pragma unroll
+      for (int I = 0; I < NT; ++I) d[I][0] = d[I][1] = double4_t{0.0, 0.0, 0.0, 0.0};
+      for (int k0 = 0; k0 < KP; k0 += 4) {
+        const int kk = k0 + q4;
+        const double b0 = __hip_atomic_load(vprev + kk + (wave * 16 + r16) * KP,
+                                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const double b1 = __hip_atomic_load(vprev + kk + ((wave + 4) * 16 + r16) * KP,
+                                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+        for (int I = 0; I < NT; ++I) {
+          const double am = sm.A[(I * 16 + r16) + kk * LDA];
+          d[I][0] = mfma_f64_16x16x4(am, b0, d[I][0]);
+          d[I][1] = mfma_f64_16x16x4(am, b1, d[I][1]);
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int I = 0; I < NT; ++I)
+#pragma unroll
+        for (int jx = 0; jx < 2; ++jx)
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            sm.A[(I * 16 + 4 * r + q4) + ((wave + 4 * jx) * 16 + r16) * LDA] = d[I][jx][r];
+      __syncthreads();
+    }
+  }
 
   bool conv;
   const int sweeps = jacobi_lds<KP, LDA, kThreads>(sm.A, sm.nrm, sm.flag, a.tol, a.max_sweeps,
@@ -305,6 +339,15 @@ __global__ __launch_bounds__(kThreads) void k_letkf_core_f64(dk_core_args a) {
     sm.nrm[j] = s;
   }
   __syncthreads();
+  if (vnext) {
+    // v_j = a_j / lambda_j for the next analysis of the chain
+    for (int t = tid; t < KP * KP; t += kThreads) {
+      const int i = t % KP, j = t / KP;
+      const double al = j < m ? sm.nrm[j] : 0.0;
+      vnext[t] = al > 0.0 && i < k ? sm.A[i + j * LDA] / sqrt(al) : 0.0;
+    }
+    __threadfence();
+  }
   // with v_j = a_j / lambda_j: W = sum_j sqrt((k-1)/lambda_j) v_j v_j^T
   // = A diag(g) A^T, g_j = sqrt((k-1)/lambda_j) / alpha_j, and
   // wbar = sum_j v_j (v_j^T b) / lambda_j = A diag(h) A^T b, h_j = 1/(lambda_j alpha_j)
@@ -360,7 +403,45 @@ __global__ __launch_bounds__(kThreads) void k_letkf_core_f64(dk_core_args a) {
   if (a.info && tid == 0) a.info[aid] = conv ? sweeps : -1;
 }
 
+
+template <int KP>
+__global__ __launch_bounds__(kThreads) void k_letkf_core_f64(dk_core_args a) {
+  __shared__ CoreSmem<KP> sm;
+  const int aid = a.list ? a.list[blockIdx.x] : a.aid0 + (int)blockIdx.x;
+  analysis_f64<KP, 6>(a, aid, sm, nullptr, nullptr);
+}
+
+// Chains of chain_len consecutive analyses drained from a work counter by
+// persistent workgroups; every analysis after the first of a chain starts its
+// eigensolve from the previous one's eigenvectors (a per-workgroup KP x KP
+// buffer in HBM).
+template <int KP>
+__global__ __launch_bounds__(kThreads) void k_letkf_chain_f64(dk_core_args a, dk_chain_args ch) {
+  __shared__ CoreSmem<KP> sm;
+  __shared__ int item_s;
+  double* vs = ch.vscr + (int64_t)blockIdx.x * KP * KP;
+  for (;;) {
+    if (threadIdx.x == 0) item_s = atomicAdd(ch.head, 1);
+    __syncthreads();
+    const int item = item_s;
+    __syncthreads();
+    if (item >= ch.nchain) break;
+    const int c0 = a.aid0 + item * ch.chain_len;
+    const int c1 = min(c0 + ch.chain_len, a.aid0 + a.nanal);
+    for (int aid = c0; aid < c1; ++aid)
+      analysis_f64<KP, 4>(a, aid, sm, aid > c0 ? vs : nullptr, aid + 1 < c1 ? vs : nullptr);
+  }
+}
+
 }  // namespace
+
+extern "C" int dk_letkf_chain_f64(const dk_core_args* a, const dk_chain_args* ch, int nblocks,
+                                  void* stream) {
+  if (a->nanal <= 0 || ch->nchain <= 0) return 0;
+  if (a->k <= 64 || a->k > 128 || ch->chain_len < 1) return (int)hipErrorInvalidValue;
+  k_letkf_chain_f64<128><<<nblocks, kThreads, 0, (hipStream_t)stream>>>(*a, *ch);
+  return (int)hipGetLastError();
+}
 
 extern "C" int dk_letkf_core_f64(const dk_core_args* a, void* stream) {
   if (a->nanal <= 0) return 0;

@@ -1,0 +1,78 @@
+"""The batched form for 64 < k <= 128: chains of analyses on the FP64 Gram,
+each warm-starting its eigensolve from the previous analysis of its chain.
+
+Bars: per-variable increment and Xa' relative error <= 1e-10 against the
+oracle, for columns (MAX_OBS binding), horizontal cubes with overlap, and a
+workspace smaller than the sweep's lists; fewer Jacobi sweeps than the plain
+form's cold starts."""
+import numpy as np
+import pytest
+
+from test_gpu_sweep import _rel_err, _var_rows
+
+pytestmark = pytest.mark.gpu
+
+
+def _check(oracle, data, params, xa, axa, cube_n=1, overlap=0):
+    h = {k: v.cpu().numpy() for k, v in data.items()}
+    o = oracle.Obs(h["lat"], h["lon"], h["yb"], h["dep"], h["rinv"])
+    perm, bs, nb, nl = oracle.bin_obs(o, params.bin_deg, params.obs_polar)
+    prm = oracle.params(params.nanals, params.nrow, params.nlat, params.nlon, params.local_dist,
+                        params.inflation_factor, max_obs=params.max_obs)
+    out = oracle.sweep_cubes(prm, h["grid_lat"], h["grid_lon"], o.permuted(perm), h["xb"],
+                             h["axb"], cube_n, overlap, bins=(bs, nb, nl, params.bin_deg))
+    assert out["nfail"] == 0
+    xa_h, axa_h = xa.cpu().numpy(), axa.cpu().numpy()
+    for var, rows in _var_rows(2).items():
+        inc_g = axa_h[:, rows] - h["axb"][:, rows]
+        inc_o = out["axa"][:, rows] - h["axb"][:, rows]
+        assert _rel_err(inc_g, inc_o) <= 1e-10, var
+        assert _rel_err(xa_h[:, rows], out["xa"][:, rows]) <= 1e-10, var
+
+
+@pytest.mark.parametrize("case", ["k100_chain5", "k128_capped", "k100_cubes", "k128_chunks"])
+def test_chained_batched_k128(cuda, dl, oracle, case):
+    torch = cuda
+    print(f"chained batched {case}", flush=True)
+    k = 128 if case.startswith("k128") else 100
+    cfg = dl.synth_cfg(20, 40, 2, k, 1500, seed=0xD1DA0500 + len(case), cluster_frac=0.5,
+                       nclusters=2, rscale=(0.5, 3.0))
+    data = dl.synth_generate(cfg)
+    p = dict(nanals=k, nrow=7, nlat=cfg.nlat, nlon=cfg.nlon, local_dist=1200.0,
+             inflation_factor=1.05, bin_deg=4.0, form=dl.FORM_BATCHED, gram=dl.GRAM_FP64,
+             chain_len=5)
+    cap, cube = 0, (1, 0)
+    if case == "k128_capped":
+        p["max_obs"] = 200
+    elif case == "k100_cubes":
+        cube = (2, 1)
+        p.update(cube_n=2, overlap=1)
+    params = dl.default_params(**p)
+    obs = dl.DeviceObs(data["lat"], data["lon"], data["yb"], data["dep"], data["rinv"])
+    if case == "k128_chunks":
+        probe = dl.Engine(params, nobs_max=cfg.nobs)
+        probe.set_grid(data["grid_lat"], data["grid_lon"])
+        x, ax = torch.empty_like(data["xb"]), torch.empty_like(data["axb"])
+        probe.sweep(obs, data["xb"], data["axb"], x, ax)
+        cap = probe.stats()["nobsl_sum"] // 3 + 1
+        probe.close()
+    eng = dl.Engine(params, nobs_max=cfg.nobs, capacity=cap)
+    eng.set_grid(data["grid_lat"], data["grid_lon"])
+    xa, axa = torch.empty_like(data["xb"]), torch.empty_like(data["axb"])
+    eng.sweep(obs, data["xb"], data["axb"], xa, axa)
+    st = eng.stats()
+    eng.close()
+    assert st["status"] == 0 and st["nonconverged"] == 0, st
+    if case == "k128_capped":
+        assert st["nobsl_capped"] > 0
+    _check(oracle, data, params, xa, axa, *cube)
+    if case == "k100_chain5":
+        # the plain form's cold starts take more sweeps
+        pp = dl.default_params(**dict(p, form=dl.FORM_PLAIN))
+        e2 = dl.Engine(pp, nobs_max=cfg.nobs)
+        e2.set_grid(data["grid_lat"], data["grid_lon"])
+        x2, a2 = torch.empty_like(xa), torch.empty_like(axa)
+        e2.sweep(obs, data["xb"], data["axb"], x2, a2)
+        s2 = e2.stats()
+        e2.close()
+        assert st["sweeps_sum"] < s2["sweeps_sum"], (st["sweeps_sum"], s2["sweeps_sum"])
