@@ -47,7 +47,9 @@ def make_engine(dl, cfg, prm, form, gram, nobs):
     n = 1 + 3 * cfg.nz
     p = dl.default_params(nanals=cfg.k, nrow=n, nlat=cfg.nlat, nlon=cfg.nlon, form=form,
                           gram=gram, **prm)
-    cap = SEL_CAPACITY if form == dl.FORM_PLAIN else 0
+    # runs of analyses over a bounded workspace: the plain form and the
+    # chained batched form (k > 64)
+    cap = SEL_CAPACITY if form == dl.FORM_PLAIN or cfg.k > 64 else 0
     return dl.Engine(p, nobs_max=nobs, capacity=cap), p
 
 
@@ -506,8 +508,8 @@ def main():
     ap.add_argument("--no-cubes", action="store_true",
                     help="skip the horizontal-cube timing")
     ap.add_argument("--sel-gb", type=float, default=0.0,
-                    help="plain form: selection workspace in GB (12 B an entry); sweeps whose "
-                         "lists exceed it run in runs of analyses")
+                    help="plain form and batched form with k > 64: selection workspace in GB "
+                         "(12 B an entry); sweeps whose lists exceed it run in runs of analyses")
     ap.add_argument("--no-domains", action="store_true",
                     help="skip the twelve-domain group-graph timing")
     ap.add_argument("--no-struct", action="store_true",
