@@ -152,7 +152,7 @@ def kernel_roofline(cfg, st, head, kernel_ms, config, scale, layout=None):
     p_sum = float(st["nobsl_sum"])
     gram = 2.0 * k * k * p_sum
     fp64 = 2.0 * k * p_sum + ncol * (11.0 * k ** 3 + 4.0 * k * k)
-    fused = head != "plain_fp64"
+    fused = head != "plain_fp64" and k <= 64   # k > 64: chained analyses, update apart
     if fused:
         fp64 += ncol * (2.0 * k * k * n + 2.0 * k * n)
     if head.endswith("fp64"):
@@ -172,7 +172,8 @@ def kernel_roofline(cfg, st, head, kernel_ms, config, scale, layout=None):
     return {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
             "frac": achieved / peak if achieved else None, "traffic": traffic,
             "kernel": "k_letkf_wave (fused Gram, eigensolve, transform, update)" if fused
-            else "k_letkf_core_f64 (Gram, eigensolve, transform)",
+            else ("k_letkf_chain_f64 (chained Gram, warm-started eigensolve, transform)"
+                  if head != "plain_fp64" else "k_letkf_core_f64 (Gram, eigensolve, transform)"),
             "kernel_ms": kernel_ms, "flops_per_launch": flops, "half_flops": half,
             "fp64_flops": fp64,
             "algorithmic_bytes": {"obs_gathered": gathered, "update": upd},

@@ -44,7 +44,10 @@ extern "C" {
 
 /* Execution form of a sweep. */
 #define DIDA_FORM_PLAIN 0       /* separate transform / update kernels, grid order */
-#define DIDA_FORM_BATCHED 1     /* obs-count-sorted work queue, fused persistent kernel */
+#define DIDA_FORM_BATCHED 1     /* chains of adjacent analyses from a work queue, each
+                                   eigensolve warm-started from the previous one: one
+                                   fused wavefront per analysis (k <= 64), chained
+                                   workgroups on the FP64 Gram (64 < k <= 128) */
 
 typedef struct dida_letkf_params {
     int32_t nanals;             /* k, ensemble size (letkf.F90:24 ens) */
