@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "letkf_kernels.h"
+
 namespace dida {
 
 constexpr int kWave = 64;
@@ -76,6 +78,35 @@ typedef double double4_t __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ double4_t mfma_f64_16x16x4(double a, double b, double4_t c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
+
+// Half-precision MFMA 16x16x32 (FP32 accumulate) of a Gram precision:
+// lane l holds A[l%16][8(l/16) + e] and B[8(l/16) + e][l%16] in element e
+// of its 8-vector operands; the result holds D[4(l/16) + r][l%16] in
+// element r (measured on gfx950).
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+template <int GRAM>
+struct H16;
+template <>
+struct H16<DK_GRAM_BF16X3> {
+  typedef bf16x8_t vec;
+  typedef __bf16 elt;
+  static __device__ __forceinline__ f32x4_t mfma(vec a, vec b, f32x4_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct H16<DK_GRAM_FP16X3> {
+  typedef f16x8_t vec;
+  typedef _Float16 elt;
+  static __device__ __forceinline__ f32x4_t mfma(vec a, vec b, f32x4_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct H16<DK_GRAM_BF16> : H16<DK_GRAM_BF16X3> {};
 
 // Per-phase clock accounting (wall_clock64), one thread per workgroup.
 struct PhaseClock {

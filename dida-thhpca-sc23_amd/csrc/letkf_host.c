@@ -90,11 +90,11 @@ static int check_params(const dida_letkf_params *p)
     if (p->nanals < 2 || p->nanals > 128)
         return set_err(DIDA_ERR_UNSUPPORTED, "nanals = %d: supported range is 2..128", p->nanals);
     if (p->form == DIDA_FORM_BATCHED && p->nanals > 64 &&
-        (p->gram != DIDA_GRAM_FP64 || p->vert_dist > 0.0))
+        (p->gram == DIDA_GRAM_FP16X3 || p->vert_dist > 0.0))
         return set_err(DIDA_ERR_UNSUPPORTED,
-                       "nanals = %d: the batched form takes k > 64 with the FP64 Gram and "
-                       "no vertical localisation only (split Grams and vertical: 2..64)",
-                       p->nanals);
+                       "nanals = %d: the batched form takes k > 64 with the FP64, BF16X3 or "
+                       "BF16 Gram and no vertical localisation only (FP16X3 and vertical: "
+                       "2..64)", p->nanals);
     if (p->nrow < 1 || p->nlat < 1 || p->nlon < 1)
         return set_err(DIDA_ERR_ARG, "grid %d x %d x %d rows is empty", p->nlat, p->nlon, p->nrow);
     if (p->max_obs < 0) return set_err(DIDA_ERR_ARG, "max_obs < 0");
@@ -777,6 +777,7 @@ static int plain_transform(dida_letkf_ctx *c, int64_t a0, int64_t a1, int64_t ba
     ca.yb = c->yb_b;
     ca.rinv = c->rinv_b;
     ca.dep = c->dep_b;
+    ca.gram = c->chained ? c->prm.gram : DIDA_GRAM_FP64;
     ca.rho = c->prm.inflation_factor;
     ca.tol = jacobi_tol_of(&c->prm);
     ca.max_sweeps = c->prm.jacobi_max_sweeps;

@@ -261,18 +261,180 @@ __device__ __forceinline__ void gram_f64_obs(const dk_core_args& a, int64_t off,
   }
 }
 
+// KP = 128 Gram in half precision (batched form): Yhat = Yb sqrt(R^-1 rloc)
+// in FP32, split as hi = h(Yhat), lo = h(Yhat - hi) (h: BF16 rounding), and
+// C = hi hi^T + hi lo^T + lo hi^T (hi hi^T alone for GRAM = BF16) on
+// 16x16x32 MFMA with FP32 accumulation; b = Yb^T (R^-1 rloc d) in FP64.
+// Blocks of 64 obs are converted once by the whole workgroup into LDS, in
+// the MFMA operand layout, while the previous block is multiplied: thread
+// (row = tid % 128, step = tid / 128) converts row `row` of the 32 obs of
+// k-step `step`.  The staging area aliases A, which is free until C is
+// written.  Wave W owns the lower tiles of tile rows W and NT-1-W.
+template <int KP>
+struct H16Stage {
+  bf16x8_t hi[2][8][KP];  // [buffer][obs octet][row]: obs 8 o .. 8 o + 7
+  bf16x8_t lo[2][8][KP];
+  double rr[2][64];       // sqrt(R^-1 rloc), 0 past p
+  double w[2][64];        // R^-1 rloc dep
+  int32_t gi[2][64];      // obs index
+};
+
+template <int KP, int GRAM, int W>
+__device__ __forceinline__ void gram_h16_rows(const dk_core_args& a, int64_t off, int p,
+                                              CoreSmem<KP>& sm) {
+  typedef typename H16<GRAM>::vec hvec;
+  typedef typename H16<GRAM>::elt helt;
+  constexpr bool kSplit = GRAM != DK_GRAM_BF16;
+  constexpr int NT = KP / 16;
+  constexpr int IA = W, IB = NT - 1 - W;
+  constexpr int NF = IB + 1;
+  constexpr int LDA = CoreSmem<KP>::LDA;
+  static_assert(sizeof(H16Stage<KP>) <= sizeof(sm.A), "staging fits in A");
+  H16Stage<KP>& st = *reinterpret_cast<H16Stage<KP>*>(sm.A);
+  const int tid = threadIdx.x, lane = tid & 63, g = lane >> 4, c = lane & 15;
+  const int row = tid % KP, step = tid / KP;
+  const int k = a.k;
+  const int rowc = row < k ? row : k - 1;
+  const int nblk = (p + 63) / 64;
+  f32x4_t accA[IA + 1], accB[IB + 1];
+#pragma unroll
+  for (int J = 0; J <= IA; ++J) accA[J] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int J = 0; J <= IB; ++J) accB[J] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+  double bacc = 0.0;
+  double raw[32];
+
+  // obs metadata of block blk into buffer buf (threads 0-63); obs past p
+  // take the index of obs 0 of the analysis and zero weights
+  auto meta = [&](int blk, int buf) {
+    if (tid < 64) {
+      const int o = blk * 64 + tid;
+      const bool ok = o < p;
+      const int oc = ok ? o : 0;
+      const int gi = a.idx ? a.idx[off + oc] : (int)(off + oc);
+      const double r2 = ok ? (double)a.rinv[gi] * a.rloc[off + oc] : 0.0;
+      st.gi[buf][tid] = gi;
+      st.rr[buf][tid] = sqrt(r2);
+      st.w[buf][tid] = r2 * a.dep[gi];
+    }
+  };
+  auto load = [&](int buf) {
+#pragma unroll
+    for (int e = 0; e < 32; ++e) raw[e] = a.yb[(int64_t)st.gi[buf][step * 32 + e] * k + rowc];
+  };
+  auto split = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      hvec hv, lv;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int o = step * 32 + q * 8 + e;
+        const double x = raw[q * 8 + e];
+        bacc = fma(x, st.w[buf][o], bacc);
+        const float xs = row < k ? (float)x * (float)st.rr[buf][o] : 0.0f;
+        const helt h = (helt)xs;
+        hv[e] = h;
+        lv[e] = (helt)(xs - (float)h);
+      }
+      st.hi[buf][step * 4 + q][row] = hv;
+      if (kSplit) st.lo[buf][step * 4 + q][row] = lv;
+    }
+  };
+  auto mma = [&](int buf) {
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      hvec fh[NF], fl[NF];
+#pragma unroll
+      for (int T = 0; T < NF; ++T) {
+        fh[T] = st.hi[buf][s * 4 + g][T * 16 + c];
+        if (kSplit) fl[T] = st.lo[buf][s * 4 + g][T * 16 + c];
+      }
+#pragma unroll
+      for (int J = 0; J <= IA; ++J) {
+        accA[J] = H16<GRAM>::mfma(fh[IA], fh[J], accA[J]);
+        if (kSplit) {
+          accA[J] = H16<GRAM>::mfma(fh[IA], fl[J], accA[J]);
+          accA[J] = H16<GRAM>::mfma(fl[IA], fh[J], accA[J]);
+        }
+      }
+#pragma unroll
+      for (int J = 0; J <= IB; ++J) {
+        accB[J] = H16<GRAM>::mfma(fh[IB], fh[J], accB[J]);
+        if (kSplit) {
+          accB[J] = H16<GRAM>::mfma(fh[IB], fl[J], accB[J]);
+          accB[J] = H16<GRAM>::mfma(fl[IB], fh[J], accB[J]);
+        }
+      }
+    }
+  };
+  if (nblk > 0) {
+    meta(0, 0);
+    __syncthreads();
+    load(0);
+    split(0);
+    if (nblk > 1) meta(1, 1);
+    __syncthreads();
+    for (int blk = 0; blk < nblk; ++blk) {
+      const int buf = blk & 1;
+      if (blk + 1 < nblk) load(buf ^ 1);
+      __builtin_amdgcn_sched_barrier(0);
+      mma(buf);
+      if (blk + 1 < nblk) split(buf ^ 1);
+      __syncthreads();
+      if (blk + 2 < nblk) meta(blk + 2, buf);
+      __syncthreads();
+    }
+  }
+  // D[4g + r][c] of tile (I, J) = C(I*16 + 4g + r, J*16 + c)
+#pragma unroll
+  for (int J = 0; J <= IA; ++J)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = IA * 16 + 4 * g + r, j = J * 16 + c;
+      sm.A[i + j * LDA] = (double)accA[J][r];
+      if (IA != J) sm.A[j + i * LDA] = (double)accA[J][r];
+    }
+#pragma unroll
+  for (int J = 0; J <= IB; ++J)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = IB * 16 + 4 * g + r, j = J * 16 + c;
+      sm.A[i + j * LDA] = (double)accB[J][r];
+      if (IB != J) sm.A[j + i * LDA] = (double)accB[J][r];
+    }
+  // b: the two k-step halves of each row meet in LDS
+  if (step == 1) sm.u[row] = bacc;
+  __syncthreads();
+  if (step == 0) sm.b[row] = row < k ? bacc + sm.u[row] : 0.0;
+}
+
+template <int KP, int GRAM>
+__device__ __forceinline__ void gram_h16_tiles(const dk_core_args& a, int64_t off, int p,
+                                               CoreSmem<KP>& sm) {
+  static_assert(KP == 128, "two k-steps of 128 rows per 64-obs block");
+  switch (threadIdx.x >> 6) {
+    case 0: gram_h16_rows<KP, GRAM, 0>(a, off, p, sm); break;
+    case 1: gram_h16_rows<KP, GRAM, 1>(a, off, p, sm); break;
+    case 2: gram_h16_rows<KP, GRAM, 2>(a, off, p, sm); break;
+    default: gram_h16_rows<KP, GRAM, 3>(a, off, p, sm); break;
+  }
+  __syncthreads();
+}
+
 // U: k = 128 Gram stage length (steps of 4 obs), traded against registers
-template <int KP, int U>
-__device__ __forceinline__ void gram_f64(const dk_core_args& a, int64_t off, int p,
+template <int KP, int U, int GRAM>
+__device__ __forceinline__ void gram_any(const dk_core_args& a, int64_t off, int p,
                                          CoreSmem<KP>& sm) {
-  if constexpr (KP > 64) gram_f64_tiles<KP, U>(a, off, p, sm);
+  if constexpr (GRAM != DK_GRAM_FP64) gram_h16_tiles<KP, GRAM>(a, off, p, sm);
+  else if constexpr (KP > 64) gram_f64_tiles<KP, U>(a, off, p, sm);
   else gram_f64_obs<KP>(a, off, p, sm);
 }
 
 // One analysis by the workgroup.  vprev: eigenvectors of the previous
 // analysis of a chain (column-major, KP x KP, in HBM), from which the
 // eigensolve starts (A = M V_prev); vnext: where this analysis leaves its own.
-template <int KP, int U>
+// GRAM: precision of C (b, the eigensolve and the transform are FP64).
+template <int KP, int U, int GRAM>
 __device__ void analysis_f64(const dk_core_args& a, int aid, CoreSmem<KP>& sm,
                              const double* vprev, double* vnext) {
   constexpr int NT = KP / 16;
@@ -286,7 +448,9 @@ __device__ void analysis_f64(const dk_core_args& a, int aid, CoreSmem<KP>& sm,
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63, r16 = lane & 15, q4 = lane >> 4;
 
-  gram_f64<KP, U>(a, off, p, sm);
+  // the previous analysis of a chain may still be reading A (its W tiles)
+  __syncthreads();
+  gram_any<KP, U, GRAM>(a, off, p, sm);
   const double km1 = (double)(k - 1);
   const double a0 = km1 / a.rho;
   // M = C + (k-1)/rho I on the first k diagonal entries (letkf_core.F90:110-113
@@ -408,14 +572,14 @@ template <int KP>
 __global__ __launch_bounds__(kThreads) void k_letkf_core_f64(dk_core_args a) {
   __shared__ CoreSmem<KP> sm;
   const int aid = a.list ? a.list[blockIdx.x] : a.aid0 + (int)blockIdx.x;
-  analysis_f64<KP, 6>(a, aid, sm, nullptr, nullptr);
+  analysis_f64<KP, 6, DK_GRAM_FP64>(a, aid, sm, nullptr, nullptr);
 }
 
 // Chains of chain_len consecutive analyses drained from a work counter by
 // persistent workgroups; every analysis after the first of a chain starts its
 // eigensolve from the previous one's eigenvectors (a per-workgroup KP x KP
 // buffer in HBM).
-template <int KP>
+template <int KP, int GRAM>
 __global__ __launch_bounds__(kThreads) void k_letkf_chain_f64(dk_core_args a, dk_chain_args ch) {
   __shared__ CoreSmem<KP> sm;
   __shared__ int item_s;
@@ -429,7 +593,8 @@ __global__ __launch_bounds__(kThreads) void k_letkf_chain_f64(dk_core_args a, dk
     const int c0 = a.aid0 + item * ch.chain_len;
     const int c1 = min(c0 + ch.chain_len, a.aid0 + a.nanal);
     for (int aid = c0; aid < c1; ++aid)
-      analysis_f64<KP, 4>(a, aid, sm, aid > c0 ? vs : nullptr, aid + 1 < c1 ? vs : nullptr);
+      analysis_f64<KP, 4, GRAM>(a, aid, sm, aid > c0 ? vs : nullptr,
+                                 aid + 1 < c1 ? vs : nullptr);
   }
 }
 
@@ -439,7 +604,13 @@ extern "C" int dk_letkf_chain_f64(const dk_core_args* a, const dk_chain_args* ch
                                   void* stream) {
   if (a->nanal <= 0 || ch->nchain <= 0) return 0;
   if (a->k <= 64 || a->k > 128 || ch->chain_len < 1) return (int)hipErrorInvalidValue;
-  k_letkf_chain_f64<128><<<nblocks, kThreads, 0, (hipStream_t)stream>>>(*a, *ch);
+  hipStream_t s = (hipStream_t)stream;
+  switch (a->gram) {
+    case DK_GRAM_FP64: k_letkf_chain_f64<128, DK_GRAM_FP64><<<nblocks, kThreads, 0, s>>>(*a, *ch); break;
+    case DK_GRAM_BF16X3: k_letkf_chain_f64<128, DK_GRAM_BF16X3><<<nblocks, kThreads, 0, s>>>(*a, *ch); break;
+    case DK_GRAM_BF16: k_letkf_chain_f64<128, DK_GRAM_BF16><<<nblocks, kThreads, 0, s>>>(*a, *ch); break;
+    default: return (int)hipErrorInvalidValue;
+  }
   return (int)hipGetLastError();
 }
 
@@ -447,7 +618,7 @@ extern "C" int dk_letkf_core_f64(const dk_core_args* a, void* stream) {
   if (a->nanal <= 0) return 0;
   hipStream_t s = (hipStream_t)stream;
   const int k = a->k;
-  if (k < 2 || k > 128) return (int)hipErrorInvalidValue;
+  if (k < 2 || k > 128 || a->gram != DK_GRAM_FP64) return (int)hipErrorInvalidValue;
   if (k <= 16) k_letkf_core_f64<16><<<a->nanal, kThreads, 0, s>>>(*a);
   else if (k <= 32) k_letkf_core_f64<32><<<a->nanal, kThreads, 0, s>>>(*a);
   else if (k <= 64) k_letkf_core_f64<64><<<a->nanal, kThreads, 0, s>>>(*a);

@@ -32,31 +32,6 @@ using namespace dida;
 
 namespace {
 
-typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
-typedef _Float16 f16x8_t __attribute__((ext_vector_type(8)));
-typedef float f32x4_t __attribute__((ext_vector_type(4)));
-
-template <int GRAM>
-struct H16;
-template <>
-struct H16<DK_GRAM_BF16X3> {
-  typedef bf16x8_t vec;
-  typedef __bf16 elt;
-  static __device__ __forceinline__ f32x4_t mfma(vec a, vec b, f32x4_t c) {
-    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
-  }
-};
-template <>
-struct H16<DK_GRAM_FP16X3> {
-  typedef f16x8_t vec;
-  typedef _Float16 elt;
-  static __device__ __forceinline__ f32x4_t mfma(vec a, vec b, f32x4_t c) {
-    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
-  }
-};
-template <>
-struct H16<DK_GRAM_BF16> : H16<DK_GRAM_BF16X3> {};
-
 template <int KP>
 struct WSmem {
   static constexpr int LDV = KP + 1;

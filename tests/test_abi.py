@@ -53,7 +53,7 @@ def test_params_default_and_errors(dl):
     rc = dl.lib().dida_letkf_create(ctypes.byref(h), ctypes.byref(bad), 10, 0)
     assert rc == -5 and b"2..128" in dl.lib().dida_letkf_last_error()
     bad = dl.default_params(nanals=96, nrow=1, nlat=2, nlon=2, form=dl.FORM_BATCHED,
-                            gram=dl.GRAM_BF16X3)
+                            gram=dl.GRAM_FP16X3)
     rc = dl.lib().dida_letkf_create(ctypes.byref(h), ctypes.byref(bad), 10, 0)
     assert rc == -5 and b"batched" in dl.lib().dida_letkf_last_error()
 
