@@ -79,6 +79,61 @@ __device__ __forceinline__ double4_t mfma_f64_16x16x4(double a, double b, double
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+// v of the lane whose byte address (4 * lane) is addr
+__device__ __forceinline__ double bperm_d(int addr, double v) {
+  const int lo = __builtin_amdgcn_ds_bpermute(addr, __double2loint(v));
+  const int hi = __builtin_amdgcn_ds_bpermute(addr, __double2hiint(v));
+  return __hiloint2double(hi, lo);
+}
+
+// 1 / sqrt(x) for positive normal x: the hardware estimate (relative error
+// about 2^-23) refined by two Newton steps, y (3 - x y^2) / 2, to a few ulp.
+__device__ __forceinline__ double rsq_nr(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double hx = 0.5 * x;
+  y = y * fma(-hx * y, y, 1.5);
+  y = y * fma(-hx * y, y, 1.5);
+  return y;
+}
+// One Newton step each (relative error about 2^-45): enough for a rotation
+// angle, whose error only delays convergence.
+__device__ __forceinline__ double rsq_nr1(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  return y * fma(-0.5 * x * y, y, 1.5);
+}
+__device__ __forceinline__ double rcp_nr1(double x) {
+  const double y = __builtin_amdgcn_rcp(x);
+  return fma(y, fma(-x, y, 1.0), y);
+}
+
+// Scaled one-sided Jacobi rotation of a column pair (p, q) stored as
+// sp x_p, sq x_q, from their squared norms al, be and gs = x_p . x_q:
+// t = tan(theta), gm = gamma = gs sp sq, cs = cos(theta), ic = 1/cos(theta).
+// A pair is rotated while gamma^2 > tol2 al be (t = 0, cs = ic = 1 below);
+// *big records gamma^2 > tolp al be (the next sweep would still rotate).
+// t = sign(d e) |e| / (|d| + sqrt(d^2 + e^2)), d = be - al, e = 2 gamma.
+__device__ __forceinline__ void rotation_angle(double gs, double al, double be, double sp,
+                                               double sq, double tol2, double tolp,
+                                               bool& rotated, bool& big, double& t, double& gm,
+                                               double& cs, double& ic) {
+  gm = gs * (sp * sq);
+  const double gg = gm * gm, ab = al * be;
+  const bool need = gg > tol2 * ab;
+  big |= gg > tolp * ab;
+  rotated |= need;
+  const double d = be - al, e = 2.0 * gm;
+  const double r2 = fma(d, d, e * e);
+  const double den = fabs(d) + r2 * rsq_nr1(r2);
+  t = fabs(e) * rcp_nr1(den);
+  if ((d < 0.0) != (e < 0.0) && d != 0.0) t = -t;
+  t = need ? t : 0.0;
+  const double u = fma(t, t, 1.0);
+  cs = rsq_nr(u);
+  ic = u * cs;
+  cs = need ? cs : 1.0;
+  ic = need ? ic : 1.0;
+}
+
 // Half-precision MFMA 16x16x32 (FP32 accumulate) of a Gram precision:
 // lane l holds A[l%16][8(l/16) + e] and B[8(l/16) + e][l%16] in element e
 // of its 8-vector operands; the result holds D[4(l/16) + r][l%16] in

@@ -751,8 +751,14 @@ static int core_launch(dida_letkf_ctx *c, dk_core_args *ca, hipStream_t s)
     ch.nchain = (ca->nanal + c->chain_len - 1) / c->chain_len;
     ch.head = c->queue_head;
     ch.vscr = c->vscr;
+    ch.phase_clk = NULL;
     int e = dk_fill_i32(c->queue_head, 0, 1, s);
     if (e) return e;
+    if ((c->timing & 2) && c->phase_clk) {
+        e = (int)hipMemsetAsync(c->phase_clk, 0, DK_NPHASE * sizeof(unsigned long long), s);
+        if (e) return e;
+        ch.phase_clk = c->phase_clk;
+    }
     return dk_letkf_chain_f64(ca, &ch, c->ncu, s);
 }
 

@@ -123,6 +123,8 @@ typedef struct dk_chain_args {
     int32_t chain_len, nchain;
     int32_t *head;
     double *vscr;
+    unsigned long long *phase_clk; /* optional [DK_NPHASE] device clock sums
+                                      (gram, warm, jacobi, transform) */
 } dk_chain_args;
 int dk_letkf_chain_f64(const dk_core_args *a, const dk_chain_args *ch, int nblocks,
                        void *stream);

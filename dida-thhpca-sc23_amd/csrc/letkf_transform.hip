@@ -436,7 +436,8 @@ __device__ __forceinline__ void gram_any(const dk_core_args& a, int64_t off, int
 // GRAM: precision of C (b, the eigensolve and the transform are FP64).
 template <int KP, int U, int GRAM>
 __device__ void analysis_f64(const dk_core_args& a, int aid, CoreSmem<KP>& sm,
-                             const double* vprev, double* vnext) {
+                             const double* vprev, double* vnext,
+                             unsigned long long* phase_clk) {
   constexpr int NT = KP / 16;
   constexpr int LDA = CoreSmem<KP>::LDA;
   const int64_t slot = (int64_t)aid - a.aid0;
@@ -450,6 +451,7 @@ __device__ void analysis_f64(const dk_core_args& a, int aid, CoreSmem<KP>& sm,
 
   // the previous analysis of a chain may still be reading A (its W tiles)
   __syncthreads();
+  PhaseClock clk(phase_clk);
   gram_any<KP, U, GRAM>(a, off, p, sm);
   const double km1 = (double)(k - 1);
   const double a0 = km1 / a.rho;
@@ -458,6 +460,7 @@ __device__ void analysis_f64(const dk_core_args& a, int aid, CoreSmem<KP>& sm,
   // converged column is a non-null lambda_j v_j)
   for (int i = tid; i < k; i += kThreads) sm.A[i + i * LDA] += a0;
   __syncthreads();
+  clk.mark(DK_PHASE_GRAM);
   if constexpr (KP == 128) {
     if (vprev) {
       // warm start A = M V_prev on MFMA: wave w owns the tiles of tile
@@ -490,10 +493,17 @@ __device__ void analysis_f64(const dk_core_args& a, int aid, CoreSmem<KP>& sm,
       __syncthreads();
     }
   }
+  clk.mark(DK_PHASE_WARM);
 
   bool conv;
-  const int sweeps = jacobi_lds<KP, LDA, kThreads>(sm.A, sm.nrm, sm.flag, a.tol, a.max_sweeps,
-                                                   a.predict != 0, &conv);
+  int sweeps;
+  if constexpr (KP == 128)
+    sweeps = jacobi_regs128<LDA, kThreads>(sm.A, sm.flag, a.tol, a.max_sweeps, a.predict != 0,
+                                           &conv);
+  else
+    sweeps = jacobi_lds<KP, LDA, kThreads>(sm.A, sm.nrm, sm.flag, a.tol, a.max_sweeps,
+                                           a.predict != 0, &conv);
+  clk.mark(DK_PHASE_JACOBI);
   const int m = (k + 1) & ~1;
 
   // exact column norms: alpha_j = lambda_j^2, lambda_j eigenvalues of M
@@ -565,6 +575,7 @@ __device__ void analysis_f64(const dk_core_args& a, int aid, CoreSmem<KP>& sm,
   if (a.wbar)
     for (int i = tid; i < k; i += kThreads) a.wbar[slot * k + i] = sm.wb[i];
   if (a.info && tid == 0) a.info[aid] = conv ? sweeps : -1;
+  clk.mark(DK_PHASE_TRANSFORM);
 }
 
 
@@ -572,7 +583,7 @@ template <int KP>
 __global__ __launch_bounds__(kThreads) void k_letkf_core_f64(dk_core_args a) {
   __shared__ CoreSmem<KP> sm;
   const int aid = a.list ? a.list[blockIdx.x] : a.aid0 + (int)blockIdx.x;
-  analysis_f64<KP, 6, DK_GRAM_FP64>(a, aid, sm, nullptr, nullptr);
+  analysis_f64<KP, 6, DK_GRAM_FP64>(a, aid, sm, nullptr, nullptr, nullptr);
 }
 
 // Chains of chain_len consecutive analyses drained from a work counter by
@@ -594,7 +605,7 @@ __global__ __launch_bounds__(kThreads) void k_letkf_chain_f64(dk_core_args a, dk
     const int c1 = min(c0 + ch.chain_len, a.aid0 + a.nanal);
     for (int aid = c0; aid < c1; ++aid)
       analysis_f64<KP, 4, GRAM>(a, aid, sm, aid > c0 ? vs : nullptr,
-                                 aid + 1 < c1 ? vs : nullptr);
+                                 aid + 1 < c1 ? vs : nullptr, ch.phase_clk);
   }
 }
 

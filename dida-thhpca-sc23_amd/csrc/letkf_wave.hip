@@ -103,33 +103,6 @@ __device__ __forceinline__ double swz_xor1(double v) {
   const int hi = __builtin_amdgcn_ds_swizzle(__double2hiint(v), 0x1F | (1 << 10));
   return __hiloint2double(hi, lo);
 }
-// v of the lane whose byte address (4 * lane) is addr
-__device__ __forceinline__ double bperm_d(int addr, double v) {
-  const int lo = __builtin_amdgcn_ds_bpermute(addr, __double2loint(v));
-  const int hi = __builtin_amdgcn_ds_bpermute(addr, __double2hiint(v));
-  return __hiloint2double(hi, lo);
-}
-
-// 1 / sqrt(x) for positive normal x: the hardware estimate (relative error
-// about 2^-23) refined by two Newton steps, y (3 - x y^2) / 2, to a few ulp.
-__device__ __forceinline__ double rsq_nr(double x) {
-  double y = __builtin_amdgcn_rsq(x);
-  const double hx = 0.5 * x;
-  y = y * fma(-hx * y, y, 1.5);
-  y = y * fma(-hx * y, y, 1.5);
-  return y;
-}
-// One Newton step each (relative error about 2^-45): enough for a rotation
-// angle, whose error only delays convergence.
-__device__ __forceinline__ double rsq_nr1(double x) {
-  const double y = __builtin_amdgcn_rsq(x);
-  return y * fma(-0.5 * x * y, y, 1.5);
-}
-__device__ __forceinline__ double rcp_nr1(double x) {
-  const double y = __builtin_amdgcn_rcp(x);
-  return fma(y, fma(-x, y, 1.0), y);
-}
-
 // Metadata of one 64-obs block of an analysis, lane l holding obs base + l:
 // binned index, sqrt(R^-1 rloc) and R^-1 rloc dep.  Entries past p take the
 // index of obs 0 and zero weights, so every load is unconditional and in
@@ -482,31 +455,6 @@ struct Wave {
     isc[Q] *= ic;
   }
 
-  // Angle of one pair from its state (al, be squared norms, sp, sq scales)
-  // and gs = x_p . x_q: t = tan(theta), gm = gamma, cs = cos(theta), ic =
-  // 1/cos(theta); t = 0, cs = ic = 1 below the threshold.
-  static __device__ __forceinline__ void angle(double gs, double al, double be, double sp,
-                                               double sq, double tol2, double tolp,
-                                               bool& rotated, bool& big, double& t, double& gm,
-                                               double& cs, double& ic) {
-    gm = gs * (sp * sq);
-    const double gg = gm * gm, ab = al * be;
-    const bool need = gg > tol2 * ab;
-    big |= gg > tolp * ab;
-    rotated |= need;
-    const double d = be - al, e = 2.0 * gm;
-    const double r2 = fma(d, d, e * e);
-    const double den = fabs(d) + r2 * rsq_nr1(r2);
-    t = fabs(e) * rcp_nr1(den);
-    if ((d < 0.0) != (e < 0.0) && d != 0.0) t = -t;
-    t = need ? t : 0.0;
-    const double u = fma(t, t, 1.0);
-    cs = rsq_nr(u);
-    ic = u * cs;
-    cs = need ? cs : 1.0;
-    ic = need ? ic : 1.0;
-  }
-
   // Apply the rotation (t, gm, cs, ic) to columns P, Q and their state.
   template <int P, int Q>
   __device__ __forceinline__ void apply(double t, double gm, double cs, double ic) {
@@ -535,9 +483,9 @@ struct Wave {
       const double gs = sum_g_pair(dot<T0, B0>(), dot<T1, B1>());
       const bool second = threadIdx.x >= 32;
       double t, gm, cs, ic;
-      angle(gs, second ? nrm[T1] : nrm[T0], second ? nrm[B1] : nrm[B0],
-            second ? sc[T1] : sc[T0], second ? sc[B1] : sc[B0], tol2, tolp, rotated, big, t,
-            gm, cs, ic);
+      rotation_angle(gs, second ? nrm[T1] : nrm[T0], second ? nrm[B1] : nrm[B0],
+                     second ? sc[T1] : sc[T0], second ? sc[B1] : sc[B0], tol2, tolp, rotated,
+                     big, t, gm, cs, ic);
       double t0, t1, gm0, gm1, cs0, cs1, ic0, ic1;
       split_g(t, t0, t1);
       split_g(gm, gm0, gm1);
