@@ -515,6 +515,9 @@ def main():
                     help="skip the twelve-domain group-graph timing")
     ap.add_argument("--no-struct", action="store_true",
                     help="skip the coupler layout timing (it allocates one 3-D variable block)")
+    ap.add_argument("--jacobi-tol", type=float, default=0.0,
+                    help="Jacobi tolerance of the half-precision batched forms (0: the "
+                         "library default)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device copy (counter scale for profiles)")
     args = ap.parse_args()
@@ -552,6 +555,10 @@ def main():
         forms = {k: v for k, v in forms.items() if k in args.forms.split(",") or k == "plain_fp64"}
     if args.no_plain:
         forms.pop("plain_fp64")
+    if args.jacobi_tol > 0:
+        for name, (form, gram, extra) in forms.items():
+            if gram != dl.GRAM_FP64:
+                extra["jacobi_tol"] = args.jacobi_tol
     res = {}
     ref = None
     for name, (form, gram, extra) in forms.items():
