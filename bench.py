@@ -474,7 +474,7 @@ def cpu_baseline(dl, O, data, cfg, prm, target_s):
         stride = max(1, ncol // m)
         cols = np.arange(0, ncol, stride, dtype=np.int64)[:m].astype(np.int32)
         idx = data["xb"].new_tensor(cols, dtype=__import__("torch").long)
-        xb_c = data["xb"][idx].cpu().numpy()
+        xb_c = data["xb"][idx].double().cpu().numpy()
         axb_c = data["axb"][idx].cpu().numpy()
         t0 = time.perf_counter()
         O.sweep_compact(op, h["grid_lat"], h["grid_lon"], ob, (bs, nb, nl, bin_deg), cols, xb_c,
@@ -518,6 +518,9 @@ def main():
     ap.add_argument("--jacobi-tol", type=float, default=0.0,
                     help="Jacobi tolerance of the half-precision batched forms (0: the "
                          "library default)")
+    ap.add_argument("--state-f32", action="store_true",
+                    help="Xb' and Xa' stored in FP32 (dida_letkf_sweep_f32; plain form, or "
+                         "batched with k > 64); skips the FP64-state extras")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device copy (counter scale for profiles)")
     args = ap.parse_args()
@@ -546,6 +549,14 @@ def main():
     t0 = time.perf_counter()
     data = dl.synth_generate(cfg)
     log(f"  generated in {time.perf_counter() - t0:.1f} s")
+    sbytes = 8
+    if args.state_f32:
+        # the perturbations move to FP32 (the FP64 copy is released); the
+        # extras below read FP64 state and are skipped
+        data["xb"] = data["xb"].float()
+        torch.cuda.empty_cache()
+        sbytes = 4
+        args.no_vertical = args.no_cubes = args.no_struct = args.no_domains = True
 
     forms = {"plain_fp64": (dl.FORM_PLAIN, dl.GRAM_FP64, {}),
              "batched_fp64": (dl.FORM_BATCHED, dl.GRAM_FP64, {}),
@@ -564,7 +575,8 @@ def main():
     for name, (form, gram, extra) in forms.items():
         log(f"timing {name}")
         r = time_form(dl, torch, data, cfg, dict(prm, **extra), form, gram, args.steps,
-                      args.warmup, tune=form == dl.FORM_BATCHED and not args.no_tune)
+                      args.warmup,
+                      tune=form == dl.FORM_BATCHED and not args.no_tune and not args.state_f32)
         if r["tuning"]:
             log(f"  autotuned layout {r['tuning']}")
         log(f"  {r['ms']:.1f} ms/step, stages {r['stages']}")
@@ -603,7 +615,7 @@ def main():
                            "kernel_phase_fraction": rr["phase_frac"],
                            "autotuned_layout": rr["tuning"],
                            "stage_ms": rr["stages"],
-                           "update_GBps": update_bytes(cfg.k, n, ncol) / (rr["stages"]["update"] / 1e3) / 1e9
+                           "update_GBps": update_bytes(cfg.k, n, ncol, sbytes) / (rr["stages"]["update"] / 1e3) / 1e9
                            if rr["stages"]["update"] > 0 else None,
                            "selection_GBps": selection_bytes(cfg.k, cfg.nobs, rr["stats"]["nobsl_sum"])
                            / ((rr["stages"]["bin"] + rr["stages"]["select"]) / 1e3) / 1e9}
@@ -626,9 +638,11 @@ def main():
     if not args.no_domains:
         log("timing twelve regional domains in one graph")
         domains = time_domains(dl, torch)
-    log("timing the observation operator")
-    obsop = time_obs_operator(dl, torch, data, cfg)
-    log(f"  {obsop}")
+    obsop = None
+    if not args.state_f32:
+        log("timing the observation operator")
+        obsop = time_obs_operator(dl, torch, data, cfg)
+        log(f"  {obsop}")
     cpu = None
     if not args.no_cpu:
         log("timing the CPU oracle on a column sample")
@@ -646,8 +660,10 @@ def main():
                                f"L={prm['local_dist']} km, "
                                + (f"RTPS {prm['rtps_alpha']:g}, " if prm.get("rtps") else
                                   f"inflation {prm['inflation_factor']:g}, ")
-                               + "MAX_OBS 10000",
-                   "form": head, "columns": ncol, "nobsl_sum": st["nobsl_sum"],
+                               + "MAX_OBS 10000"
+                               + (", Xb'/Xa' stored in FP32" if args.state_f32 else ""),
+                   "form": head, "state": "f32" if args.state_f32 else "f64",
+                   "columns": ncol, "nobsl_sum": st["nobsl_sum"],
                    "nobsl_max": st["nobsl_max"], "nobsl_capped": st["nobsl_capped"],
                    "nobsl_histogram": r["histogram"],
                    "jacobi_max_sweeps_used": st["max_sweeps_used"],

@@ -191,6 +191,9 @@ struct dida_letkf_ctx {
     void *vsort_tmp;
     size_t vsort_tmp_bytes;
     unsigned long long *phase_clk; /* [DK_NPHASE], allocated when timing is on */
+    /* FP32 perturbations of the dida_letkf_sweep_f32 call in progress */
+    const float *xb32;
+    float *xa32;
     /* per-stage HIP event timing (eager sweeps only) */
     int timing;
     int nmark;
@@ -628,7 +631,9 @@ static int cube_update(dida_letkf_ctx *c, const double *xb, const double *axb, d
     ua.step = c->step;
     ua.nalat = c->nalat;
     ua.nalon = c->nalon;
-    if (ua.k <= 64) HIP_TRY(dk_update_cols(&ua, s));
+    ua.xb32 = c->xb32;
+    ua.xa32 = c->xa32;
+    if (ua.k <= 64 && !ua.xa32) HIP_TRY(dk_update_cols(&ua, s));
     else HIP_TRY(dk_update_f64(&ua, s));
     return DIDA_OK;
 }
@@ -800,6 +805,8 @@ static int plain_transform(dida_letkf_ctx *c, int64_t a0, int64_t a1, int64_t ba
     ua.axb = axb;
     ua.xa = xa;
     ua.axa = axa;
+    ua.xb32 = c->xb32;
+    ua.xa32 = c->xa32;
     ua.rtps = c->prm.rtps;
     ua.rtps_alpha = c->prm.rtps_alpha;
     if (c->prm.cube_n > 1) {
@@ -825,7 +832,7 @@ static int plain_transform(dida_letkf_ctx *c, int64_t a0, int64_t a1, int64_t ba
         tmark(c, DIDA_STAGE_TRANSFORM, s);
         ua.col0 = (int32_t)c0;
         ua.ncol = n;
-        if (k <= 64) HIP_TRY(dk_update_cols(&ua, s));
+        if (k <= 64 && !ua.xa32) HIP_TRY(dk_update_cols(&ua, s));
         else HIP_TRY(dk_update_f64(&ua, s));
         tmark(c, DIDA_STAGE_UPDATE, s);
     }
@@ -894,8 +901,8 @@ static int sweep_rest(dida_letkf_ctx *c, dk_select_args *sa, const double *xb,
     return plain_transform(c, 0, c->nsel, 0, 1, xb, axb, xa, axa, s);
 }
 
-static int check_sweep_args(dida_letkf_ctx *c, const dida_letkf_obs *o, const double *xb,
-                            const double *axb, double *xa, double *axa)
+static int check_sweep_args(dida_letkf_ctx *c, const dida_letkf_obs *o, const void *xb,
+                            const double *axb, void *xa, double *axa)
 {
     if (!c || !o || !xb || !axb || !xa || !axa) return set_err(DIDA_ERR_ARG, "null argument");
     if (!c->grid_lat) return set_err(DIDA_ERR_ARG, "grid not set (dida_letkf_set_grid)");
@@ -912,11 +919,12 @@ static int check_sweep_args(dida_letkf_ctx *c, const dida_letkf_obs *o, const do
     return DIDA_OK;
 }
 
-int dida_letkf_sweep(dida_letkf_ctx *c, const dida_letkf_obs *o, const double *xb,
-                     const double *axb, double *xa, double *axa, void *stream)
+/* Selection, transforms and update of one sweep (arguments checked); with
+ * c->xa32 set the perturbations are the FP32 ones and xb, xa are unused. */
+static int sweep_impl(dida_letkf_ctx *c, const dida_letkf_obs *o, const double *xb,
+                      const double *axb, double *xa, double *axa, void *stream)
 {
-    int rc = check_sweep_args(c, o, xb, axb, xa, axa);
-    if (rc) return rc;
+    int rc;
     hipStream_t s = (hipStream_t)stream;
     dk_select_args sa;
     rc = sweep_select_count(c, o, s, &sa);
@@ -929,6 +937,31 @@ int dida_letkf_sweep(dida_letkf_ctx *c, const dida_letkf_obs *o, const double *x
         rc = sweep_rest(c, &sa, xb, axb, xa, axa, s);
     c->last_stream = s;
     c->last_nobs = o->nobs;
+    return rc;
+}
+
+int dida_letkf_sweep(dida_letkf_ctx *c, const dida_letkf_obs *o, const double *xb,
+                     const double *axb, double *xa, double *axa, void *stream)
+{
+    int rc = check_sweep_args(c, o, xb, axb, xa, axa);
+    if (rc) return rc;
+    return sweep_impl(c, o, xb, axb, xa, axa, stream);
+}
+
+int dida_letkf_sweep_f32(dida_letkf_ctx *c, const dida_letkf_obs *o, const float *xb,
+                         const double *axb, float *xa, double *axa, void *stream)
+{
+    int rc = check_sweep_args(c, o, xb, axb, xa, axa);
+    if (rc) return rc;
+    if (c->prm.form == DIDA_FORM_BATCHED && !c->chained)
+        return set_err(DIDA_ERR_UNSUPPORTED,
+                       "FP32 state: plain form, or batched form with 64 < k <= 128 (the fused "
+                       "k <= 64 kernel streams FP64 state)");
+    c->xb32 = xb;
+    c->xa32 = xa;
+    rc = sweep_impl(c, o, NULL, axb, NULL, axa, stream);
+    c->xb32 = NULL;
+    c->xa32 = NULL;
     return rc;
 }
 

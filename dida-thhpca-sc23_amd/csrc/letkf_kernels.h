@@ -147,6 +147,10 @@ typedef struct dk_update_args {
      * aj + nalat ai over cube starts aj * step (latitude), ai * step
      * (longitude), each cube cube_n columns wide (clipped at the last) */
     int32_t nlat, nlon, cube_n, step, nalat, nalon;
+    /* FP32 perturbations (dk_update_f64 only): when xa32 is set, Xb' is read
+     * from xb32 and Xa' written to xa32 instead of xb / xa */
+    const float *xb32;
+    float *xa32;
 } dk_update_args;
 
 int dk_update_f64(const dk_update_args *a, void *stream);

@@ -19,7 +19,7 @@ constexpr int kThreads = 256;
 // Sum of v over the KP lanes of this thread's row slot.
 template <int KP>
 __device__ __forceinline__ double slot_sum(double v, double* sred) {
-  if (KP <= 64) {
+  if constexpr (KP <= 64) {
 #pragma unroll
     for (int s = 1; s < KP; s <<= 1) v += shfl_xor_d(v, s);
     return v;
@@ -38,7 +38,9 @@ __device__ __forceinline__ double slot_sum(double v, double* sred) {
   }
 }
 
-template <int KP>
+// S: storage type of Xb' and Xa' (double, or float read from xb32 and
+// written to xa32); the arithmetic is FP64 either way.
+template <int KP, typename S>
 __global__ __launch_bounds__(kThreads) void k_update_f64(dk_update_args u) {
   constexpr int RP = kThreads / KP;
   constexpr int LDT = KP + 1;
@@ -75,6 +77,15 @@ __global__ __launch_bounds__(kThreads) void k_update_f64(dk_update_args u) {
       sT[i * LDT + e] = T[t];
     }
   }
+  const S* xbs;
+  S* xas;
+  if constexpr (sizeof(S) == sizeof(float)) {
+    xbs = u.xb32;
+    xas = u.xa32;
+  } else {
+    xbs = u.xb;
+    xas = u.xa;
+  }
   const int e = threadIdx.x % KP, slot = threadIdx.x / KP;
   const double rk = 1.0 / (double)k;
   for (int row0 = 0; row0 < nrow; row0 += RP) {
@@ -82,7 +93,7 @@ __global__ __launch_bounds__(kThreads) void k_update_f64(dk_update_args u) {
     const bool rowok = row < nrow;
     const bool ok = rowok && e < k;
     const int64_t cr = col * nrow + row;
-    const double xv = ok ? u.xb[cr * k + e] : 0.0;
+    const double xv = ok ? (double)xbs[cr * k + e] : 0.0;
     __syncthreads();
     sX[slot * KP + e] = xv;
     __syncthreads();
@@ -98,7 +109,7 @@ __global__ __launch_bounds__(kThreads) void k_update_f64(dk_update_args u) {
       const double sa2 = slot_sum<KP>(xa * xa, sred) * rk;
       if (sa2 > 0.0) xa *= u.rtps_alpha * sqrt(sb2) / sqrt(sa2) + 1.0 - u.rtps_alpha;
     }
-    if (ok) u.xa[cr * k + e] = xa;
+    if (ok) xas[cr * k + e] = (S)xa;
     if (rowok && e == 0) u.axa[cr] = mean;
   }
 }
@@ -236,10 +247,17 @@ extern "C" int dk_update_f64(const dk_update_args* u, void* stream) {
   hipStream_t s = (hipStream_t)stream;
   const int k = u->k;
   if (k < 2 || k > 128) return (int)hipErrorInvalidValue;
-  if (k <= 16) k_update_f64<16><<<u->ncol, kThreads, 0, s>>>(*u);
-  else if (k <= 32) k_update_f64<32><<<u->ncol, kThreads, 0, s>>>(*u);
-  else if (k <= 64) k_update_f64<64><<<u->ncol, kThreads, 0, s>>>(*u);
-  else k_update_f64<128><<<u->ncol, kThreads, 0, s>>>(*u);
+  if (u->xa32) {
+    if (k <= 16) k_update_f64<16, float><<<u->ncol, kThreads, 0, s>>>(*u);
+    else if (k <= 32) k_update_f64<32, float><<<u->ncol, kThreads, 0, s>>>(*u);
+    else if (k <= 64) k_update_f64<64, float><<<u->ncol, kThreads, 0, s>>>(*u);
+    else k_update_f64<128, float><<<u->ncol, kThreads, 0, s>>>(*u);
+  } else {
+    if (k <= 16) k_update_f64<16, double><<<u->ncol, kThreads, 0, s>>>(*u);
+    else if (k <= 32) k_update_f64<32, double><<<u->ncol, kThreads, 0, s>>>(*u);
+    else if (k <= 64) k_update_f64<64, double><<<u->ncol, kThreads, 0, s>>>(*u);
+    else k_update_f64<128, double><<<u->ncol, kThreads, 0, s>>>(*u);
+  }
   return (int)hipGetLastError();
 }
 

@@ -26,7 +26,8 @@ TIMING_STAGES, TIMING_PHASES = 1, 2
 # Every entry point declared in include/dida_letkf.h.
 EXPORTED = ["dida_letkf_params_default", "letkf_core_", "letkf_core_batch_",
             "dida_letkf_core_batch_dev", "dida_letkf_create", "dida_letkf_destroy",
-            "dida_letkf_set_grid", "dida_letkf_sweep", "dida_letkf_graph_capture",
+            "dida_letkf_set_grid", "dida_letkf_sweep", "dida_letkf_sweep_f32",
+            "dida_letkf_graph_capture",
             "dida_letkf_graph_launch", "dida_letkf_stats_get", "dida_letkf_debug_arrays",
             "dida_letkf_post", "dida_letkf_last_error", "dida_letkf_set_timing",
             "dida_letkf_stage_ms", "dida_letkf_phase_cycles", "dida_letkf_group_capture",
@@ -114,6 +115,7 @@ def lib():
         L.dida_letkf_destroy.argtypes = [vp]
         L.dida_letkf_set_grid.argtypes = [vp, vp, vp]
         L.dida_letkf_sweep.argtypes = [vp, C.POINTER(ObsC), vp, vp, vp, vp, vp]
+        L.dida_letkf_sweep_f32.argtypes = [vp, C.POINTER(ObsC), vp, vp, vp, vp, vp]
         L.dida_letkf_graph_capture.argtypes = [vp, C.POINTER(ObsC), vp, vp, vp, vp, vp]
         L.dida_letkf_graph_launch.argtypes = [vp, vp]
         L.dida_letkf_stats_get.argtypes = [vp, C.POINTER(StatsC)]
@@ -299,9 +301,15 @@ class Engine:
                                           vs.ctypes.data, vl.ctypes.data), "set_levels")
 
     def sweep(self, obs: DeviceObs, xb, axb, xa, axa, stream=None) -> int:
+        """dida_letkf_sweep, or dida_letkf_sweep_f32 when Xb' and Xa' are
+        float32 tensors (means stay float64)."""
         oc = obs.c()
-        return check(lib().dida_letkf_sweep(self.h, C.byref(oc), _ptr(xb), _ptr(axb), _ptr(xa),
-                                            _ptr(axa), _stream_ptr(stream)), "sweep")
+        f32 = getattr(xb, "dtype", None) is not None and str(xb.dtype).endswith("float32")
+        if f32 != str(xa.dtype).endswith("float32"):
+            raise TypeError("xb and xa must both be float32 or both float64")
+        fn = lib().dida_letkf_sweep_f32 if f32 else lib().dida_letkf_sweep
+        return check(fn(self.h, C.byref(oc), _ptr(xb), _ptr(axb), _ptr(xa), _ptr(axa),
+                        _stream_ptr(stream)), "sweep")
 
     def graph_capture(self, obs: DeviceObs, xb, axb, xa, axa, stream) -> int:
         oc = obs.c()

@@ -181,10 +181,11 @@ int dida_letkf_core_batch_dev(int32_t batch, int32_t max_obs, int32_t nanals,
  * nobs_max bounds the observation count of later sweeps; sel_capacity is the
  * number of (column, obs) selection entries to reserve (0: sized by the first
  * sweep, which then synchronises once).  With a reserved workspace the plain
- * form runs each sweep in runs of consecutive analyses whose lists fit it (it
- * reads the list offsets back once per sweep), so workloads whose selection
- * lists exceed the device memory left next to the state still run; the
- * batched form and captured graphs report DIDA_ERR_CAPACITY instead. */
+ * form and the batched form with k > 64 run each sweep in runs of
+ * consecutive analyses whose lists fit it (the list offsets are read back
+ * once per sweep), so workloads whose selection lists exceed the device
+ * memory left next to the state still run; the fused batched form (k <= 64)
+ * and captured graphs report DIDA_ERR_CAPACITY instead. */
 int dida_letkf_create(dida_letkf_ctx **ctx, const dida_letkf_params *prm, int32_t nobs_max,
                       int64_t sel_capacity);
 int dida_letkf_destroy(dida_letkf_ctx *ctx);
@@ -199,6 +200,16 @@ int dida_letkf_set_grid(dida_letkf_ctx *ctx, const float *grid_lat, const float 
  * `stream`; no host synchronisation once the selection workspace is sized. */
 int dida_letkf_sweep(dida_letkf_ctx *ctx, const dida_letkf_obs *obs, const double *xb,
                      const double *axb, double *xa, double *axa, void *stream);
+
+/* The same sweep with the ensemble perturbations stored in FP32 (the
+ * reference's REAL(4) state option, yb_size in da_config.inc:7-10): xb = Xb'
+ * and xa = Xa' as float in the layout above, means axb, axa FP64; the update
+ * reads Xb' once, computes in FP64 and rounds Xa' once.  Plain form (any k)
+ * or batched form with 64 < k <= 128, horizontal cubes included; the fused
+ * batched form (k <= 64) returns DIDA_ERR_UNSUPPORTED.  Runs eagerly (no
+ * graph capture form). */
+int dida_letkf_sweep_f32(dida_letkf_ctx *ctx, const dida_letkf_obs *obs, const float *xb,
+                         const double *axb, float *xa, double *axa, void *stream);
 
 /* Capture one sweep into a HIP graph owned by the context (inputs and
  * outputs fixed at capture), then replay it with dida_letkf_graph_launch. */
