@@ -150,6 +150,7 @@ struct dida_letkf_ctx {
     void *sort_tmp;
     size_t sort_tmp_bytes;
     double *cx, *sx, *yr, *ux, *uy, *uz, *yb_b, *dep_b;
+    float *yb32_b;       /* chained half-precision Grams: binned Yb in FP32 (no yb_b) */
     int32_t ldy;         /* row stride of yb_b: k (plain), the padded k (batched) */
     float *rinv_b;
     /* selection */
@@ -221,7 +222,7 @@ int dida_letkf_destroy(dida_letkf_ctx *c)
     if (c->graph) (void)hipGraphDestroy(c->graph);
     void *ptrs[] = {c->key, c->key_sorted, c->iota, c->perm, c->bin_count, c->bin_start,
                     c->sort_tmp, c->cx, c->sx, c->yr, c->ux, c->uy, c->uz, c->yb_b,
-                    c->dep_b, c->rinv_b,
+                    c->yb32_b, c->dep_b, c->rinv_b,
                     c->nobsl, c->offset, c->scan_tmp, c->csr_idx, c->csr_w, c->flags,
                     c->trans, c->info, c->chain_key, c->chain_key_sorted, c->chain_id,
                     c->chain_order, c->queue_head, c->chain_tmp, c->ymax, c->phase_clk,
@@ -346,7 +347,10 @@ int dida_letkf_create(dida_letkf_ctx **out, const dida_letkf_params *prm, int32_
     DMALLOC(c->uz, no * sizeof(double));
     c->chained = prm->form == DIDA_FORM_BATCHED && k > 64;
     c->ldy = prm->form == DIDA_FORM_BATCHED && !c->chained ? (k <= 32 ? 32 : 64) : k;
-    DMALLOC(c->yb_b, no * (size_t)c->ldy * sizeof(double));
+    if (c->chained && prm->gram != DIDA_GRAM_FP64)
+        DMALLOC(c->yb32_b, no * (size_t)c->ldy * sizeof(float));
+    else
+        DMALLOC(c->yb_b, no * (size_t)c->ldy * sizeof(double));
     DMALLOC(c->dep_b, no * sizeof(double));
     DMALLOC(c->rinv_b, no * sizeof(float));
 
@@ -551,8 +555,8 @@ static int sweep_select_count(dida_letkf_ctx *c, const dida_letkf_obs *o, hipStr
     }
     HIP_TRY(dk_bin_scan(c->geom.nbins, c->bin_count, c->bin_start, s));
     HIP_TRY(dk_obs_gather(o->nobs, k, c->ldy, c->perm, o->lat, o->lon, o->yb, o->dep, o->rinv,
-                          c->cx, c->sx, c->yr, c->ux, c->uy, c->uz, c->yb_b, c->dep_b,
-                          c->rinv_b, s));
+                          c->cx, c->sx, c->yr, c->ux, c->uy, c->uz, c->yb_b, c->yb32_b,
+                          c->dep_b, c->rinv_b, s));
     if (c->ncube > 0) HIP_TRY(dk_obs_lnp(o->nobs, c->perm, o->ph, c->obs_lnp, s));
     tmark(c, DIDA_STAGE_BIN, s);
     memset(sa, 0, sizeof *sa);
@@ -786,6 +790,7 @@ static int plain_transform(dida_letkf_ctx *c, int64_t a0, int64_t a1, int64_t ba
     ca.capacity = c->capacity;
     ca.max_p = c->prm.max_obs;
     ca.yb = c->yb_b;
+    ca.yb32 = c->yb32_b;
     ca.rinv = c->rinv_b;
     ca.dep = c->dep_b;
     ca.gram = c->chained ? c->prm.gram : DIDA_GRAM_FP64;

@@ -71,12 +71,13 @@ int dk_sort_pairs_u32(void *tmp, size_t *tmp_bytes, const uint32_t *kin, uint32_
                       const int32_t *vin, int32_t *vout, int32_t n, int end_bit, void *stream);
 /* Exclusive scan of nbins + 1 counts into bin_start[nbins + 2] (one block). */
 int dk_bin_scan(int32_t nbins, const int32_t *count, int32_t *bin_start, void *stream);
-/* Gather obs into binned order and precompute the distance factors. */
-/* yb rows are copied into rows of ldy >= k values, zero past k. */
+/* Gather obs into binned order and precompute the distance factors.  yb
+ * rows are copied into rows of ldy >= k values, zero past k: in FP64 into
+ * yb_b and/or rounded to FP32 into yb32_b (either may be NULL). */
 int dk_obs_gather(int32_t nobs, int32_t k, int32_t ldy, const int32_t *perm, const double *lat,
                   const double *lon, const double *yb, const double *dep, const float *rinv,
                   double *cx, double *sx, double *yr, double *ux, double *uy, double *uz,
-                  double *yb_b, double *dep_b, float *rinv_b, void *stream);
+                  double *yb_b, float *yb32_b, double *dep_b, float *rinv_b, void *stream);
 /* Selection passes, one wave per column. */
 int dk_select_count(const dk_select_args *a, void *stream);
 int dk_select_fill(const dk_select_args *a, void *stream);
@@ -110,6 +111,8 @@ typedef struct dk_core_args {
     double *trans;          /* [(id - aid0) * k * k] trans(i, j) at i + j k */
     double *wbar;           /* [(id - aid0) * k] (optional) */
     int32_t *info;          /* [id]: sweeps, or -1 when not converged (optional) */
+    const float *yb32;      /* half-precision Grams (chained): yb rounded to FP32,
+                               read instead of yb */
 } dk_core_args;
 
 int dk_letkf_core_f64(const dk_core_args *a, void *stream);

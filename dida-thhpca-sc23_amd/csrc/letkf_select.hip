@@ -101,15 +101,18 @@ __global__ void k_obs_gather(int32_t nobs, const int32_t* __restrict__ perm,
 }
 
 // Binned rows of ldy >= k values, zero past k.
+// Yb rows in bin order (row stride ldy, zero past k), in FP64 or rounded to
+// FP32 (the half-precision Gram of the chained form splits FP32 values).
+template <typename T>
 __global__ void k_obs_gather_yb(int32_t nobs, int32_t k, int32_t ldy,
                                 const int32_t* __restrict__ perm, const double* __restrict__ yb,
-                                double* __restrict__ yb_b) {
+                                T* __restrict__ yb_b) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int64_t total = (int64_t)nobs * ldy;
   if (t >= total) return;
   int64_t i = t / ldy;
   int e = (int)(t - i * ldy);
-  yb_b[t] = e < k ? yb[(int64_t)perm[i] * k + e] : 0.0;
+  yb_b[t] = e < k ? (T)yb[(int64_t)perm[i] * k + e] : (T)0;
 }
 
 __global__ void k_obs_ymax(int32_t nobs, int32_t k, int32_t ldy, const double* __restrict__ yb,
@@ -264,14 +267,16 @@ extern "C" int dk_obs_gather(int32_t nobs, int32_t k, int32_t ldy, const int32_t
                              const double* lat,
                              const double* lon, const double* yb, const double* dep,
                              const float* rinv, double* cx, double* sx, double* yr,
-                             double* ux, double* uy, double* uz, double* yb_b, double* dep_b,
-                             float* rinv_b, void* stream) {
+                             double* ux, double* uy, double* uz, double* yb_b, float* yb32_b,
+                             double* dep_b, float* rinv_b, void* stream) {
   if (nobs <= 0) return 0;
   hipStream_t s = (hipStream_t)stream;
   k_obs_gather<<<(nobs + 255) / 256, 256, 0, s>>>(nobs, perm, lat, lon, dep, rinv, cx, sx, yr,
                                                   ux, uy, uz, dep_b, rinv_b);
-  int64_t total = (int64_t)nobs * ldy;
-  k_obs_gather_yb<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(nobs, k, ldy, perm, yb, yb_b);
+  const int64_t total = (int64_t)nobs * ldy;
+  const unsigned blocks = (unsigned)((total + 255) / 256);
+  if (yb_b) k_obs_gather_yb<double><<<blocks, 256, 0, s>>>(nobs, k, ldy, perm, yb, yb_b);
+  if (yb32_b) k_obs_gather_yb<float><<<blocks, 256, 0, s>>>(nobs, k, ldy, perm, yb, yb32_b);
   return (int)hipGetLastError();
 }
 

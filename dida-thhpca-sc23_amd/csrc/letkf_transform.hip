@@ -262,9 +262,10 @@ __device__ __forceinline__ void gram_f64_obs(const dk_core_args& a, int64_t off,
 }
 
 // KP = 128 Gram in half precision (batched form): Yhat = Yb sqrt(R^-1 rloc)
-// in FP32, split as hi = h(Yhat), lo = h(Yhat - hi) (h: BF16 rounding), and
-// C = hi hi^T + hi lo^T + lo hi^T (hi hi^T alone for GRAM = BF16) on
-// 16x16x32 MFMA with FP32 accumulation; b = Yb^T (R^-1 rloc d) in FP64.
+// in FP32 (Yb read from its FP32 copy, half the bytes of the gather), split
+// as hi = h(Yhat), lo = h(Yhat - hi) (h: BF16 rounding), and C = hi hi^T +
+// hi lo^T + lo hi^T (hi hi^T alone for GRAM = BF16) on 16x16x32 MFMA with
+// FP32 accumulation; b = Yb^T (R^-1 rloc d) accumulated in FP64.
 // Blocks of 64 obs are converted once by the whole workgroup into LDS, in
 // the MFMA operand layout, while the previous block is multiplied: thread
 // (row = tid % 128, step = tid / 128) converts row `row` of the 32 obs of
@@ -302,25 +303,47 @@ __device__ __forceinline__ void gram_h16_rows(const dk_core_args& a, int64_t off
 #pragma unroll
   for (int J = 0; J <= IB; ++J) accB[J] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
   double bacc = 0.0;
-  double raw[32];
+  float raw[32];
 
   // obs metadata of block blk into buffer buf (threads 0-63); obs past p
   // take the index of obs 0 of the analysis and zero weights
+  // obs metadata of block blk, threads 0-63, in two steps so that their
+  // latencies hide behind the block being multiplied and split: the index
+  // (meta_idx), then R^-1, rloc and d (meta_vals), then the LDS stores
+  struct Meta {
+    int gi, oc;
+    bool ok;
+    float ri;
+    double rl, dp;
+  };
+  auto meta_idx = [&](int blk, Meta& m) {
+    const int o = blk * 64 + tid;
+    m.ok = o < p;
+    m.oc = m.ok ? o : 0;
+    m.gi = a.idx ? a.idx[off + m.oc] : (int)(off + m.oc);
+  };
+  auto meta_vals = [&](Meta& m) {
+    m.ri = a.rinv[m.gi];
+    m.rl = a.rloc[off + m.oc];
+    m.dp = a.dep[m.gi];
+  };
+  auto meta_store = [&](const Meta& m, int buf) {
+    const double r2 = m.ok ? (double)m.ri * m.rl : 0.0;
+    st.gi[buf][tid] = m.gi;
+    st.rr[buf][tid] = sqrt(r2);
+    st.w[buf][tid] = r2 * m.dp;
+  };
   auto meta = [&](int blk, int buf) {
     if (tid < 64) {
-      const int o = blk * 64 + tid;
-      const bool ok = o < p;
-      const int oc = ok ? o : 0;
-      const int gi = a.idx ? a.idx[off + oc] : (int)(off + oc);
-      const double r2 = ok ? (double)a.rinv[gi] * a.rloc[off + oc] : 0.0;
-      st.gi[buf][tid] = gi;
-      st.rr[buf][tid] = sqrt(r2);
-      st.w[buf][tid] = r2 * a.dep[gi];
+      Meta m;
+      meta_idx(blk, m);
+      meta_vals(m);
+      meta_store(m, buf);
     }
   };
   auto load = [&](int buf) {
 #pragma unroll
-    for (int e = 0; e < 32; ++e) raw[e] = a.yb[(int64_t)st.gi[buf][step * 32 + e] * k + rowc];
+    for (int e = 0; e < 32; ++e) raw[e] = a.yb32[(int64_t)st.gi[buf][step * 32 + e] * k + rowc];
   };
   auto split = [&](int buf) {
 #pragma unroll
@@ -329,9 +352,9 @@ __device__ __forceinline__ void gram_h16_rows(const dk_core_args& a, int64_t off
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
         const int o = step * 32 + q * 8 + e;
-        const double x = raw[q * 8 + e];
-        bacc = fma(x, st.w[buf][o], bacc);
-        const float xs = row < k ? (float)x * (float)st.rr[buf][o] : 0.0f;
+        const float x = raw[q * 8 + e];
+        bacc = fma((double)x, st.w[buf][o], bacc);
+        const float xs = row < k ? x * (float)st.rr[buf][o] : 0.0f;
         const helt h = (helt)xs;
         hv[e] = h;
         lv[e] = (helt)(xs - (float)h);
@@ -376,12 +399,16 @@ __device__ __forceinline__ void gram_h16_rows(const dk_core_args& a, int64_t off
     __syncthreads();
     for (int blk = 0; blk < nblk; ++blk) {
       const int buf = blk & 1;
+      const bool more = tid < 64 && blk + 2 < nblk;
+      Meta m;
+      if (more) meta_idx(blk + 2, m);
       if (blk + 1 < nblk) load(buf ^ 1);
       __builtin_amdgcn_sched_barrier(0);
       mma(buf);
+      if (more) meta_vals(m);
       if (blk + 1 < nblk) split(buf ^ 1);
       __syncthreads();
-      if (blk + 2 < nblk) meta(blk + 2, buf);
+      if (more) meta_store(m, buf);
       __syncthreads();
     }
   }
@@ -615,6 +642,7 @@ extern "C" int dk_letkf_chain_f64(const dk_core_args* a, const dk_chain_args* ch
                                   void* stream) {
   if (a->nanal <= 0 || ch->nchain <= 0) return 0;
   if (a->k <= 64 || a->k > 128 || ch->chain_len < 1) return (int)hipErrorInvalidValue;
+  if (a->gram != DK_GRAM_FP64 && !a->yb32) return (int)hipErrorInvalidValue;
   hipStream_t s = (hipStream_t)stream;
   switch (a->gram) {
     case DK_GRAM_FP64: k_letkf_chain_f64<128, DK_GRAM_FP64><<<nblocks, kThreads, 0, s>>>(*a, *ch); break;
