@@ -515,6 +515,9 @@ def main():
                     help="skip the twelve-domain group-graph timing")
     ap.add_argument("--no-struct", action="store_true",
                     help="skip the coupler layout timing (it allocates one 3-D variable block)")
+    ap.add_argument("--chain-len", type=int, default=0,
+                    help="chain length of the batched forms when not autotuned (0: the "
+                         "library default)")
     ap.add_argument("--jacobi-tol", type=float, default=0.0,
                     help="Jacobi tolerance of the half-precision batched forms (0: the "
                          "library default)")
@@ -566,10 +569,11 @@ def main():
         forms = {k: v for k, v in forms.items() if k in args.forms.split(",") or k == "plain_fp64"}
     if args.no_plain:
         forms.pop("plain_fp64")
-    if args.jacobi_tol > 0:
-        for name, (form, gram, extra) in forms.items():
-            if gram != dl.GRAM_FP64:
-                extra["jacobi_tol"] = args.jacobi_tol
+    for name, (form, gram, extra) in forms.items():
+        if args.jacobi_tol > 0 and gram != dl.GRAM_FP64:
+            extra["jacobi_tol"] = args.jacobi_tol
+        if args.chain_len > 0 and form == dl.FORM_BATCHED:
+            extra["chain_len"] = args.chain_len
     res = {}
     ref = None
     for name, (form, gram, extra) in forms.items():

@@ -202,7 +202,8 @@ int dida_letkf_sweep(dida_letkf_ctx *ctx, const dida_letkf_obs *obs, const doubl
                      const double *axb, double *xa, double *axa, void *stream);
 
 /* The same sweep with the ensemble perturbations stored in FP32 (the
- * reference's REAL(4) state option, yb_size in da_config.inc:7-10): xb = Xb'
+ * state type of the reference is the xb_type macro of
+ * code/x86/TradDA/DA/utils/da_config.inc:4-5): xb = Xb'
  * and xa = Xa' as float in the layout above, means axb, axa FP64; the update
  * reads Xb' once, computes in FP64 and rounds Xa' once.  Plain form (any k)
  * or batched form with 64 < k <= 128, horizontal cubes included; the fused
