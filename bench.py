@@ -606,8 +606,11 @@ def main():
     ms = r["ms"]
     value = ncol / (ms / 1e3)
 
+    layout = r["tuning"]
+    if layout is None and head != "plain_fp64":
+        layout = {"chain_len": args.chain_len or 8, "xcd_tiles": 0}   # library defaults
     roofline = kernel_roofline(cfg, st, head, r["stages"]["transform"], args.config,
-                               args.scale, r["tuning"])
+                               args.scale, layout)
 
     out_forms = {}
     for name, rr in res.items():
