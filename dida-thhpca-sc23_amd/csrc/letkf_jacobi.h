@@ -133,22 +133,25 @@ __device__ int jacobi_lds(double* A, double* nrm, int* flag, double tol, int max
 // One-sided Jacobi on a 128 x 128 block by 256 threads with the columns in
 // registers (same result contract and stopping rule as jacobi_lds).
 //
-// Pair slot s = tid / 4 (64 slots, 16 per wavefront) holds a top and a
-// bottom column; the four threads of a slot hold rows 4u + sub (u < 32) of
-// both.  Column j is kept as sc_j x_j with the scaled rotations of
-// rotation_angle (two FMAs a row).  Ordering (KP - 1 = 127 rounds a sweep):
-// for G = 64, 32, ..., 1, each block of G slots runs G rounds in which the
-// top of slot s meets the bottom that started the level in slot
-// s ^ gray(r); between rounds the bottoms move by the one bit in which
-// gray(r) and gray(r + 1) differ; then the bottom of every slot in the
-// first half of its block swaps with the top of slot s + G/2.  Moves along
-// slot bits 0-3 stay inside a wavefront (ds_bpermute); the six a sweep
-// along bits 4 and 5 go through LDS, which aliases A (the matrix is in
-// registers meanwhile).  Every column returns to A at the index it came
-// from, so zero padding columns stay at >= k.
+// Groups of eight threads (gp = tid / 8: 32 groups, 8 per wavefront) hold
+// four column slots X[0..3] each (initially columns J * 32 + gp); thread sub
+// of a group holds rows 8u + sub (u < 16) of its four columns.  Columns are
+// kept as sc_j x_j with the scaled rotations of rotation_angle (two FMAs a
+// row).  Each round rotates two independent pairs per group, (X0, X1) and
+// (X2, X3) or (X0, X3) and (X2, X1), so renaming the slots changes partners
+// for free.  Ordering (127 rounds a sweep; the fused kernel's order for two
+// pairs per lane group, over 32 groups): for G = 64, 32, ..., 4, blocks of
+// G rounds in steps of four, slots 1 and 3 moving once inside the step (from
+// group gp ^ 1) and once between steps (from gp ^ lowbit(r + 4) / 2); then
+// slots 1 and 3 of the first half of every block of G/4 groups swap with
+// slots 0 and 2 of the partner group; three final rounds pair the four
+// slots of each group.  Moves along group bits 0-2 stay in the wavefront
+// (ds_bpermute), the six a sweep along bits 3-4 go through LDS, which
+// aliases A (the matrix is in registers meanwhile).  Every column returns to
+// A at the index it came from, so zero padding columns stay at >= k.
 struct JacobiXbuf {
-  static constexpr int LD = 132;  // slot stride: 8 slots of 4 rows cover the banks
-  double col[64 * LD];
+  static constexpr int LD = 132;  // column stride (bank spread)
+  double col[64 * LD];            // [2 gp + q][row]: two columns a group
   double nrm[64], sc[64], isc[64];
   int32_t id[64];
 };
@@ -159,154 +162,184 @@ __device__ __forceinline__ double dpp_d(double v) {
   const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
   return __hiloint2double(hi, lo);
 }
-// sum over the four lanes of a quad (quad_perm [1,0,3,2], then [2,3,0,1]);
-// every lane of the quad gets the same bits
-__device__ __forceinline__ double quad_sum(double v) {
+// sum over the eight lanes of a group (quad_perm [1,0,3,2], [2,3,0,1], then
+// row_half_mirror); every lane of the group gets the same bits
+__device__ __forceinline__ double oct_sum(double v) {
   v = v + dpp_d<0xB1>(v);
-  return v + dpp_d<0x4E>(v);
+  v = v + dpp_d<0x4E>(v);
+  return v + dpp_d<0x141>(v);
 }
 
 struct Jacobi128 {
-  static constexpr int NR = 32;  // rows per thread
-  double x[NR], y[NR];           // top and bottom column of the slot
-  double nt, nb, st, sb, it, ib; // squared norms, scales, inverse scales
-  int idt, idb;                  // their column index in A
+  static constexpr int NR = 16;  // rows per thread
+  double X[4][NR];               // column slots of the group
+  double nrm[4], sc[4], isc[4];  // squared norms, scales, inverse scales
+  int id[4];                     // column index in A
 
-  __device__ __forceinline__ void round(double tol2, double tolp, bool& rotated, bool& big) {
+  template <int P, int Q>
+  __device__ __forceinline__ double dot() const {
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
 #pragma unroll
     for (int u = 0; u < NR; u += 4) {
-      a0 = fma(x[u], y[u], a0);
-      a1 = fma(x[u + 1], y[u + 1], a1);
-      a2 = fma(x[u + 2], y[u + 2], a2);
-      a3 = fma(x[u + 3], y[u + 3], a3);
+      a0 = fma(X[P][u], X[Q][u], a0);
+      a1 = fma(X[P][u + 1], X[Q][u + 1], a1);
+      a2 = fma(X[P][u + 2], X[Q][u + 2], a2);
+      a3 = fma(X[P][u + 3], X[Q][u + 3], a3);
     }
-    const double gs = quad_sum((a0 + a1) + (a2 + a3));
-    double t, gm, cs, ic;
-    rotation_angle(gs, nt, nb, st, sb, tol2, tolp, rotated, big, t, gm, cs, ic);
-    const double ta = t * (sb * it), tb = t * (st * ib);
+    return (a0 + a1) + (a2 + a3);
+  }
+  template <int P, int Q>
+  __device__ __forceinline__ void apply(double t, double gm, double cs, double ic) {
+    const double ta = t * (sc[Q] * isc[P]), tb = t * (sc[P] * isc[Q]);
 #pragma unroll
     for (int u = 0; u < NR; ++u) {
-      const double xx = x[u], yy = y[u];
-      x[u] = fma(-ta, yy, xx);
-      y[u] = fma(tb, xx, yy);
+      const double x = X[P][u], y = X[Q][u];
+      X[P][u] = fma(-ta, y, x);
+      X[Q][u] = fma(tb, x, y);
     }
-    nt = fma(-t, gm, nt);
-    nb = fma(t, gm, nb);
-    st *= cs;
-    sb *= cs;
-    it *= ic;
-    ib *= ic;
+    nrm[P] = fma(-t, gm, nrm[P]);
+    nrm[Q] = fma(t, gm, nrm[Q]);
+    sc[P] *= cs;
+    sc[Q] *= cs;
+    isc[P] *= ic;
+    isc[Q] *= ic;
   }
-  // the bottom of the slot whose lanes sit at byte address addr
+  template <int T0, int B0, int T1, int B1>
+  __device__ __forceinline__ void round(double tol2, double tolp, bool& rotated, bool& big) {
+    const double g0 = oct_sum(dot<T0, B0>());
+    const double g1 = oct_sum(dot<T1, B1>());
+    double t0, gm0, cs0, ic0, t1, gm1, cs1, ic1;
+    rotation_angle(g0, nrm[T0], nrm[B0], sc[T0], sc[B0], tol2, tolp, rotated, big, t0, gm0,
+                   cs0, ic0);
+    rotation_angle(g1, nrm[T1], nrm[B1], sc[T1], sc[B1], tol2, tolp, rotated, big, t1, gm1,
+                   cs1, ic1);
+    apply<T0, B0>(t0, gm0, cs0, ic0);
+    apply<T1, B1>(t1, gm1, cs1, ic1);
+  }
+  // slot J takes slot J of the group whose lanes sit at byte address addr
+  template <int J>
   __device__ __forceinline__ void move_lane(int addr) {
 #pragma unroll
-    for (int u = 0; u < NR; ++u) y[u] = bperm_d(addr, y[u]);
-    nb = bperm_d(addr, nb);
-    sb = bperm_d(addr, sb);
-    ib = bperm_d(addr, ib);
-    idb = __builtin_amdgcn_ds_bpermute(addr, idb);
+    for (int u = 0; u < NR; ++u) X[J][u] = bperm_d(addr, X[J][u]);
+    nrm[J] = bperm_d(addr, nrm[J]);
+    sc[J] = bperm_d(addr, sc[J]);
+    isc[J] = bperm_d(addr, isc[J]);
+    id[J] = __builtin_amdgcn_ds_bpermute(addr, id[J]);
   }
-  // the bottom of slot src, through LDS
-  __device__ __forceinline__ void move_lds(JacobiXbuf& xb, int s, int sub, int src) {
+  // the column of slot J (q: 0 or 1 of the group's two exchange places) to
+  // the exchange area, and back from group src
+  template <int J>
+  __device__ __forceinline__ void put(JacobiXbuf& xb, int gp, int sub, int q) const {
 #pragma unroll
-    for (int u = 0; u < NR; ++u) xb.col[s * JacobiXbuf::LD + 4 * u + sub] = y[u];
+    for (int u = 0; u < NR; ++u) xb.col[(2 * gp + q) * JacobiXbuf::LD + 8 * u + sub] = X[J][u];
     if (sub == 0) {
-      xb.nrm[s] = nb;
-      xb.sc[s] = sb;
-      xb.isc[s] = ib;
-      xb.id[s] = idb;
+      xb.nrm[2 * gp + q] = nrm[J];
+      xb.sc[2 * gp + q] = sc[J];
+      xb.isc[2 * gp + q] = isc[J];
+      xb.id[2 * gp + q] = id[J];
     }
-    __syncthreads();
-#pragma unroll
-    for (int u = 0; u < NR; ++u) y[u] = xb.col[src * JacobiXbuf::LD + 4 * u + sub];
-    nb = xb.nrm[src];
-    sb = xb.sc[src];
-    ib = xb.isc[src];
-    idb = xb.id[src];
-    __syncthreads();
   }
-  // first: this slot's bottom takes the partner's top; otherwise this
-  // slot's top takes the partner's bottom
+  template <int J>
+  __device__ __forceinline__ void get(const JacobiXbuf& xb, int src, int sub, int q) {
+#pragma unroll
+    for (int u = 0; u < NR; ++u) X[J][u] = xb.col[(2 * src + q) * JacobiXbuf::LD + 8 * u + sub];
+    nrm[J] = xb.nrm[2 * src + q];
+    sc[J] = xb.sc[2 * src + q];
+    isc[J] = xb.isc[2 * src + q];
+    id[J] = xb.id[2 * src + q];
+  }
+  // slots 1 and 3 from group gp ^ v
+  __device__ __forceinline__ void move_bottoms(JacobiXbuf& xb, int lane, int gp, int sub, int v) {
+    if (v < 8) {
+      const int addr = (lane ^ (8 * v)) << 2;
+      move_lane<1>(addr);
+      move_lane<3>(addr);
+    } else {
+      put<1>(xb, gp, sub, 0);
+      put<3>(xb, gp, sub, 1);
+      __syncthreads();
+      get<1>(xb, gp ^ v, sub, 0);
+      get<3>(xb, gp ^ v, sub, 1);
+      __syncthreads();
+    }
+  }
+  // first: slot B takes the partner's slot T; otherwise slot T takes the
+  // partner's slot B
+  template <int T, int B>
   __device__ __forceinline__ void xswap_lane(int addr, bool first) {
 #pragma unroll
     for (int u = 0; u < NR; ++u) {
-      const double w = bperm_d(addr, first ? y[u] : x[u]);
-      y[u] = first ? w : y[u];
-      x[u] = first ? x[u] : w;
+      const double w = bperm_d(addr, first ? X[B][u] : X[T][u]);
+      X[B][u] = first ? w : X[B][u];
+      X[T][u] = first ? X[T][u] : w;
     }
-    double w = bperm_d(addr, first ? nb : nt);
-    nb = first ? w : nb;
-    nt = first ? nt : w;
-    w = bperm_d(addr, first ? sb : st);
-    sb = first ? w : sb;
-    st = first ? st : w;
-    w = bperm_d(addr, first ? ib : it);
-    ib = first ? w : ib;
-    it = first ? it : w;
-    const int wi = __builtin_amdgcn_ds_bpermute(addr, first ? idb : idt);
-    idb = first ? wi : idb;
-    idt = first ? idt : wi;
+    double w = bperm_d(addr, first ? nrm[B] : nrm[T]);
+    nrm[B] = first ? w : nrm[B];
+    nrm[T] = first ? nrm[T] : w;
+    w = bperm_d(addr, first ? sc[B] : sc[T]);
+    sc[B] = first ? w : sc[B];
+    sc[T] = first ? sc[T] : w;
+    w = bperm_d(addr, first ? isc[B] : isc[T]);
+    isc[B] = first ? w : isc[B];
+    isc[T] = first ? isc[T] : w;
+    const int wi = __builtin_amdgcn_ds_bpermute(addr, first ? id[B] : id[T]);
+    id[B] = first ? wi : id[B];
+    id[T] = first ? id[T] : wi;
   }
-  __device__ __forceinline__ void xswap_lds(JacobiXbuf& xb, int s, int sub, int src,
-                                            bool first) {
-#pragma unroll
-    for (int u = 0; u < NR; ++u) xb.col[s * JacobiXbuf::LD + 4 * u + sub] = first ? y[u] : x[u];
-    if (sub == 0) {
-      xb.nrm[s] = first ? nb : nt;
-      xb.sc[s] = first ? sb : st;
-      xb.isc[s] = first ? ib : it;
-      xb.id[s] = first ? idb : idt;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int u = 0; u < NR; ++u) {
-      const double w = xb.col[src * JacobiXbuf::LD + 4 * u + sub];
-      y[u] = first ? w : y[u];
-      x[u] = first ? x[u] : w;
-    }
-    const double wn = xb.nrm[src], ws = xb.sc[src], wi = xb.isc[src];
-    const int wd = xb.id[src];
-    if (first) {
-      nb = wn;
-      sb = ws;
-      ib = wi;
-      idb = wd;
+  // (0, 1) and (2, 3) swapped with group gp ^ d
+  __device__ __forceinline__ void xswap(JacobiXbuf& xb, int lane, int gp, int sub, int d) {
+    const bool first = (gp & d) == 0;
+    if (d < 8) {
+      const int addr = (lane ^ (8 * d)) << 2;
+      xswap_lane<0, 1>(addr, first);
+      xswap_lane<2, 3>(addr, first);
     } else {
-      nt = wn;
-      st = ws;
-      it = wi;
-      idt = wd;
+      if (first) {
+        put<1>(xb, gp, sub, 0);
+        put<3>(xb, gp, sub, 1);
+      } else {
+        put<0>(xb, gp, sub, 0);
+        put<2>(xb, gp, sub, 1);
+      }
+      __syncthreads();
+      if (first) {
+        get<1>(xb, gp ^ d, sub, 0);
+        get<3>(xb, gp ^ d, sub, 1);
+      } else {
+        get<0>(xb, gp ^ d, sub, 0);
+        get<2>(xb, gp ^ d, sub, 1);
+      }
+      __syncthreads();
     }
-    __syncthreads();
   }
   __device__ __forceinline__ void fold() {
 #pragma unroll
-    for (int u = 0; u < NR; ++u) {
-      x[u] *= st;
-      y[u] *= sb;
+    for (int J = 0; J < 4; ++J) {
+#pragma unroll
+      for (int u = 0; u < NR; ++u) X[J][u] *= sc[J];
+      sc[J] = 1.0;
+      isc[J] = 1.0;
     }
-    st = sb = it = ib = 1.0;
   }
 };
 
 template <int LDA, int NTHREADS>
 __device__ int jacobi_regs128(double* A, int* flag, double tol, int max_sweeps, bool predict,
                               bool* converged) {
-  static_assert(NTHREADS == 256, "64 pair slots of four threads");
+  static_assert(NTHREADS == 256, "32 groups of eight threads");
   static_assert(sizeof(JacobiXbuf) <= sizeof(double) * 128 * LDA, "exchange area fits in A");
   constexpr int NR = Jacobi128::NR;
-  const int tid = threadIdx.x, lane = tid & 63, sub = tid & 3, s = tid >> 2;
+  const int tid = threadIdx.x, lane = tid & 63, sub = tid & 7, gp = tid >> 3;
   JacobiXbuf& xb = *reinterpret_cast<JacobiXbuf*>(A);
   Jacobi128 j;
-  j.idt = 2 * s;
-  j.idb = 2 * s + 1;
 #pragma unroll
-  for (int u = 0; u < NR; ++u) {
-    j.x[u] = A[4 * u + sub + j.idt * LDA];
-    j.y[u] = A[4 * u + sub + j.idb * LDA];
+  for (int J = 0; J < 4; ++J) {
+    j.id[J] = J * 32 + gp;
+#pragma unroll
+    for (int u = 0; u < NR; ++u) j.X[J][u] = A[8 * u + sub + j.id[J] * LDA];
+    j.sc[J] = 1.0;
+    j.isc[J] = 1.0;
   }
-  j.st = j.sb = j.it = j.ib = 1.0;
   if (tid == 0) {
     flag[0] = 0;
     flag[2] = 0;
@@ -318,31 +351,31 @@ __device__ int jacobi_regs128(double* A, int* flag, double tol, int max_sweeps, 
   *converged = false;
   for (; sweep < max_sweeps; ++sweep) {
     j.fold();
-    double a0 = 0.0, a1 = 0.0;
 #pragma unroll
-    for (int u = 0; u < NR; ++u) {
-      a0 = fma(j.x[u], j.x[u], a0);
-      a1 = fma(j.y[u], j.y[u], a1);
+    for (int J = 0; J < 4; ++J) {
+      double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+      for (int u = 0; u < NR; u += 2) {
+        a0 = fma(j.X[J][u], j.X[J][u], a0);
+        a1 = fma(j.X[J][u + 1], j.X[J][u + 1], a1);
+      }
+      j.nrm[J] = oct_sum(a0 + a1);
     }
-    j.nt = quad_sum(a0);
-    j.nb = quad_sum(a1);
     bool rotated = false, big = false;
-    for (int G = 64; G >= 1; G >>= 1) {
-      for (int r = 0; r < G; ++r) {
-        j.round(tol2, tolp, rotated, big);
-        if (r + 1 < G) {
-          const int m = (r + 1) & -(r + 1);
-          if (m < 16) j.move_lane((lane ^ (4 * m)) << 2);
-          else j.move_lds(xb, s, sub, s ^ m);
-        }
+    for (int G = 64; G >= 4; G >>= 1) {
+      for (int r = 0; r < G; r += 4) {
+        j.round<0, 1, 2, 3>(tol2, tolp, rotated, big);
+        j.round<0, 3, 2, 1>(tol2, tolp, rotated, big);
+        j.move_bottoms(xb, lane, gp, sub, 1);
+        j.round<0, 3, 2, 1>(tol2, tolp, rotated, big);
+        j.round<0, 1, 2, 3>(tol2, tolp, rotated, big);
+        if (r + 4 < G) j.move_bottoms(xb, lane, gp, sub, ((r + 4) & -(r + 4)) >> 1);
       }
-      if (G > 1) {
-        const int d = G >> 1;
-        const bool first = (s & d) == 0;
-        if (d < 16) j.xswap_lane((lane ^ (4 * d)) << 2, first);
-        else j.xswap_lds(xb, s, sub, s ^ d, first);
-      }
+      j.xswap(xb, lane, gp, sub, G >> 2);
     }
+    j.round<0, 1, 2, 3>(tol2, tolp, rotated, big);
+    j.round<0, 3, 2, 1>(tol2, tolp, rotated, big);
+    j.round<0, 2, 3, 1>(tol2, tolp, rotated, big);
     const bool any_rot = __any(rotated), any_big = __any(big);
     if (lane == 0) {
       if (any_rot) flag[sweep & 1] = 1;
@@ -362,10 +395,9 @@ __device__ int jacobi_regs128(double* A, int* flag, double tol, int max_sweeps, 
   j.fold();
   __syncthreads();  // the exchange area is no longer read
 #pragma unroll
-  for (int u = 0; u < NR; ++u) {
-    A[4 * u + sub + j.idt * LDA] = j.x[u];
-    A[4 * u + sub + j.idb * LDA] = j.y[u];
-  }
+  for (int J = 0; J < 4; ++J)
+#pragma unroll
+    for (int u = 0; u < NR; ++u) A[8 * u + sub + j.id[J] * LDA] = j.X[J][u];
   __syncthreads();
   return sweep;
 }

@@ -51,11 +51,13 @@ def _slot_schedule(nslot):
     return rounds
 
 
-def _wave_np2_schedule():
-    """KP = 64: lane group c (16 of them) holds column slots J = 0..3, first
-    column J * 16 + c; rounds pair (X0, X1), (X2, X3) or (X0, X3), (X2, X1)."""
-    kp, ng = 64, 16
-    x = [[j * 16 + c for j in range(4)] for c in range(ng)]
+def _wave_np2_schedule(kp=64):
+    """Two pairs per lane group: group c (kp / 4 of them) holds column slots
+    J = 0..3, first column J * (kp / 4) + c; rounds pair (X0, X1), (X2, X3)
+    or (X0, X3), (X2, X1).  kp = 64: the fused kernel's 16 lane groups;
+    kp = 128: the workgroup form's 32 groups of eight threads."""
+    ng = kp // 4
+    x = [[j * ng + c for j in range(4)] for c in range(ng)]
     rounds = []
 
     def rnd(t0, b0, t1, b1):
@@ -108,6 +110,10 @@ def test_wave_one_pair_per_lane_group_kp32():
 
 def test_workgroup_register_jacobi_kp128():
     _check(_slot_schedule(64), 128)
+
+
+def test_workgroup_two_pairs_per_group_kp128():
+    _check(_wave_np2_schedule(128), 128)
 
 
 def test_workgroup_cross_wave_moves_are_six_a_sweep():
