@@ -114,6 +114,155 @@ __global__ __launch_bounds__(kThreads) void k_update_f64(dk_update_args u) {
   }
 }
 
+// 64 < k <= 128: the same update on FP64 MFMA 16x16x4, one 256-thread
+// workgroup per column.  Xa(row, e) = sum_i Xb'(i, row) trans(i, e) + xbar_b
+// with rows in tiles of 16 (M), members in tiles of 16 (N; wave w owns
+// member tiles 2w and 2w + 1 and holds their trans columns in registers)
+// and i in steps of 4 (K).  Each tile of 16 rows of Xb' is staged once in
+// LDS as FP64 and read by all four waves; the row mean (and the RTPS
+// spreads) meet across the waves in LDS.  Horizontal cubes take the mean
+// transform of the covering cubes, as k_update_f64.
+template <typename S>
+__global__ __launch_bounds__(kThreads) void k_update_mfma128(dk_update_args u) {
+  constexpr int KP = 128, NS = KP / 4, LDX = KP + 4;
+  __shared__ double sX[16 * LDX];
+  __shared__ double sv[4][16], sb[4][16], sa[4][16];
+  const int c = blockIdx.x;
+  const int64_t col = (int64_t)u.col0 + c;
+  const int k = u.k, nrow = u.nrow;
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, r16 = lane & 15, q4 = lane >> 4;
+  const S* xbs;
+  S* xas;
+  if constexpr (sizeof(S) == sizeof(float)) {
+    xbs = u.xb32;
+    xas = u.xa32;
+  } else {
+    xbs = u.xb;
+    xas = u.xa;
+  }
+  // B operands: trans(i = 4s + q4, e = (2w + j) * 16 + r16), zero past k
+  double bv[2][NS];
+  if (u.cube_n > 1) {
+    const int ilat = (int)(col % u.nlat), ilon = (int)(col / u.nlat);
+    const int st = u.step, cn = u.cube_n;
+    const int aj0 = ilat - cn + 1 > 0 ? (ilat - cn + 1 + st - 1) / st : 0;
+    const int aj1 = min(ilat / st, u.nalat - 1);
+    const int ai0 = ilon - cn + 1 > 0 ? (ilon - cn + 1 + st - 1) / st : 0;
+    const int ai1 = min(ilon / st, u.nalon - 1);
+    const double cnt = (double)((aj1 - aj0 + 1) * (ai1 - ai0 + 1));
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+        const int i = 4 * s + q4, e = (2 * w + j) * 16 + r16;
+        double sum = 0.0;
+        if (i < k && e < k)
+          for (int ai = ai0; ai <= ai1; ++ai)
+            for (int aj = aj0; aj <= aj1; ++aj)
+              sum += u.trans[((int64_t)aj + (int64_t)u.nalat * ai) * k * k + i + (int64_t)e * k];
+        bv[j][s] = sum / cnt;
+      }
+  } else {
+    const double* T = u.trans + (int64_t)c * k * k;
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+        const int i = 4 * s + q4, e = (2 * w + j) * 16 + r16;
+        bv[j][s] = i < k && e < k ? T[i + (int64_t)e * k] : 0.0;
+      }
+  }
+  const double rk = 1.0 / (double)k;
+  for (int r0 = 0; r0 < nrow; r0 += 16) {
+    __syncthreads();  // the previous tile's sX, sv, sb, sa are read
+    for (int t = tid; t < 16 * KP; t += kThreads) {
+      const int rr = t / KP, e = t % KP, row = r0 + rr;
+      sX[rr * LDX + e] = row < nrow && e < k ? (double)xbs[(col * nrow + row) * k + e] : 0.0;
+    }
+    __syncthreads();
+    double4_t acc[2] = {double4_t{0.0, 0.0, 0.0, 0.0}, double4_t{0.0, 0.0, 0.0, 0.0}};
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      const double a = sX[r16 * LDX + 4 * s + q4];
+      acc[0] = mfma_f64_16x16x4(a, bv[0][s], acc[0]);
+      acc[1] = mfma_f64_16x16x4(a, bv[1][s], acc[1]);
+    }
+    // acc[j][r]: row r0 + 4r + q4, member (2w + j) * 16 + r16
+    double v[2][4], pv[4], pb[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int rl = 4 * r + q4, row = r0 + rl;
+      const double xbar = row < nrow ? u.axb[col * nrow + row] : 0.0;
+      pv[r] = 0.0;
+      pb[r] = 0.0;
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int e = (2 * w + j) * 16 + r16;
+        const bool ok = row < nrow && e < k;
+        v[j][r] = ok ? acc[j][r] + xbar : 0.0;
+        const double xv = ok ? sX[rl * LDX + e] : 0.0;
+        pv[r] += v[j][r];
+        pb[r] += xv * xv;
+      }
+#pragma unroll
+      for (int m = 1; m < 16; m <<= 1) {
+        pv[r] += shfl_xor_d(pv[r], m);
+        pb[r] += shfl_xor_d(pb[r], m);
+      }
+      if (r16 == 0) {
+        sv[w][rl] = pv[r];
+        sb[w][rl] = pb[r];
+      }
+    }
+    __syncthreads();
+    double mean[4], xa[2][4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int rl = 4 * r + q4;
+      mean[r] = ((sv[0][rl] + sv[1][rl]) + (sv[2][rl] + sv[3][rl])) * rk;
+      double pa = 0.0;
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int e = (2 * w + j) * 16 + r16;
+        const bool ok = r0 + rl < nrow && e < k;
+        xa[j][r] = ok ? v[j][r] - mean[r] : 0.0;
+        pa += xa[j][r] * xa[j][r];
+      }
+      if (u.rtps) {
+#pragma unroll
+        for (int m = 1; m < 16; m <<= 1) pa += shfl_xor_d(pa, m);
+        if (r16 == 0) sa[w][rl] = pa;
+      }
+    }
+    if (u.rtps) {
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int rl = 4 * r + q4;
+        const double sb2 = ((sb[0][rl] + sb[1][rl]) + (sb[2][rl] + sb[3][rl])) * rk;
+        const double sa2 = ((sa[0][rl] + sa[1][rl]) + (sa[2][rl] + sa[3][rl])) * rk;
+        if (sa2 > 0.0) {
+          const double f = u.rtps_alpha * sqrt(sb2) / sqrt(sa2) + 1.0 - u.rtps_alpha;
+          xa[0][r] *= f;
+          xa[1][r] *= f;
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int rl = 4 * r + q4, row = r0 + rl;
+      if (row < nrow) {
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          const int e = (2 * w + j) * 16 + r16;
+          if (e < k) xas[(col * nrow + row) * k + e] = (S)xa[j][r];
+        }
+        if (w == 0 && r16 == 0) u.axa[col * nrow + row] = mean[r];
+      }
+    }
+  }
+}
+
 __global__ void k_cube_centres(int32_t nlat, int32_t nlon, int32_t cube_n, int32_t step,
                                const float* grid_lat, const float* grid_lon, float* ca_lat,
                                float* ca_lon) {
@@ -251,12 +400,12 @@ extern "C" int dk_update_f64(const dk_update_args* u, void* stream) {
     if (k <= 16) k_update_f64<16, float><<<u->ncol, kThreads, 0, s>>>(*u);
     else if (k <= 32) k_update_f64<32, float><<<u->ncol, kThreads, 0, s>>>(*u);
     else if (k <= 64) k_update_f64<64, float><<<u->ncol, kThreads, 0, s>>>(*u);
-    else k_update_f64<128, float><<<u->ncol, kThreads, 0, s>>>(*u);
+    else k_update_mfma128<float><<<u->ncol, kThreads, 0, s>>>(*u);
   } else {
     if (k <= 16) k_update_f64<16, double><<<u->ncol, kThreads, 0, s>>>(*u);
     else if (k <= 32) k_update_f64<32, double><<<u->ncol, kThreads, 0, s>>>(*u);
     else if (k <= 64) k_update_f64<64, double><<<u->ncol, kThreads, 0, s>>>(*u);
-    else k_update_f64<128, double><<<u->ncol, kThreads, 0, s>>>(*u);
+    else k_update_mfma128<double><<<u->ncol, kThreads, 0, s>>>(*u);
   }
   return (int)hipGetLastError();
 }
