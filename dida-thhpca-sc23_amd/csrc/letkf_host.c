@@ -347,6 +347,7 @@ int dida_letkf_create(dida_letkf_ctx **out, const dida_letkf_params *prm, int32_
     DMALLOC(c->uz, no * sizeof(double));
     c->chained = prm->form == DIDA_FORM_BATCHED && k > 64;
     c->ldy = prm->form == DIDA_FORM_BATCHED && !c->chained ? (k <= 32 ? 32 : 64) : k;
+    if (c->chained && prm->gram != DIDA_GRAM_FP64) c->ldy = (k + 3) & ~3; /* float4 rows */
     if (c->chained && prm->gram != DIDA_GRAM_FP64)
         DMALLOC(c->yb32_b, no * (size_t)c->ldy * sizeof(float));
     else
@@ -791,6 +792,7 @@ static int plain_transform(dida_letkf_ctx *c, int64_t a0, int64_t a1, int64_t ba
     ca.max_p = c->prm.max_obs;
     ca.yb = c->yb_b;
     ca.yb32 = c->yb32_b;
+    ca.ldy32 = c->ldy;
     ca.rinv = c->rinv_b;
     ca.dep = c->dep_b;
     ca.gram = c->chained ? c->prm.gram : DIDA_GRAM_FP64;

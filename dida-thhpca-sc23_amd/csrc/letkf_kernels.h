@@ -112,7 +112,10 @@ typedef struct dk_core_args {
     double *wbar;           /* [(id - aid0) * k] (optional) */
     int32_t *info;          /* [id]: sweeps, or -1 when not converged (optional) */
     const float *yb32;      /* half-precision Grams (chained): yb rounded to FP32,
-                               read instead of yb */
+                               read instead of yb, rows ldy32 apart (a multiple of 4,
+                               zero past k) */
+    int32_t ldy32;
+    int32_t pad2_;
 } dk_core_args;
 
 int dk_letkf_core_f64(const dk_core_args *a, void *stream);

@@ -268,8 +268,9 @@ __device__ __forceinline__ void gram_f64_obs(const dk_core_args& a, int64_t off,
 // FP32 accumulation; b = Yb^T (R^-1 rloc d) accumulated in FP64.
 // Blocks of 64 obs are converted once by the whole workgroup into LDS, in
 // the MFMA operand layout, while the previous block is multiplied: thread
-// (row = tid % 128, step = tid / 128) converts row `row` of the 32 obs of
-// k-step `step`.  The staging area aliases A, which is free until C is
+// (rq = tid % 32, og = tid / 32) loads rows 4 rq .. 4 rq + 3 of the 8 obs
+// of octet og (one 16-byte load each) and writes their four 16-byte hi and
+// lo vectors.  The staging area aliases A, which is free until C is
 // written.  Wave W owns the lower tiles of tile rows W and NT-1-W.
 template <int KP>
 struct H16Stage {
@@ -293,17 +294,19 @@ __device__ __forceinline__ void gram_h16_rows(const dk_core_args& a, int64_t off
   static_assert(sizeof(H16Stage<KP>) <= sizeof(sm.A), "staging fits in A");
   H16Stage<KP>& st = *reinterpret_cast<H16Stage<KP>*>(sm.A);
   const int tid = threadIdx.x, lane = tid & 63, g = lane >> 4, c = lane & 15;
-  const int row = tid % KP, step = tid / KP;
+  // staging: thread (rq = tid % 32, og = tid / 32) converts rows 4 rq .. 4 rq + 3
+  // of obs 8 og .. 8 og + 7 of a block (one 16-byte load per obs)
+  const int rq = tid & 31, og = tid >> 5;
   const int k = a.k;
-  const int rowc = row < k ? row : k - 1;
+  const bool rows_ok = 4 * rq < k;
   const int nblk = (p + 63) / 64;
   f32x4_t accA[IA + 1], accB[IB + 1];
 #pragma unroll
   for (int J = 0; J <= IA; ++J) accA[J] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
   for (int J = 0; J <= IB; ++J) accB[J] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
-  double bacc = 0.0;
-  float raw[32];
+  double bacc[4] = {0.0, 0.0, 0.0, 0.0};
+  f32x4_t raw[8];
 
   // obs metadata of block blk into buffer buf (threads 0-63); obs past p
   // take the index of obs 0 of the analysis and zero weights
@@ -343,24 +346,32 @@ __device__ __forceinline__ void gram_h16_rows(const dk_core_args& a, int64_t off
   };
   auto load = [&](int buf) {
 #pragma unroll
-    for (int e = 0; e < 32; ++e) raw[e] = a.yb32[(int64_t)st.gi[buf][step * 32 + e] * k + rowc];
+    for (int e = 0; e < 8; ++e)
+      raw[e] = rows_ok ? *reinterpret_cast<const f32x4_t*>(
+                             a.yb32 + (int64_t)st.gi[buf][og * 8 + e] * a.ldy32 + 4 * rq)
+                       : f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
   };
   auto split = [&](int buf) {
+    float rs[8];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int e = 0; e < 8; ++e) {
+      const double w = st.w[buf][og * 8 + e];
+      rs[e] = (float)st.rr[buf][og * 8 + e];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) bacc[r] = fma((double)raw[e][r], w, bacc[r]);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
       hvec hv, lv;
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
-        const int o = step * 32 + q * 8 + e;
-        const float x = raw[q * 8 + e];
-        bacc = fma((double)x, st.w[buf][o], bacc);
-        const float xs = row < k ? x * (float)st.rr[buf][o] : 0.0f;
+        const float xs = raw[e][r] * rs[e];
         const helt h = (helt)xs;
         hv[e] = h;
         lv[e] = (helt)(xs - (float)h);
       }
-      st.hi[buf][step * 4 + q][row] = hv;
-      if (kSplit) st.lo[buf][step * 4 + q][row] = lv;
+      st.hi[buf][og][4 * rq + r] = hv;
+      if (kSplit) st.lo[buf][og][4 * rq + r] = lv;
     }
   };
   auto mma = [&](int buf) {
@@ -429,10 +440,35 @@ __device__ __forceinline__ void gram_h16_rows(const dk_core_args& a, int64_t off
       sm.A[i + j * LDA] = (double)accB[J][r];
       if (IB != J) sm.A[j + i * LDA] = (double)accB[J][r];
     }
-  // b: the two k-step halves of each row meet in LDS
-  if (step == 1) sm.u[row] = bacc;
+  // b: the partials of the eight obs groups of each row meet in LDS (sm.u
+  // and sm.sg, which the transform fills later)
   __syncthreads();
-  if (step == 0) sm.b[row] = row < k ? bacc + sm.u[row] : 0.0;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) bacc[r] += shfl_xor_d(bacc[r], 32);  // og pairs of a wave
+  const int wv = tid >> 6;
+  if (lane < 32) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      if (wv == 0) sm.u[4 * rq + r] = bacc[r];
+      if (wv == 1) sm.sg[4 * rq + r] = bacc[r];
+    }
+  }
+  __syncthreads();
+  if (lane < 32 && wv >= 2) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      if (wv == 2) sm.u[4 * rq + r] += bacc[r];
+      if (wv == 3) bacc[r] += sm.sg[4 * rq + r];
+    }
+  }
+  __syncthreads();
+  if (lane < 32 && wv == 3) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = 4 * rq + r;
+      sm.b[i] = i < k ? bacc[r] + sm.u[i] : 0.0;
+    }
+  }
 }
 
 template <int KP, int GRAM>
