@@ -577,13 +577,22 @@ __device__ void analysis_f64(const dk_core_args& a, int aid, CoreSmem<KP>& sm,
   }
   __syncthreads();
   if (vnext) {
-    // v_j = a_j / lambda_j for the next analysis of the chain
+    // v_j = a_j / lambda_j for the next analysis of the chain (1/lambda_j
+    // staged in u, which is filled again below).  Only this workgroup reads
+    // vnext back, on the same CU and L2, with loads that bypass the L1: the
+    // stores need to have reached the L2, which the workgroup-scope release
+    // and the barrier at the start of the next analysis ensure (an agent-
+    // scope fence would also write back the L2 of the whole die).
+    for (int j = tid; j < KP; j += kThreads) {
+      const double al = j < m ? sm.nrm[j] : 0.0;
+      sm.u[j] = al > 0.0 ? 1.0 / sqrt(al) : 0.0;
+    }
+    __syncthreads();
     for (int t = tid; t < KP * KP; t += kThreads) {
       const int i = t % KP, j = t / KP;
-      const double al = j < m ? sm.nrm[j] : 0.0;
-      vnext[t] = al > 0.0 && i < k ? sm.A[i + j * LDA] / sqrt(al) : 0.0;
+      vnext[t] = i < k ? sm.A[i + j * LDA] * sm.u[j] : 0.0;
     }
-    __threadfence();
+    __threadfence_block();
   }
   // with v_j = a_j / lambda_j: W = sum_j sqrt((k-1)/lambda_j) v_j v_j^T
   // = A diag(g) A^T, g_j = sqrt((k-1)/lambda_j) / alpha_j, and
@@ -615,23 +624,29 @@ __device__ void analysis_f64(const dk_core_args& a, int aid, CoreSmem<KP>& sm,
   }
   __syncthreads();
 
-  // W = (A diag(sg)) A^T on MFMA, written with wbar as trans.
+  // W = (A diag(sg)) A^T on MFMA, written with wbar as trans.  The tile of
+  // trans rows I, columns I2 is formed as W(I2 block, I block), whose result
+  // layout puts 16 consecutive rows of a trans column in 16 lanes (W is
+  // symmetric), so the stores are whole 128-byte lines.
   double* trans = a.trans + slot * k * k;
+  // (columns past m are zero and sg is zero there, so the contraction runs
+  // over all KP columns at compile-time trip counts, in two chains)
   for (int t = wave; t < NT * NT; t += 4) {
     const int I = t / NT, I2 = t % NT;
-    double4_t d = double4_t{0.0, 0.0, 0.0, 0.0};
-    for (int j0 = 0; j0 < m; j0 += 4) {
-      const int jj = j0 + q4;
-      double fa = 0.0, fb = 0.0;
-      if (jj < m) {
-        fa = sm.A[I * 16 + r16 + jj * LDA] * sm.sg[jj];
-        fb = sm.A[I2 * 16 + r16 + jj * LDA];
-      }
-      d = mfma_f64_16x16x4(fa, fb, d);
+    double4_t d = double4_t{0.0, 0.0, 0.0, 0.0}, d1 = d;
+#pragma unroll
+    for (int j0 = 0; j0 < KP; j0 += 8) {
+      const int ja = j0 + q4, jb = j0 + 4 + q4;
+      d = mfma_f64_16x16x4(sm.A[I2 * 16 + r16 + ja * LDA] * sm.sg[ja],
+                           sm.A[I * 16 + r16 + ja * LDA], d);
+      d1 = mfma_f64_16x16x4(sm.A[I2 * 16 + r16 + jb * LDA] * sm.sg[jb],
+                            sm.A[I * 16 + r16 + jb * LDA], d1);
     }
+    d += d1;
+    // d[r] = W(I2*16 + 4r + q4, I*16 + r16) = W(i, i2)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int i = I * 16 + 4 * r + q4, i2 = I2 * 16 + r16;
+      const int i = I * 16 + r16, i2 = I2 * 16 + 4 * r + q4;
       if (i < k && i2 < k) trans[i + (int64_t)i2 * k] = d[r] + sm.wb[i];
     }
   }
