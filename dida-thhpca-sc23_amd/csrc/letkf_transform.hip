@@ -274,11 +274,11 @@ __device__ __forceinline__ void gram_f64_obs(const dk_core_args& a, int64_t off,
 // written.  Wave W owns the lower tiles of tile rows W and NT-1-W.
 template <int KP>
 struct H16Stage {
-  bf16x8_t hi[2][8][KP];  // [buffer][obs octet][row]: obs 8 o .. 8 o + 7
+  bf16x8_t hi[2][8][KP];  // [block & 1][obs octet][row]: obs 8 o .. 8 o + 7
   bf16x8_t lo[2][8][KP];
-  double rr[2][64];       // sqrt(R^-1 rloc), 0 past p
-  double w[2][64];        // R^-1 rloc dep
-  int32_t gi[2][64];      // obs index
+  double rr[4][64];       // [block & 3]: sqrt(R^-1 rloc), 0 past p
+  double w[4][64];        // R^-1 rloc dep
+  int32_t gi[4][64];      // obs index
 };
 
 template <int KP, int GRAM, int W>
@@ -306,11 +306,10 @@ __device__ __forceinline__ void gram_h16_rows(const dk_core_args& a, int64_t off
 #pragma unroll
   for (int J = 0; J <= IB; ++J) accB[J] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
   double bacc[4] = {0.0, 0.0, 0.0, 0.0};
-  f32x4_t raw[8];
+  f32x4_t rawA[8], rawB[8];  // rows of the even and the odd blocks in flight
 
-  // obs metadata of block blk into buffer buf (threads 0-63); obs past p
-  // take the index of obs 0 of the analysis and zero weights
-  // obs metadata of block blk, threads 0-63, in two steps so that their
+  // obs metadata of block blk, threads 0-63 (obs past p take the index of
+  // obs 0 of the analysis and zero weights), in two steps so that their
   // latencies hide behind the block being multiplied and split: the index
   // (meta_idx), then R^-1, rloc and d (meta_vals), then the LDS stores
   struct Meta {
@@ -336,27 +335,29 @@ __device__ __forceinline__ void gram_h16_rows(const dk_core_args& a, int64_t off
     st.rr[buf][tid] = sqrt(r2);
     st.w[buf][tid] = r2 * m.dp;
   };
-  auto meta = [&](int blk, int buf) {
+  auto meta = [&](int blk) {
     if (tid < 64) {
       Meta m;
       meta_idx(blk, m);
       meta_vals(m);
-      meta_store(m, buf);
+      meta_store(m, blk & 3);
     }
   };
-  auto load = [&](int buf) {
+  auto load = [&](int blk, f32x4_t (&raw)[8]) {
+    const int ms = blk & 3;
 #pragma unroll
     for (int e = 0; e < 8; ++e)
       raw[e] = rows_ok ? *reinterpret_cast<const f32x4_t*>(
-                             a.yb32 + (int64_t)st.gi[buf][og * 8 + e] * a.ldy32 + 4 * rq)
+                             a.yb32 + (int64_t)st.gi[ms][og * 8 + e] * a.ldy32 + 4 * rq)
                        : f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
   };
-  auto split = [&](int buf) {
+  auto split = [&](int blk, const f32x4_t (&raw)[8]) {
+    const int ms = blk & 3, buf = blk & 1;
     float rs[8];
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
-      const double w = st.w[buf][og * 8 + e];
-      rs[e] = (float)st.rr[buf][og * 8 + e];
+      const double w = st.w[ms][og * 8 + e];
+      rs[e] = (float)st.rr[ms][og * 8 + e];
 #pragma unroll
       for (int r = 0; r < 4; ++r) bacc[r] = fma((double)raw[e][r], w, bacc[r]);
     }
@@ -374,7 +375,8 @@ __device__ __forceinline__ void gram_h16_rows(const dk_core_args& a, int64_t off
       if (kSplit) st.lo[buf][og][4 * rq + r] = lv;
     }
   };
-  auto mma = [&](int buf) {
+  auto mma = [&](int blk) {
+    const int buf = blk & 1;
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
       hvec fh[NF], fl[NF];
@@ -401,26 +403,33 @@ __device__ __forceinline__ void gram_h16_rows(const dk_core_args& a, int64_t off
       }
     }
   };
+  // one step: block blk is multiplied, blk + 1 split (its rows arrived
+  // during the previous step, in rs), the rows of blk + 2 requested (into
+  // rl) and the metadata of blk + 3 staged
+  auto pipe_step = [&](int blk, f32x4_t (&rl)[8], const f32x4_t (&rs)[8]) {
+    const bool more = tid < 64 && blk + 3 < nblk;
+    Meta m;
+    if (more) meta_idx(blk + 3, m);
+    if (blk + 2 < nblk) load(blk + 2, rl);
+    __builtin_amdgcn_sched_barrier(0);
+    mma(blk);
+    if (more) meta_vals(m);
+    if (blk + 1 < nblk) split(blk + 1, rs);
+    __syncthreads();
+    if (more) meta_store(m, (blk + 3) & 3);
+    __syncthreads();
+  };
   if (nblk > 0) {
-    meta(0, 0);
+    for (int b = 0; b < 3 && b < nblk; ++b) meta(b);
     __syncthreads();
-    load(0);
-    split(0);
-    if (nblk > 1) meta(1, 1);
+    load(0, rawA);
+    if (nblk > 1) load(1, rawB);
+    split(0, rawA);
     __syncthreads();
-    for (int blk = 0; blk < nblk; ++blk) {
-      const int buf = blk & 1;
-      const bool more = tid < 64 && blk + 2 < nblk;
-      Meta m;
-      if (more) meta_idx(blk + 2, m);
-      if (blk + 1 < nblk) load(buf ^ 1);
-      __builtin_amdgcn_sched_barrier(0);
-      mma(buf);
-      if (more) meta_vals(m);
-      if (blk + 1 < nblk) split(buf ^ 1);
-      __syncthreads();
-      if (more) meta_store(m, buf);
-      __syncthreads();
+    for (int blk = 0; blk < nblk; blk += 2) {
+      pipe_step(blk, rawA, rawB);
+      if (blk + 1 >= nblk) break;
+      pipe_step(blk + 1, rawB, rawA);
     }
   }
   // D[4g + r][c] of tile (I, J) = C(I*16 + 4g + r, J*16 + c)
