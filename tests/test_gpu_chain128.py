@@ -5,7 +5,8 @@ Bars: per-variable increment and Xa' relative error against the oracle
 <= 1e-10 on the FP64 Gram, for columns (MAX_OBS binding), horizontal cubes
 with overlap, and a workspace smaller than the sweep's lists; <= 1e-4 on the
 BF16x3 Gram (clustered obs with R spanning 0.5-3, and MAX_OBS binding);
-fewer Jacobi sweeps than the plain form's cold starts."""
+fewer Jacobi sweeps than the plain form's cold starts; captured-graph
+replays bit-identical to the eager sweep."""
 import numpy as np
 import pytest
 
@@ -108,3 +109,36 @@ def test_chained_batched_k128_split_gram(cuda, dl, oracle, case):
     if case.endswith("capped"):
         assert st["nobsl_capped"] > 0
     _check(oracle, data, params, xa, axa, tol=1e-4)
+
+
+def test_chained_graph_replays(cuda, dl):
+    """The chained form captured in a HIP graph: every replay reproduces the
+    eager sweep bit for bit (the queue is dynamic, but a chain's analyses and
+    warm starts do not depend on the workgroup that takes it)."""
+    torch = cuda
+    print("chained graph replays", flush=True)
+    cfg = dl.synth_cfg(16, 32, 2, 100, 1200, seed=0xD1DA0800, cluster_frac=0.5, nclusters=2)
+    data = dl.synth_generate(cfg)
+    params = dl.default_params(nanals=100, nrow=7, nlat=cfg.nlat, nlon=cfg.nlon,
+                               local_dist=1200.0, bin_deg=4.0, form=dl.FORM_BATCHED,
+                               gram=dl.GRAM_BF16X3, chain_len=4)
+    eng = dl.Engine(params, nobs_max=cfg.nobs)
+    eng.set_grid(data["grid_lat"], data["grid_lon"])
+    obs = dl.DeviceObs(data["lat"], data["lon"], data["yb"], data["dep"], data["rinv"])
+    xa, axa = torch.empty_like(data["xb"]), torch.empty_like(data["axb"])
+    eng.sweep(obs, data["xb"], data["axb"], xa, axa)
+    torch.cuda.synchronize()
+    s = torch.cuda.Stream()
+    xa2, axa2 = torch.zeros_like(xa), torch.zeros_like(axa)
+    with torch.cuda.stream(s):
+        eng.graph_capture(obs, data["xb"], data["axb"], xa2, axa2, s)
+    s.synchronize()
+    for rep in range(3):
+        xa2.zero_()
+        axa2.zero_()
+        torch.cuda.synchronize()
+        eng.graph_launch(s)
+        s.synchronize()
+        _ = (data["xb"] * 3).sum().item()
+        assert torch.equal(xa2, xa) and torch.equal(axa2, axa), rep
+    eng.close()
