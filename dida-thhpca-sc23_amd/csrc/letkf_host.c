@@ -816,9 +816,14 @@ static int plain_transform(dida_letkf_ctx *c, int64_t a0, int64_t a1, int64_t ba
     ua.xa32 = c->xa32;
     ua.rtps = c->prm.rtps;
     ua.rtps_alpha = c->prm.rtps_alpha;
+    /* chained analyses: launches start at multiples of the chain length, so
+     * every chain is the same whichever runs and launches a sweep is cut in */
+    const int64_t chunk = c->chained && c->chunk >= c->chain_len
+                              ? c->chunk / c->chain_len * c->chain_len
+                              : c->chunk;
     if (c->prm.cube_n > 1) {
-        for (int64_t c0 = a0; c0 < a1; c0 += c->chunk) {
-            int32_t n = (int32_t)((a1 - c0) < c->chunk ? (a1 - c0) : c->chunk);
+        for (int64_t c0 = a0; c0 < a1; c0 += chunk) {
+            int32_t n = (int32_t)((a1 - c0) < chunk ? (a1 - c0) : chunk);
             ca.aid0 = (int32_t)c0;
             ca.nanal = n;
             ca.trans = c->trans_all + c0 * k * k;
@@ -831,8 +836,8 @@ static int plain_transform(dida_letkf_ctx *c, int64_t a0, int64_t a1, int64_t ba
         }
         return DIDA_OK;
     }
-    for (int64_t c0 = a0; c0 < a1; c0 += c->chunk) {
-        int32_t n = (int32_t)((a1 - c0) < c->chunk ? (a1 - c0) : c->chunk);
+    for (int64_t c0 = a0; c0 < a1; c0 += chunk) {
+        int32_t n = (int32_t)((a1 - c0) < chunk ? (a1 - c0) : chunk);
         ca.aid0 = (int32_t)c0;
         ca.nanal = n;
         HIP_TRY(core_launch(c, &ca, s));
@@ -876,7 +881,12 @@ static int sweep_plain_chunked(dida_letkf_ctx *c, dk_select_args *sa, const doub
             if (off[mid] - off[a0] <= cap) lo = mid;
             else hi = mid - 1;
         }
-        const int64_t a1 = lo;
+        int64_t a1 = lo;
+        if (c->chained && a1 < n) {
+            /* end the run at a chain boundary when a whole chain fits */
+            const int64_t al = a1 / c->chain_len * c->chain_len;
+            if (al > a0) a1 = al;
+        }
         sa->col0 = a0;
         sa->ncols = a1 - a0;
         sa->base = off[a0];

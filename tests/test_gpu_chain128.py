@@ -69,6 +69,9 @@ def test_chained_batched_k128(cuda, dl, oracle, case):
     assert st["status"] == 0 and st["nonconverged"] == 0, st
     if case == "k128_capped":
         assert st["nobsl_capped"] > 0
+    if cap:
+        # runs start at chain boundaries: bit-identical to one whole-workspace sweep
+        assert torch.equal(xa, x) and torch.equal(axa, ax)
     _check(oracle, data, params, xa, axa, *cube)
     if case == "k100_chain5":
         # the plain form's cold starts take more sweeps
@@ -82,10 +85,11 @@ def test_chained_batched_k128(cuda, dl, oracle, case):
         assert st["sweeps_sum"] < s2["sweeps_sum"], (st["sweeps_sum"], s2["sweeps_sum"])
 
 
-@pytest.mark.parametrize("case", ["k128_bf16x3", "k100_bf16x3_capped"])
+@pytest.mark.parametrize("case", ["k128_bf16x3", "k100_bf16x3_capped", "k128_bf16x3_chunks"])
 def test_chained_batched_k128_split_gram(cuda, dl, oracle, case):
     """The BF16x3 Gram (split hi/lo on BF16 MFMA, FP32 accumulation) in the
-    chained form, against the FP64 oracle."""
+    chained form, against the FP64 oracle; also with a workspace that holds a
+    third of the sweep's selection lists (runs of analyses)."""
     torch = cuda
     print(f"chained batched {case}", flush=True)
     k = 128 if case.startswith("k128") else 100
@@ -99,7 +103,15 @@ def test_chained_batched_k128_split_gram(cuda, dl, oracle, case):
         p["max_obs"] = 300
     params = dl.default_params(**p)
     obs = dl.DeviceObs(data["lat"], data["lon"], data["yb"], data["dep"], data["rinv"])
-    eng = dl.Engine(params, nobs_max=cfg.nobs)
+    cap = 0
+    if case.endswith("chunks"):
+        probe = dl.Engine(params, nobs_max=cfg.nobs)
+        probe.set_grid(data["grid_lat"], data["grid_lon"])
+        x, ax = torch.empty_like(data["xb"]), torch.empty_like(data["axb"])
+        probe.sweep(obs, data["xb"], data["axb"], x, ax)
+        cap = probe.stats()["nobsl_sum"] // 3 + 1
+        probe.close()
+    eng = dl.Engine(params, nobs_max=cfg.nobs, capacity=cap)
     eng.set_grid(data["grid_lat"], data["grid_lon"])
     xa, axa = torch.empty_like(data["xb"]), torch.empty_like(data["axb"])
     eng.sweep(obs, data["xb"], data["axb"], xa, axa)
@@ -108,6 +120,9 @@ def test_chained_batched_k128_split_gram(cuda, dl, oracle, case):
     assert st["status"] == 0 and st["nonconverged"] == 0, st
     if case.endswith("capped"):
         assert st["nobsl_capped"] > 0
+    if cap:
+        # the runs reproduce one whole-workspace sweep bit for bit
+        assert torch.equal(xa, x) and torch.equal(axa, ax)
     _check(oracle, data, params, xa, axa, tol=1e-4)
 
 

@@ -27,11 +27,13 @@ def _oracle(oracle, h, params, cube_n, overlap):
     return out
 
 
-@pytest.mark.parametrize("case", ["plain_k40", "plain_k100_cubes", "chained_k128_bf16x3"])
+@pytest.mark.parametrize("case", ["plain_k40", "plain_k100_cubes", "chained_k128_bf16x3",
+                                  "plain_k128_chunks"])
 def test_sweep_f32_state(cuda, dl, oracle, case):
     torch = cuda
     print(f"fp32 state {case}", flush=True)
-    k = {"plain_k40": 40, "plain_k100_cubes": 100, "chained_k128_bf16x3": 128}[case]
+    k = {"plain_k40": 40, "plain_k100_cubes": 100, "chained_k128_bf16x3": 128,
+         "plain_k128_chunks": 128}[case]
     cfg = dl.synth_cfg(20, 40, 2, k, 1500, seed=0xD1DA0700 + len(case), cluster_frac=0.5,
                        nclusters=2, rscale=(0.5, 3.0))
     data = dl.synth_generate(cfg)
@@ -46,9 +48,17 @@ def test_sweep_f32_state(cuda, dl, oracle, case):
         tol_inc = tol_xa = 1e-4
     params = dl.default_params(**p)
     xb32 = data["xb"].float()
-    eng = dl.Engine(params, nobs_max=cfg.nobs)
-    eng.set_grid(data["grid_lat"], data["grid_lon"])
     obs = dl.DeviceObs(data["lat"], data["lon"], data["yb"], data["dep"], data["rinv"])
+    cap = 0
+    if case.endswith("chunks"):
+        # a workspace of a third of the lists: the sweep runs in runs of analyses
+        probe = dl.Engine(params, nobs_max=cfg.nobs)
+        probe.set_grid(data["grid_lat"], data["grid_lon"])
+        probe.sweep(obs, xb32, data["axb"], torch.empty_like(xb32), torch.empty_like(data["axb"]))
+        cap = probe.stats()["nobsl_sum"] // 3 + 1
+        probe.close()
+    eng = dl.Engine(params, nobs_max=cfg.nobs, capacity=cap)
+    eng.set_grid(data["grid_lat"], data["grid_lon"])
     xa32 = torch.empty_like(xb32)
     axa = torch.empty_like(data["axb"])
     eng.sweep(obs, xb32, data["axb"], xa32, axa)
