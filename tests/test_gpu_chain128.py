@@ -85,14 +85,15 @@ def test_chained_batched_k128(cuda, dl, oracle, case):
         assert st["sweeps_sum"] < s2["sweeps_sum"], (st["sweeps_sum"], s2["sweeps_sum"])
 
 
-@pytest.mark.parametrize("case", ["k128_bf16x3", "k100_bf16x3_capped", "k128_bf16x3_chunks"])
+@pytest.mark.parametrize("case", ["k128_bf16x3", "k100_bf16x3_capped", "k128_bf16x3_chunks",
+                                  "k65_bf16x3"])
 def test_chained_batched_k128_split_gram(cuda, dl, oracle, case):
     """The BF16x3 Gram (split hi/lo on BF16 MFMA, FP32 accumulation) in the
     chained form, against the FP64 oracle; also with a workspace that holds a
     third of the sweep's selection lists (runs of analyses)."""
     torch = cuda
     print(f"chained batched {case}", flush=True)
-    k = 128 if case.startswith("k128") else 100
+    k = int(case[1:].split("_")[0])
     cfg = dl.synth_cfg(20, 40, 2, k, 2000, seed=0xD1DA0600 + len(case), cluster_frac=0.7,
                        nclusters=3, rscale=(0.5, 3.0))
     data = dl.synth_generate(cfg)
