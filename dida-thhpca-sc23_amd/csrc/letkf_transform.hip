@@ -267,11 +267,13 @@ __device__ __forceinline__ void gram_f64_obs(const dk_core_args& a, int64_t off,
 // hi lo^T + lo hi^T (hi hi^T alone for GRAM = BF16) on 16x16x32 MFMA with
 // FP32 accumulation; b = Yb^T (R^-1 rloc d) accumulated in FP64.
 // Blocks of 64 obs are converted once by the whole workgroup into LDS, in
-// the MFMA operand layout, while the previous block is multiplied: thread
-// (rq = tid % 32, og = tid / 32) loads rows 4 rq .. 4 rq + 3 of the 8 obs
-// of octet og (one 16-byte load each) and writes their four 16-byte hi and
-// lo vectors.  The staging area aliases A, which is free until C is
-// written.  Wave W owns the lower tiles of tile rows W and NT-1-W.
+// the MFMA operand layout: while block i is multiplied, block i + 1 is split
+// and the rows of block i + 2 are requested (the obs metadata runs one block
+// further ahead).  Thread (rq = tid % 32, og = tid / 32) loads rows
+// 4 rq .. 4 rq + 3 of the 8 obs of octet og (one 16-byte load each) and
+// writes their four 16-byte hi and lo vectors.  The staging area aliases A,
+// which is free until C is written.  Wave W owns the lower tiles of tile
+// rows W and NT-1-W.
 template <int KP>
 struct H16Stage {
   bf16x8_t hi[2][8][KP];  // [block & 1][obs octet][row]: obs 8 o .. 8 o + 7
@@ -294,9 +296,7 @@ __device__ __forceinline__ void gram_h16_rows(const dk_core_args& a, int64_t off
   static_assert(sizeof(H16Stage<KP>) <= sizeof(sm.A), "staging fits in A");
   H16Stage<KP>& st = *reinterpret_cast<H16Stage<KP>*>(sm.A);
   const int tid = threadIdx.x, lane = tid & 63, g = lane >> 4, c = lane & 15;
-  // staging: thread (rq = tid % 32, og = tid / 32) converts rows 4 rq .. 4 rq + 3
-  // of obs 8 og .. 8 og + 7 of a block (one 16-byte load per obs)
-  const int rq = tid & 31, og = tid >> 5;
+  const int rq = tid & 31, og = tid >> 5;  // staging rows 4 rq.., obs octet og
   const int k = a.k;
   const bool rows_ok = 4 * rq < k;
   const int nblk = (p + 63) / 64;
