@@ -608,7 +608,8 @@ def main():
 
     layout = r["tuning"]
     if layout is None and head != "plain_fp64":
-        layout = {"chain_len": args.chain_len or 8, "xcd_tiles": 0}   # library defaults
+        layout = {"chain_len": args.chain_len or (32 if cfg.k > 64 else 8),
+                  "xcd_tiles": 0}   # library defaults
     roofline = kernel_roofline(cfg, st, head, r["stages"]["transform"], args.config,
                                args.scale, layout)
 

@@ -317,7 +317,10 @@ int dida_letkf_create(dida_letkf_ctx **out, const dida_letkf_params *prm, int32_
         c->nbins_max = g0.nbins > g1.nbins ? g0.nbins : g1.nbins;
     }
     c->nchain_max = (int32_t)c->nsel;
-    rc = set_layout(c, bd, prm->chain_len > 0 ? prm->chain_len : 8, prm->xcd_tiles);
+    /* automatic chain length: 8 columns for the fused kernel (its autotuner
+     * refines it), 32 analyses for the chained workgroups (measured best) */
+    const int32_t cl0 = prm->form == DIDA_FORM_BATCHED && k > 64 ? 32 : 8;
+    rc = set_layout(c, bd, prm->chain_len > 0 ? prm->chain_len : cl0, prm->xcd_tiles);
     if (rc) {
         free(c);
         return rc;
@@ -374,7 +377,10 @@ int dida_letkf_create(dida_letkf_ctx **out, const dida_letkf_params *prm, int32_
             DMALLOC(c->trans_all, (size_t)c->nsel * k * k * sizeof(double));
     }
     if (prm->form == DIDA_FORM_PLAIN || c->chained) {
-        c->chunk = (int32_t)(c->nsel < 32768 ? c->nsel : 32768);
+        /* analyses per transform launch; the chained form takes more, since
+         * each launch ends with a tail of whole chains on few workgroups */
+        const int64_t cmax = c->chained ? 131072 : 32768;
+        c->chunk = (int32_t)(c->nsel < cmax ? c->nsel : cmax);
         if (prm->cube_n == 1) DMALLOC(c->trans, (size_t)c->chunk * k * k * sizeof(double));
     }
     if (prm->form == DIDA_FORM_BATCHED) {

@@ -138,8 +138,8 @@ typedef struct dida_letkf_stats {
 typedef struct dida_letkf_ctx dida_letkf_ctx;
 
 /* Defaults: max_obs 10000, gc_filter 1, obs_polar 90, rho 1, plain FP64,
- * automatic Jacobi tolerance, 30 sweeps, automatic bins, chains of 8 with
- * warm start (batched form). */
+ * automatic Jacobi tolerance, 30 sweeps, automatic bins, automatic chains
+ * (8 columns, or 32 analyses for k > 64) with warm start (batched form). */
 void dida_letkf_params_default(dida_letkf_params *prm);
 
 /* ------------------------------------------------------------------------ */
