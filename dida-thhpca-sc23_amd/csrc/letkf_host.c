@@ -768,7 +768,15 @@ static int core_launch(dida_letkf_ctx *c, dk_core_args *ca, hipStream_t s)
     ch.head = c->queue_head;
     ch.vscr = c->vscr;
     ch.phase_clk = NULL;
-    int e = dk_fill_i32(c->queue_head, 0, 1, s);
+    /* chains drained most expensive first, so the launch ends on cheap ones */
+    int e = dk_chain_costs(ca, ch.chain_len, ch.nchain, c->chain_key, c->chain_id, s);
+    if (e) return e;
+    size_t tb = c->chain_tmp_bytes;
+    e = dk_sort_pairs_u32(c->chain_tmp, &tb, c->chain_key, c->chain_key_sorted, c->chain_id,
+                          c->chain_order, ch.nchain, 32, s);
+    if (e) return e;
+    ch.order = c->chain_order;
+    e = dk_fill_i32(c->queue_head, 0, 1, s);
     if (e) return e;
     if ((c->timing & 2) && c->phase_clk) {
         e = (int)hipMemsetAsync(c->phase_clk, 0, DK_NPHASE * sizeof(unsigned long long), s);

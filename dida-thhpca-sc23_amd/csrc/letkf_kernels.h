@@ -131,7 +131,13 @@ typedef struct dk_chain_args {
     double *vscr;
     unsigned long long *phase_clk; /* optional [DK_NPHASE] device clock sums
                                       (gram, warm, jacobi, transform) */
+    const int32_t *order;          /* optional: chain taken at queue position q */
 } dk_chain_args;
+/* Sort keys of the chains of one chained launch: 2^32-1 - (sum of the local
+ * obs counts of the chain's analyses + 10 k each), so that an ascending
+ * sort drains the most expensive chains first. */
+int dk_chain_costs(const dk_core_args *a, int32_t chain_len, int32_t nchain, uint32_t *key,
+                   int32_t *id, void *stream);
 int dk_letkf_chain_f64(const dk_core_args *a, const dk_chain_args *ch, int nblocks,
                        void *stream);
 

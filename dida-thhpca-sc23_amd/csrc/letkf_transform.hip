@@ -688,7 +688,7 @@ __global__ __launch_bounds__(kThreads) void k_letkf_chain_f64(dk_core_args a, dk
     const int item = item_s;
     __syncthreads();
     if (item >= ch.nchain) break;
-    const int c0 = a.aid0 + item * ch.chain_len;
+    const int c0 = a.aid0 + (ch.order ? ch.order[item] : item) * ch.chain_len;
     const int c1 = min(c0 + ch.chain_len, a.aid0 + a.nanal);
     for (int aid = c0; aid < c1; ++aid)
       analysis_f64<KP, 4, GRAM>(a, aid, sm, aid > c0 ? vs : nullptr,
@@ -696,7 +696,31 @@ __global__ __launch_bounds__(kThreads) void k_letkf_chain_f64(dk_core_args a, dk
   }
 }
 
+__global__ void k_chain_costs(dk_core_args a, int32_t chain_len, int32_t nchain, uint32_t* key,
+                              int32_t* id) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nchain) return;
+  const int c0 = a.aid0 + q * chain_len;
+  const int c1 = min(c0 + chain_len, a.aid0 + a.nanal);
+  double cost = 0.0;
+  for (int aid = c0; aid < c1; ++aid) {
+    const int p = a.nobsl[aid];
+    cost += (double)(p < a.max_p ? p : a.max_p) + 10.0 * a.k;
+  }
+  key[q] = 0xFFFFFFFFu - (uint32_t)fmin(cost, 4.0e9);
+  id[q] = q;
+}
+
 }  // namespace
+
+extern "C" int dk_chain_costs(const dk_core_args* a, int32_t chain_len, int32_t nchain,
+                              uint32_t* key, int32_t* id, void* stream) {
+  if (nchain <= 0) return 0;
+  if (chain_len < 1) return (int)hipErrorInvalidValue;
+  k_chain_costs<<<(nchain + 255) / 256, 256, 0, (hipStream_t)stream>>>(*a, chain_len, nchain,
+                                                                       key, id);
+  return (int)hipGetLastError();
+}
 
 extern "C" int dk_letkf_chain_f64(const dk_core_args* a, const dk_chain_args* ch, int nblocks,
                                   void* stream) {
