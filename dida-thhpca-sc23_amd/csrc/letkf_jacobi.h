@@ -204,17 +204,27 @@ struct Jacobi128 {
     isc[P] *= ic;
     isc[Q] *= ic;
   }
+  // Lanes 0-3 of a group take the angle of the first pair, lanes 4-7 that
+  // of the second: the quad sums of both dot products are exchanged across
+  // the halves (row_half_mirror pairs lane i with 7 - i), each half finishes
+  // its own pair's sum and angle, and the angles come back the same way.
   template <int T0, int B0, int T1, int B1>
   __device__ __forceinline__ void round(double tol2, double tolp, bool& rotated, bool& big) {
-    const double g0 = oct_sum(dot<T0, B0>());
-    const double g1 = oct_sum(dot<T1, B1>());
-    double t0, gm0, cs0, ic0, t1, gm1, cs1, ic1;
-    rotation_angle(g0, nrm[T0], nrm[B0], sc[T0], sc[B0], tol2, tolp, rotated, big, t0, gm0,
-                   cs0, ic0);
-    rotation_angle(g1, nrm[T1], nrm[B1], sc[T1], sc[B1], tol2, tolp, rotated, big, t1, gm1,
-                   cs1, ic1);
-    apply<T0, B0>(t0, gm0, cs0, ic0);
-    apply<T1, B1>(t1, gm1, cs1, ic1);
+    double x = dot<T0, B0>(), y = dot<T1, B1>();
+    x = x + dpp_d<0xB1>(x);
+    y = y + dpp_d<0xB1>(y);
+    x = x + dpp_d<0x4E>(x);
+    y = y + dpp_d<0x4E>(y);
+    const bool second = (threadIdx.x & 4) != 0;
+    const double gs = (second ? y : x) + dpp_d<0x141>(second ? x : y);
+    double t, gm, cs, ic;
+    rotation_angle(gs, second ? nrm[T1] : nrm[T0], second ? nrm[B1] : nrm[B0],
+                   second ? sc[T1] : sc[T0], second ? sc[B1] : sc[B0], tol2, tolp, rotated, big,
+                   t, gm, cs, ic);
+    const double to = dpp_d<0x141>(t), gmo = dpp_d<0x141>(gm);
+    const double cso = dpp_d<0x141>(cs), ico = dpp_d<0x141>(ic);
+    apply<T0, B0>(second ? to : t, second ? gmo : gm, second ? cso : cs, second ? ico : ic);
+    apply<T1, B1>(second ? t : to, second ? gm : gmo, second ? cs : cso, second ? ic : ico);
   }
   // slot J takes slot J of the group whose lanes sit at byte address addr
   template <int J>
