@@ -146,8 +146,8 @@ __device__ int jacobi_lds(double* A, double* nrm, int* flag, double tol, int max
 // slots 1 and 3 of the first half of every block of G/4 groups swap with
 // slots 0 and 2 of the partner group; three final rounds pair the four
 // slots of each group.  Moves along group bits 0-2 stay in the wavefront
-// (ds_bpermute), the six a sweep along bits 3-4 go through LDS, which
-// aliases A (the matrix is in registers meanwhile).  Every column returns to
+// (DPP and lane-swap VALU ops), the six a sweep along bits 3-4 go through
+// LDS, which aliases A (the matrix is in registers meanwhile).  Every column returns to
 // A at the index it came from, so zero padding columns stay at >= k.
 struct JacobiXbuf {
   static constexpr int LD = 132;  // column stride (bank spread)
@@ -168,6 +168,28 @@ __device__ __forceinline__ double oct_sum(double v) {
   v = v + dpp_d<0xB1>(v);
   v = v + dpp_d<0x4E>(v);
   return v + dpp_d<0x141>(v);
+}
+
+// v of lane l ^ V (V = 8: DPP row_ror:8 inside each 16-lane row; V = 16, 32:
+// v_permlane16/32_swap of v with itself, whose first result holds the value
+// of the lower row (lanes l & ~V) and the second that of the upper one)
+template <int V>
+__device__ __forceinline__ unsigned lane_xor_u(unsigned v) {
+  if constexpr (V == 8) {
+    return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x128, 0xF, 0xF, false);
+  } else if constexpr (V == 16) {
+    const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    return (threadIdx.x & 16) ? r[0] : r[1];
+  } else {
+    static_assert(V == 32, "lane xor of 8, 16 or 32");
+    const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    return (threadIdx.x & 32) ? r[0] : r[1];
+  }
+}
+template <int V>
+__device__ __forceinline__ double lane_xor(double v) {
+  return __hiloint2double((int)lane_xor_u<V>((unsigned)__double2hiint(v)),
+                          (int)lane_xor_u<V>((unsigned)__double2loint(v)));
 }
 
 struct Jacobi128 {
@@ -226,15 +248,15 @@ struct Jacobi128 {
     apply<T0, B0>(second ? to : t, second ? gmo : gm, second ? cso : cs, second ? ico : ic);
     apply<T1, B1>(second ? t : to, second ? gm : gmo, second ? cs : cso, second ? ic : ico);
   }
-  // slot J takes slot J of the group whose lanes sit at byte address addr
-  template <int J>
-  __device__ __forceinline__ void move_lane(int addr) {
+  // slot J takes slot J of the group V / 8 groups away (lanes l ^ V)
+  template <int J, int V>
+  __device__ __forceinline__ void move_lane() {
 #pragma unroll
-    for (int u = 0; u < NR; ++u) X[J][u] = bperm_d(addr, X[J][u]);
-    nrm[J] = bperm_d(addr, nrm[J]);
-    sc[J] = bperm_d(addr, sc[J]);
-    isc[J] = bperm_d(addr, isc[J]);
-    id[J] = __builtin_amdgcn_ds_bpermute(addr, id[J]);
+    for (int u = 0; u < NR; ++u) X[J][u] = lane_xor<V>(X[J][u]);
+    nrm[J] = lane_xor<V>(nrm[J]);
+    sc[J] = lane_xor<V>(sc[J]);
+    isc[J] = lane_xor<V>(isc[J]);
+    id[J] = (int)lane_xor_u<V>((unsigned)id[J]);
   }
   // the column of slot J (q: 0 or 1 of the group's two exchange places) to
   // the exchange area, and back from group src
@@ -259,11 +281,16 @@ struct Jacobi128 {
     id[J] = xb.id[2 * src + q];
   }
   // slots 1 and 3 from group gp ^ v
-  __device__ __forceinline__ void move_bottoms(JacobiXbuf& xb, int lane, int gp, int sub, int v) {
-    if (v < 8) {
-      const int addr = (lane ^ (8 * v)) << 2;
-      move_lane<1>(addr);
-      move_lane<3>(addr);
+  __device__ __forceinline__ void move_bottoms(JacobiXbuf& xb, int gp, int sub, int v) {
+    if (v == 1) {
+      move_lane<1, 8>();
+      move_lane<3, 8>();
+    } else if (v == 2) {
+      move_lane<1, 16>();
+      move_lane<3, 16>();
+    } else if (v == 4) {
+      move_lane<1, 32>();
+      move_lane<3, 32>();
     } else {
       put<1>(xb, gp, sub, 0);
       put<3>(xb, gp, sub, 1);
@@ -274,35 +301,40 @@ struct Jacobi128 {
     }
   }
   // first: slot B takes the partner's slot T; otherwise slot T takes the
-  // partner's slot B
-  template <int T, int B>
-  __device__ __forceinline__ void xswap_lane(int addr, bool first) {
+  // partner's slot B (partner: lanes l ^ V)
+  template <int T, int B, int V>
+  __device__ __forceinline__ void xswap_lane(bool first) {
 #pragma unroll
     for (int u = 0; u < NR; ++u) {
-      const double w = bperm_d(addr, first ? X[B][u] : X[T][u]);
+      const double w = lane_xor<V>(first ? X[B][u] : X[T][u]);
       X[B][u] = first ? w : X[B][u];
       X[T][u] = first ? X[T][u] : w;
     }
-    double w = bperm_d(addr, first ? nrm[B] : nrm[T]);
+    double w = lane_xor<V>(first ? nrm[B] : nrm[T]);
     nrm[B] = first ? w : nrm[B];
     nrm[T] = first ? nrm[T] : w;
-    w = bperm_d(addr, first ? sc[B] : sc[T]);
+    w = lane_xor<V>(first ? sc[B] : sc[T]);
     sc[B] = first ? w : sc[B];
     sc[T] = first ? sc[T] : w;
-    w = bperm_d(addr, first ? isc[B] : isc[T]);
+    w = lane_xor<V>(first ? isc[B] : isc[T]);
     isc[B] = first ? w : isc[B];
     isc[T] = first ? isc[T] : w;
-    const int wi = __builtin_amdgcn_ds_bpermute(addr, first ? id[B] : id[T]);
+    const int wi = (int)lane_xor_u<V>((unsigned)(first ? id[B] : id[T]));
     id[B] = first ? wi : id[B];
     id[T] = first ? id[T] : wi;
   }
   // (0, 1) and (2, 3) swapped with group gp ^ d
-  __device__ __forceinline__ void xswap(JacobiXbuf& xb, int lane, int gp, int sub, int d) {
+  __device__ __forceinline__ void xswap(JacobiXbuf& xb, int gp, int sub, int d) {
     const bool first = (gp & d) == 0;
-    if (d < 8) {
-      const int addr = (lane ^ (8 * d)) << 2;
-      xswap_lane<0, 1>(addr, first);
-      xswap_lane<2, 3>(addr, first);
+    if (d == 1) {
+      xswap_lane<0, 1, 8>(first);
+      xswap_lane<2, 3, 8>(first);
+    } else if (d == 2) {
+      xswap_lane<0, 1, 16>(first);
+      xswap_lane<2, 3, 16>(first);
+    } else if (d == 4) {
+      xswap_lane<0, 1, 32>(first);
+      xswap_lane<2, 3, 32>(first);
     } else {
       if (first) {
         put<1>(xb, gp, sub, 0);
@@ -376,12 +408,12 @@ __device__ int jacobi_regs128(double* A, int* flag, double tol, int max_sweeps, 
       for (int r = 0; r < G; r += 4) {
         j.round<0, 1, 2, 3>(tol2, tolp, rotated, big);
         j.round<0, 3, 2, 1>(tol2, tolp, rotated, big);
-        j.move_bottoms(xb, lane, gp, sub, 1);
+        j.move_bottoms(xb, gp, sub, 1);
         j.round<0, 3, 2, 1>(tol2, tolp, rotated, big);
         j.round<0, 1, 2, 3>(tol2, tolp, rotated, big);
-        if (r + 4 < G) j.move_bottoms(xb, lane, gp, sub, ((r + 4) & -(r + 4)) >> 1);
+        if (r + 4 < G) j.move_bottoms(xb, gp, sub, ((r + 4) & -(r + 4)) >> 1);
       }
-      j.xswap(xb, lane, gp, sub, G >> 2);
+      j.xswap(xb, gp, sub, G >> 2);
     }
     j.round<0, 1, 2, 3>(tol2, tolp, rotated, big);
     j.round<0, 3, 2, 1>(tol2, tolp, rotated, big);
