@@ -268,7 +268,7 @@ __device__ __forceinline__ void gram_f64_obs(const dk_core_args& a, int64_t off,
 // FP32 accumulation; b = Yb^T (R^-1 rloc d) accumulated in FP64.
 // Blocks of 64 obs are converted once by the whole workgroup into LDS, in
 // the MFMA operand layout: while block i is multiplied, block i + 1 is split
-// and the rows of block i + 2 are requested (the obs metadata runs one block
+// and the rows of block i + 2 are requested (the obs metadata runs two blocks
 // further ahead).  Thread (rq = tid % 32, og = tid / 32) loads rows
 // 4 rq .. 4 rq + 3 of the 8 obs of octet og (one 16-byte load each) and
 // writes their four 16-byte hi and lo vectors.  The staging area aliases A,
@@ -405,22 +405,23 @@ __device__ __forceinline__ void gram_h16_rows(const dk_core_args& a, int64_t off
   };
   // one step: block blk is multiplied, blk + 1 split (its rows arrived
   // during the previous step, in rs), the rows of blk + 2 requested (into
-  // rl) and the metadata of blk + 3 staged
+  // rl) and the metadata of blk + 4 staged in the slot of block blk, which
+  // no step reads any more; it is first read two steps later, so one
+  // barrier a step orders everything
   auto pipe_step = [&](int blk, f32x4_t (&rl)[8], const f32x4_t (&rs)[8]) {
-    const bool more = tid < 64 && blk + 3 < nblk;
+    const bool more = tid < 64 && blk + 4 < nblk;
     Meta m;
-    if (more) meta_idx(blk + 3, m);
+    if (more) meta_idx(blk + 4, m);
     if (blk + 2 < nblk) load(blk + 2, rl);
     __builtin_amdgcn_sched_barrier(0);
     mma(blk);
     if (more) meta_vals(m);
     if (blk + 1 < nblk) split(blk + 1, rs);
-    __syncthreads();
-    if (more) meta_store(m, (blk + 3) & 3);
+    if (more) meta_store(m, (blk + 4) & 3);
     __syncthreads();
   };
   if (nblk > 0) {
-    for (int b = 0; b < 3 && b < nblk; ++b) meta(b);
+    for (int b = 0; b < 4 && b < nblk; ++b) meta(b);
     __syncthreads();
     load(0, rawA);
     if (nblk > 1) load(1, rawB);
