@@ -89,12 +89,10 @@ static int check_params(const dida_letkf_params *p)
 {
     if (p->nanals < 2 || p->nanals > 128)
         return set_err(DIDA_ERR_UNSUPPORTED, "nanals = %d: supported range is 2..128", p->nanals);
-    if (p->form == DIDA_FORM_BATCHED && p->nanals > 64 &&
-        (p->gram == DIDA_GRAM_FP16X3 || p->vert_dist > 0.0))
+    if (p->form == DIDA_FORM_BATCHED && p->nanals > 64 && p->vert_dist > 0.0)
         return set_err(DIDA_ERR_UNSUPPORTED,
-                       "nanals = %d: the batched form takes k > 64 with the FP64, BF16X3 or "
-                       "BF16 Gram and no vertical localisation only (FP16X3 and vertical: "
-                       "2..64)", p->nanals);
+                       "nanals = %d: the batched form takes k > 64 without vertical "
+                       "localisation only (vertical: 2..64)", p->nanals);
     if (p->nrow < 1 || p->nlat < 1 || p->nlon < 1)
         return set_err(DIDA_ERR_ARG, "grid %d x %d x %d rows is empty", p->nlat, p->nlon, p->nrow);
     if (p->max_obs < 0) return set_err(DIDA_ERR_ARG, "max_obs < 0");
@@ -565,6 +563,8 @@ static int sweep_select_count(dida_letkf_ctx *c, const dida_letkf_obs *o, hipStr
                           c->cx, c->sx, c->yr, c->ux, c->uy, c->uz, c->yb_b, c->yb32_b,
                           c->dep_b, c->rinv_b, s));
     if (c->ncube > 0) HIP_TRY(dk_obs_lnp(o->nobs, c->perm, o->ph, c->obs_lnp, s));
+    if (c->chained && c->prm.gram == DIDA_GRAM_FP16X3)
+        HIP_TRY(dk_obs_ymax32(o->nobs, k, c->ldy, c->yb32_b, c->rinv_b, c->ymax, s));
     tmark(c, DIDA_STAGE_BIN, s);
     memset(sa, 0, sizeof *sa);
     sa->nlat = c->nalat;
@@ -807,6 +807,7 @@ static int plain_transform(dida_letkf_ctx *c, int64_t a0, int64_t a1, int64_t ba
     ca.yb = c->yb_b;
     ca.yb32 = c->yb32_b;
     ca.ldy32 = c->ldy;
+    ca.ymax = c->ymax;
     ca.rinv = c->rinv_b;
     ca.dep = c->dep_b;
     ca.gram = c->chained ? c->prm.gram : DIDA_GRAM_FP64;

@@ -116,6 +116,7 @@ typedef struct dk_core_args {
                                zero past k) */
     int32_t ldy32;
     int32_t pad2_;
+    const double *ymax;     /* FP16X3 (chained): max_e |yb| sqrt(R^-1) per obs index */
 } dk_core_args;
 
 int dk_letkf_core_f64(const dk_core_args *a, void *stream);
@@ -251,9 +252,11 @@ int dk_chain_groups(const dk_batched_args *a, int32_t *order, void *stream);
  * workgroups per CU for (k, gram), and the persistent launch. */
 int dk_wave_blocks_per_cu(int k, int gram);
 int dk_letkf_wave(const dk_batched_args *a, int nblocks, void *stream);
-/* ymax[i] = max_e |yb[i k + e]| sqrt(rinv[i]). */
+/* ymax[i] = max_e |yb[i k + e]| sqrt(rinv[i]); the second form reads FP32 rows. */
 int dk_obs_ymax(int32_t nobs, int32_t k, int32_t ldy, const double *yb, const float *rinv,
                 double *ymax, void *stream);
+int dk_obs_ymax32(int32_t nobs, int32_t k, int32_t ldy, const float *yb, const float *rinv,
+                  double *ymax, void *stream);
 
 /* Vertical localisation inputs (letkf_compute.F90:424-447, obs_make.F90:418-437):
  * cube_lnp[col * ncube + q] = ln of the full-level pressure of level

@@ -115,12 +115,13 @@ __global__ void k_obs_gather_yb(int32_t nobs, int32_t k, int32_t ldy,
   yb_b[t] = e < k ? (T)yb[(int64_t)perm[i] * k + e] : (T)0;
 }
 
-__global__ void k_obs_ymax(int32_t nobs, int32_t k, int32_t ldy, const double* __restrict__ yb,
+template <typename T>
+__global__ void k_obs_ymax(int32_t nobs, int32_t k, int32_t ldy, const T* __restrict__ yb,
                            const float* __restrict__ rinv, double* __restrict__ ymax) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nobs) return;
   double m = 0.0;
-  for (int e = 0; e < k; ++e) m = fmax(m, fabs(yb[(int64_t)i * ldy + e]));
+  for (int e = 0; e < k; ++e) m = fmax(m, fabs((double)yb[(int64_t)i * ldy + e]));
   ymax[i] = m * sqrt((double)rinv[i]);
 }
 
@@ -283,7 +284,16 @@ extern "C" int dk_obs_gather(int32_t nobs, int32_t k, int32_t ldy, const int32_t
 extern "C" int dk_obs_ymax(int32_t nobs, int32_t k, int32_t ldy, const double* yb,
                            const float* rinv, double* ymax, void* stream) {
   if (nobs <= 0) return 0;
-  k_obs_ymax<<<(nobs + 255) / 256, 256, 0, (hipStream_t)stream>>>(nobs, k, ldy, yb, rinv, ymax);
+  k_obs_ymax<double><<<(nobs + 255) / 256, 256, 0, (hipStream_t)stream>>>(nobs, k, ldy, yb,
+                                                                         rinv, ymax);
+  return (int)hipGetLastError();
+}
+
+extern "C" int dk_obs_ymax32(int32_t nobs, int32_t k, int32_t ldy, const float* yb,
+                             const float* rinv, double* ymax, void* stream) {
+  if (nobs <= 0) return 0;
+  k_obs_ymax<float><<<(nobs + 255) / 256, 256, 0, (hipStream_t)stream>>>(nobs, k, ldy, yb,
+                                                                        rinv, ymax);
   return (int)hipGetLastError();
 }
 

@@ -264,7 +264,8 @@ __device__ __forceinline__ void gram_f64_obs(const dk_core_args& a, int64_t off,
 // KP = 128 Gram in half precision (batched form): Yhat = Yb sqrt(R^-1 rloc)
 // in FP32 (Yb read from its FP32 copy, half the bytes of the gather), split
 // as hi = h(Yhat), lo = h(Yhat - hi) (h: BF16 rounding), and C = hi hi^T +
-// hi lo^T + lo hi^T (hi hi^T alone for GRAM = BF16) on 16x16x32 MFMA with
+// hi lo^T + lo hi^T (hi hi^T alone for GRAM = BF16; h: FP16 rounding of a
+// power-of-two scaled Yhat for GRAM = FP16X3) on 16x16x32 MFMA with
 // FP32 accumulation; b = Yb^T (R^-1 rloc d) accumulated in FP64.
 // Blocks of 64 obs are converted once by the whole workgroup into LDS, in
 // the MFMA operand layout: while block i is multiplied, block i + 1 is split
@@ -274,10 +275,10 @@ __device__ __forceinline__ void gram_f64_obs(const dk_core_args& a, int64_t off,
 // writes their four 16-byte hi and lo vectors.  The staging area aliases A,
 // which is free until C is written.  Wave W owns the lower tiles of tile
 // rows W and NT-1-W.
-template <int KP>
+template <int KP, typename HV>
 struct H16Stage {
-  bf16x8_t hi[2][8][KP];  // [block & 1][obs octet][row]: obs 8 o .. 8 o + 7
-  bf16x8_t lo[2][8][KP];
+  HV hi[2][8][KP];        // [block & 1][obs octet][row]: obs 8 o .. 8 o + 7
+  HV lo[2][8][KP];
   double rr[4][64];       // [block & 3]: sqrt(R^-1 rloc), 0 past p
   double w[4][64];        // R^-1 rloc dep
   int32_t gi[4][64];      // obs index
@@ -293,8 +294,8 @@ __device__ __forceinline__ void gram_h16_rows(const dk_core_args& a, int64_t off
   constexpr int IA = W, IB = NT - 1 - W;
   constexpr int NF = IB + 1;
   constexpr int LDA = CoreSmem<KP>::LDA;
-  static_assert(sizeof(H16Stage<KP>) <= sizeof(sm.A), "staging fits in A");
-  H16Stage<KP>& st = *reinterpret_cast<H16Stage<KP>*>(sm.A);
+  static_assert(sizeof(H16Stage<KP, hvec>) <= sizeof(sm.A), "staging fits in A");
+  H16Stage<KP, hvec>& st = *reinterpret_cast<H16Stage<KP, hvec>*>(sm.A);
   const int tid = threadIdx.x, lane = tid & 63, g = lane >> 4, c = lane & 15;
   const int rq = tid & 31, og = tid >> 5;  // staging rows 4 rq.., obs octet og
   const int k = a.k;
@@ -307,6 +308,27 @@ __device__ __forceinline__ void gram_h16_rows(const dk_core_args& a, int64_t off
   for (int J = 0; J <= IB; ++J) accB[J] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
   double bacc[4] = {0.0, 0.0, 0.0, 0.0};
   f32x4_t rawA[8], rawB[8];  // rows of the even and the odd blocks in flight
+  // FP16: a power-of-two scale per analysis puts the largest |Yhat| near
+  // 2^11 (FP16's range); C is scaled back by its square
+  double scale = 1.0, inv2 = 1.0;
+  if constexpr (GRAM == DK_GRAM_FP16X3) {
+    double mx = 0.0;
+    for (int o = tid; o < p; o += 256) {
+      const int gi = a.idx ? a.idx[off + o] : (int)(off + o);
+      mx = fmax(mx, a.ymax[gi] * sqrt(a.rloc[off + o]));
+    }
+#pragma unroll
+    for (int s = 32; s >= 1; s >>= 1) mx = fmax(mx, __shfl_xor(mx, s, kWave));
+    if (lane == 0) sm.nrm[tid >> 6] = mx;
+    __syncthreads();
+    mx = fmax(fmax(sm.nrm[0], sm.nrm[1]), fmax(sm.nrm[2], sm.nrm[3]));
+    if (mx > 0.0) {
+      int e;
+      frexp(mx, &e);
+      scale = ldexp(1.0, 11 - e);
+      inv2 = ldexp(1.0, 2 * (e - 11));
+    }
+  }
 
   // obs metadata of block blk, threads 0-63 (obs past p take the index of
   // obs 0 of the analysis and zero weights), in two steps so that their
@@ -357,7 +379,7 @@ __device__ __forceinline__ void gram_h16_rows(const dk_core_args& a, int64_t off
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
       const double w = st.w[ms][og * 8 + e];
-      rs[e] = (float)st.rr[ms][og * 8 + e];
+      rs[e] = (float)(st.rr[ms][og * 8 + e] * scale);
 #pragma unroll
       for (int r = 0; r < 4; ++r) bacc[r] = fma((double)raw[e][r], w, bacc[r]);
     }
@@ -439,16 +461,16 @@ __device__ __forceinline__ void gram_h16_rows(const dk_core_args& a, int64_t off
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int i = IA * 16 + 4 * g + r, j = J * 16 + c;
-      sm.A[i + j * LDA] = (double)accA[J][r];
-      if (IA != J) sm.A[j + i * LDA] = (double)accA[J][r];
+      sm.A[i + j * LDA] = (double)accA[J][r] * inv2;
+      if (IA != J) sm.A[j + i * LDA] = (double)accA[J][r] * inv2;
     }
 #pragma unroll
   for (int J = 0; J <= IB; ++J)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int i = IB * 16 + 4 * g + r, j = J * 16 + c;
-      sm.A[i + j * LDA] = (double)accB[J][r];
-      if (IB != J) sm.A[j + i * LDA] = (double)accB[J][r];
+      sm.A[i + j * LDA] = (double)accB[J][r] * inv2;
+      if (IB != J) sm.A[j + i * LDA] = (double)accB[J][r] * inv2;
     }
   // b: the partials of the eight obs groups of each row meet in LDS (sm.u
   // and sm.sg, which the transform fills later)
@@ -733,6 +755,10 @@ extern "C" int dk_letkf_chain_f64(const dk_core_args* a, const dk_chain_args* ch
     case DK_GRAM_FP64: k_letkf_chain_f64<128, DK_GRAM_FP64><<<nblocks, kThreads, 0, s>>>(*a, *ch); break;
     case DK_GRAM_BF16X3: k_letkf_chain_f64<128, DK_GRAM_BF16X3><<<nblocks, kThreads, 0, s>>>(*a, *ch); break;
     case DK_GRAM_BF16: k_letkf_chain_f64<128, DK_GRAM_BF16><<<nblocks, kThreads, 0, s>>>(*a, *ch); break;
+    case DK_GRAM_FP16X3:
+      if (!a->ymax) return (int)hipErrorInvalidValue;
+      k_letkf_chain_f64<128, DK_GRAM_FP16X3><<<nblocks, kThreads, 0, s>>>(*a, *ch);
+      break;
     default: return (int)hipErrorInvalidValue;
   }
   return (int)hipGetLastError();

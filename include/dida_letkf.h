@@ -47,8 +47,7 @@ extern "C" {
 #define DIDA_FORM_BATCHED 1     /* chains of adjacent analyses from a work queue, each
                                    eigensolve warm-started from the previous one: one
                                    fused wavefront per analysis (k <= 64), chained
-                                   workgroups on the FP64, BF16X3 or BF16 Gram
-                                   (64 < k <= 128) */
+                                   workgroups (64 < k <= 128) */
 
 typedef struct dida_letkf_params {
     int32_t nanals;             /* k, ensemble size (letkf.F90:24 ens) */

@@ -52,10 +52,6 @@ def test_params_default_and_errors(dl):
     bad = dl.default_params(nanals=129, nrow=1, nlat=2, nlon=2)
     rc = dl.lib().dida_letkf_create(ctypes.byref(h), ctypes.byref(bad), 10, 0)
     assert rc == -5 and b"2..128" in dl.lib().dida_letkf_last_error()
-    bad = dl.default_params(nanals=96, nrow=1, nlat=2, nlon=2, form=dl.FORM_BATCHED,
-                            gram=dl.GRAM_FP16X3)
-    rc = dl.lib().dida_letkf_create(ctypes.byref(h), ctypes.byref(bad), 10, 0)
-    assert rc == -5 and b"batched" in dl.lib().dida_letkf_last_error()
     # the chained form (64 < k <= 128) takes no vertical localisation
     bad = dl.default_params(nanals=96, nrow=1, nlat=2, nlon=2, form=dl.FORM_BATCHED,
                             gram=dl.GRAM_BF16X3, vert_dist=0.5)

@@ -4,7 +4,8 @@ its eigensolve from the previous analysis of its chain.
 Bars: per-variable increment and Xa' relative error against the oracle
 <= 1e-10 on the FP64 Gram, for columns (MAX_OBS binding), horizontal cubes
 with overlap, and a workspace smaller than the sweep's lists; <= 1e-4 on the
-BF16x3 Gram (clustered obs with R spanning 0.5-3, and MAX_OBS binding);
+BF16x3 and FP16x3 Grams (clustered obs with R spanning 0.5-3, and MAX_OBS
+binding);
 fewer Jacobi sweeps than the plain form's cold starts; captured-graph
 replays bit-identical to the eager sweep."""
 import numpy as np
@@ -86,9 +87,10 @@ def test_chained_batched_k128(cuda, dl, oracle, case):
 
 
 @pytest.mark.parametrize("case", ["k128_bf16x3", "k100_bf16x3_capped", "k128_bf16x3_chunks",
-                                  "k65_bf16x3"])
+                                  "k65_bf16x3", "k128_fp16x3", "k100_fp16x3_capped"])
 def test_chained_batched_k128_split_gram(cuda, dl, oracle, case):
-    """The BF16x3 Gram (split hi/lo on BF16 MFMA, FP32 accumulation) in the
+    """The BF16x3 and FP16x3 Grams (split hi/lo on half-precision MFMA, FP32
+    accumulation; FP16 with a power-of-two scale per analysis) in the
     chained form, against the FP64 oracle; also with a workspace that holds a
     third of the sweep's selection lists (runs of analyses)."""
     torch = cuda
@@ -97,8 +99,9 @@ def test_chained_batched_k128_split_gram(cuda, dl, oracle, case):
     cfg = dl.synth_cfg(20, 40, 2, k, 2000, seed=0xD1DA0600 + len(case), cluster_frac=0.7,
                        nclusters=3, rscale=(0.5, 3.0))
     data = dl.synth_generate(cfg)
+    gram = dl.GRAM_FP16X3 if "fp16x3" in case else dl.GRAM_BF16X3
     p = dict(nanals=k, nrow=7, nlat=cfg.nlat, nlon=cfg.nlon, local_dist=1200.0,
-             inflation_factor=1.05, bin_deg=4.0, form=dl.FORM_BATCHED, gram=dl.GRAM_BF16X3,
+             inflation_factor=1.05, bin_deg=4.0, form=dl.FORM_BATCHED, gram=gram,
              chain_len=4)
     if case.endswith("capped"):
         p["max_obs"] = 300
