@@ -91,7 +91,7 @@ typedef struct dk_core_args {
     int32_t nanal;          /* analyses in this launch */
     const int32_t *list;    /* analysis ids (NULL: aid0 + block index) */
     int32_t aid0;           /* first analysis id; trans/wbar hold id - aid0 */
-    int32_t gram;           /* DK_GRAM_*: FP64 (every kernel); BF16X3, BF16 (chained) */
+    int32_t gram;           /* DK_GRAM_*: FP64 (every kernel); half-precision ones (chained) */
     const int64_t *offset;  /* obs offset of each analysis */
     const int32_t *nobsl;   /* obs count of each analysis */
     int64_t capacity;       /* entries readable at idx/rloc; an analysis whose
