@@ -153,6 +153,7 @@ struct dida_letkf_ctx {
     float *rinv_b;
     /* selection */
     int32_t *nobsl;
+    int32_t *thr;        /* caller-index threshold of columns capped at MAX_OBS */
     int64_t *offset;
     void *scan_tmp;
     size_t scan_tmp_bytes;
@@ -204,6 +205,7 @@ struct dida_letkf_ctx {
     hipStream_t last_stream;
     int32_t last_nobs;
     int32_t last_nobs_pending; /* obs count of the sweep being enqueued */
+    int64_t generation;        /* bumped when the selection workspace is replaced */
 };
 
 static void dfree(void *p)
@@ -221,7 +223,7 @@ int dida_letkf_destroy(dida_letkf_ctx *c)
     void *ptrs[] = {c->key, c->key_sorted, c->iota, c->perm, c->bin_count, c->bin_start,
                     c->sort_tmp, c->cx, c->sx, c->yr, c->ux, c->uy, c->uz, c->yb_b,
                     c->yb32_b, c->dep_b, c->rinv_b,
-                    c->nobsl, c->offset, c->scan_tmp, c->csr_idx, c->csr_w, c->flags,
+                    c->nobsl, c->thr, c->offset, c->scan_tmp, c->csr_idx, c->csr_w, c->flags,
                     c->trans, c->info, c->chain_key, c->chain_key_sorted, c->chain_id,
                     c->chain_order, c->queue_head, c->chain_tmp, c->ymax, c->phase_clk,
                     c->obs_lnp, c->cube_lnp, c->cube_row_off, c->cube_rows, c->ca_lat,
@@ -357,6 +359,7 @@ int dida_letkf_create(dida_letkf_ctx **out, const dida_letkf_params *prm, int32_
     DMALLOC(c->rinv_b, no * sizeof(float));
 
     DMALLOC(c->nobsl, (size_t)c->nsel * sizeof(int32_t));
+    DMALLOC(c->thr, (size_t)c->nsel * sizeof(int32_t));
     DMALLOC(c->offset, (size_t)(c->nsel + 1) * sizeof(int64_t));
     dk_scan_offsets(NULL, &c->scan_tmp_bytes, NULL, NULL, (int32_t)c->nsel, NULL);
     DMALLOC(c->scan_tmp, c->scan_tmp_bytes);
@@ -598,18 +601,61 @@ static int sweep_select_count(dida_letkf_ctx *c, const dida_letkf_obs *o, hipStr
     sa->nobsl = c->nobsl;
     sa->offset = c->offset;
     sa->overflow = c->flags;
+    sa->perm = c->perm;
+    sa->thr = c->thr;
+    sa->perm_bits = 1;
+    while (sa->perm_bits < 31 && ((int64_t)1 << sa->perm_bits) < (int64_t)o->nobs) ++sa->perm_bits;
     HIP_TRY(dk_select_count(sa, s));
+    /* columns capped at MAX_OBS: the caller-order threshold of what they keep */
+    HIP_TRY(dk_select_cap(sa, s));
     size_t tb = c->scan_tmp_bytes;
     HIP_TRY(dk_scan_offsets(c->scan_tmp, &tb, c->nobsl, c->offset, (int32_t)c->nsel, s));
     return DIDA_OK;
 }
 
+/* Drop the context's captured graph (it refers to buffers being replaced). */
+static void drop_graph(dida_letkf_ctx *c)
+{
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->gexec = NULL;
+    c->graph = NULL;
+}
+
+/* Eager sweeps: the selection entries of this sweep (offset[nsel], read
+ * back once, a stream synchronisation) against the workspace.  Sized by the
+ * sweep (no capacity given at create): the workspace grows when a sweep's
+ * lists exceed it, which replaces its buffers, so graphs captured before
+ * are dropped (group graphs refuse to launch).  Fixed at create: the plain
+ * form and the chained form run in runs that fit; the fused batched form
+ * reports DIDA_ERR_CAPACITY before any analysis runs. */
 static int ensure_capacity(dida_letkf_ctx *c, hipStream_t s)
 {
-    if (c->capacity > 0) return DIDA_OK;
     int64_t total = 0;
     HIP_TRY(hipMemcpyAsync(&total, c->offset + c->nsel, sizeof total, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
+    if (c->user_capacity) {
+        if (total > c->capacity && c->prm.form == DIDA_FORM_BATCHED && !c->chained)
+            return set_err(DIDA_ERR_CAPACITY,
+                           "the sweep's selection lists hold %lld entries, the workspace %lld "
+                           "(the fused batched form runs whole sweeps only)",
+                           (long long)total, (long long)c->capacity);
+        return DIDA_OK;
+    }
+    if (total <= c->capacity && c->capacity > 0) return DIDA_OK;
+    drop_graph(c);
+    c->generation++;
+    dfree(c->csr_idx);
+    dfree(c->csr_w);
+    c->csr_idx = NULL;
+    c->csr_w = NULL;
+    void *vb[] = {c->vkey_in, c->vkey_out, c->vpos_in, c->vpos_out, c->vidx, c->vw, c->vsort_tmp};
+    for (size_t i = 0; i < sizeof vb / sizeof vb[0]; ++i) dfree(vb[i]);
+    c->vkey_in = c->vkey_out = NULL;
+    c->vpos_in = c->vpos_out = c->vidx = NULL;
+    c->vw = NULL;
+    c->vsort_tmp = NULL;
+    c->capacity = 0;
     int64_t cap = total + total / 8 + 1024;
     if (hipMalloc((void **)&c->csr_idx, (size_t)cap * sizeof(int32_t)) != hipSuccess ||
         hipMalloc((void **)&c->csr_w, (size_t)cap * sizeof(double)) != hipSuccess)
@@ -1062,9 +1108,6 @@ int dida_letkf_autotune(dida_letkf_ctx *c, const dida_letkf_obs *o, const double
     hipStream_t s = (hipStream_t)stream;
     rc = dida_letkf_sweep(c, o, xb, axb, xa, axa, stream); /* sizes the lists */
     if (rc < 0) return rc;
-    dida_letkf_stats st;
-    rc = dida_letkf_stats_get(c, &st);
-    if (rc < 0) return rc;
     const int timing = c->timing;
     c->timing = 0;
     hipEvent_t e0 = NULL, e1 = NULL;
@@ -1081,7 +1124,8 @@ int dida_letkf_autotune(dida_letkf_ctx *c, const dida_letkf_obs *o, const double
     out->ms_initial = best_ms;
     out->ncand = 1;
     const double bins[3] = {c->theta_deg / 2.0, c->theta_deg / 4.0, c->theta_deg / 8.0};
-    const int nbin = st.nobsl_capped > 0 ? 1 : 3;
+    /* a capped column keeps the same obs at every bin size */
+    const int nbin = 3;
     const int32_t chains[6] = {2, 4, 8, 16, 32, 64};
     const int nchn = c->prm.form == DIDA_FORM_BATCHED ? 6 : 1;
     const int ntl = c->prm.form == DIDA_FORM_BATCHED ? 2 : 1;
@@ -1127,6 +1171,8 @@ struct dida_letkf_group {
     hipEvent_t *ev;      /* [n + 1]: fork, then one join event per branch */
     hipGraph_t graph;
     hipGraphExec_t exec;
+    dida_letkf_ctx **ctx;  /* the contexts and their workspace generation at capture */
+    int64_t *gen;
 };
 
 int dida_letkf_group_destroy(dida_letkf_group *g)
@@ -1140,6 +1186,8 @@ int dida_letkf_group_destroy(dida_letkf_group *g)
         if (g->branch[i]) (void)hipStreamDestroy(g->branch[i]);
     free(g->ev);
     free(g->branch);
+    free(g->ctx);
+    free(g->gen);
     free(g);
     return DIDA_OK;
 }
@@ -1165,7 +1213,9 @@ int dida_letkf_group_capture(dida_letkf_group **out, int32_t n, dida_letkf_ctx *
     g->n = n;
     g->branch = (hipStream_t *)calloc((size_t)n, sizeof(hipStream_t));
     g->ev = (hipEvent_t *)calloc((size_t)n + 1, sizeof(hipEvent_t));
-    if (!g->branch || !g->ev) {
+    g->ctx = (dida_letkf_ctx **)calloc((size_t)n, sizeof(dida_letkf_ctx *));
+    g->gen = (int64_t *)calloc((size_t)n, sizeof(int64_t));
+    if (!g->branch || !g->ev || !g->ctx || !g->gen) {
         dida_letkf_group_destroy(g);
         return set_err(DIDA_ERR_NOMEM, "group");
     }
@@ -1190,6 +1240,8 @@ int dida_letkf_group_capture(dida_letkf_group **out, int32_t n, dida_letkf_ctx *
     for (int i = 0; i < n && !rc; ++i) {
         dida_letkf_ctx *c = ctx[i];
         hipStream_t b = g->branch[i];
+        g->ctx[i] = c;
+        g->gen[i] = c->generation;
         if (hipStreamWaitEvent(b, g->ev[0], 0) != hipSuccess) {
             rc = set_err(DIDA_ERR_HIP, "group: fork wait");
             break;
@@ -1228,6 +1280,10 @@ int dida_letkf_group_capture(dida_letkf_group **out, int32_t n, dida_letkf_ctx *
 int dida_letkf_group_launch(dida_letkf_group *g, void *stream)
 {
     if (!g || !g->exec) return set_err(DIDA_ERR_ARG, "no group graph");
+    for (int i = 0; i < g->n; ++i)
+        if (g->ctx[i]->generation != g->gen[i])
+            return set_err(DIDA_ERR_ARG, "group graph: context %d replaced its selection "
+                           "workspace after the capture (capture again)", i);
     HIP_TRY(hipGraphLaunch(g->exec, (hipStream_t)stream));
     return DIDA_OK;
 }
@@ -1305,7 +1361,8 @@ int dida_letkf_stats_get(dida_letkf_ctx *c, dida_letkf_stats *st)
             st->nobsl_sum += p;
             if (p > st->nobsl_max) st->nobsl_max = p;
             if (p >= c->prm.max_obs) st->nobsl_capped++;
-            if (inf[i] < 0) st->nonconverged++;
+            if (inf[i] == -2) st->nonfinite++;
+            else if (inf[i] < 0) st->nonconverged++;
             else {
                 st->sweeps_sum += inf[i];
                 if (inf[i] > st->max_sweeps_used) st->max_sweeps_used = inf[i];
@@ -1313,6 +1370,9 @@ int dida_letkf_stats_get(dida_letkf_ctx *c, dida_letkf_stats *st)
         }
         st->capacity_overflow = flags[0];
         if (flags[0]) rc = set_err(DIDA_ERR_CAPACITY, "selection lists exceeded the workspace");
+        else if (st->nonfinite)
+            rc = set_err(DIDA_ERR_NONFINITE, "%d analyses met non-finite inputs (NaN transforms)",
+                         st->nonfinite);
         else if (st->nonconverged) rc = DIDA_WARN_NOCONV;
     }
     free(nb);
@@ -1474,6 +1534,7 @@ void letkf_core_(const int32_t *nanals, const int32_t *nobsl, const double *hxen
     int rc = core_batch_host(1, p, *nanals, &p, hxens, rdiaginv, dep, rloc, trans, overline_w,
                              *inflation_factor, &info);
     if (rc) fprintf(stderr, "letkf_core_: %s\n", dida_letkf_last_error());
+    else if (info == -2) fprintf(stderr, "letkf_core_: non-finite input, trans is NaN\n");
     else if (info < 0) fprintf(stderr, "letkf_core_: eigensolver did not converge\n");
     if (pflag && *pflag) {
         printf("nobsl %d  jacobi sweeps %d\n", p, info);

@@ -58,6 +58,14 @@ typedef struct dk_select_args {
     /* fill pass over analyses [col0, col0 + ncols) only (ncols 0: all),
      * writing entry offset[col] - base */
     int64_t col0, ncols, base;
+    /* MAX_OBS truncation in the caller's obs order: perm[l] = caller index of
+     * binned obs l (< 2^perm_bits); thr[col] (cap pass output, fill pass
+     * input) = one past the largest caller index a column capped at max_obs
+     * keeps, INT32_MAX for a column that keeps every obs with gc > 0 */
+    const int32_t *perm;
+    int32_t *thr;
+    int32_t perm_bits;
+    int32_t pad_;
 } dk_select_args;
 
 /* Fill n int32 with v (a kernel, so that captured graphs hold no memset). */
@@ -78,8 +86,12 @@ int dk_obs_gather(int32_t nobs, int32_t k, int32_t ldy, const int32_t *perm, con
                   const double *lon, const double *yb, const double *dep, const float *rinv,
                   double *cx, double *sx, double *yr, double *ux, double *uy, double *uz,
                   double *yb_b, float *yb32_b, double *dep_b, float *rinv_b, void *stream);
-/* Selection passes, one wave per column. */
+/* Selection passes, one wave per column: count (capped at max_obs), cap
+ * (thr of every column; a column whose count reached max_obs scans all its
+ * candidates for the caller-order threshold by a radix select over perm),
+ * fill (entries with gc > 0 and perm < thr, in bin order). */
 int dk_select_count(const dk_select_args *a, void *stream);
+int dk_select_cap(const dk_select_args *a, void *stream);
 int dk_select_fill(const dk_select_args *a, void *stream);
 /* offset[0] = 0, offset[i+1] = offset[i] + nobsl[i]; tmp as dk_sort_pairs. */
 int dk_scan_offsets(void *tmp, size_t *tmp_bytes, const int32_t *nobsl, int64_t *offset,

@@ -3,10 +3,18 @@
 // Observations are binned on a regular lat/lon lattice (stable sort by bin
 // id), then one wavefront per grid column visits the candidate bins in
 // ascending bin id, evaluates the reference's distance and Gaspari-Cohn
-// weight per candidate, and compacts the kept obs with a ballot.  Two
-// passes: count (capped at MAX_OBS) and fill into a CSR list.  The kept
-// sequence is exactly the one a full scan of the binned list yields, so
-// MAX_OBS truncation matches the reference's in-order truncation.
+// weight per candidate, and compacts the kept obs with a ballot.  Three
+// passes: count (capped at MAX_OBS), cap and fill into a CSR list.
+//
+// MAX_OBS truncation.  gen_yb_dis appends the kept obs in its scan order and
+// stops at MAX_OBS (letkf_yb.F90:98-115, letkf.F90:155); over one global
+// block that order is the caller's obs order, so a column with more than
+// max_obs kept obs keeps the max_obs of smallest caller index.  The cap pass
+// finds, for every column whose count reached max_obs, the caller index of
+// its max_obs-th kept obs by a radix select (11-bit digits of perm, one
+// histogram in LDS per wave, one scan of the candidates per digit); the fill
+// pass then keeps the obs below that threshold, so what a capped column
+// keeps does not depend on the bin size.
 #include "letkf_device.h"
 #include "letkf_kernels.h"
 
@@ -173,6 +181,49 @@ __device__ __forceinline__ void for_candidate_ranges(const dk_select_args& a, do
   }
 }
 
+// The column's geometry for the candidate test.
+struct ColGeo {
+  double cx2, sx2, y2r;     // cos / sin of the latitude, longitude (reference radians)
+  double vx, vy, vz;        // unit vector
+  double far_lo, near_hi;   // dot-product margins of the support radius
+};
+
+__device__ __forceinline__ ColGeo col_geo(const dk_select_args& a, double clat, double clon) {
+  ColGeo q;
+  const double x2r = deg2rad_ref(clat);
+  q.cx2 = cos(x2r);
+  q.sx2 = sin(x2r);
+  q.y2r = deg2rad_ref(clon);
+  q.vx = q.cx2 * cos(q.y2r);
+  q.vy = q.cx2 * sin(q.y2r);
+  q.vz = q.sx2;
+  // the unit-vector dot product differs from the reference's cosine by a few
+  // ulp; with these margins it decides every candidate outside a thin shell
+  // around the support radius exactly as the reference formula would
+  q.far_lo = a.cos_far - 1e-12;
+  q.near_hi = a.cos_near + 1e-12;
+  return q;
+}
+
+// Whether binned obs l is kept (gc > 0, letkf_yb.F90:84-96).  kWeight: gc is
+// the Gaspari-Cohn weight of a kept obs; otherwise candidates surely inside
+// the support are kept without it.
+template <bool kWeight>
+__device__ __forceinline__ bool keep_candidate(const dk_select_args& a, const ColGeo& q, int l,
+                                               double& gc) {
+#pragma clang fp contract(off)
+  gc = 0.0;
+  const double vd = a.obs.ux[l] * q.vx + a.obs.uy[l] * q.vy + a.obs.uz[l] * q.vz;
+  if (vd < q.far_lo) return false;
+  if (!kWeight && vd > q.near_hi) return true;
+  const double v = central_cos(a.obs.cx[l], a.obs.sx[l], a.obs.yr[l], q.cx2, q.sx2, q.y2r);
+  if (!(v >= a.cos_far)) return false;
+  const double d = dist_from_cos(v);
+  const double r = d * 1.0 / a.local_dist;
+  gc = gc_weight(r, a.gc_filter);
+  return gc > 0.0;
+}
+
 template <bool kFill>
 __global__ __launch_bounds__(256) void k_select(dk_select_args a) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -182,14 +233,10 @@ __global__ __launch_bounds__(256) void k_select(dk_select_args a) {
   const int ilat = (int)(col % a.nlat), ilon = (int)(col / a.nlat);
   const double clat = (double)a.grid_lat[ilat];
   const double clon = (double)a.grid_lon[ilon];
-  const double x2r = deg2rad_ref(clat);
-  const double cx2 = cos(x2r), sx2 = sin(x2r), y2r = deg2rad_ref(clon);
-  const double vx = cx2 * cos(y2r), vy = cx2 * sin(y2r), vz = sx2;
-  // the unit-vector dot product differs from the reference's cosine by a few
-  // ulp; with these margins it decides every candidate outside a thin shell
-  // around the support radius exactly as the reference formula would
-  const double far_lo = a.cos_far - 1e-12, near_hi = a.cos_near + 1e-12;
+  const ColGeo q = col_geo(a, clat, clon);
   const int limit = kFill ? a.nobsl[col] : a.max_obs;
+  // capped columns keep the obs below the caller-order threshold
+  const int32_t thr = kFill ? a.thr[col] : INT32_MAX;
   int64_t out = kFill ? a.offset[col] - a.base : 0;
   if (kFill && out + limit > a.capacity && lane == 0) *a.overflow = 1;
   int cnt = 0;
@@ -197,25 +244,9 @@ __global__ __launch_bounds__(256) void k_select(dk_select_args a) {
     for_candidate_ranges(a, clat, clon, [&](int l0, int l1) -> bool {
       for (int base = l0; base < l1; base += kWave) {
         const int l = base + lane;
-        bool keep = false;
         double gc = 0.0;
-        if (l < l1) {
-          const double vd = a.obs.ux[l] * vx + a.obs.uy[l] * vy + a.obs.uz[l] * vz;
-          if (vd >= far_lo) {
-            if (!kFill && vd > near_hi) {
-              keep = true;  // the count pass needs no weight
-            } else {
-              double v = central_cos(a.obs.cx[l], a.obs.sx[l], a.obs.yr[l], cx2, sx2, y2r);
-              if (v >= a.cos_far) {
-#pragma clang fp contract(off)
-                double d = dist_from_cos(v);
-                double r = d * 1.0 / a.local_dist;
-                gc = gc_weight(r, a.gc_filter);
-                keep = gc > 0.0;
-              }
-            }
-          }
-        }
+        bool keep = l < l1 && keep_candidate<kFill>(a, q, l, gc);
+        if (kFill && thr != INT32_MAX && keep) keep = a.perm[l] < thr;
         const uint64_t m = __ballot(keep);
         if (kFill && keep) {
           const int pos = cnt + (int)__builtin_amdgcn_mbcnt_hi(
@@ -233,6 +264,108 @@ __global__ __launch_bounds__(256) void k_select(dk_select_args a) {
     });
   }
   if (!kFill && lane == 0) a.nobsl[col] = cnt < a.max_obs ? cnt : a.max_obs;
+}
+
+// LDS writes of this wave visible to its later LDS reads.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+constexpr int kCapDigit = 11;               // bits of caller index per radix level
+constexpr int kCapBins = 1 << kCapDigit;
+
+// thr[col] of every column: INT32_MAX unless the count pass stopped at
+// max_obs; then, if more than max_obs obs are kept, one past the caller index
+// of the max_obs-th smallest kept one.  Level by level, the kept obs whose
+// caller index lies in the open range [lo, lo + 2^width) are counted per
+// 11-bit digit, and the digit holding rank `rank` narrows the range.
+__global__ __launch_bounds__(256) void k_select_cap(dk_select_args a) {
+  __shared__ uint32_t hist_all[4][kCapBins];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t ncol = (int64_t)a.nlat * a.nlon;
+  const int64_t col = (int64_t)blockIdx.x * 4 + wave;
+  if (col >= ncol) return;
+  if (a.max_obs <= 0 || a.nobsl[col] < a.max_obs) {
+    if (lane == 0) a.thr[col] = INT32_MAX;
+    return;
+  }
+  uint32_t* hist = hist_all[wave];
+  const int ilat = (int)(col % a.nlat), ilon = (int)(col / a.nlat);
+  const double clat = (double)a.grid_lat[ilat];
+  const double clon = (double)a.grid_lon[ilon];
+  const ColGeo q = col_geo(a, clat, clon);
+  int width = a.perm_bits;  // caller indices < 2^perm_bits
+  uint32_t lo = 0;
+  int rank = a.max_obs;
+  int32_t thr = INT32_MAX;
+  for (int level = 0;; ++level) {
+    const int dbits = width < kCapDigit ? width : kCapDigit;
+    const int shift = width - dbits;
+    const int nb = 1 << dbits;
+    for (int i = lane; i < nb; i += kWave) hist[i] = 0u;
+    wave_lds_sync();
+    const uint32_t span = width >= 31 ? 0x80000000u : (1u << width);
+    int total = 0;
+    for_candidate_ranges(a, clat, clon, [&](int l0, int l1) -> bool {
+      for (int base = l0; base < l1; base += kWave) {
+        const int l = base + lane;
+        bool keep = false;
+        uint32_t rel = 0;
+        if (l < l1) {
+          rel = (uint32_t)a.perm[l] - lo;
+          double gc;
+          keep = rel < span && keep_candidate<false>(a, q, l, gc);
+        }
+        if (keep) atomicAdd(&hist[rel >> shift], 1u);
+        total += __popcll(__ballot(keep));
+      }
+      return true;
+    });
+    wave_lds_sync();
+    if (level == 0 && total <= a.max_obs) break;  // every kept obs fits
+    // the digit whose cumulative count reaches rank: bins split over lanes,
+    // a wave scan of the lane sums, then the owning lane walks its bins
+    const int per = (nb + kWave - 1) / kWave;
+    const int b0 = lane * per;
+    int sum = 0;
+    for (int i = 0; i < per; ++i)
+      if (b0 + i < nb) sum += (int)hist[b0 + i];
+    int inc = sum;
+    for (int o = 1; o < kWave; o <<= 1) {
+      const int t = __shfl_up(inc, o, kWave);
+      if (lane >= o) inc += t;
+    }
+    const int exc = inc - sum;
+    const uint64_t m = __ballot(exc < rank && rank <= inc);
+    if (m == 0) break;  // cannot happen with consistent passes: keep bin order
+    const int src = __ffsll((unsigned long long)m) - 1;
+    int dsel = 0, before = 0;
+    if (lane == src) {
+      int cum = exc;
+      for (int i = 0; i < per; ++i) {
+        const int h = b0 + i < nb ? (int)hist[b0 + i] : 0;
+        if (cum + h >= rank) {
+          dsel = b0 + i;
+          before = cum;
+          break;
+        }
+        cum += h;
+      }
+    }
+    dsel = __shfl(dsel, src, kWave);
+    before = __shfl(before, src, kWave);
+    lo += (uint32_t)dsel << shift;
+    rank -= before;
+    width = shift;
+    if (width == 0) {
+      thr = (int32_t)(lo + 1u);
+      break;
+    }
+    wave_lds_sync();  // the bins are read before the next level clears them
+  }
+  if (lane == 0) a.thr[col] = thr;
 }
 
 }  // namespace
@@ -301,6 +434,14 @@ extern "C" int dk_select_count(const dk_select_args* a, void* stream) {
   int64_t ncol = a->ncols > 0 ? a->ncols : (int64_t)a->nlat * a->nlon;
   if (ncol <= 0) return 0;
   k_select<false><<<(unsigned)((ncol + 3) / 4), 256, 0, (hipStream_t)stream>>>(*a);
+  return (int)hipGetLastError();
+}
+
+extern "C" int dk_select_cap(const dk_select_args* a, void* stream) {
+  const int64_t ncol = (int64_t)a->nlat * a->nlon;
+  if (ncol <= 0) return 0;
+  if (a->perm_bits < 1 || a->perm_bits > 31) return (int)hipErrorInvalidValue;
+  k_select_cap<<<(unsigned)((ncol + 3) / 4), 256, 0, (hipStream_t)stream>>>(*a);
   return (int)hipGetLastError();
 }
 

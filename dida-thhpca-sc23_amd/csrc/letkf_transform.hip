@@ -529,8 +529,14 @@ __device__ __forceinline__ void gram_any(const dk_core_args& a, int64_t off, int
 // analysis of a chain (column-major, KP x KP, in HBM), from which the
 // eigensolve starts (A = M V_prev); vnext: where this analysis leaves its own.
 // GRAM: precision of C (b, the eigensolve and the transform are FP64).
+// A non-finite Yb, d or R^-1 reaches M (then an eigenvalue) or b; the
+// transform is then NaN, as the reference's NaN would propagate (the Sunway
+// caller checks trans and overline_w for it, letkf_compute.F90:168-187), and
+// info is -2.  Returns whether the eigenvectors may warm-start the next
+// analysis: not after a non-finite one, nor after one whose eigensolve hit
+// the sweep cap (its basis is not orthonormal).
 template <int KP, int U, int GRAM>
-__device__ void analysis_f64(const dk_core_args& a, int aid, CoreSmem<KP>& sm,
+__device__ bool analysis_f64(const dk_core_args& a, int aid, CoreSmem<KP>& sm,
                              const double* vprev, double* vnext,
                              unsigned long long* phase_clk) {
   constexpr int NT = KP / 16;
@@ -602,12 +608,16 @@ __device__ void analysis_f64(const dk_core_args& a, int aid, CoreSmem<KP>& sm,
   const int m = (k + 1) & ~1;
 
   // exact column norms: alpha_j = lambda_j^2, lambda_j eigenvalues of M
+  int nonfinite = 0;
   for (int j = tid; j < m; j += kThreads) {
     double s = 0.0;
     for (int i = 0; i < k; ++i) s += sm.A[i + j * LDA] * sm.A[i + j * LDA];
     sm.nrm[j] = s;
+    nonfinite |= !__builtin_isfinite(s);
   }
-  __syncthreads();
+  for (int i = tid; i < k; i += kThreads) nonfinite |= !__builtin_isfinite(sm.b[i]);
+  const bool bad = __syncthreads_or(nonfinite) != 0;
+  const double nan = __builtin_nan("");
   if (vnext) {
     // v_j = a_j / lambda_j for the next analysis of the chain (1/lambda_j
     // staged in u, which is filled again below).  Only this workgroup reads
@@ -679,13 +689,14 @@ __device__ void analysis_f64(const dk_core_args& a, int aid, CoreSmem<KP>& sm,
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int i = I * 16 + r16, i2 = I2 * 16 + 4 * r + q4;
-      if (i < k && i2 < k) trans[i + (int64_t)i2 * k] = d[r] + sm.wb[i];
+      if (i < k && i2 < k) trans[i + (int64_t)i2 * k] = bad ? nan : d[r] + sm.wb[i];
     }
   }
   if (a.wbar)
-    for (int i = tid; i < k; i += kThreads) a.wbar[slot * k + i] = sm.wb[i];
-  if (a.info && tid == 0) a.info[aid] = conv ? sweeps : -1;
+    for (int i = tid; i < k; i += kThreads) a.wbar[slot * k + i] = bad ? nan : sm.wb[i];
+  if (a.info && tid == 0) a.info[aid] = bad ? -2 : (conv ? sweeps : -1);
   clk.mark(DK_PHASE_TRANSFORM);
+  return conv && !bad;
 }
 
 
@@ -713,9 +724,10 @@ __global__ __launch_bounds__(kThreads) void k_letkf_chain_f64(dk_core_args a, dk
     if (item >= ch.nchain) break;
     const int c0 = a.aid0 + (ch.order ? ch.order[item] : item) * ch.chain_len;
     const int c1 = min(c0 + ch.chain_len, a.aid0 + a.nanal);
+    bool prev_ok = false;  // whether the previous analysis may warm-start this one
     for (int aid = c0; aid < c1; ++aid)
-      analysis_f64<KP, 4, GRAM>(a, aid, sm, aid > c0 ? vs : nullptr,
-                                 aid + 1 < c1 ? vs : nullptr, ch.phase_clk);
+      prev_ok = analysis_f64<KP, 4, GRAM>(a, aid, sm, prev_ok ? vs : nullptr,
+                                          aid + 1 < c1 ? vs : nullptr, ch.phase_clk);
   }
 }
 

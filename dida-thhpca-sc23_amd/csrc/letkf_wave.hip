@@ -724,8 +724,12 @@ __device__ __forceinline__ void update_rows(int k, int nrow, const double* __res
 // MODE: kModeColumn (one analysis per column, fused update), kModeVertical
 // (one per column and level cube, fused update of the cube's rows) or
 // kModeTransOut (one per horizontal cube, transform written to HBM)
+// Returns whether the analysis's eigenvectors may warm-start the next one:
+// not when it was not finite (its transform is NaN) or its eigensolve hit the
+// sweep cap (its basis is not orthonormal, and A = M V_prev would then not
+// carry the eigenvectors of M).
 template <int KP, int GRAM, int MODE>
-__device__ void wave_column(const dk_batched_args& a, int64_t col, int cube, bool warm,
+__device__ bool wave_column(const dk_batched_args& a, int64_t col, int cube, bool warm,
                             WSmem<KP>& sm, Wave<KP, GRAM>& w) {
   constexpr int NJ = Wave<KP, GRAM>::NJ;
   constexpr int NQ = Wave<KP, GRAM>::NQ;
@@ -778,8 +782,16 @@ __device__ void wave_column(const dk_batched_args& a, int64_t col, int cube, boo
   // lambda_j = |a_j|; V = A / lambda (to LDS); f = sqrt((k-1)/lambda);
   // u = V^T b / lambda
   double lam[NJ];
+  // a non-finite Yb, d or R^-1 reaches M (then lambda) or b: the analysis is
+  // flagged (info -2) and its transform set to NaN, as the reference's NaN
+  // would propagate (letkf_compute.F90:168-187 of the Sunway path checks for it)
+  bool bad = false;
 #pragma unroll
-  for (int J = 0; J < NJ; ++J) lam[J] = sqrt(w.colnorm(J));
+  for (int J = 0; J < NJ; ++J) {
+    lam[J] = sqrt(w.colnorm(J));
+    bad |= !__builtin_isfinite(lam[J]) || !__builtin_isfinite(sm.b[J * 16 + c]);
+  }
+  bad = __any(bad);
   __syncthreads();
 #pragma unroll
   for (int J = 0; J < NJ; ++J) {
@@ -823,13 +835,14 @@ __device__ void wave_column(const dk_batched_args& a, int64_t col, int cube, boo
       for (int I2 = 0; I2 < NJ; ++I2) T[I][I2] = mfma_f64_16x16x4(v[I] * fj, v[I2], T[I][I2]);
     }
   }
+  const double nan = __builtin_nan("");
 #pragma unroll
   for (int I = 0; I < NJ; ++I)
 #pragma unroll
     for (int I2 = 0; I2 < NJ; ++I2)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) T[I][I2][r] += wb[I][r];
-  if (a.info && lane == 0) a.info[aid] = conv ? sweeps : -1;
+      for (int r = 0; r < 4; ++r) T[I][I2][r] = bad ? nan : T[I][I2][r] + wb[I][r];
+  if (a.info && lane == 0) a.info[aid] = bad ? -2 : (conv ? sweeps : -1);
   if (a.wbar_out && c == 0)
 #pragma unroll
     for (int I = 0; I < NJ; ++I)
@@ -852,7 +865,7 @@ __device__ void wave_column(const dk_batched_args& a, int64_t col, int cube, boo
           const int i = rowB(I, r, g), e = I2 * 16 + c;
           if (i < k && e < k) tr[i + (int64_t)e * k] = T[I][I2][r];
         }
-    return;
+    return conv && !bad;
   }
 
   // update: all nrow rows of the column, or with vertical localisation the
@@ -875,6 +888,7 @@ __device__ void wave_column(const dk_batched_args& a, int64_t col, int cube, boo
   }
   clk.mark(DK_PHASE_UPDATE);
   (void)NQ;
+  return conv && !bad;
 }
 
 // First longitude of stripe g of n groups (groups of whole longitudes).
@@ -910,6 +924,9 @@ __global__ __launch_bounds__(64, 1) void k_letkf_wave(dk_batched_args a) {
     const int ilon = chain / a.chains_per_lon;
     const int lat0 = (chain % a.chains_per_lon) * a.chain_len;
     const int lat1 = min(lat0 + a.chain_len, a.nlat);
+    // an analysis warm-starts from the previous one of its chain unless that
+    // one was not finite or did not converge
+    bool prev_ok = false;
     for (int ilat = lat0; ilat < lat1; ++ilat) {
       const int64_t col = (int64_t)ilon * a.nlat + ilat;
       if constexpr (MODE == kModeVertical) {
@@ -918,10 +935,10 @@ __global__ __launch_bounds__(64, 1) void k_letkf_wave(dk_batched_args a) {
         // of the previous column
         const bool down = ((ilat - lat0) & 1) != 0;
         for (int t = 0; t < a.ncube; ++t)
-          wave_column<KP, GRAM, MODE>(a, col, down ? a.ncube - 1 - t : t,
-                                      a.warm_start && (ilat > lat0 || t > 0), sm, w);
+          prev_ok = wave_column<KP, GRAM, MODE>(a, col, down ? a.ncube - 1 - t : t,
+                                                a.warm_start && prev_ok, sm, w);
       } else {
-        wave_column<KP, GRAM, MODE>(a, col, 0, a.warm_start && ilat > lat0, sm, w);
+        prev_ok = wave_column<KP, GRAM, MODE>(a, col, 0, a.warm_start && prev_ok, sm, w);
       }
     }
   }

@@ -19,6 +19,8 @@ SYNTH_PATH = os.path.join(HERE, "lib", "libdida_synth.so")
 
 DIDA_OK = 0
 DIDA_WARN_NOCONV = 1
+DIDA_ERR_CAPACITY = -4
+DIDA_ERR_NONFINITE = -6
 GRAM_FP64, GRAM_BF16X3, GRAM_FP16X3, GRAM_BF16 = 0, 1, 2, 3
 FORM_PLAIN, FORM_BATCHED = 0, 1
 TIMING_STAGES, TIMING_PHASES = 1, 2
@@ -79,7 +81,7 @@ class StatsC(C.Structure):
     _fields_ = [("nanal", C.c_int64), ("nobsl_sum", C.c_int64), ("nobsl_max", C.c_int32),
                 ("nobsl_capped", C.c_int32), ("nonconverged", C.c_int32),
                 ("max_sweeps_used", C.c_int32), ("capacity_overflow", C.c_int32),
-                ("pad_", C.c_int32), ("sweeps_sum", C.c_int64)]
+                ("nonfinite", C.c_int32), ("sweeps_sum", C.c_int64)]
 
 
 class SynthCfg(C.Structure):
@@ -342,9 +344,14 @@ class Engine:
         check(lib().dida_letkf_phase_cycles(self.h, cyc), "phase_cycles")
         return {name: cyc[i] for i, name in enumerate(PHASES)}
 
-    def stats(self) -> dict:
+    def stats(self, strict: bool = True) -> dict:
+        """Diagnostics of the last sweep.  strict: raise on an error status
+        (capacity overflow, non-finite analyses); otherwise return it in
+        "status" with the counts."""
         st = StatsC()
-        rc = check(lib().dida_letkf_stats_get(self.h, C.byref(st)), "stats")
+        rc = lib().dida_letkf_stats_get(self.h, C.byref(st))
+        if strict:
+            check(rc, "stats")
         d = {f: getattr(st, f) for f, _ in StatsC._fields_ if f != "pad_"}
         d["status"] = rc
         return d

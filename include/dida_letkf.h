@@ -35,6 +35,9 @@ extern "C" {
 #define DIDA_ERR_NOMEM (-3)     /* device allocation failed */
 #define DIDA_ERR_CAPACITY (-4)  /* selection lists exceeded the workspace */
 #define DIDA_ERR_UNSUPPORTED (-5)
+#define DIDA_ERR_NONFINITE (-6) /* some analyses met a non-finite Yb, d or R^-1: their
+                                   transforms (and so their columns of Xa', xbar_a)
+                                   are NaN; the others are unaffected */
 
 /* Precision of the Yb^T R^-1 Yb contraction. */
 #define DIDA_GRAM_FP64 0        /* FP64 MFMA (the plain drop-in) */
@@ -130,7 +133,7 @@ typedef struct dida_letkf_stats {
     int32_t nonconverged;       /* eigensolves that hit the sweep cap */
     int32_t max_sweeps_used;
     int32_t capacity_overflow;  /* 1 if the selection lists did not fit */
-    int32_t pad_;
+    int32_t nonfinite;          /* analyses with a non-finite M or b (info -2) */
     int64_t sweeps_sum;         /* Jacobi sweeps summed over converged analyses */
 } dida_letkf_stats;
 
@@ -165,7 +168,9 @@ void letkf_core_batch_(const int32_t *batch, const int32_t *batch_large_matrix,
 
 /* Status-returning batched core on device pointers (same layout as
  * letkf_core_batch_), enqueued on `stream` (a hipStream_t, NULL = default).
- * info[batch] (optional) receives the Jacobi sweep count or -1. */
+ * info[batch] (optional) receives the Jacobi sweep count, -1 when the
+ * eigensolve hit the sweep cap, or -2 when M or b was not finite (trans and
+ * overline_w are then NaN). */
 int dida_letkf_core_batch_dev(int32_t batch, int32_t max_obs, int32_t nanals,
                               const int32_t *nobsl, const double *hxens, const float *rdiaginv,
                               const double *dep, const double *rloc, double *trans,
@@ -178,13 +183,19 @@ int dida_letkf_core_batch_dev(int32_t batch, int32_t max_obs, int32_t nanals,
  * letkf_compute_nooverlap (letkf_compute.F90:12-126) with cube_n = 1.
  *
  * nobs_max bounds the observation count of later sweeps; sel_capacity is the
- * number of (column, obs) selection entries to reserve (0: sized by the first
- * sweep, which then synchronises once).  With a reserved workspace the plain
- * form and the batched form with k > 64 run each sweep in runs of
- * consecutive analyses whose lists fit it (the list offsets are read back
- * once per sweep), so workloads whose selection lists exceed the device
- * memory left next to the state still run; the fused batched form (k <= 64)
- * and captured graphs report DIDA_ERR_CAPACITY instead. */
+ * number of (column, obs) selection entries to reserve (0: sized by the
+ * sweeps).  Every eager sweep reads its number of selection entries back once
+ * (one stream synchronisation, after the count pass).  Sized by the sweeps,
+ * the workspace is allocated by the first and grows (by 1/8 more) when a
+ * later one needs more; growing replaces its buffers, so a graph captured on
+ * the context before is dropped (capture again) and a group graph holding the
+ * context refuses to launch.  With a reserved workspace the plain form and
+ * the batched form with k > 64 run each sweep in runs of consecutive
+ * analyses whose lists fit it, so workloads whose selection lists exceed the
+ * device memory left next to the state still run; the fused batched form
+ * (k <= 64) returns DIDA_ERR_CAPACITY before any analysis runs.  A captured
+ * graph whose lists outgrow the workspace on replay analyses the columns past
+ * it without obs and dida_letkf_stats_get reports DIDA_ERR_CAPACITY. */
 int dida_letkf_create(dida_letkf_ctx **ctx, const dida_letkf_params *prm, int32_t nobs_max,
                       int64_t sel_capacity);
 int dida_letkf_destroy(dida_letkf_ctx *ctx);
@@ -221,11 +232,11 @@ int dida_letkf_graph_launch(dida_letkf_ctx *ctx, void *stream);
  * dida_letkf_sweep) over candidate layouts and keeps the fastest for the
  * sweeps that follow: the length of the warm-start chains of the batched form
  * (2, 4, 8, 16, 32, 64 adjacent columns per work item: longer chains warm-start
- * more eigensolves, shorter ones balance the cost-sorted queue better) and,
- * when no column reaches MAX_OBS, the obs bin size (theta/2, theta/4,
- * theta/8, theta = support radius: coarser bins scan more candidates, finer
- * ones visit more bins).  With MAX_OBS binding the bin size stays, because
- * the bin order decides which obs a capped column keeps.  Each candidate is
+ * more eigensolves, shorter ones balance the cost-sorted queue better) and
+ * the obs bin size (theta/2, theta/4, theta/8, theta = support radius:
+ * coarser bins scan more candidates, finer ones visit more bins; a column
+ * capped at MAX_OBS keeps the obs of smallest caller index whatever the bin
+ * size, as the reference's in-order scan does).  Each candidate is
  * timed twice (HIP events, minimum); xa/axa hold the result of a sweep with
  * the kept layout on return.  Synchronises. */
 typedef struct dida_letkf_tuning {
@@ -282,13 +293,18 @@ int dida_letkf_stage_ms(dida_letkf_ctx *ctx, double ms[DIDA_NSTAGES]);
 #define DIDA_NPHASES 5
 int dida_letkf_phase_cycles(dida_letkf_ctx *ctx, double cyc[DIDA_NPHASES]);
 
-/* Diagnostics of the last sweep; synchronises the context's last stream. */
+/* Diagnostics of the last sweep; synchronises the context's last stream.
+ * Returns DIDA_ERR_CAPACITY, else DIDA_ERR_NONFINITE, else DIDA_WARN_NOCONV
+ * when the sweep had such analyses (the reference prints a failed eigensolve,
+ * letkf_core.F90:119-122, and its Sunway caller stops on a NaN transform,
+ * letkf_compute.F90:168-187), DIDA_OK otherwise. */
 int dida_letkf_stats_get(dida_letkf_ctx *ctx, dida_letkf_stats *st);
 
 /* Device arrays of the last sweep (valid until the next sweep):
  * nobsl[ncol], offset[ncol + 1] (int64), csr_idx / csr_w (binned obs index
  * and Gaspari-Cohn weight per selection entry), perm[nobs] (binned position
- * -> caller's obs index), info[ncol] (Jacobi sweeps or -1). */
+ * -> caller's obs index), info[ncol] (Jacobi sweeps, -1 not converged,
+ * -2 not finite). */
 int dida_letkf_debug_arrays(dida_letkf_ctx *ctx, const int32_t **nobsl, const int64_t **offset,
                             const int32_t **csr_idx, const double **csr_w, const int32_t **perm,
                             const int32_t **info);

@@ -419,24 +419,71 @@ int orc_candidate_ranges(double clat, double clon, double local_dist,
     return nr;
 }
 
+static int cmp_i32(const void *a, const void *b)
+{
+    const int32_t x = *(const int32_t *)a, y = *(const int32_t *)b;
+    return (x > y) - (x < y);
+}
+
+/* The binned list is the caller's list permuted; perm[l] (NULL: l) is the
+ * caller index of binned obs l.  gen_yb_dis appends kept obs in its scan
+ * order and stops at MAX_OBS (letkf_yb.F90:98-115, letkf.F90:155); with one
+ * global block that order is the caller's, so when more than max_obs obs
+ * are kept the list holds the max_obs of smallest caller index (listed in
+ * bin order).  Uncapped columns keep every obs with gc > 0. */
 int orc_select_column_binned(const orc_params *prm, const orc_obs *obs,
                              const int32_t *bin_start, int nband, int nlonbin,
-                             double bin_deg, double center_lat,
+                             double bin_deg, const int32_t *perm, double center_lat,
                              double center_lon, int32_t *idx, double *rloc)
 {
     int32_t ranges[4 * 400];
     int nr = orc_candidate_ranges(center_lat, center_lon, prm->local_dist, bin_deg,
                                   nband, nlonbin, ranges);
+    const int max_obs = prm->max_obs;
+    if (max_obs <= 0) return 0;
+    int32_t *keys = NULL;
+    size_t nkeys = 0, cap = 0;
     int p = 0;
     for (int q = 0; q < nr; ++q) {
         int l0 = bin_start[ranges[2 * q]], l1 = bin_start[ranges[2 * q + 1] + 1];
         for (int l = l0; l < l1; ++l) {
             double gc;
-            if (select_one(prm, obs, l, center_lat, center_lon, &gc)) {
+            if (!select_one(prm, obs, l, center_lat, center_lon, &gc)) continue;
+            if (p < max_obs) {
                 idx[p] = l;
                 rloc[p] = gc;
                 ++p;
-                if (p >= prm->max_obs) return p;
+            }
+            if (nkeys == cap) {
+                cap = cap ? 2 * cap : 4096;
+                int32_t *nk = (int32_t *)realloc(keys, cap * sizeof(int32_t));
+                if (!nk) {
+                    free(keys);
+                    return -1;
+                }
+                keys = nk;
+            }
+            keys[nkeys++] = perm ? perm[l] : l;
+        }
+    }
+    if (nkeys <= (size_t)max_obs) {
+        free(keys);
+        return p;
+    }
+    /* MAX_OBS binds: keep the caller indices up to the max_obs-th smallest */
+    qsort(keys, nkeys, sizeof(int32_t), cmp_i32);
+    const int32_t last = keys[max_obs - 1];
+    free(keys);
+    p = 0;
+    for (int q = 0; q < nr; ++q) {
+        int l0 = bin_start[ranges[2 * q]], l1 = bin_start[ranges[2 * q + 1] + 1];
+        for (int l = l0; l < l1; ++l) {
+            double gc;
+            if (!select_one(prm, obs, l, center_lat, center_lon, &gc)) continue;
+            if ((perm ? perm[l] : l) <= last) {
+                idx[p] = l;
+                rloc[p] = gc;
+                ++p;
             }
         }
     }
@@ -474,7 +521,7 @@ int orc_sweep(const orc_params *prm, const float *grid_lat,
             int ilat = (int)(col % prm->nlat), ilon = (int)(col / prm->nlat);
             double clat = (double)grid_lat[ilat], clon = (double)grid_lon[ilon];
             int p = bins ? orc_select_column_binned(prm, obs, bins->bin_start, bins->nband,
-                                                    bins->nlonbin, bins->bin_deg, clat,
+                                                    bins->nlonbin, bins->bin_deg, bins->perm, clat,
                                                     clon, idx, rl)
                          : orc_select_column(prm, obs, clat, clon, idx, rl);
             for (int o = 0; o < p; ++o) {
@@ -538,7 +585,7 @@ int orc_sweep_cubes(const orc_params *prm, const float *grid_lat, const float *g
             const int cj = (sj + 1 + ej + 1) / 2 - 1, ci = (si + 1 + ei + 1) / 2 - 1;
             const double clat = (double)grid_lat[cj], clon = (double)grid_lon[ci];
             int p = bins ? orc_select_column_binned(prm, obs, bins->bin_start, bins->nband,
-                                                    bins->nlonbin, bins->bin_deg, clat,
+                                                    bins->nlonbin, bins->bin_deg, bins->perm, clat,
                                                     clon, idx, rl)
                          : orc_select_column(prm, obs, clat, clon, idx, rl);
             for (int o = 0; o < p; ++o) {
@@ -638,7 +685,7 @@ int orc_sweep_compact(const orc_params *prm, const float *grid_lat, const float 
             int ilat = (int)(col % prm->nlat), ilon = (int)(col / prm->nlat);
             double clat = (double)grid_lat[ilat], clon = (double)grid_lon[ilon];
             int p = bins ? orc_select_column_binned(prm, obs, bins->bin_start, bins->nband,
-                                                    bins->nlonbin, bins->bin_deg, clat,
+                                                    bins->nlonbin, bins->bin_deg, bins->perm, clat,
                                                     clon, idx, rl)
                          : orc_select_column(prm, obs, clat, clon, idx, rl);
             for (int o = 0; o < p; ++o) {
@@ -883,7 +930,7 @@ int orc_sweep_vertical(const orc_params *prm, const float *grid_lat, const float
             const int64_t ccol = cj + (int64_t)nlat * ci;
             double clat = (double)grid_lat[cj], clon = (double)grid_lon[ci];
             int p = bins ? orc_select_column_binned(prm, obs, bins->bin_start, bins->nband,
-                                                    bins->nlonbin, bins->bin_deg, clat,
+                                                    bins->nlonbin, bins->bin_deg, bins->perm, clat,
                                                     clon, idx, rl)
                          : orc_select_column(prm, obs, clat, clon, idx, rl);
             for (int o = 0; o < p; ++o) {

@@ -120,18 +120,22 @@ void orc_bin_sort(int nobs, const int32_t *key, int nbins, int32_t *perm,
 int orc_candidate_ranges(double clat, double clon, double local_dist,
                          double bin_deg, int nband, int nlonbin, int32_t *ranges);
 
-/* Selection over the binned list (obs already permuted into binned order),
- * visiting only candidate bins; same result as orc_select_column on the
- * binned list. */
+/* Selection over the binned list (obs already permuted into binned order,
+ * perm[l] = caller index of binned obs l, NULL for the identity), visiting
+ * only candidate bins.  The kept set is that of orc_select_column on the
+ * caller's list: every obs with gc > 0 or, when more than max_obs are kept,
+ * the max_obs of smallest caller index (gen_yb_dis stops at MAX_OBS in its
+ * scan order); listed in bin order.  Returns the count (-1: no memory). */
 int orc_select_column_binned(const orc_params *prm, const orc_obs *obs,
                              const int32_t *bin_start, int nband, int nlonbin,
-                             double bin_deg, double center_lat,
+                             double bin_deg, const int32_t *perm, double center_lat,
                              double center_lon, int32_t *idx, double *rloc);
 
 typedef struct orc_bins {
     const int32_t *bin_start; /* nbins + 2 */
     int32_t nband, nlonbin;
     double bin_deg;
+    const int32_t *perm;      /* binned position -> caller index (NULL: identity) */
 } orc_bins;
 
 /* Whole sweep (letkf_compute_nooverlap with cube_n = 1): one analysis per

@@ -41,7 +41,21 @@ class OrcParams(C.Structure):
 
 class OrcBins(C.Structure):
     _fields_ = [("bin_start", C.c_void_p), ("nband", C.c_int32), ("nlonbin", C.c_int32),
-                ("bin_deg", C.c_double)]
+                ("bin_deg", C.c_double), ("perm", C.c_void_p)]
+
+
+def _bins_c(bins):
+    """OrcBins of bins = (bin_start, nband, nlonbin, bin_deg[, perm]) and the
+    arrays it points into (keep them alive while it is used).  perm maps a
+    binned position to the caller's obs index; without it the binned list's
+    own order stands for the caller's (MAX_OBS truncation)."""
+    if bins is None:
+        return None, ()
+    bs = np.ascontiguousarray(bins[0], dtype=np.int32)
+    pm = None if len(bins) < 5 or bins[4] is None else np.ascontiguousarray(bins[4], np.int32)
+    b = OrcBins(bs.ctypes.data, int(bins[1]), int(bins[2]), float(bins[3]),
+                None if pm is None else pm.ctypes.data)
+    return b, (bs, pm)
 
 
 def build(force: bool = False) -> str:
@@ -76,8 +90,8 @@ def lib():
                                         C.c_double, _ip, _dp]
         L.orc_select_column_binned.restype = C.c_int
         L.orc_select_column_binned.argtypes = [C.POINTER(OrcParams), C.POINTER(OrcObs), _ip,
-                                               C.c_int, C.c_int, C.c_double, C.c_double,
-                                               C.c_double, _ip, _dp]
+                                               C.c_int, C.c_int, C.c_double, C.c_void_p,
+                                               C.c_double, C.c_double, _ip, _dp]
         L.orc_bin_keys.restype = C.c_int
         L.orc_bin_keys.argtypes = [C.POINTER(OrcObs), C.c_double, C.c_double, _ip,
                                    C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
@@ -222,13 +236,21 @@ def bin_obs(obs: Obs, bin_deg: float, obs_polar: float):
     return perm[:len(obs.lat)].copy(), bin_start, nb.value, nl.value
 
 
-def select_column_binned(prm, obs_binned: Obs, bin_start, nband, nlonbin, bin_deg, clat, clon):
+def select_column_binned(prm, obs_binned: Obs, bin_start, nband, nlonbin, bin_deg, clat, clon,
+                         perm=None):
+    """Selection over the binned list; perm (binned position -> caller index)
+    decides which obs a column capped at MAX_OBS keeps (the smallest caller
+    indices).  Returns binned indices and weights in bin order."""
     idx = np.zeros(max(prm.max_obs, 1), np.int32)
     rl = np.zeros(max(prm.max_obs, 1))
+    pm = None if perm is None else np.ascontiguousarray(perm, dtype=np.int32)
     p = lib().orc_select_column_binned(C.byref(prm), C.byref(obs_binned.c),
                                        np.ascontiguousarray(bin_start, dtype=np.int32),
-                                       nband, nlonbin, float(bin_deg), float(clat),
+                                       nband, nlonbin, float(bin_deg),
+                                       None if pm is None else pm.ctypes.data, float(clat),
                                        float(clon), idx, rl)
+    if p < 0:
+        raise MemoryError("orc_select_column_binned")
     return idx[:p].copy(), rl[:p].copy()
 
 
@@ -246,11 +268,7 @@ def sweep(prm: OrcParams, grid_lat, grid_lon, obs: Obs, xb, axb, cols=None, bins
     trans = np.zeros(ncol * k * k) if want_trans else None
     wbar = np.zeros(ncol * k) if want_trans else None
     cols_a = None if cols is None else np.ascontiguousarray(cols, dtype=np.int32)
-    bins_c = None
-    if bins is not None:
-        bin_start, nband, nlonbin, bin_deg = bins
-        bs = np.ascontiguousarray(bin_start, dtype=np.int32)
-        bins_c = OrcBins(bs.ctypes.data, nband, nlonbin, float(bin_deg))
+    bins_c, _keep = _bins_c(bins)
     nfail = lib().orc_sweep(C.byref(prm), np.ascontiguousarray(grid_lat, dtype=np.float32),
                             np.ascontiguousarray(grid_lon, dtype=np.float32), C.byref(obs.c),
                             xb, axb, xa, axa,
@@ -270,7 +288,7 @@ def sweep(prm: OrcParams, grid_lat, grid_lon, obs: Obs, xb, axb, cols=None, bins
 def sweep_compact(prm: OrcParams, grid_lat, grid_lon, obs: Obs, bins, cols, xb_c, axb_c,
                   nthreads=0):
     """Sweep over the listed columns; xb_c (ncols, nrow, k), axb_c (ncols, nrow)
-    hold those columns.  bins = (bin_start, nband, nlonbin, bin_deg) or None.
+    hold those columns.  bins = (bin_start, nband, nlonbin, bin_deg[, perm]) or None.
     Returns (xa_c, axa_c, nobsl, nfail)."""
     cols = np.ascontiguousarray(cols, dtype=np.int32)
     xb_c = np.ascontiguousarray(xb_c, dtype=np.float64)
@@ -278,10 +296,7 @@ def sweep_compact(prm: OrcParams, grid_lat, grid_lon, obs: Obs, bins, cols, xb_c
     xa = np.zeros_like(xb_c)
     axa = np.zeros_like(axb_c)
     nobsl = np.zeros(len(cols), np.int32)
-    bins_c = None
-    if bins is not None:
-        bs = np.ascontiguousarray(bins[0], dtype=np.int32)
-        bins_c = OrcBins(bs.ctypes.data, bins[1], bins[2], float(bins[3]))
+    bins_c, _keep = _bins_c(bins)
     nfail = lib().orc_sweep_compact(C.byref(prm), np.ascontiguousarray(grid_lat, np.float32),
                                     np.ascontiguousarray(grid_lon, np.float32), C.byref(obs.c),
                                     None if bins_c is None else C.byref(bins_c), cols, len(cols),
@@ -362,11 +377,7 @@ def sweep_cubes(prm: OrcParams, grid_lat, grid_lon, obs: Obs, xb, axb, cube_n, o
     xa = np.zeros(ncol * n * k)
     axa = np.zeros(ncol * n)
     trans = np.zeros(nanal * k * k) if want_trans else None
-    bins_c = None
-    if bins is not None:
-        bin_start, nband, nlonbin, bin_deg = bins
-        bs = np.ascontiguousarray(bin_start, dtype=np.int32)
-        bins_c = OrcBins(bs.ctypes.data, nband, nlonbin, float(bin_deg))
+    bins_c, _keep = _bins_c(bins)
     nfail = lib().orc_sweep_cubes(
         C.byref(prm), np.ascontiguousarray(grid_lat, np.float32),
         np.ascontiguousarray(grid_lon, np.float32), C.byref(obs.c),
@@ -396,11 +407,7 @@ def sweep_vertical(prm: OrcParams, grid_lat, grid_lon, obs: Obs, obs_ph, xb, axb
     c2 = np.ascontiguousarray(vcoord.get("c2", vcoord["c1"]), np.float64)
     vc = OrcVcoord(int(vcoord["type"]), len(c1) - 1, c1.ctypes.data, c2.ctypes.data,
                    float(vcoord["ptop"]), float(vcoord.get("p0", 0.0)))
-    bins_c = None
-    if bins is not None:
-        bin_start, nband, nlonbin, bin_deg = bins
-        bs = np.ascontiguousarray(bin_start, dtype=np.int32)
-        bins_c = OrcBins(bs.ctypes.data, nband, nlonbin, float(bin_deg))
+    bins_c, _keep = _bins_c(bins)
     vs = np.ascontiguousarray(var_start, np.int32)
     vl = np.ascontiguousarray(var_lev, np.int32)
     nfail = lib().orc_sweep_vertical(

@@ -23,7 +23,7 @@ def _check(oracle, data, params, xa, axa, cube_n=1, overlap=0, tol=1e-10):
     prm = oracle.params(params.nanals, params.nrow, params.nlat, params.nlon, params.local_dist,
                         params.inflation_factor, max_obs=params.max_obs)
     out = oracle.sweep_cubes(prm, h["grid_lat"], h["grid_lon"], o.permuted(perm), h["xb"],
-                             h["axb"], cube_n, overlap, bins=(bs, nb, nl, params.bin_deg))
+                             h["axb"], cube_n, overlap, bins=(bs, nb, nl, params.bin_deg, perm))
     assert out["nfail"] == 0
     xa_h, axa_h = xa.cpu().numpy(), axa.cpu().numpy()
     for var, rows in _var_rows(2).items():

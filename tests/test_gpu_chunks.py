@@ -44,7 +44,7 @@ def test_chunked_plain_sweep(cuda, dl, oracle, cube_n, overlap):
     prm = oracle.params(params.nanals, params.nrow, cfg.nlat, cfg.nlon, params.local_dist,
                         params.inflation_factor)
     out = oracle.sweep_cubes(prm, h["grid_lat"], h["grid_lon"], o.permuted(perm), h["xb"],
-                             h["axb"], cube_n, overlap, bins=(bs, nb, nl, params.bin_deg))
+                             h["axb"], cube_n, overlap, bins=(bs, nb, nl, params.bin_deg, perm))
     inc_o = out["axa"] - h["axb"]
     assert np.linalg.norm(axa2.cpu().numpy() - h["axb"] - inc_o) <= 1e-10 * np.linalg.norm(inc_o)
 

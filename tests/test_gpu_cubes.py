@@ -70,7 +70,7 @@ def test_cube_sweep_matches_oracle(cuda, dl, oracle, case):
     prm = oracle.params(params.nanals, n, cfg.nlat, cfg.nlon, params.local_dist,
                         params.inflation_factor, max_obs=params.max_obs)
     out = oracle.sweep_cubes(prm, h["grid_lat"], h["grid_lon"], o.permuted(perm), h["xb"],
-                             h["axb"], cube_n, overlap, bins=(bin_start, nband, nlonbin, bin_deg))
+                             h["axb"], cube_n, overlap, bins=(bin_start, nband, nlonbin, bin_deg, perm))
     assert out["nfail"] == 0
     exp_xa, exp_axa = out["xa"], out["axa"]
     if params.rtps:

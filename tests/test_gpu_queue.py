@@ -48,7 +48,7 @@ def test_grouped_queue_matches_oracle(cuda, dl, oracle):
     prm = oracle.params(params.nanals, params.nrow, cfg.nlat, cfg.nlon, params.local_dist,
                         params.inflation_factor)
     out = oracle.sweep(prm, h["grid_lat"], h["grid_lon"], o.permuted(perm), h["xb"], h["axb"],
-                       bins=(bin_start, nband, nlonbin, params.bin_deg))
+                       bins=(bin_start, nband, nlonbin, params.bin_deg, perm))
     inc_g = axa.cpu().numpy() - h["axb"]
     inc_o = out["axa"] - h["axb"]
     assert np.linalg.norm(inc_g - inc_o) <= 1e-10 * np.linalg.norm(inc_o)

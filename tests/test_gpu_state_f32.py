@@ -22,7 +22,7 @@ def _oracle(oracle, h, params, cube_n, overlap):
     prm = oracle.params(params.nanals, params.nrow, params.nlat, params.nlon, params.local_dist,
                         params.inflation_factor, max_obs=params.max_obs)
     out = oracle.sweep_cubes(prm, h["grid_lat"], h["grid_lon"], o.permuted(perm), h["xb"],
-                             h["axb"], cube_n, overlap, bins=(bs, nb, nl, params.bin_deg))
+                             h["axb"], cube_n, overlap, bins=(bs, nb, nl, params.bin_deg, perm))
     assert out["nfail"] == 0
     return out
 

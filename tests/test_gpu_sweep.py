@@ -44,7 +44,7 @@ def _oracle_sweep(oracle, data_h, params, bin_deg, cols=None):
                         params.local_dist, params.inflation_factor, max_obs=params.max_obs,
                         gc_filter=bool(params.gc_filter), obs_polar=params.obs_polar)
     out = oracle.sweep(prm, data_h["grid_lat"], data_h["grid_lon"], ob, data_h["xb"],
-                       data_h["axb"], cols=cols, bins=(bin_start, nband, nlonbin, bin_deg))
+                       data_h["axb"], cols=cols, bins=(bin_start, nband, nlonbin, bin_deg, perm))
     return out, perm, bin_start, nband, nlonbin, ob, prm
 
 
@@ -55,7 +55,7 @@ def _check_selection(oracle, dbg, perm, bin_start, nband, nlonbin, ob, prm, bin_
     for col in cols:
         clat, clon = glat[col % nlat], glon[col // nlat]
         idx, w = oracle.select_column_binned(prm, ob, bin_start, nband, nlonbin, bin_deg,
-                                             clat, clon)
+                                             clat, clon, perm=perm)
         off, p = int(dbg["offset"][col]), int(dbg["nobsl"][col])
         assert p == len(idx), col
         assert np.array_equal(dbg["csr_idx"][off:off + p], idx), col
@@ -98,6 +98,44 @@ def test_sweep_matches_oracle(cuda, dl, oracle, case):
         inc_o = out["axa"][:, rows] - h["axb"][:, rows]
         assert _rel_err(inc_g, inc_o) <= TOL, var
         assert _rel_err(xa_h[:, rows], out["xa"][:, rows]) <= TOL, var
+
+
+def test_capped_selection_follows_caller_order(cuda, dl, oracle):
+    """MAX_OBS truncation keeps the obs of smallest caller index (gen_yb_dis
+    scans in the caller's order and stops at MAX_OBS, letkf_yb.F90:98-115):
+    for two bin sizes the device lists, mapped back through perm, equal the
+    full-scan oracle on the caller's list, and the analyses agree."""
+    torch = cuda
+    cfg = dl.synth_cfg(45, 90, 3, 32, 6000, seed=0xD1DA0001, cluster_frac=0.7, nclusters=3,
+                       cluster_radius_km=600.0, rscale=(0.1, 10.0))
+    h = None
+    res = []
+    for bin_deg in (3.0, 1.3):
+        print(f"capped selection bin_deg {bin_deg}", flush=True)
+        data, params, eng, obs, xa, axa = _setup(dl, torch, cfg, local_dist=700.0, max_obs=150,
+                                                 bin_deg=bin_deg, obs_polar=80.0)
+        eng.sweep(obs, data["xb"], data["axb"], xa, axa)
+        st = eng.stats()
+        assert st["status"] == 0 and st["nobsl_capped"] > 20
+        if h is None:
+            h = {k: v.cpu().numpy() for k, v in data.items()}
+        dbg = eng.debug(cfg.nobs)
+        o = oracle.Obs(h["lat"], h["lon"], h["yb"], h["dep"], h["rinv"])
+        prm = oracle.params(params.nanals, params.nrow, params.nlat, params.nlon,
+                            params.local_dist, 1.0, max_obs=params.max_obs, obs_polar=80.0)
+        capped = np.nonzero(dbg["nobsl"] >= params.max_obs)[0]
+        cols = np.concatenate([capped[::3], np.arange(0, cfg.nlat * cfg.nlon, 97)])
+        for col in cols:
+            clat, clon = h["grid_lat"][col % cfg.nlat], h["grid_lon"][col // cfg.nlat]
+            idx, w = oracle.select_column(prm, o, clat, clon)
+            off, p = int(dbg["offset"][col]), int(dbg["nobsl"][col])
+            got = dbg["perm"][dbg["csr_idx"][off:off + p]]
+            order = np.argsort(got)
+            assert np.array_equal(got[order], idx), (bin_deg, col)
+            assert np.abs(dbg["csr_w"][off:off + p][order] - w).max(initial=0.0) <= 1e-12
+        res.append(axa.cpu().numpy())
+        eng.close()
+    assert _rel_err(res[1] - h["axb"], res[0] - h["axb"]) <= 1e-12
 
 
 def test_rtps_and_post(cuda, dl, oracle):

@@ -59,7 +59,7 @@ def _oracle(oracle, h, ph, params, vc, vs, vl):
     return oracle.sweep_vertical(prm, h["grid_lat"], h["grid_lon"], o.permuted(perm), ph[perm],
                                  h["xb"], h["axb"], vc, 0, vs, vl, params.vert_dist,
                                  params.cube_nz, bool(params.gc_filter_vertical),
-                                 bins=(bin_start, nband, nlonbin, params.bin_deg),
+                                 bins=(bin_start, nband, nlonbin, params.bin_deg, perm),
                                  cube_n=params.cube_n)
 
 

@@ -131,6 +131,36 @@ def test_binned_selection_equals_full_scan(oracle, L, bin_deg):
             assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), (clat, clon)
 
 
+@pytest.mark.parametrize("bin_deg", [1.1, 2.7, 6.3])
+def test_binned_truncation_follows_caller_order(oracle, bin_deg):
+    """With MAX_OBS binding, the binned selection keeps the obs a full scan of
+    the caller's list keeps (gen_yb_dis stops at MAX_OBS in its scan order,
+    letkf_yb.F90:98-115), whatever the bin size; the bin order alone would
+    keep a different set."""
+    rng = np.random.default_rng(77)
+    lat, lon, yb, dep, ri = _obs(rng, 6000, 3, clustered=True)
+    obs = oracle.Obs(lat, lon, yb, dep, ri)
+    perm, bin_start, nband, nlonbin = oracle.bin_obs(obs, bin_deg, 90.0)
+    ob = obs.permuted(perm)
+    prm = oracle.params(3, 1, 1, 1, 1200.0, 1.0, max_obs=300)
+    ncapped = 0
+    for clat in (-60.0, -10.0, 0.0, 25.0, 45.0, 70.0):
+        for clon in (0.0, 90.0, 100.0, 200.0, 300.0):
+            a_idx, a_w = oracle.select_column(prm, obs, clat, clon)
+            b_idx, b_w = oracle.select_column_binned(prm, ob, bin_start, nband, nlonbin,
+                                                     bin_deg, clat, clon, perm=perm)
+            got = perm[b_idx]
+            order = np.argsort(got)
+            assert np.array_equal(got[order], a_idx), (clat, clon)
+            assert np.array_equal(b_w[order], a_w), (clat, clon)
+            if len(a_idx) == prm.max_obs:
+                ncapped += 1
+                c_idx, _ = oracle.select_column_binned(prm, ob, bin_start, nband, nlonbin,
+                                                       bin_deg, clat, clon)
+                assert not np.array_equal(np.sort(perm[c_idx]), a_idx), (clat, clon)
+    assert ncapped >= 3
+
+
 def test_bin_sort_is_stable(oracle):
     rng = np.random.default_rng(5)
     lat, lon, yb, dep, ri = _obs(rng, 2000, 2)
