@@ -29,7 +29,7 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
 C_FLAGS = ["-std=c11", "-O2", "-fPIC", "-Wall", "-Wextra", f"-I{ROCM}/include"]
 
 LIB_HIP = ["letkf_select.hip", "letkf_transform.hip", "letkf_update.hip", "letkf_wave.hip",
-           "letkf_obsop.hip"]
+           "letkf_obsop.hip", "letkf_noalloc.hip"]
 LIB_C = ["letkf_host.c"]
 SYNTH_HIP = ["synth.hip"]
 

@@ -30,6 +30,12 @@ static int set_err(int code, const char *fmt, ...)
 
 const char *dida_letkf_last_error(void) { return g_err; }
 
+/* The error message of an entry point implemented in a HIP source. */
+int dk_set_error(int code, const char *msg)
+{
+    return set_err(code, "%s", msg);
+}
+
 #define HIP_TRY(expr)                                                                    \
     do {                                                                                 \
         int e_ = (int)(expr);                                                            \

@@ -133,6 +133,10 @@ typedef struct dk_core_args {
 
 int dk_letkf_core_f64(const dk_core_args *a, void *stream);
 
+/* Sets the thread's last error message (dida_letkf_last_error) and returns
+ * code; for entry points implemented in HIP sources. */
+int dk_set_error(int code, const char *msg);
+
 /* Chained analyses (64 < k <= 128): chains of chain_len consecutive
  * analyses of [aid0, aid0 + nanal), drained by nblocks persistent
  * workgroups from *head (zero before the launch); each workgroup keeps the

@@ -27,7 +27,8 @@ TIMING_STAGES, TIMING_PHASES = 1, 2
 
 # Every entry point declared in include/dida_letkf.h.
 EXPORTED = ["dida_letkf_params_default", "letkf_core_", "letkf_core_batch_",
-            "dida_letkf_core_batch_dev", "dida_letkf_create", "dida_letkf_destroy",
+            "dida_letkf_core_batch_dev", "dida_letkf_core_batch_dev_noalloc",
+            "dida_letkf_create", "dida_letkf_destroy",
             "dida_letkf_set_grid", "dida_letkf_sweep", "dida_letkf_sweep_f32",
             "dida_letkf_graph_capture",
             "dida_letkf_graph_launch", "dida_letkf_stats_get", "dida_letkf_debug_arrays",
@@ -124,6 +125,7 @@ def lib():
         L.dida_letkf_debug_arrays.argtypes = [vp] + [C.POINTER(vp)] * 6
         L.dida_letkf_core_batch_dev.argtypes = [C.c_int32, C.c_int32, C.c_int32, vp, vp, vp, vp,
                                                 vp, vp, vp, C.c_double, C.c_int32, vp, vp]
+        L.dida_letkf_core_batch_dev_noalloc.argtypes = L.dida_letkf_core_batch_dev.argtypes
         L.dida_letkf_post.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, vp, C.c_double,
                                       C.c_double, C.c_double, vp, vp, vp, vp, vp, vp, vp, vp]
         L.dida_letkf_set_timing.argtypes = [vp, C.c_int32]

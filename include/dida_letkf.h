@@ -177,6 +177,24 @@ int dida_letkf_core_batch_dev(int32_t batch, int32_t max_obs, int32_t nanals,
                               double *overline_w, double inflation_factor, int32_t gram,
                               int32_t *info, void *stream);
 
+/* The same batched core without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory -- code recorded
+ * in a HIP graph, tight serving loops, harnesses that forbid hipMalloc.  The
+ * per-analysis obs offsets that dida_letkf_core_batch_dev stages in a buffer
+ * it allocates live here in a fixed array of the library on each device (8
+ * MiB, held until the process ends), written by a kernel on `stream`; no host
+ * copy, no synchronisation.  batch is at most 1048576 (larger batches return
+ * DIDA_ERR_ARG before anything runs).  Every call on a device shares that
+ * device's array: calls that overlap on one device (unordered streams) race
+ * on it; calls ordered on one stream do not.  Same arguments, layouts and
+ * results as dida_letkf_core_batch_dev. */
+int dida_letkf_core_batch_dev_noalloc(int32_t batch, int32_t max_obs, int32_t nanals,
+                                      const int32_t *nobsl, const double *hxens,
+                                      const float *rdiaginv, const double *dep,
+                                      const double *rloc, double *trans, double *overline_w,
+                                      double inflation_factor, int32_t gram, int32_t *info,
+                                      void *stream);
+
 /* ------------------------------------------------------------------------ */
 /* Whole-sweep engine: selection -> contractions -> eigensolve -> transform ->
  * update for every column of the grid, the per-cube loop of
