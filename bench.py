@@ -388,7 +388,7 @@ def time_domains(dl, torch, steps=3):
     res, outs = {}, {}
     for name, gram, nstep in (("batched_fp64", dl.GRAM_FP64, 1),
                               ("batched_bf16x3", dl.GRAM_BF16X3, steps)):
-        engs, obs, xa, axa = [], [], [], []
+        engs, obs, xa, axa, tunings = [], [], [], [], []
         for cfg, g in data:
             p = dl.default_params(nanals=cfg.k, nrow=1 + 3 * nz, nlat=cfg.nlat, nlon=cfg.nlon,
                                   local_dist=150.0, inflation_factor=1.05, form=dl.FORM_BATCHED,
@@ -397,7 +397,11 @@ def time_domains(dl, torch, steps=3):
             e.set_grid(g["grid_lat"], g["grid_lon"])
             o = dl.DeviceObs(g["lat"], g["lon"], g["yb"], g["dep"], g["rinv"])
             x, ax = torch.empty_like(g["xb"]), torch.empty_like(g["axb"])
-            e.sweep(o, g["xb"], g["axb"], x, ax)     # sizes the selection lists
+            if gram == dl.GRAM_BF16X3:
+                # each domain keeps its own tuned layout in the graph (untimed)
+                tunings.append(e.autotune(o, g["xb"], g["axb"], x, ax))
+            else:
+                e.sweep(o, g["xb"], g["axb"], x, ax)     # sizes the selection lists
             engs.append(e)
             obs.append(o)
             xa.append(x)
@@ -424,6 +428,11 @@ def time_domains(dl, torch, steps=3):
         res[name] = {"ms_per_graph": ms, "analyses_per_s": ncol / (ms / 1e3),
                      "nobsl_mean": sum(t["nobsl_sum"] for t in st) / ncol,
                      "nonconverged": sum(t["nonconverged"] for t in st)}
+        if tunings:
+            res[name]["tuned_layouts"] = [
+                {key: t[key] for key in ("bin_deg", "chain_len", "xcd_tiles", "bucketed",
+                                         "bucket_chain", "bucket_count", "ms_initial",
+                                         "ms_best")} for t in tunings]
         log(f"  domains {name}: {res[name]}")
     err = {}
     for var, rows in var_rows(nz).items():

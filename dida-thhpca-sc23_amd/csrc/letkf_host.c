@@ -116,6 +116,15 @@ static int check_params(const dida_letkf_params *p)
     if (p->cube_n < 1 || p->overlap < 0 || cube_step(p) < 1)
         return set_err(DIDA_ERR_ARG, "cube_n = %d, overlap = %d: need cube_n >= 1 and "
                        "0 <= overlap < cube_n", p->cube_n, p->overlap);
+    int nbucket_set = 0;
+    for (int b = 0; b < 4; ++b) {
+        if (p->bucket_chain[b] < 0) return set_err(DIDA_ERR_ARG, "bucket_chain must be >= 0");
+        nbucket_set += p->bucket_chain[b] > 0;
+    }
+    if (nbucket_set && !(p->form == DIDA_FORM_BATCHED && p->nanals <= 64))
+        return set_err(DIDA_ERR_UNSUPPORTED,
+                       "bucket_chain: chains per obs-count bucket are a batched-form option "
+                       "for k <= 64");
     if (p->vert_dist > 0.0) {
         if (p->overlap != 0)
             return set_err(DIDA_ERR_UNSUPPORTED,
@@ -184,6 +193,13 @@ struct dida_letkf_ctx {
     void *chain_tmp;
     size_t chain_tmp_bytes;
     double *ymax;
+    /* per-bucket chains (fused batched form): on when bucketed, chain length
+     * bucket_len[b] for analyses of local count in bucket b (thresholds
+     * kBucketLo); the chain descriptors are built on the device every sweep */
+    int32_t bucketed;
+    int32_t bucket_len[DK_NBUCKET];
+    int32_t *line_cnt, *chain_col0, *chain_ncol;
+    int64_t *line_off;
     /* vertical localisation: analyses per column, binned obs ln p, cube
      * centre ln p [ncol * ncube], rows per cube (CSR over cubes) */
     int32_t ncube, ps_row;
@@ -234,7 +250,8 @@ int dida_letkf_destroy(dida_letkf_ctx *c)
                     c->chain_order, c->queue_head, c->chain_tmp, c->ymax, c->phase_clk,
                     c->obs_lnp, c->cube_lnp, c->cube_row_off, c->cube_rows, c->ca_lat,
                     c->ca_lon, c->trans_all, c->vkey_in, c->vkey_out, c->vpos_in,
-                    c->vpos_out, c->vidx, c->vw, c->vsort_tmp, c->vscr};
+                    c->vpos_out, c->vidx, c->vw, c->vsort_tmp, c->vscr, c->line_cnt,
+                    c->chain_col0, c->chain_ncol, c->line_off};
     for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; ++i) dfree(ptrs[i]);
     free(c);
     return DIDA_OK;
@@ -266,9 +283,14 @@ static dk_bin_geom bin_geom(double bd, double obs_polar)
     return g;
 }
 
+/* Lower bounds of the local-obs-count buckets of the per-bucket chains. */
+static const int32_t kBucketLo[DK_NBUCKET] = {0, 64, 512, 2048};
+
 /* Bin size, chain length and queue order of the sweeps that follow (within
- * the sizes the context was created for). */
-static int set_layout(dida_letkf_ctx *c, double bin_deg, int32_t chain_len, int32_t tiles)
+ * the sizes the context was created for); blen (fused batched form): one
+ * chain length per obs-count bucket instead of chain_len, one queue. */
+static int set_layout(dida_letkf_ctx *c, double bin_deg, int32_t chain_len, int32_t tiles,
+                      const int32_t *blen)
 {
     dk_bin_geom g = bin_geom(bin_deg, c->prm.obs_polar);
     if (g.nbins > c->nbins_max) return set_err(DIDA_ERR_ARG, "bins finer than allocated");
@@ -283,6 +305,12 @@ static int set_layout(dida_letkf_ctx *c, double bin_deg, int32_t chain_len, int3
         c->chains_per_lon = (c->nalat + chain_len - 1) / chain_len;
         c->nchain = c->chains_per_lon * c->nalon;
         c->ngroup = tiles ? DK_XCD_GROUPS : 0;
+        c->bucketed = blen != NULL && !c->chained;
+        if (c->bucketed) {
+            c->ngroup = 0;
+            for (int b = 0; b < DK_NBUCKET; ++b)
+                c->bucket_len[b] = blen[b] < 1 ? 1 : (blen[b] > c->nalat ? c->nalat : blen[b]);
+        }
     }
     return DIDA_OK;
 }
@@ -326,7 +354,14 @@ int dida_letkf_create(dida_letkf_ctx **out, const dida_letkf_params *prm, int32_
     /* automatic chain length: 8 columns for the fused kernel (its autotuner
      * refines it), 32 analyses for the chained workgroups (measured best) */
     const int32_t cl0 = prm->form == DIDA_FORM_BATCHED && k > 64 ? 32 : 8;
-    rc = set_layout(c, bd, prm->chain_len > 0 ? prm->chain_len : cl0, prm->xcd_tiles);
+    const int32_t cl1 = prm->chain_len > 0 ? prm->chain_len : cl0;
+    int32_t blen[DK_NBUCKET];
+    int bset = 0;
+    for (int b = 0; b < DK_NBUCKET; ++b) {
+        blen[b] = prm->bucket_chain[b] > 0 ? prm->bucket_chain[b] : cl1;
+        bset |= prm->bucket_chain[b] > 0;
+    }
+    rc = set_layout(c, bd, cl1, prm->xcd_tiles, bset ? blen : NULL);
     if (rc) {
         free(c);
         return rc;
@@ -355,7 +390,9 @@ int dida_letkf_create(dida_letkf_ctx **out, const dida_letkf_params *prm, int32_
     DMALLOC(c->uy, no * sizeof(double));
     DMALLOC(c->uz, no * sizeof(double));
     c->chained = prm->form == DIDA_FORM_BATCHED && k > 64;
-    c->ldy = prm->form == DIDA_FORM_BATCHED && !c->chained ? (k <= 32 ? 32 : 64) : k;
+    /* the one-wavefront kernels (batched form, and the plain form with k <= 64)
+     * read binned Yb rows of 32 or 64 values, zero past k */
+    c->ldy = k <= 64 && !c->chained ? (k <= 32 ? 32 : 64) : k;
     if (c->chained && prm->gram != DIDA_GRAM_FP64) c->ldy = (k + 3) & ~3; /* float4 rows */
     if (c->chained && prm->gram != DIDA_GRAM_FP64)
         DMALLOC(c->yb32_b, no * (size_t)c->ldy * sizeof(float));
@@ -410,6 +447,12 @@ int dida_letkf_create(dida_letkf_ctx **out, const dida_letkf_params *prm, int32_
             ncu = 256;
         c->nblocks = ncu * dk_wave_blocks_per_cu(k, prm->gram);
         c->ncu = ncu;
+        if (!c->chained) {
+            DMALLOC(c->line_cnt, (size_t)c->nalon * sizeof(int32_t));
+            DMALLOC(c->line_off, (size_t)(c->nalon + 1) * sizeof(int64_t));
+            DMALLOC(c->chain_col0, (size_t)c->nchain_max * sizeof(int32_t));
+            DMALLOC(c->chain_ncol, (size_t)c->nchain_max * sizeof(int32_t));
+        }
         if (prm->vert_dist > 0.0) DMALLOC(c->obs_lnp, no * sizeof(double));
         /* chained analyses: one workgroup per CU, 128 x 128 eigenvectors each */
         if (c->chained) DMALLOC(c->vscr, (size_t)ncu * 128 * 128 * sizeof(double));
@@ -790,7 +833,37 @@ static int sweep_batched(dida_letkf_ctx *c, const double *xb, const double *axb,
     if (c->prm.gram == DIDA_GRAM_FP16X3)
         HIP_TRY(dk_obs_ymax(c->last_nobs_pending, ba.k, c->ldy, c->yb_b, c->rinv_b, c->ymax, s));
     ba.ngroup = c->ngroup;
-    if (c->ngroup > 0) {
+    if (c->bucketed) {
+        /* chains of one obs-count bucket, built from this sweep's counts */
+        dk_chain_build_args cb;
+        memset(&cb, 0, sizeof cb);
+        cb.nlat = c->nalat;
+        cb.nlon = c->nalon;
+        cb.nbucket = DK_NBUCKET;
+        for (int b = 0; b < DK_NBUCKET; ++b) {
+            cb.lo[b] = kBucketLo[b];
+            cb.len[b] = c->bucket_len[b];
+        }
+        cb.nobsl = c->nobsl;
+        cb.line_cnt = c->line_cnt;
+        cb.line_off = c->line_off;
+        cb.chain_col0 = c->chain_col0;
+        cb.chain_ncol = c->chain_ncol;
+        HIP_TRY(dk_chain_build(&cb, 0, s));
+        size_t tb = c->scan_tmp_bytes;
+        HIP_TRY(dk_scan_offsets(c->scan_tmp, &tb, c->line_cnt, c->line_off, c->nalon, s));
+        HIP_TRY(dk_chain_build(&cb, 1, s));
+        ba.bucketed = 1;
+        ba.chain_col0 = c->chain_col0;
+        ba.chain_ncol = c->chain_ncol;
+        ba.nchain_dev = c->line_off + c->nalon;
+        ba.nchain = c->nchain_max;
+        ba.ngroup = 0;
+        HIP_TRY(dk_chain_keys(&ba, c->chain_key, c->chain_id, s));
+        tb = c->chain_tmp_bytes;
+        HIP_TRY(dk_sort_pairs_u32(c->chain_tmp, &tb, c->chain_key, c->chain_key_sorted,
+                                  c->chain_id, c->chain_order, c->nchain_max, 32, s));
+    } else if (c->ngroup > 0) {
         HIP_TRY(dk_chain_groups(&ba, c->chain_order, s));
     } else {
         HIP_TRY(dk_chain_keys(&ba, c->chain_key, c->chain_id, s));
@@ -813,7 +886,8 @@ static int sweep_batched(dida_letkf_ctx *c, const double *xb, const double *axb,
  * the batched form with k > 64, chains of warm-started analyses. */
 static int core_launch(dida_letkf_ctx *c, dk_core_args *ca, hipStream_t s)
 {
-    if (!c->chained) return dk_letkf_core_f64(ca, s);
+    if (!c->chained)
+        return c->prm.nanals <= 64 ? dk_letkf_plain_wave(ca, s) : dk_letkf_core_f64(ca, s);
     dk_chain_args ch;
     ch.chain_len = c->chain_len;
     ch.nchain = (ca->nanal + c->chain_len - 1) / c->chain_len;
@@ -1124,50 +1198,100 @@ int dida_letkf_autotune(dida_letkf_ctx *c, const dida_letkf_obs *o, const double
     }
     const double bd0 = c->bin_deg;
     const int32_t cl0 = c->chain_len, tl0 = c->ngroup > 0;
+    const int bucketed0 = c->bucketed;
+    int32_t bl0[DK_NBUCKET];
+    memcpy(bl0, c->bucket_len, sizeof bl0);
     double best_bd = bd0, best_ms = 0.0;
     int32_t best_cl = cl0, best_tl = tl0;
+    int best_bk = bucketed0;
+    int32_t best_bl[DK_NBUCKET];
+    memcpy(best_bl, bl0, sizeof best_bl);
     rc = timed_sweep(c, o, xb, axb, xa, axa, s, e0, e1, &best_ms);
     out->ms_initial = best_ms;
     out->ncand = 1;
+    const int batched = c->prm.form == DIDA_FORM_BATCHED;
+    /* one candidate: time it and keep it if faster */
+#define TRY_LAYOUT(BD, CL, TL, BL)                                                       \
+    do {                                                                                 \
+        rc = set_layout(c, (BD), (CL), (TL), (BL));                                      \
+        if (rc) break;                                                                   \
+        double ms_ = 0.0;                                                                \
+        rc = timed_sweep(c, o, xb, axb, xa, axa, s, e0, e1, &ms_);                       \
+        out->ncand++;                                                                    \
+        if (!rc && ms_ < best_ms) {                                                      \
+            best_ms = ms_;                                                               \
+            best_bd = (BD);                                                              \
+            best_cl = (CL);                                                              \
+            best_tl = (TL);                                                              \
+            best_bk = (BL) != NULL;                                                      \
+            if (BL) memcpy(best_bl, (BL), sizeof best_bl);                               \
+        }                                                                                \
+    } while (0)
+    /* 1. the bin size (a capped column keeps the same obs at every size) */
     const double bins[3] = {c->theta_deg / 2.0, c->theta_deg / 4.0, c->theta_deg / 8.0};
-    /* a capped column keeps the same obs at every bin size */
-    const int nbin = 3;
+    for (int ib = 0; ib < 3 && !rc; ++ib)
+        if (bins[ib] != bd0 || bucketed0) TRY_LAYOUT(bins[ib], cl0, tl0, NULL);
+    /* 2. one chain length for all analyses, and the queue order */
     const int32_t chains[6] = {2, 4, 8, 16, 32, 64};
-    const int nchn = c->prm.form == DIDA_FORM_BATCHED ? 6 : 1;
-    const int ntl = c->prm.form == DIDA_FORM_BATCHED ? 2 : 1;
-    for (int ib = 0; ib < nbin && !rc; ++ib) {
-        const double bd = nbin == 1 ? bd0 : bins[ib];
-        for (int ic = 0; ic < nchn && !rc; ++ic) {
-            const int32_t cl = nchn == 1 ? cl0 : chains[ic];
-            for (int it = 0; it < ntl && !rc; ++it) {
-                const int32_t tl = ntl == 1 ? tl0 : it;
-                if (bd == bd0 && cl == cl0 && tl == tl0) continue;
-                rc = set_layout(c, bd, cl, tl);
-                if (rc) break;
-                double ms = 0.0;
-                rc = timed_sweep(c, o, xb, axb, xa, axa, s, e0, e1, &ms);
-                out->ncand++;
-                if (!rc && ms < best_ms) {
-                    best_ms = ms;
-                    best_bd = bd;
-                    best_cl = cl;
-                    best_tl = tl;
-                }
+    const double bd1 = best_bd;
+    for (int ic = 0; ic < 6 && batched && !rc; ++ic)
+        for (int it = 0; it < 2 && !rc; ++it)
+            if (!(bd1 == best_bd && chains[ic] == best_cl && it == best_tl && !best_bk))
+                TRY_LAYOUT(bd1, chains[ic], it, NULL);
+    /* 3. fused form: a chain length per obs-count bucket (1: cold starts),
+     * one bucket at a time from the best single length */
+    int64_t bcount[DK_NBUCKET] = {0, 0, 0, 0};
+    if (batched && !c->chained && !rc) {
+        int32_t *nb = (int32_t *)malloc((size_t)c->nsel * sizeof(int32_t));
+        if (!nb) rc = set_err(DIDA_ERR_NOMEM, "autotune: host counts");
+        else if (hipMemcpy(nb, c->nobsl, (size_t)c->nsel * sizeof(int32_t),
+                           hipMemcpyDeviceToHost) != hipSuccess)
+            rc = set_err(DIDA_ERR_HIP, "autotune: counts copy");
+        for (int64_t i = 0; i < c->nsel && !rc; ++i) {
+            int b = 0;
+            while (b + 1 < DK_NBUCKET && nb[i] >= kBucketLo[b + 1]) ++b;
+            bcount[b]++;
+        }
+        free(nb);
+        int32_t bl[DK_NBUCKET];
+        for (int b = 0; b < DK_NBUCKET; ++b) bl[b] = best_bk ? best_bl[b] : best_cl;
+        const int32_t blens[7] = {1, 2, 4, 8, 16, 32, 64};
+        const double bd2 = best_bd;
+        const int32_t cl2 = best_cl;
+        if (!best_bk && !rc) TRY_LAYOUT(bd2, cl2, 0, bl);
+        for (int b = 0; b < DK_NBUCKET && !rc; ++b) {
+            if (!bcount[b]) continue;
+            const int32_t keep = bl[b];
+            int32_t bestb = keep;
+            for (int j = 0; j < 7 && !rc; ++j) {
+                if (blens[j] == keep) continue;
+                bl[b] = blens[j];
+                const double before = best_ms;
+                TRY_LAYOUT(bd2, cl2, 0, bl);
+                if (best_ms < before) bestb = blens[j];
             }
+            bl[b] = bestb;
         }
     }
+#undef TRY_LAYOUT
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     c->timing = timing;
-    int rc2 = set_layout(c, best_bd, best_cl, best_tl);
+    int rc2 = set_layout(c, best_bd, best_cl, best_tl, best_bk ? best_bl : NULL);
     if (!rc) rc = rc2;
     if (rc) return rc;
     rc = dida_letkf_sweep(c, o, xb, axb, xa, axa, stream);
     if (rc < 0) return rc;
-    out->chain_len = c->prm.form == DIDA_FORM_BATCHED ? c->chain_len : 0;
+    out->chain_len = batched ? c->chain_len : 0;
     out->bin_deg = c->bin_deg;
     out->ms_best = best_ms;
-    out->xcd_tiles = c->prm.form == DIDA_FORM_BATCHED ? c->ngroup > 0 : 0;
+    out->xcd_tiles = batched ? c->ngroup > 0 : 0;
+    out->bucketed = c->bucketed;
+    for (int b = 0; b < DK_NBUCKET; ++b) {
+        out->bucket_lo[b] = kBucketLo[b];
+        out->bucket_chain[b] = c->bucketed ? c->bucket_len[b] : 0;
+        out->bucket_count[b] = bcount[b];
+    }
     return DIDA_OK;
 }
 
@@ -1561,6 +1685,78 @@ void letkf_core_batch_(const int32_t *batch, const int32_t *batch_large_matrix,
                              overline_w, *inflation_factor, NULL);
     if (rc) fprintf(stderr, "letkf_core_batch_: %s\n", dida_letkf_last_error());
     if (pflag && *pflag) printf("letkf_core_batch_: batch %d done\n", *batch);
+}
+
+/* ------------------------------------------------------------------------ */
+/* REAL(4) forms: the reference's letkf_core / letkf_core_batch built with
+ * yb_type = ayb_type = real(4) (da_config.inc:7-10), where hxens, dep, rloc,
+ * trans and overline_w are REAL(4).  The values are widened to FP64 on the
+ * host, analysed by the FP64 kernels and rounded once to REAL(4). */
+static double *widen(const float *x, size_t n)
+{
+    double *y = (double *)malloc((n ? n : 1) * sizeof(double));
+    if (y)
+        for (size_t i = 0; i < n; ++i) y[i] = (double)x[i];
+    return y;
+}
+
+static int core_batch_host_r4(int32_t batch, int32_t max_obs, int32_t k, const int32_t *nobsl,
+                              const float *hxens, const float *rdiaginv, const float *dep,
+                              const float *rloc, float *trans, float *wbar, double rho,
+                              int32_t *info)
+{
+    if (batch < 0 || max_obs < 0 || k < 1) return set_err(DIDA_ERR_ARG, "bad sizes");
+    const size_t nob = (size_t)batch * (size_t)max_obs;
+    const size_t ntr = (size_t)batch * k * k, nwb = (size_t)batch * k;
+    double *hx = widen(hxens, nob * (size_t)k), *dp = widen(dep, nob), *rl = widen(rloc, nob);
+    double *tr = (double *)malloc((ntr ? ntr : 1) * sizeof(double));
+    double *wb = (double *)malloc((nwb ? nwb : 1) * sizeof(double));
+    int rc = DIDA_OK;
+    if (!hx || !dp || !rl || !tr || !wb) rc = set_err(DIDA_ERR_NOMEM, "host buffers");
+    if (!rc)
+        rc = core_batch_host(batch, max_obs, k, nobsl, hx, rdiaginv, dp, rl, tr, wb, rho, info);
+    if (!rc) {
+        for (size_t i = 0; i < ntr; ++i) trans[i] = (float)tr[i];
+        for (size_t i = 0; i < nwb; ++i) wbar[i] = (float)wb[i];
+    }
+    free(hx);
+    free(dp);
+    free(rl);
+    free(tr);
+    free(wb);
+    return rc;
+}
+
+void letkf_core_r4_(const int32_t *nanals, const int32_t *nobsl, const float *hxens,
+                    const float *rdiaginv, const float *dep, const float *rloc, float *trans,
+                    float *overline_w, const double *inflation_factor, const int32_t *pflag)
+{
+    int32_t info = 0;
+    int32_t p = *nobsl;
+    int rc = core_batch_host_r4(1, p, *nanals, &p, hxens, rdiaginv, dep, rloc, trans,
+                                overline_w, *inflation_factor, &info);
+    if (rc) fprintf(stderr, "letkf_core_r4_: %s\n", dida_letkf_last_error());
+    else if (info == -2) fprintf(stderr, "letkf_core_r4_: non-finite input, trans is NaN\n");
+    else if (info < 0) fprintf(stderr, "letkf_core_r4_: eigensolver did not converge\n");
+    if (pflag && *pflag) {
+        printf("nobsl %d  jacobi sweeps %d\n", p, info);
+        printf("overline_w");
+        for (int i = 0; i < *nanals; ++i) printf(" %.6g", (double)overline_w[i]);
+        printf("\n");
+    }
+}
+
+void letkf_core_batch_r4_(const int32_t *batch, const int32_t *batch_large_matrix,
+                          const int32_t *max_obs, const int32_t *nanals, const int32_t *nobsl,
+                          float *hxens, const float *rdiaginv, const float *dep,
+                          const float *rloc, float *trans, float *overline_w,
+                          const double *inflation_factor, const int32_t *pflag)
+{
+    (void)batch_large_matrix;
+    int rc = core_batch_host_r4(*batch, *max_obs, *nanals, nobsl, hxens, rdiaginv, dep, rloc,
+                                trans, overline_w, *inflation_factor, NULL);
+    if (rc) fprintf(stderr, "letkf_core_batch_r4_: %s\n", dida_letkf_last_error());
+    if (pflag && *pflag) printf("letkf_core_batch_r4_: batch %d done\n", *batch);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -248,9 +248,35 @@ typedef struct dk_batched_args {
      * XCD, so neighbouring analyses share an L2), then the others;
      * queue_head holds ngroup counters */
     int32_t ngroup, pad3_;
+    /* per-bucket chains (bucketed != 0, one queue): chain c covers the
+     * chain_ncol[c] analyses of one longitude from column chain_col0[c];
+     * *nchain_dev chains (queue entries past it are never taken) */
+    int32_t bucketed, pad5_;
+    const int32_t *chain_col0, *chain_ncol;
+    const int64_t *nchain_dev;
+    /* selection entries of analysis a at offset[a] - base (runs of a sweep
+     * over a fixed workspace); trans_out holds analysis a at (a - trans_aid0) k^2 */
+    int64_t base, trans_aid0;
 } dk_batched_args;
 
 #define DK_XCD_GROUPS 8
+
+/* Chains per obs-count bucket: analysis of local count p is in bucket b, the
+ * last with p >= lo[b] (lo[0] = 0); a chain is a run of adjacent latitudes
+ * of one longitude in the same bucket, at most len[b] long (1: every analysis
+ * starts cold).  Built per sweep: count pass (line_cnt[ilon]), exclusive scan
+ * of the counts into line_off[nlon + 1], write pass (descriptors). */
+#define DK_NBUCKET 4
+typedef struct dk_chain_build_args {
+    int32_t nlat, nlon;
+    int32_t nbucket, pad_;
+    int32_t lo[DK_NBUCKET], len[DK_NBUCKET];
+    const int32_t *nobsl;
+    int32_t *line_cnt;
+    const int64_t *line_off;
+    int32_t *chain_col0, *chain_ncol;
+} dk_chain_build_args;
+int dk_chain_build(const dk_chain_build_args *a, int write, void *stream);
 
 /* Phases of the batched kernel timed by phase_clk (wave 0 of each block). */
 #define DK_PHASE_GRAM 0
@@ -268,6 +294,11 @@ int dk_chain_groups(const dk_batched_args *a, int32_t *order, void *stream);
  * workgroups per CU for (k, gram), and the persistent launch. */
 int dk_wave_blocks_per_cu(int k, int gram);
 int dk_letkf_wave(const dk_batched_args *a, int nblocks, void *stream);
+/* Plain form, k <= 64: the transforms of analyses [aid0, aid0 + nanal) in
+ * grid order, one wavefront each, FP64 Gram, cold-started register Jacobi;
+ * trans (and wbar, info when set) as dk_letkf_core_f64.  The binned Yb rows
+ * are KP = 32 or 64 values apart (zero past k), as for the batched form. */
+int dk_letkf_plain_wave(const dk_core_args *a, void *stream);
 /* ymax[i] = max_e |yb[i k + e]| sqrt(rinv[i]); the second form reads FP32 rows. */
 int dk_obs_ymax(int32_t nobs, int32_t k, int32_t ldy, const double *yb, const float *rinv,
                 double *ymax, void *stream);

@@ -25,6 +25,8 @@
 //             on FP64 MFMA; trans = W + wbar 1^T (letkf_core.F90:129-225).
 //   update    Xa = trans^T Xb' + xbar_b, mean, re-centre, RTPS (xb_2_xa,
 //             letkf_compute.F90:261-299) on FP64 MFMA.
+#include <cstring>
+
 #include "letkf_device.h"
 #include "letkf_kernels.h"
 
@@ -738,7 +740,7 @@ __device__ bool wave_column(const dk_batched_args& a, int64_t col, int cube, boo
   const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
   const int k = a.k;
   PhaseClock clk(a.phase_clk);
-  const int64_t off = a.offset[col];
+  const int64_t off = a.offset[col] - a.base;
   int p = a.nobsl[col];
   p = p < 0 ? 0 : (p > a.max_p ? a.max_p : p);
   if (off < 0 || off + p > a.capacity) p = 0;
@@ -849,13 +851,13 @@ __device__ bool wave_column(const dk_batched_args& a, int64_t col, int cube, boo
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int i = rowB(I, r, g);
-        if (i < k) a.wbar_out[aid * k + i] = wb[I][r];
+        if (i < k) a.wbar_out[(aid - a.trans_aid0) * k + i] = wb[I][r];
       }
   clk.mark(DK_PHASE_TRANSFORM);
   if constexpr (MODE == kModeTransOut) {
     // horizontal cubes: the transform goes to HBM; a per-column kernel
     // applies the mean of the covering cubes' transforms
-    double* tr = a.trans_out + aid * k * k;
+    double* tr = a.trans_out + (aid - a.trans_aid0) * k * k;
 #pragma unroll
     for (int I = 0; I < NJ; ++I)
 #pragma unroll
@@ -891,6 +893,15 @@ __device__ bool wave_column(const dk_batched_args& a, int64_t col, int cube, boo
   return conv && !bad;
 }
 
+// The plain form: one wavefront per analysis of [aid0, aid0 + gridDim.x) in
+// grid order, every eigensolve cold (no chains), transform to HBM.
+template <int KP>
+__global__ __launch_bounds__(64, 1) void k_letkf_plain(dk_batched_args a, int32_t aid0) {
+  __shared__ WSmem<KP> sm;
+  Wave<KP, DK_GRAM_FP64> w;
+  wave_column<KP, DK_GRAM_FP64, kModeTransOut>(a, (int64_t)aid0 + blockIdx.x, 0, false, sm, w);
+}
+
 // First longitude of stripe g of n groups (groups of whole longitudes).
 __device__ __forceinline__ int stripe_lo(int g, int n, int nlon) {
   return (int)(((int64_t)g * nlon) / n);
@@ -904,7 +915,8 @@ __global__ __launch_bounds__(64, 1) void k_letkf_wave(dk_batched_args a) {
   // one queue (ngroup 0) or ngroup group queues, own group first
   const int ng = a.ngroup > 0 ? a.ngroup : 1;
   const int g0 = a.ngroup > 0 ? (int)(blockIdx.x % (unsigned)ng) : 0;
-  int s = 0, gq = g0, beg = 0, cnt = a.nchain;
+  int s = 0, gq = g0, beg = 0;
+  int cnt = a.bucketed ? (int)*a.nchain_dev : a.nchain;
   if (a.ngroup > 0) {
     beg = stripe_lo(gq, ng, a.nlon) * a.chains_per_lon;
     cnt = stripe_lo(gq + 1, ng, a.nlon) * a.chains_per_lon - beg;
@@ -921,9 +933,17 @@ __global__ __launch_bounds__(64, 1) void k_letkf_wave(dk_batched_args a) {
       continue;
     }
     const int chain = a.queue ? a.queue[beg + item] : beg + item;
-    const int ilon = chain / a.chains_per_lon;
-    const int lat0 = (chain % a.chains_per_lon) * a.chain_len;
-    const int lat1 = min(lat0 + a.chain_len, a.nlat);
+    int ilon, lat0, lat1;
+    if (a.bucketed) {
+      const int c0 = a.chain_col0[chain];
+      ilon = c0 / a.nlat;
+      lat0 = c0 - ilon * a.nlat;
+      lat1 = lat0 + a.chain_ncol[chain];
+    } else {
+      ilon = chain / a.chains_per_lon;
+      lat0 = (chain % a.chains_per_lon) * a.chain_len;
+      lat1 = min(lat0 + a.chain_len, a.nlat);
+    }
     // an analysis warm-starts from the previous one of its chain unless that
     // one was not finite or did not converge
     bool prev_ok = false;
@@ -951,9 +971,23 @@ __global__ __launch_bounds__(64, 1) void k_letkf_wave(dk_batched_args a) {
 __global__ void k_chain_keys(dk_batched_args a, uint32_t* key, int32_t* id) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= a.nchain) return;
-  const int ilon = c / a.chains_per_lon;
-  const int lat0 = (c % a.chains_per_lon) * a.chain_len;
-  const int lat1 = min(lat0 + a.chain_len, a.nlat);
+  int ilon, lat0, lat1;
+  if (a.bucketed) {
+    // a.nchain entries are sorted; those past the built chains go last
+    if (c >= (int)*a.nchain_dev) {
+      key[c] = 0xFFFFFFFFu;
+      id[c] = c;
+      return;
+    }
+    const int c0 = a.chain_col0[c];
+    ilon = c0 / a.nlat;
+    lat0 = c0 - ilon * a.nlat;
+    lat1 = lat0 + a.chain_ncol[c];
+  } else {
+    ilon = c / a.chains_per_lon;
+    lat0 = (c % a.chains_per_lon) * a.chain_len;
+    lat1 = min(lat0 + a.chain_len, a.nlat);
+  }
   double cost = 0.0;
   for (int ilat = lat0; ilat < lat1; ++ilat)
     cost += (double)a.nobsl[(int64_t)ilon * a.nlat + ilat] + 10.0 * a.k;
@@ -1022,6 +1056,36 @@ __global__ __launch_bounds__(64) void k_update_cols(dk_update_args u) {
                          nullptr, u.nrow);
 }
 
+__device__ __forceinline__ int bucket_of(int p, const dk_chain_build_args& a) {
+  int b = 0;
+  while (b + 1 < a.nbucket && p >= a.lo[b + 1]) ++b;
+  return b;
+}
+
+// One thread per longitude: the chains of its latitudes (runs of one bucket,
+// at most the bucket's length), counted (kWrite false) or written.
+template <bool kWrite>
+__global__ void k_chain_build(dk_chain_build_args a) {
+  const int ilon = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ilon >= a.nlon) return;
+  const int32_t* p = a.nobsl + (int64_t)ilon * a.nlat;
+  const int64_t out = kWrite ? a.line_off[ilon] : 0;
+  int cnt = 0;
+  for (int i = 0; i < a.nlat;) {
+    const int b = bucket_of(p[i], a);
+    const int len = a.len[b] > 0 ? a.len[b] : 1;
+    int j = i + 1;
+    while (j < a.nlat && j - i < len && bucket_of(p[j], a) == b) ++j;
+    if (kWrite) {
+      a.chain_col0[out + cnt] = ilon * a.nlat + i;
+      a.chain_ncol[out + cnt] = j - i;
+    }
+    ++cnt;
+    i = j;
+  }
+  if (!kWrite) a.line_cnt[ilon] = cnt;
+}
+
 // Grouped queue: chain (ilon, latchain) of stripe g goes to position
 // stripe_lo(g) cpl + latchain * width(g) + (ilon - stripe_lo(g)).
 __global__ void k_chain_groups(dk_batched_args a, int32_t* order) {
@@ -1081,6 +1145,37 @@ extern "C" int dk_wave_blocks_per_cu(int k, int gram) {
   return wocc_dispatch<64>(gram);
 }
 
+extern "C" int dk_letkf_plain_wave(const dk_core_args* c, void* stream) {
+  if (c->nanal <= 0) return 0;
+  if (c->k < 2 || c->k > 64 || c->gram != DK_GRAM_FP64 || !c->idx) return (int)hipErrorInvalidValue;
+  dk_batched_args a;
+  memset(&a, 0, sizeof a);
+  a.k = c->k;
+  a.ldy = c->k <= 32 ? 32 : 64;
+  a.max_sweeps = c->max_sweeps;
+  a.max_p = c->max_p;
+  a.predict = c->predict;
+  a.rho = c->rho;
+  a.tol = c->tol;
+  a.capacity = c->capacity;
+  a.offset = c->offset;
+  a.nobsl = c->nobsl;
+  a.idx = c->idx;
+  a.rloc = c->rloc;
+  a.yb = c->yb;
+  a.rinv = c->rinv;
+  a.dep = c->dep;
+  a.info = c->info;
+  a.base = c->base;
+  a.trans_out = c->trans;
+  a.trans_aid0 = c->aid0;
+  a.wbar_out = c->wbar;
+  hipStream_t s = (hipStream_t)stream;
+  if (c->k <= 32) k_letkf_plain<32><<<c->nanal, 64, 0, s>>>(a, c->aid0);
+  else k_letkf_plain<64><<<c->nanal, 64, 0, s>>>(a, c->aid0);
+  return (int)hipGetLastError();
+}
+
 extern "C" int dk_update_cols(const dk_update_args* u, void* stream) {
   if (u->ncol <= 0) return 0;
   if (u->k < 2 || u->k > 64) return (int)hipErrorInvalidValue;
@@ -1093,6 +1188,15 @@ extern "C" int dk_update_cols(const dk_update_args* u, void* stream) {
 extern "C" int dk_chain_groups(const dk_batched_args* a, int32_t* order, void* stream) {
   if (a->nchain <= 0 || a->ngroup <= 0) return 0;
   k_chain_groups<<<(a->nchain + 255) / 256, 256, 0, (hipStream_t)stream>>>(*a, order);
+  return (int)hipGetLastError();
+}
+
+extern "C" int dk_chain_build(const dk_chain_build_args* a, int write, void* stream) {
+  if (a->nlon <= 0) return 0;
+  if (a->nbucket < 1 || a->nbucket > DK_NBUCKET) return (int)hipErrorInvalidValue;
+  const unsigned nb = (unsigned)((a->nlon + 127) / 128);
+  if (write) k_chain_build<true><<<nb, 128, 0, (hipStream_t)stream>>>(*a);
+  else k_chain_build<false><<<nb, 128, 0, (hipStream_t)stream>>>(*a);
   return (int)hipGetLastError();
 }
 

@@ -27,6 +27,7 @@ TIMING_STAGES, TIMING_PHASES = 1, 2
 
 # Every entry point declared in include/dida_letkf.h.
 EXPORTED = ["dida_letkf_params_default", "letkf_core_", "letkf_core_batch_",
+            "letkf_core_r4_", "letkf_core_batch_r4_",
             "dida_letkf_core_batch_dev", "dida_letkf_core_batch_dev_noalloc",
             "dida_letkf_create", "dida_letkf_destroy",
             "dida_letkf_set_grid", "dida_letkf_sweep", "dida_letkf_sweep_f32",
@@ -52,13 +53,14 @@ class Params(C.Structure):
                 ("jacobi_predict", C.c_int32), ("vert_dist", C.c_double),
                 ("cube_nz", C.c_int32), ("gc_filter_vertical", C.c_int32),
                 ("cube_n", C.c_int32), ("overlap", C.c_int32), ("xcd_tiles", C.c_int32),
-                ("pad_", C.c_int32)]
+                ("bucket_chain", C.c_int32 * 4), ("pad_", C.c_int32)]
 
 
 class TuningC(C.Structure):
     _fields_ = [("chain_len", C.c_int32), ("ncand", C.c_int32), ("bin_deg", C.c_double),
                 ("ms_initial", C.c_double), ("ms_best", C.c_double), ("xcd_tiles", C.c_int32),
-                ("pad_", C.c_int32)]
+                ("bucketed", C.c_int32), ("bucket_lo", C.c_int32 * 4),
+                ("bucket_chain", C.c_int32 * 4), ("bucket_count", C.c_int64 * 4)]
 
 
 class VertCoordC(C.Structure):
@@ -152,6 +154,10 @@ def lib():
         L.letkf_core_.restype = None
         L.letkf_core_batch_.argtypes = [vp] * 13
         L.letkf_core_batch_.restype = None
+        L.letkf_core_r4_.argtypes = [vp] * 10
+        L.letkf_core_r4_.restype = None
+        L.letkf_core_batch_r4_.argtypes = [vp] * 13
+        L.letkf_core_batch_r4_.restype = None
         _lib = L
     return _lib
 
@@ -203,6 +209,8 @@ def default_params(**kw) -> Params:
     for k, v in kw.items():
         if not hasattr(p, k):
             raise KeyError(k)
+        if k == "bucket_chain":
+            v = (C.c_int32 * 4)(*v)
         setattr(p, k, v)
     return p
 
@@ -326,7 +334,11 @@ class Engine:
         oc, t = obs.c(), TuningC()
         check(lib().dida_letkf_autotune(self.h, C.byref(oc), _ptr(xb), _ptr(axb), _ptr(xa),
                                         _ptr(axa), _stream_ptr(stream), C.byref(t)), "autotune")
-        return {f: getattr(t, f) for f, _ in TuningC._fields_ if f != "pad_"}
+        out = {}
+        for f, _ in TuningC._fields_:
+            v = getattr(t, f)
+            out[f] = list(v) if hasattr(v, "__len__") else v
+        return out
 
     def graph_launch(self, stream=None) -> int:
         return check(lib().dida_letkf_graph_launch(self.h, _stream_ptr(stream)), "graph_launch")
@@ -427,6 +439,43 @@ def letkf_core_batch(nobsl, hxens, rdiaginv, dep, rloc, inflation: float):
     lib().letkf_core_batch_(*[C.byref(v) for v in vals], nb.ctypes.data, hx.ctypes.data,
                             ri.ctypes.data, dp.ctypes.data, rl.ctypes.data, trans.ctypes.data,
                             wbar.ctypes.data, C.byref(rho), C.byref(pf))
+    return trans.reshape(batch, k, k).transpose(0, 2, 1).copy(), wbar.reshape(batch, k)
+
+
+def letkf_core_r4(hxens: np.ndarray, rdiaginv, dep, rloc, inflation: float, pflag: int = 0):
+    """letkf_core_r4_ (REAL(4) arrays) on host arrays.  hxens: (p, k).
+    Returns float32 (trans[i, j], wbar)."""
+    hx = np.ascontiguousarray(hxens, dtype=np.float32)
+    p, k = hx.shape
+    ri = np.ascontiguousarray(rdiaginv, dtype=np.float32)
+    dp = np.ascontiguousarray(dep, dtype=np.float32)
+    rl = np.ascontiguousarray(rloc, dtype=np.float32)
+    trans = np.zeros(k * k, np.float32)
+    wbar = np.zeros(k, np.float32)
+    ik, ip, pf = C.c_int32(k), C.c_int32(p), C.c_int32(pflag)
+    rho = C.c_double(inflation)
+    lib().letkf_core_r4_(C.byref(ik), C.byref(ip), hx.ctypes.data, ri.ctypes.data,
+                         dp.ctypes.data, rl.ctypes.data, trans.ctypes.data, wbar.ctypes.data,
+                         C.byref(rho), C.byref(pf))
+    return trans.reshape(k, k).T.copy(), wbar
+
+
+def letkf_core_batch_r4(nobsl, hxens, rdiaginv, dep, rloc, inflation: float):
+    """letkf_core_batch_r4_ on host arrays: hxens (batch, max_obs, k), all
+    REAL(4).  Returns float32 (trans[b, i, j], wbar[b])."""
+    hx = np.ascontiguousarray(hxens, dtype=np.float32)
+    batch, max_obs, k = hx.shape
+    nb = np.ascontiguousarray(nobsl, dtype=np.int32)
+    ri = np.ascontiguousarray(rdiaginv, dtype=np.float32)
+    dp = np.ascontiguousarray(dep, dtype=np.float32)
+    rl = np.ascontiguousarray(rloc, dtype=np.float32)
+    trans = np.zeros(batch * k * k, np.float32)
+    wbar = np.zeros(batch * k, np.float32)
+    vals = [C.c_int32(batch), C.c_int32(0), C.c_int32(max_obs), C.c_int32(k)]
+    rho, pf = C.c_double(inflation), C.c_int32(0)
+    lib().letkf_core_batch_r4_(*[C.byref(v) for v in vals], nb.ctypes.data, hx.ctypes.data,
+                               ri.ctypes.data, dp.ctypes.data, rl.ctypes.data,
+                               trans.ctypes.data, wbar.ctypes.data, C.byref(rho), C.byref(pf))
     return trans.reshape(batch, k, k).transpose(0, 2, 1).copy(), wbar.reshape(batch, k)
 
 

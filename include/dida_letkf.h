@@ -9,6 +9,8 @@
  * Reference interfaces replaced (paths relative to the reference tree):
  *   letkf_core_        code/x86/TradDA/DA/LETKF/src/letkf_core.F90:14-15
  *   letkf_core_batch_  code/sw/TradDA/DA/LETKF/src/letkf_core.F90:16-17
+ *   letkf_core_r4_, letkf_core_batch_r4_  the same with yb_type = real(4)
+ *                      (code/x86/TradDA/DA/utils/da_config.inc:7-10)
  *   dida_letkf_sweep   the per-cube loop of letkf_compute_nooverlap,
  *                      code/x86/TradDA/DA/LETKF/src/letkf_compute.F90:12-126
  *                      (gen_yb_dis :72, letkf_core :97, xb_2_xa :111)
@@ -107,6 +109,10 @@ typedef struct dida_letkf_params {
                                    one stripe per accelerator die (XCD) first, so that
                                    concurrent analyses of a die share its L2; 0 (default)
                                    one queue, most expensive chains first */
+    int32_t bucket_chain[4];    /* batched form, k <= 64: > 0 gives analyses of local
+                                   count p >= 0, 64, 512, 2048 (the four buckets) their
+                                   own chain length (1: cold starts; 0: chain_len);
+                                   all 0 (default): chain_len for every analysis */
     int32_t pad_;
 } dida_letkf_params;
 
@@ -165,6 +171,25 @@ void letkf_core_batch_(const int32_t *batch, const int32_t *batch_large_matrix,
                        double *hxens, const float *rdiaginv, const double *dep,
                        const double *rloc, double *trans, double *overline_w,
                        const double *inflation_factor, const int32_t *pflag);
+
+/* REAL(4) forms of the two drop-ins: the reference's letkf_core and
+ * letkf_core_batch compiled with yb_type = ayb_type = real(4)
+ * (code/x86/TradDA/DA/utils/da_config.inc:7-10; letkf_core.F90:14-29 and
+ * code/sw/TradDA/DA/LETKF/src/letkf_core.F90:16-35 with those types), where
+ * hxens, dep, rloc, trans and overline_w are REAL(4) (rdiaginv is REAL(4) in
+ * both builds).  Same arguments and layouts otherwise.  The reference then
+ * computes in single precision (sgemm, ssyevd); here the values are analysed
+ * in FP64 and the results rounded once to REAL(4), so they agree with the
+ * FP64 analysis of the same values to one REAL(4) rounding and with the
+ * reference's single-precision arithmetic to its own rounding error. */
+void letkf_core_r4_(const int32_t *nanals, const int32_t *nobsl, const float *hxens,
+                    const float *rdiaginv, const float *dep, const float *rloc, float *trans,
+                    float *overline_w, const double *inflation_factor, const int32_t *pflag);
+void letkf_core_batch_r4_(const int32_t *batch, const int32_t *batch_large_matrix,
+                          const int32_t *max_obs, const int32_t *nanals, const int32_t *nobsl,
+                          float *hxens, const float *rdiaginv, const float *dep,
+                          const float *rloc, float *trans, float *overline_w,
+                          const double *inflation_factor, const int32_t *pflag);
 
 /* Status-returning batched core on device pointers (same layout as
  * letkf_core_batch_), enqueued on `stream` (a hipStream_t, NULL = default).
@@ -254,7 +279,12 @@ int dida_letkf_graph_launch(dida_letkf_ctx *ctx, void *stream);
  * the obs bin size (theta/2, theta/4, theta/8, theta = support radius:
  * coarser bins scan more candidates, finer ones visit more bins; a column
  * capped at MAX_OBS keeps the obs of smallest caller index whatever the bin
- * size, as the reference's in-order scan does).  Each candidate is
+ * size, as the reference's in-order scan does); then, in the fused batched
+ * form (k <= 64), a chain length per local-obs-count bucket (p < 64, < 512,
+ * < 2048, >= 2048; 1, 2, 4, ..., 64, where 1 starts every analysis of the
+ * bucket cold), one bucket at a time: chains are then runs of adjacent
+ * columns of one bucket, built on the device every sweep from its counts,
+ * drained from one cost-sorted queue.  Each candidate is
  * timed twice (HIP events, minimum); xa/axa hold the result of a sweep with
  * the kept layout on return.  Synchronises. */
 typedef struct dida_letkf_tuning {
@@ -264,7 +294,11 @@ typedef struct dida_letkf_tuning {
     double ms_initial;          /* sweep time with the layout on entry */
     double ms_best;             /* sweep time with the kept layout */
     int32_t xcd_tiles;          /* kept queue order (batched form) */
-    int32_t pad_;
+    int32_t bucketed;           /* 1: chain length per obs-count bucket (below) */
+    int32_t bucket_lo[4];       /* buckets: analyses with local count p >= bucket_lo[b]
+                                   (and below the next bound) */
+    int32_t bucket_chain[4];    /* kept chain length per bucket (1: cold starts) */
+    int64_t bucket_count[4];    /* analyses per bucket in the tuned sweep */
 } dida_letkf_tuning;
 int dida_letkf_autotune(dida_letkf_ctx *ctx, const dida_letkf_obs *obs, const double *xb,
                         const double *axb, double *xa, double *axa, void *stream,

@@ -180,6 +180,38 @@ def letkf_core(hxens: np.ndarray, rdiaginv: np.ndarray, dep: np.ndarray, rloc: n
     return trans.reshape(k, k).T.copy(), wbar, s
 
 
+def letkf_core_r4(hxens: np.ndarray, rdiaginv: np.ndarray, dep: np.ndarray,
+                  rloc: np.ndarray, inflation: float):
+    """letkf_core.F90:14-233 built with yb_type = ayb_type = real(4)
+    (da_config.inc:7-10): hxens, dep, rloc, trans, overline_w and every work
+    array REAL(4); the products are sgemm (:78, :141, :168, :205), the
+    eigensolve ssyevd (:101), here numpy's float32 matmul and LAPACK syevd
+    through numpy.linalg.eigh.  Mixed expressions with REAL(8) operands
+    (:111, :201) are evaluated in double and rounded on assignment, as
+    Fortran does.  hxens: (p, k).  Returns float32 (trans[i, j], wbar)."""
+    f4 = np.float32
+    hx = np.asarray(hxens, f4).reshape(-1, np.asarray(hxens).shape[-1])
+    p, k = hx.shape
+    rr = np.sqrt(np.asarray(rdiaginv, f4) * np.asarray(rloc, f4))        # :64-66
+    h = hx * rr[:, None]                                                  # :67
+    work1 = (h.T @ h).astype(f4)                                          # :78
+    eival, eivec = np.linalg.eigh(work1)                                  # :101
+    eival = (eival.astype(np.float64) * 1.0 + (k - 1) / inflation).astype(f4)  # :110-113
+    work1 = (eivec / eival[None, :]).astype(f4)                           # :129-135
+    pa = (work1 @ eivec.T).astype(f4)                                     # :141
+    h = h * rr[:, None]                                                   # :146-148
+    work2 = (pa @ h.T).astype(f4)                                         # :168
+    dep4 = np.asarray(dep, f4)
+    wbar = np.zeros(k, f4)
+    for o in range(p):                                                    # :190-195
+        wbar = (wbar + work2[:, o] * dep4[o]).astype(f4)
+    rho = np.sqrt((k - 1) / eival.astype(np.float64)).astype(f4)          # :199-200
+    work1 = (eivec * rho[None, :]).astype(f4)
+    trans = (work1 @ eivec.T).astype(f4)                                  # :205
+    trans = (trans + wbar[:, None]).astype(f4)                            # :221-225
+    return trans, wbar
+
+
 def xb_2_xa(xb: np.ndarray, axb: np.ndarray, trans: np.ndarray):
     """letkf_compute.F90:231-301.  xb: (ncells, k), axb: (ncells,), trans: (k, k)
     with trans[i, j] as returned by letkf_core.  Returns (xa, axa)."""

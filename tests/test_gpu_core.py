@@ -91,3 +91,34 @@ def test_ill_conditioned_obs_errors(cuda, dl, oracle):
     trans, wbar = dl.letkf_core(h, ri, dep, rl, 1.0)
     t_o, w_o, _ = oracle.letkf_core(h, ri, dep, rl, 1.0)
     assert _rel(trans, t_o) < TOL and _rel(wbar, w_o) < TOL
+
+
+@pytest.mark.parametrize("k", [8, 20, 64, 128])
+def test_r4_drop_ins(cuda, dl, oracle, k):
+    """letkf_core_r4_ / letkf_core_batch_r4_ (yb_type = real(4)): the FP64
+    analysis of the REAL(4) values rounded once (within one REAL(4) rounding
+    of the oracle's FP64 result), and within 1e-4 of the reference's own
+    single-precision arithmetic (the oracle's float32 restatement)."""
+    rng = np.random.default_rng(500 + k)
+    counts = [0, 1, 7, k + 5, 400]
+    max_obs = max(counts)
+    hx, ri, dep, rl = _batch(rng, k, counts, max_obs)
+    hx4, dep4, rl4 = hx.astype(np.float32), dep.astype(np.float32), rl.astype(np.float32)
+    rho = 1.1
+    print(f"r4 drop-ins k={k}", flush=True)
+    tb, wb = dl.letkf_core_batch_r4(np.array(counts), hx4, ri, dep4, rl4, rho)
+    for j, p in enumerate(counts):
+        t1, w1 = dl.letkf_core_r4(hx4[j, :p], ri[j, :p], dep4[j, :p], rl4[j, :p], rho)
+        assert t1.dtype == np.float32 and np.array_equal(t1, tb[j]) and np.array_equal(w1, wb[j])
+        t8, w8, _ = oracle.letkf_core(hx4[j, :p].astype(np.float64), ri[j, :p],
+                                      dep4[j, :p].astype(np.float64),
+                                      rl4[j, :p].astype(np.float64), rho)
+        # one REAL(4) rounding of a value within the FP64 bar (1e-10 of the largest)
+        ulp = 2.0 ** -24
+        assert np.all(np.abs(t1 - t8) <= ulp * np.abs(t8) + 1e-10 * np.abs(t8).max()), (k, p)
+        assert np.all(np.abs(w1 - w8) <= ulp * np.abs(w8) + 1e-10 * np.abs(w8).max() + 1e-300), \
+            (k, p)
+        t4, w4 = oracle.letkf_core_r4(hx4[j, :p], ri[j, :p], dep4[j, :p], rl4[j, :p], rho)
+        assert np.abs(t1 - t4).max() <= 1e-4 * np.abs(t4).max(), (k, p)
+        if p:
+            assert np.abs(w1 - w4).max() <= 1e-4 * np.abs(w4).max(), (k, p)

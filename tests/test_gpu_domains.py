@@ -41,14 +41,20 @@ def _domains(dl, torch, gram):
     return out
 
 
-@pytest.mark.parametrize("gram", ["FP64", "BF16X3"])
-def test_domains_group_graph(cuda, dl, oracle, gram):
+@pytest.mark.parametrize("gram,tuned", [("FP64", False), ("BF16X3", False), ("FP64", True)])
+def test_domains_group_graph(cuda, dl, oracle, gram, tuned):
+    """tuned: every domain's context autotuned first (its own bin size, chain
+    lengths per obs-count bucket, queue order), so the graph records each
+    domain with its own layout."""
     torch = cuda
-    print(f"twelve domains, {gram}", flush=True)
+    print(f"twelve domains, {gram}, tuned {tuned}", flush=True)
     doms = _domains(dl, torch, getattr(dl, "GRAM_" + gram))
     eager = []
     for r, params, eng, obs in doms:
         xa, axa = torch.empty_like(r["xb"]), torch.empty_like(r["axb"])
+        if tuned:
+            t = eng.autotune(obs, r["xb"], r["axb"], xa, axa)
+            assert t["ms_best"] <= t["ms_initial"]
         eng.sweep(obs, r["xb"], r["axb"], xa, axa)
         st = eng.stats()
         assert st["status"] == 0 and st["nonconverged"] == 0, st

@@ -115,3 +115,59 @@ def test_planted_mistake_is_caught(oracle):
     w = trans - wbar[:, None]
     wrong = w * 1.001  # e.g. a sqrt((k-1)) vs sqrt(k) slip
     assert np.abs(wrong @ a @ wrong - (k - 1) * np.eye(k)).max() > 1e-6 * np.abs(a).max()
+
+
+# ---------------------------------------------------------------------------
+# The REAL(4) build of letkf_core (yb_type = real(4), da_config.inc:7-10),
+# restated in float32 numpy: pinned by the same closed forms and invariants at
+# single-precision tolerance and against the FP64 analysis of the same values.
+
+def _r4(x):
+    return np.asarray(x, np.float32)
+
+
+@pytest.mark.parametrize("k", [4, 20, 64])
+def test_r4_no_obs_gives_inflated_identity(oracle, k):
+    trans, wbar = oracle.letkf_core_r4(np.zeros((0, k)), np.zeros(0), np.zeros(0), np.zeros(0),
+                                       1.1)
+    assert trans.dtype == np.float32 and wbar.dtype == np.float32
+    assert np.allclose(trans, np.sqrt(1.1) * np.eye(k), atol=2e-6, rtol=0)
+    assert np.all(wbar == 0.0)
+
+
+@pytest.mark.parametrize("k", [3, 20, 40])
+def test_r4_single_obs_sherman_morrison(oracle, k):
+    rng = np.random.default_rng(100 + k)
+    rho = 1.2
+    y = _r4(rng.normal(size=k))
+    y -= y.mean()
+    ri, rl, d = np.float32(1.7), np.float32(0.6), np.float32(0.9)
+    trans, wbar = oracle.letkf_core_r4(y[None, :], [ri], [d], [rl], rho)
+    yd = y.astype(np.float64)
+    r = float(ri) * float(rl)
+    a0 = (k - 1) / rho
+    lam = r * (yd @ yd)
+    yy = np.outer(yd, yd) / (yd @ yd)
+    w_expect = np.sqrt(rho) * (np.eye(k) - yy) + np.sqrt((k - 1) / (lam + a0)) * yy
+    wbar_expect = yd * r * float(d) / (lam + a0)
+    assert np.abs(wbar - wbar_expect).max() <= 1e-5 * np.abs(wbar_expect).max()
+    assert np.abs(trans - (w_expect + wbar_expect[:, None])).max() <= 2e-5
+
+
+@pytest.mark.parametrize("k,p", [(8, 30), (20, 300), (64, 1000)])
+def test_r4_close_to_the_fp64_analysis_of_the_same_values(oracle, k, p):
+    """Single-precision arithmetic differs from the FP64 analysis of the same
+    REAL(4) values by its own rounding only; a wrong step (a missing sqrt, a
+    transposed product) is far outside this."""
+    rng = np.random.default_rng(k * 1000 + p)
+    hx, rinv, dep, rloc = _random_case(rng, k, p)
+    hx4, dep4, rl4 = _r4(hx), _r4(dep), _r4(rloc)
+    t4, w4 = oracle.letkf_core_r4(hx4, rinv, dep4, rl4, 1.05)
+    t8, w8, _ = oracle.letkf_core(hx4.astype(np.float64), rinv, dep4.astype(np.float64),
+                                  rl4.astype(np.float64), 1.05)
+    assert np.abs(t4 - t8).max() <= 1e-4 * np.abs(t8).max()
+    assert np.abs(w4 - w8).max() <= 1e-4 * np.abs(w8).max()
+    # planted: rloc applied once instead of sqrt(R^-1 rloc) twice
+    bad, _ = oracle.letkf_core_r4(hx4 * np.sqrt(rl4)[:, None], rinv, dep4, np.ones_like(rl4),
+                                  1.05)
+    assert np.abs(bad - t8).max() > 1e-2 * np.abs(t8).max()
