@@ -162,7 +162,8 @@ struct dida_letkf_ctx {
     int32_t *iota, *perm, *bin_count, *bin_start;
     void *sort_tmp;
     size_t sort_tmp_bytes;
-    double *cx, *sx, *yr, *ux, *uy, *uz, *yb_b, *dep_b;
+    double *cx, *sx, *yr, *yb_b, *dep_b;
+    float *u4;           /* FP32 unit vector of every binned obs (x, y, z, 0) */
     float *yb32_b;       /* chained half-precision Grams: binned Yb in FP32 (no yb_b) */
     int32_t ldy;         /* row stride of yb_b: k (plain), the padded k (batched) */
     float *rinv_b;
@@ -196,6 +197,7 @@ struct dida_letkf_ctx {
     /* per-bucket chains (fused batched form): on when bucketed, chain length
      * bucket_len[b] for analyses of local count in bucket b (thresholds
      * kBucketLo); the chain descriptors are built on the device every sweep */
+    int32_t sel_tile;    /* DK_SELECT_TILE_* of the count and fill passes */
     int32_t bucketed;
     int32_t bucket_len[DK_NBUCKET];
     int32_t *line_cnt, *chain_col0, *chain_ncol;
@@ -243,7 +245,7 @@ int dida_letkf_destroy(dida_letkf_ctx *c)
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
     void *ptrs[] = {c->key, c->key_sorted, c->iota, c->perm, c->bin_count, c->bin_start,
-                    c->sort_tmp, c->cx, c->sx, c->yr, c->ux, c->uy, c->uz, c->yb_b,
+                    c->sort_tmp, c->cx, c->sx, c->yr, c->u4, c->yb_b,
                     c->yb32_b, c->dep_b, c->rinv_b,
                     c->nobsl, c->thr, c->offset, c->scan_tmp, c->csr_idx, c->csr_w, c->flags,
                     c->trans, c->info, c->chain_key, c->chain_key_sorted, c->chain_id,
@@ -386,9 +388,7 @@ int dida_letkf_create(dida_letkf_ctx **out, const dida_letkf_params *prm, int32_
     DMALLOC(c->cx, no * sizeof(double));
     DMALLOC(c->sx, no * sizeof(double));
     DMALLOC(c->yr, no * sizeof(double));
-    DMALLOC(c->ux, no * sizeof(double));
-    DMALLOC(c->uy, no * sizeof(double));
-    DMALLOC(c->uz, no * sizeof(double));
+    DMALLOC(c->u4, no * 4 * sizeof(float));
     c->chained = prm->form == DIDA_FORM_BATCHED && k > 64;
     /* the one-wavefront kernels (batched form, and the plain form with k <= 64)
      * read binned Yb rows of 32 or 64 values, zero past k */
@@ -612,7 +612,7 @@ static int sweep_select_count(dida_letkf_ctx *c, const dida_letkf_obs *o, hipStr
     }
     HIP_TRY(dk_bin_scan(c->geom.nbins, c->bin_count, c->bin_start, s));
     HIP_TRY(dk_obs_gather(o->nobs, k, c->ldy, c->perm, o->lat, o->lon, o->yb, o->dep, o->rinv,
-                          c->cx, c->sx, c->yr, c->ux, c->uy, c->uz, c->yb_b, c->yb32_b,
+                          c->cx, c->sx, c->yr, c->u4, c->yb_b, c->yb32_b,
                           c->dep_b, c->rinv_b, s));
     if (c->ncube > 0) HIP_TRY(dk_obs_lnp(o->nobs, c->perm, o->ph, c->obs_lnp, s));
     if (c->chained && c->prm.gram == DIDA_GRAM_FP16X3)
@@ -634,9 +634,7 @@ static int sweep_select_count(dida_letkf_ctx *c, const dida_letkf_obs *o, hipStr
     sa->obs.cx = c->cx;
     sa->obs.sx = c->sx;
     sa->obs.yr = c->yr;
-    sa->obs.ux = c->ux;
-    sa->obs.uy = c->uy;
-    sa->obs.uz = c->uz;
+    sa->obs.u4 = c->u4;
     sa->obs.yb = c->yb_b;
     sa->obs.dep = c->dep_b;
     sa->obs.rinv = c->rinv_b;
@@ -652,6 +650,7 @@ static int sweep_select_count(dida_letkf_ctx *c, const dida_letkf_obs *o, hipStr
     sa->overflow = c->flags;
     sa->perm = c->perm;
     sa->thr = c->thr;
+    sa->tile = c->sel_tile;
     sa->perm_bits = 1;
     while (sa->perm_bits < 31 && ((int64_t)1 << sa->perm_bits) < (int64_t)o->nobs) ++sa->perm_bits;
     HIP_TRY(dk_select_count(sa, s));
@@ -1227,6 +1226,21 @@ int dida_letkf_autotune(dida_letkf_ctx *c, const dida_letkf_obs *o, const double
             if (BL) memcpy(best_bl, (BL), sizeof best_bl);                               \
         }                                                                                \
     } while (0)
+    /* 0. the tile of columns of the selection passes */
+    const int32_t tile0 = c->sel_tile;
+    int32_t best_tile = tile0;
+    for (int32_t tt = DK_SELECT_TILE_4X2; tt <= DK_SELECT_TILE_4X4 && !rc; ++tt) {
+        if (tt == tile0) continue;
+        c->sel_tile = tt;
+        double ms = 0.0;
+        rc = timed_sweep(c, o, xb, axb, xa, axa, s, e0, e1, &ms);
+        out->ncand++;
+        if (!rc && ms < best_ms) {
+            best_ms = ms;
+            best_tile = tt;
+        }
+    }
+    c->sel_tile = best_tile;
     /* 1. the bin size (a capped column keeps the same obs at every size) */
     const double bins[3] = {c->theta_deg / 2.0, c->theta_deg / 4.0, c->theta_deg / 8.0};
     for (int ib = 0; ib < 3 && !rc; ++ib)
@@ -1287,6 +1301,7 @@ int dida_letkf_autotune(dida_letkf_ctx *c, const dida_letkf_obs *o, const double
     out->ms_best = best_ms;
     out->xcd_tiles = batched ? c->ngroup > 0 : 0;
     out->bucketed = c->bucketed;
+    out->select_tile = c->sel_tile;
     for (int b = 0; b < DK_NBUCKET; ++b) {
         out->bucket_lo[b] = kBucketLo[b];
         out->bucket_chain[b] = c->bucketed ? c->bucket_len[b] : 0;

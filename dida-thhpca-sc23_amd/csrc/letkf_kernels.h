@@ -28,10 +28,11 @@ typedef struct dk_bin_geom {
 /* Observations in binned order, with the per-obs factors of the distance
  * formula precomputed (cos/sin of the latitude in the reference's radians
  * and the longitude in the reference's radians) and the unit vector
- * (cx cos yr, cx sin yr, sx) used to decide far-from-threshold candidates. */
+ * (cx cos yr, cx sin yr, sx) rounded to FP32, used to decide the candidates
+ * away from the support radius. */
 typedef struct dk_obs_binned {
     const double *cx, *sx, *yr;   /* [nobs] */
-    const double *ux, *uy, *uz;   /* [nobs] */
+    const float *u4;              /* [nobs * 4]: unit vector x, y, z, 0 */
     const double *yb;             /* [nobs * k] ensemble fastest */
     const double *dep;            /* [nobs] */
     const float *rinv;            /* [nobs] */
@@ -65,8 +66,14 @@ typedef struct dk_select_args {
     const int32_t *perm;
     int32_t *thr;
     int32_t perm_bits;
-    int32_t pad_;
+    int32_t tile;        /* count and fill: DK_SELECT_TILE_* columns per wavefront */
 } dk_select_args;
+
+/* Tile shapes (latitudes x longitudes of the analysis grid) of the count and
+ * fill passes. */
+#define DK_SELECT_TILE_4X2 0
+#define DK_SELECT_TILE_4X1 1
+#define DK_SELECT_TILE_4X4 2
 
 /* Fill n int32 with v (a kernel, so that captured graphs hold no memset). */
 int dk_fill_i32(int32_t *p, int32_t v, int64_t n, void *stream);
@@ -84,12 +91,14 @@ int dk_bin_scan(int32_t nbins, const int32_t *count, int32_t *bin_start, void *s
  * yb_b and/or rounded to FP32 into yb32_b (either may be NULL). */
 int dk_obs_gather(int32_t nobs, int32_t k, int32_t ldy, const int32_t *perm, const double *lat,
                   const double *lon, const double *yb, const double *dep, const float *rinv,
-                  double *cx, double *sx, double *yr, double *ux, double *uy, double *uz,
-                  double *yb_b, float *yb32_b, double *dep_b, float *rinv_b, void *stream);
-/* Selection passes, one wave per column: count (capped at max_obs), cap
- * (thr of every column; a column whose count reached max_obs scans all its
- * candidates for the caller-order threshold by a radix select over perm),
- * fill (entries with gc > 0 and perm < thr, in bin order). */
+                  double *cx, double *sx, double *yr, float *u4, double *yb_b, float *yb32_b,
+                  double *dep_b, float *rinv_b, void *stream);
+/* Selection passes: count (capped at max_obs) and fill (entries with gc > 0
+ * and perm < thr, in bin order) by tiles of columns (DK_SELECT_TILE_*), one
+ * wavefront a tile reading each candidate once for its columns; cap (thr of every
+ * column; a column whose count reached max_obs scans all its candidates for
+ * the caller-order threshold by a radix select over perm), one wavefront a
+ * column. */
 int dk_select_count(const dk_select_args *a, void *stream);
 int dk_select_cap(const dk_select_args *a, void *stream);
 int dk_select_fill(const dk_select_args *a, void *stream);

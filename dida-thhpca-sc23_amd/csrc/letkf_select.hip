@@ -90,8 +90,7 @@ __global__ void k_obs_gather(int32_t nobs, const int32_t* __restrict__ perm,
                              const double* __restrict__ lat, const double* __restrict__ lon,
                              const double* __restrict__ dep, const float* __restrict__ rinv,
                              double* __restrict__ cx, double* __restrict__ sx,
-                             double* __restrict__ yr, double* __restrict__ ux,
-                             double* __restrict__ uy, double* __restrict__ uz,
+                             double* __restrict__ yr, float4* __restrict__ u4,
                              double* __restrict__ dep_b, float* __restrict__ rinv_b) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nobs) return;
@@ -101,9 +100,7 @@ __global__ void k_obs_gather(int32_t nobs, const int32_t* __restrict__ perm,
   cx[i] = c1;
   sx[i] = s1;
   yr[i] = y1;
-  ux[i] = c1 * cos(y1);
-  uy[i] = c1 * sin(y1);
-  uz[i] = s1;
+  u4[i] = make_float4((float)(c1 * cos(y1)), (float)(c1 * sin(y1)), (float)s1, 0.0f);
   dep_b[i] = dep[s];
   rinv_b[i] = rinv[s];
 }
@@ -133,36 +130,57 @@ __global__ void k_obs_ymax(int32_t nobs, int32_t k, int32_t ldy, const T* __rest
   ymax[i] = m * sqrt((double)rinv[i]);
 }
 
-// Visit the candidate bins of one column; call f(l0, l1) per contiguous obs
-// range in ascending bin order; f returns false to stop.
-template <class F>
-__device__ __forceinline__ void for_candidate_ranges(const dk_select_args& a, double clat,
-                                                     double clon, F&& f) {
+// Candidate region of analysis columns: bands b0..b1 and, unless `full`,
+// the longitudes [lo_deg, hi_deg] (degrees, unwrapped: lo_deg may be below 0
+// and hi_deg above 360).  For one column: the spherical cap of angular radius
+// theta_deg around it, widened so that no kept obs falls outside the visited
+// bins (the reference's distance uses a float-rounded pi).
+struct CandRegion {
+  int b0, b1;
+  bool full;
+  double lo_deg, hi_deg;
+};
+
+__device__ __forceinline__ CandRegion column_region(const dk_select_args& a, double clat,
+                                                    double cw) {
   const double kPi = 3.14159265358979323846;
   const double th = a.theta_deg;
   const dk_bin_geom& g = a.geom;
-  int b0 = (int)floor((clat - th + 90.0) / g.dlat);
-  int b1 = (int)floor((clat + th + 90.0) / g.dlat);
-  b0 = b0 < 0 ? 0 : b0;
-  b1 = b1 > g.nband - 1 ? g.nband - 1 : b1;
-  bool full = false;
+  CandRegion r;
+  r.b0 = (int)floor((clat - th + 90.0) / g.dlat);
+  r.b1 = (int)floor((clat + th + 90.0) / g.dlat);
+  r.b0 = r.b0 < 0 ? 0 : r.b0;
+  r.b1 = r.b1 > g.nband - 1 ? g.nband - 1 : r.b1;
+  r.full = false;
   double dlon = 0.0;
   if (fabs(clat) + th >= 90.0) {
-    full = true;
+    r.full = true;
   } else {
-    double sv = sin(th * kPi / 180.0) / cos(clat * kPi / 180.0);
-    if (sv >= 1.0) full = true;
+    const double sv = sin(th * kPi / 180.0) / cos(clat * kPi / 180.0);
+    if (sv >= 1.0) r.full = true;
     else dlon = asin(sv) * 180.0 / kPi * (1.0 + 1e-6) + 1e-6;
   }
+  r.lo_deg = cw - dlon;
+  r.hi_deg = cw + dlon;
+  return r;
+}
+
+// Visit the candidate bins of a region; call f(l0, l1) per contiguous obs
+// range in ascending bin order (so every column's kept obs come in the order
+// of a full scan of the binned list); f returns false to stop.
+template <class F>
+__device__ __forceinline__ void for_region_ranges(const dk_select_args& a, const CandRegion& r,
+                                                  F&& f) {
+  const dk_bin_geom& g = a.geom;
+  bool full = r.full;
   int lo = 0, hi = g.nlonbin - 1;
   if (!full) {
-    double cw = wrap360(clon);
-    lo = (int)floor((cw - dlon) / g.dlon);
-    hi = (int)floor((cw + dlon) / g.dlon);
+    lo = (int)floor(r.lo_deg / g.dlon);
+    hi = (int)floor(r.hi_deg / g.dlon);
     if (hi - lo + 1 >= g.nlonbin) full = true;
   }
   const int32_t* bs = a.obs.bin_start;
-  for (int b = b0; b <= b1; ++b) {
+  for (int b = r.b0; b <= r.b1; ++b) {
     int base = b * g.nlonbin;
     int r0a, r0b, r1a = -1, r1b = -2;
     if (full) {
@@ -183,39 +201,44 @@ __device__ __forceinline__ void for_candidate_ranges(const dk_select_args& a, do
 
 // The column's geometry for the candidate test.
 struct ColGeo {
-  double cx2, sx2, y2r;     // cos / sin of the latitude, longitude (reference radians)
-  double vx, vy, vz;        // unit vector
-  double far_lo, near_hi;   // dot-product margins of the support radius
+  double cx2, sx2, y2r;  // cos / sin of the latitude, longitude (reference radians)
+  float vx, vy, vz;      // unit vector, FP32
 };
 
-__device__ __forceinline__ ColGeo col_geo(const dk_select_args& a, double clat, double clon) {
+__device__ __forceinline__ ColGeo col_geo(double clat, double clon) {
   ColGeo q;
   const double x2r = deg2rad_ref(clat);
   q.cx2 = cos(x2r);
   q.sx2 = sin(x2r);
   q.y2r = deg2rad_ref(clon);
-  q.vx = q.cx2 * cos(q.y2r);
-  q.vy = q.cx2 * sin(q.y2r);
-  q.vz = q.sx2;
-  // the unit-vector dot product differs from the reference's cosine by a few
-  // ulp; with these margins it decides every candidate outside a thin shell
-  // around the support radius exactly as the reference formula would
-  q.far_lo = a.cos_far - 1e-12;
-  q.near_hi = a.cos_near + 1e-12;
+  q.vx = (float)(q.cx2 * cos(q.y2r));
+  q.vy = (float)(q.cx2 * sin(q.y2r));
+  q.vz = (float)q.sx2;
   return q;
 }
 
-// Whether binned obs l is kept (gc > 0, letkf_yb.F90:84-96).  kWeight: gc is
-// the Gaspari-Cohn weight of a kept obs; otherwise candidates surely inside
-// the support are kept without it.
-template <bool kWeight>
-__device__ __forceinline__ bool keep_candidate(const dk_select_args& a, const ColGeo& q, int l,
-                                               double& gc) {
+// Margins of the FP32 unit-vector test: its dot product differs from the
+// exact cosine by less than 4e-7 (FP32 rounding of both vectors, the
+// products and the sums), so outside [cos_far - 1e-6, cos_near + 1e-6] it
+// decides a candidate as the reference formula would (candidates surely
+// beyond the support are rejected, those surely inside it have gc > 0); the
+// thin shell between is decided by the reference formula itself.
+__device__ __forceinline__ float far_lo32(const dk_select_args& a) {
+  return (float)(a.cos_far - 1e-6);
+}
+__device__ __forceinline__ float near_hi32(const dk_select_args& a) {
+  return (float)(a.cos_near + 1e-6);
+}
+__device__ __forceinline__ float dot32(const float4& u, float vx, float vy, float vz) {
+  return fmaf(u.z, vz, fmaf(u.y, vy, u.x * vx));
+}
+
+// The reference's test of binned obs l (letkf_yb.F90:84-96): gc > 0 with gc
+// the Gaspari-Cohn weight of the distance in the reference's formula.
+__device__ __forceinline__ bool exact_keep(const dk_select_args& a, const ColGeo& q, int l,
+                                           double& gc) {
 #pragma clang fp contract(off)
   gc = 0.0;
-  const double vd = a.obs.ux[l] * q.vx + a.obs.uy[l] * q.vy + a.obs.uz[l] * q.vz;
-  if (vd < q.far_lo) return false;
-  if (!kWeight && vd > q.near_hi) return true;
   const double v = central_cos(a.obs.cx[l], a.obs.sx[l], a.obs.yr[l], q.cx2, q.sx2, q.y2r);
   if (!(v >= a.cos_far)) return false;
   const double d = dist_from_cos(v);
@@ -224,46 +247,220 @@ __device__ __forceinline__ bool keep_candidate(const dk_select_args& a, const Co
   return gc > 0.0;
 }
 
-template <bool kFill>
-__global__ __launch_bounds__(256) void k_select(dk_select_args a) {
+// Whether binned obs l is kept (gc > 0), without its weight.
+__device__ __forceinline__ bool keep_candidate(const dk_select_args& a, const ColGeo& q, int l,
+                                               const float4& u) {
+  const float d = dot32(u, q.vx, q.vy, q.vz);
+  if (d < far_lo32(a)) return false;
+  if (d > near_hi32(a)) return true;
+  double gc;
+  return exact_keep(a, q, l, gc);
+}
+
+// Tiles of TJ x TI analysis columns (latitude-fastest within the tile), one
+// wavefront a tile, four tiles a workgroup; TJ x TI <= kNT.
+constexpr int kNT = 16;
+
+struct TileCols {
+  double cx2[kNT], sx2[kNT], y2r[kNT];
+  int64_t out[kNT];
+  float vx[kNT], vy[kNT], vz[kNT];
+  int32_t limit[kNT], thr[kNT], cnt[kNT];
+};
+
+// Fill pass: per column, the candidates that passed the FP32 test, in scan
+// order, waiting for a full wavefront of exact tests (up to 63 waiting plus
+// the 64 of one chunk).
+template <int NTT>
+struct TileBuf {
+  int32_t l[NTT][128];
+  int32_t n[NTT];
+};
+
+__device__ __forceinline__ double wave_min(double v) {
+  for (int s = 32; s >= 1; s >>= 1) v = fmin(v, __shfl_xor(v, s, kWave));
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+  for (int s = 32; s >= 1; s >>= 1) v = fmax(v, __shfl_xor(v, s, kWave));
+  return v;
+}
+__device__ __forceinline__ int wave_min_i(int v) {
+  for (int s = 32; s >= 1; s >>= 1) v = min(v, __shfl_xor(v, s, kWave));
+  return v;
+}
+__device__ __forceinline__ int wave_max_i(int v) {
+  for (int s = 32; s >= 1; s >>= 1) v = max(v, __shfl_xor(v, s, kWave));
+  return v;
+}
+
+// Count (kFill false: nobsl[col] = min(kept, max_obs)) or fill (kept obs with
+// caller index below thr[col], in bin order, at offset[col] - base) of the
+// columns of a tile: the union of the columns' candidate regions is scanned
+// once, 64 candidates at a time, each chunk tested against every column of
+// the tile.  Fill: columns outside [col0, col0 + ncols) are skipped.
+__device__ __forceinline__ void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <bool kFill, int TJ, int TI>
+__global__ __launch_bounds__(256) void k_select_tiles(dk_select_args a) {
+  constexpr int kTJ = TJ, kTI = TI, kNTT = TJ * TI;
+  static_assert(kNTT <= kNT, "tile larger than its column state");
+  __shared__ TileCols tc_all[4];
+  __shared__ TileBuf<kFill ? kNTT : 1> tb_all[4];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int64_t ncol = a.ncols > 0 ? a.col0 + a.ncols : (int64_t)a.nlat * a.nlon;
-  const int64_t col = a.col0 + (int64_t)blockIdx.x * 4 + wave;
-  if (col >= ncol) return;
-  const int ilat = (int)(col % a.nlat), ilon = (int)(col / a.nlat);
-  const double clat = (double)a.grid_lat[ilat];
-  const double clon = (double)a.grid_lon[ilon];
-  const ColGeo q = col_geo(a, clat, clon);
-  const int limit = kFill ? a.nobsl[col] : a.max_obs;
-  // capped columns keep the obs below the caller-order threshold
-  const int32_t thr = kFill ? a.thr[col] : INT32_MAX;
-  int64_t out = kFill ? a.offset[col] - a.base : 0;
-  if (kFill && out + limit > a.capacity && lane == 0) *a.overflow = 1;
-  int cnt = 0;
-  if (limit > 0) {
-    for_candidate_ranges(a, clat, clon, [&](int l0, int l1) -> bool {
+  TileCols& tc = tc_all[wave];
+  auto& tb = tb_all[wave];
+  const int ntj = (a.nlat + kTJ - 1) / kTJ;
+  const int ti0 = kFill && a.ncols > 0 ? (int)(a.col0 / a.nlat) / kTI : 0;
+  const int64_t tile = (int64_t)ti0 * ntj + (int64_t)blockIdx.x * 4 + wave;
+  const int tj = (int)(tile % ntj), ti = (int)(tile / ntj);
+  const int64_t c_lo = kFill && a.ncols > 0 ? a.col0 : 0;
+  const int64_t c_hi = kFill && a.ncols > 0 ? a.col0 + a.ncols : (int64_t)a.nlat * a.nlon;
+  // lane j < kNTT: column j of the tile
+  const int jj = lane % kTJ, ii = lane / kTJ;
+  const int ilat = tj * kTJ + jj, ilon = ti * kTI + ii;
+  const int64_t col = (int64_t)ilat + (int64_t)a.nlat * ilon;
+  const bool mine = lane < kNTT && ilat < a.nlat && ilon < a.nlon && col >= c_lo && col < c_hi;
+  if (!__any(mine)) return;
+  double clat = 0.0, clon = 0.0;
+  if (mine) {
+    clat = (double)a.grid_lat[ilat];
+    clon = (double)a.grid_lon[ilon];
+  }
+  // the union of the columns' regions, longitudes unwrapped around the first
+  // column's
+  const int src = __ffsll((unsigned long long)__ballot(mine)) - 1;
+  const double ref = wrap360(__shfl(clon, src, kWave));
+  double cw = wrap360(clon);
+  if (cw - ref > 180.0) cw -= 360.0;
+  if (cw - ref < -180.0) cw += 360.0;
+  const CandRegion rc = column_region(a, clat, cw);
+  CandRegion reg;
+  reg.b0 = wave_min_i(mine ? rc.b0 : 1 << 30);
+  reg.b1 = wave_max_i(mine ? rc.b1 : -1);
+  reg.full = __any(mine && rc.full);
+  reg.lo_deg = wave_min(mine ? rc.lo_deg : 1e300);
+  reg.hi_deg = wave_max(mine ? rc.hi_deg : -1e300);
+  if (lane < kNTT) {
+    const ColGeo q = col_geo(clat, clon);
+    tc.cx2[lane] = q.cx2;
+    tc.sx2[lane] = q.sx2;
+    tc.y2r[lane] = q.y2r;
+    tc.vx[lane] = q.vx;
+    tc.vy[lane] = q.vy;
+    tc.vz[lane] = q.vz;
+    tc.limit[lane] = !mine ? 0 : (kFill ? a.nobsl[col] : a.max_obs);
+    tc.thr[lane] = kFill && mine ? a.thr[col] : INT32_MAX;
+    tc.out[lane] = kFill && mine ? a.offset[col] - a.base : 0;
+    tc.cnt[lane] = 0;
+    if constexpr (kFill) tb.n[lane] = 0;
+    if (kFill && mine && a.offset[col] - a.base + a.nobsl[col] > a.capacity) *a.overflow = 1;
+  }
+  wave_sync_lds();
+  const bool capped = __any(lane < kNTT && tc.thr[lane < kNTT ? lane : 0] != INT32_MAX);
+  // columns still counting / filling (bit j)
+  uint32_t open = (uint32_t)__ballot(lane < kNTT && tc.limit[lane < kNTT ? lane : 0] > 0);
+  const float flo = far_lo32(a), nhi = near_hi32(a);
+  // Fill: the exact test (and weight) of the first n buffered candidates of
+  // column j, one per lane; kept ones are written in buffer (scan) order.
+  auto flush = [&](int j, int n) {
+    ColGeo q;
+    q.cx2 = tc.cx2[j];
+    q.sx2 = tc.sx2[j];
+    q.y2r = tc.y2r[j];
+    const int l = lane < n ? tb.l[j][lane] : 0;
+    double gc = 0.0;
+    const bool keep = lane < n && exact_keep(a, q, l, gc);
+    const uint64_t bm = __ballot(keep);
+    const int c0 = tc.cnt[j], lim = tc.limit[j];
+    if (keep) {
+      const int pos = c0 + (int)__builtin_amdgcn_mbcnt_hi(
+                               (uint32_t)(bm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bm, 0u));
+      const int64_t o = tc.out[j] + pos;
+      if (pos < lim && o < a.capacity) {
+        a.csr_idx[o] = l;
+        a.csr_w[o] = gc;
+      }
+    }
+    const int c1 = c0 + __popcll(bm);
+    wave_sync_lds();
+    if (lane == 0) tc.cnt[j] = c1;
+    if (c1 >= lim) open &= ~(1u << j);
+  };
+  if (open) {
+    for_region_ranges(a, reg, [&](int l0, int l1) -> bool {
       for (int base = l0; base < l1; base += kWave) {
         const int l = base + lane;
-        double gc = 0.0;
-        bool keep = l < l1 && keep_candidate<kFill>(a, q, l, gc);
-        if (kFill && thr != INT32_MAX && keep) keep = a.perm[l] < thr;
-        const uint64_t m = __ballot(keep);
-        if (kFill && keep) {
-          const int pos = cnt + (int)__builtin_amdgcn_mbcnt_hi(
-                                    (uint32_t)(m >> 32),
-                                    __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-          if (pos < limit && out + pos < a.capacity) {
-            a.csr_idx[out + pos] = l;
-            a.csr_w[out + pos] = gc;
+        const bool valid = l < l1;
+        const float4 u = valid ? reinterpret_cast<const float4*>(a.obs.u4)[l]
+                               : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        const int32_t pv = kFill && capped && valid ? a.perm[l] : 0;
+        for (uint32_t m = open; m; m &= m - 1) {
+          const int j = __builtin_ctz(m);
+          const float d = dot32(u, tc.vx[j], tc.vy[j], tc.vz[j]);
+          if constexpr (kFill) {
+            // a capped column's caller-index threshold first, then the FP32
+            // test; survivors wait in the column's buffer
+            const int32_t thr = tc.thr[j];
+            const bool cand = valid && d >= flo && (thr == INT32_MAX || pv < thr);
+            const uint64_t bm = __ballot(cand);
+            if (!bm) continue;
+            const int nb0 = tb.n[j];
+            if (cand)
+              tb.l[j][nb0 + (int)__builtin_amdgcn_mbcnt_hi(
+                                (uint32_t)(bm >> 32),
+                                __builtin_amdgcn_mbcnt_lo((uint32_t)bm, 0u))] = l;
+            int nb = nb0 + __popcll(bm);
+            wave_sync_lds();
+            if (nb >= kWave) {
+              flush(j, kWave);
+              // the rest moves to the front of the buffer
+              const int rest = nb - kWave;
+              const int v = lane < rest ? tb.l[j][kWave + lane] : 0;
+              wave_sync_lds();
+              if (lane < rest) tb.l[j][lane] = v;
+              nb = rest;
+            }
+            if (lane == 0) tb.n[j] = nb;
+            wave_sync_lds();
+          } else {
+            const bool cand = valid && d >= flo;
+            if (!__any(cand)) continue;
+            ColGeo q;
+            q.cx2 = tc.cx2[j];
+            q.sx2 = tc.sx2[j];
+            q.y2r = tc.y2r[j];
+            double gc;
+            const bool keep = cand && (d > nhi || exact_keep(a, q, l, gc));
+            const int c1 = tc.cnt[j] + __popcll(__ballot(keep));
+            wave_sync_lds();
+            if (lane == 0) tc.cnt[j] = c1;
+            if (c1 >= tc.limit[j]) open &= ~(1u << j);
           }
         }
-        cnt += __popcll(m);
-        if (cnt >= limit) return false;
+        wave_sync_lds();
+        if (!open) return false;
       }
       return true;
     });
+    if constexpr (kFill) {
+      // the candidates still waiting
+      for (uint32_t m = open; m; m &= m - 1) {
+        const int j = __builtin_ctz(m);
+        const int nb = tb.n[j];
+        if (nb > 0) flush(j, nb);
+      }
+    }
   }
-  if (!kFill && lane == 0) a.nobsl[col] = cnt < a.max_obs ? cnt : a.max_obs;
+  wave_sync_lds();
+  if (!kFill && mine) {
+    const int c = tc.cnt[lane];
+    a.nobsl[col] = c < a.max_obs ? c : a.max_obs;
+  }
 }
 
 // LDS writes of this wave visible to its later LDS reads.
@@ -295,7 +492,8 @@ __global__ __launch_bounds__(256) void k_select_cap(dk_select_args a) {
   const int ilat = (int)(col % a.nlat), ilon = (int)(col / a.nlat);
   const double clat = (double)a.grid_lat[ilat];
   const double clon = (double)a.grid_lon[ilon];
-  const ColGeo q = col_geo(a, clat, clon);
+  const ColGeo q = col_geo(clat, clon);
+  const CandRegion reg = column_region(a, clat, wrap360(clon));
   int width = a.perm_bits;  // caller indices < 2^perm_bits
   uint32_t lo = 0;
   int rank = a.max_obs;
@@ -308,15 +506,15 @@ __global__ __launch_bounds__(256) void k_select_cap(dk_select_args a) {
     wave_lds_sync();
     const uint32_t span = width >= 31 ? 0x80000000u : (1u << width);
     int total = 0;
-    for_candidate_ranges(a, clat, clon, [&](int l0, int l1) -> bool {
+    for_region_ranges(a, reg, [&](int l0, int l1) -> bool {
       for (int base = l0; base < l1; base += kWave) {
         const int l = base + lane;
         bool keep = false;
         uint32_t rel = 0;
         if (l < l1) {
           rel = (uint32_t)a.perm[l] - lo;
-          double gc;
-          keep = rel < span && keep_candidate<false>(a, q, l, gc);
+          keep = rel < span &&
+                 keep_candidate(a, q, l, reinterpret_cast<const float4*>(a.obs.u4)[l]);
         }
         if (keep) atomicAdd(&hist[rel >> shift], 1u);
         total += __popcll(__ballot(keep));
@@ -400,13 +598,13 @@ extern "C" int dk_bin_scan(int32_t nbins, const int32_t* count, int32_t* bin_sta
 extern "C" int dk_obs_gather(int32_t nobs, int32_t k, int32_t ldy, const int32_t* perm,
                              const double* lat,
                              const double* lon, const double* yb, const double* dep,
-                             const float* rinv, double* cx, double* sx, double* yr,
-                             double* ux, double* uy, double* uz, double* yb_b, float* yb32_b,
-                             double* dep_b, float* rinv_b, void* stream) {
+                             const float* rinv, double* cx, double* sx, double* yr, float* u4,
+                             double* yb_b, float* yb32_b, double* dep_b, float* rinv_b,
+                             void* stream) {
   if (nobs <= 0) return 0;
   hipStream_t s = (hipStream_t)stream;
   k_obs_gather<<<(nobs + 255) / 256, 256, 0, s>>>(nobs, perm, lat, lon, dep, rinv, cx, sx, yr,
-                                                  ux, uy, uz, dep_b, rinv_b);
+                                                  (float4*)u4, dep_b, rinv_b);
   const int64_t total = (int64_t)nobs * ldy;
   const unsigned blocks = (unsigned)((total + 255) / 256);
   if (yb_b) k_obs_gather_yb<double><<<blocks, 256, 0, s>>>(nobs, k, ldy, perm, yb, yb_b);
@@ -430,11 +628,38 @@ extern "C" int dk_obs_ymax32(int32_t nobs, int32_t k, int32_t ldy, const float* 
   return (int)hipGetLastError();
 }
 
-extern "C" int dk_select_count(const dk_select_args* a, void* stream) {
-  int64_t ncol = a->ncols > 0 ? a->ncols : (int64_t)a->nlat * a->nlon;
-  if (ncol <= 0) return 0;
-  k_select<false><<<(unsigned)((ncol + 3) / 4), 256, 0, (hipStream_t)stream>>>(*a);
+// Workgroups of four tiles over the tiles of the analysis grid (fill over a
+// run of columns: the tiles of its longitudes).
+static unsigned tile_blocks(const dk_select_args* a, bool fill, int tj, int ti) {
+  const int64_t ntj = (a->nlat + tj - 1) / tj;
+  int64_t ti0 = 0, ti1 = (a->nlon + ti - 1) / ti;
+  if (fill && a->ncols > 0) {
+    ti0 = (a->col0 / a->nlat) / ti;
+    ti1 = ((a->col0 + a->ncols - 1) / a->nlat) / ti + 1;
+  }
+  return (unsigned)(((ti1 - ti0) * ntj + 3) / 4);
+}
+
+template <bool kFill>
+static int select_tiles(const dk_select_args* a, hipStream_t s) {
+  switch (a->tile) {
+    case DK_SELECT_TILE_4X1:
+      k_select_tiles<kFill, 4, 1><<<tile_blocks(a, kFill, 4, 1), 256, 0, s>>>(*a);
+      break;
+    case DK_SELECT_TILE_4X4:
+      k_select_tiles<kFill, 4, 4><<<tile_blocks(a, kFill, 4, 4), 256, 0, s>>>(*a);
+      break;
+    default:
+      k_select_tiles<kFill, 4, 2><<<tile_blocks(a, kFill, 4, 2), 256, 0, s>>>(*a);
+      break;
+  }
   return (int)hipGetLastError();
+}
+
+extern "C" int dk_select_count(const dk_select_args* a, void* stream) {
+  const int64_t ncol = (int64_t)a->nlat * a->nlon;
+  if (ncol <= 0) return 0;
+  return select_tiles<false>(a, (hipStream_t)stream);
 }
 
 extern "C" int dk_select_cap(const dk_select_args* a, void* stream) {
@@ -446,10 +671,9 @@ extern "C" int dk_select_cap(const dk_select_args* a, void* stream) {
 }
 
 extern "C" int dk_select_fill(const dk_select_args* a, void* stream) {
-  int64_t ncol = a->ncols > 0 ? a->ncols : (int64_t)a->nlat * a->nlon;
+  const int64_t ncol = a->ncols > 0 ? a->ncols : (int64_t)a->nlat * a->nlon;
   if (ncol <= 0) return 0;
-  k_select<true><<<(unsigned)((ncol + 3) / 4), 256, 0, (hipStream_t)stream>>>(*a);
-  return (int)hipGetLastError();
+  return select_tiles<true>(a, (hipStream_t)stream);
 }
 
 namespace {

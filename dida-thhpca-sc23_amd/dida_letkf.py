@@ -60,7 +60,8 @@ class TuningC(C.Structure):
     _fields_ = [("chain_len", C.c_int32), ("ncand", C.c_int32), ("bin_deg", C.c_double),
                 ("ms_initial", C.c_double), ("ms_best", C.c_double), ("xcd_tiles", C.c_int32),
                 ("bucketed", C.c_int32), ("bucket_lo", C.c_int32 * 4),
-                ("bucket_chain", C.c_int32 * 4), ("bucket_count", C.c_int64 * 4)]
+                ("bucket_chain", C.c_int32 * 4), ("bucket_count", C.c_int64 * 4),
+                ("select_tile", C.c_int32), ("pad_", C.c_int32)]
 
 
 class VertCoordC(C.Structure):
@@ -336,6 +337,8 @@ class Engine:
                                         _ptr(axa), _stream_ptr(stream), C.byref(t)), "autotune")
         out = {}
         for f, _ in TuningC._fields_:
+            if f == "pad_":
+                continue
             v = getattr(t, f)
             out[f] = list(v) if hasattr(v, "__len__") else v
         return out

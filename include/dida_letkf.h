@@ -284,7 +284,9 @@ int dida_letkf_graph_launch(dida_letkf_ctx *ctx, void *stream);
  * < 2048, >= 2048; 1, 2, 4, ..., 64, where 1 starts every analysis of the
  * bucket cold), one bucket at a time: chains are then runs of adjacent
  * columns of one bucket, built on the device every sweep from its counts,
- * drained from one cost-sorted queue.  Each candidate is
+ * drained from one cost-sorted queue.  First of all it picks the tile of
+ * columns (4 x 1, 4 x 2 or 4 x 4) that one wavefront of the selection passes
+ * tests each candidate obs against.  Each candidate is
  * timed twice (HIP events, minimum); xa/axa hold the result of a sweep with
  * the kept layout on return.  Synchronises. */
 typedef struct dida_letkf_tuning {
@@ -299,6 +301,9 @@ typedef struct dida_letkf_tuning {
                                    (and below the next bound) */
     int32_t bucket_chain[4];    /* kept chain length per bucket (1: cold starts) */
     int64_t bucket_count[4];    /* analyses per bucket in the tuned sweep */
+    int32_t select_tile;        /* kept tile of columns of the selection passes:
+                                   0: 4 x 2, 1: 4 x 1, 2: 4 x 4 (latitudes x longitudes) */
+    int32_t pad_;
 } dida_letkf_tuning;
 int dida_letkf_autotune(dida_letkf_ctx *ctx, const dida_letkf_obs *obs, const double *xb,
                         const double *axb, double *xa, double *axa, void *stream,
