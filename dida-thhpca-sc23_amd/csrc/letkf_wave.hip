@@ -34,10 +34,14 @@ using namespace dida;
 
 namespace {
 
-template <int KP>
+// FULLV: the whole eigenvector basis V (the warm start of the next analysis
+// of a chain, column slot j at V[i + j LDV]); otherwise 16 eigenvectors at a
+// time (column slot j at V[(j % 16) LDV + i]), for cold starts only, so that
+// two wavefronts fit a SIMD.
+template <int KP, bool FULLV>
 struct WSmem {
   static constexpr int LDV = KP + 1;
-  double V[KP * LDV];  // eigenvector basis: column slot j at V[i + j LDV]
+  double V[(FULLV ? KP : 16) * LDV];
   double b[KP];
   double f[KP];
   double u[KP];
@@ -132,15 +136,15 @@ struct ObsMeta {
     dp = a.dep[gi];
     if constexpr (VERT) lp = a.obs_lnp[gi];
   }
-  template <int KP>
-  __device__ __forceinline__ void store_index(WSmem<KP>& sm, int buf) const {
+  template <class SM>
+  __device__ __forceinline__ void store_index(SM& sm, int buf) const {
     sm.ogi[buf][threadIdx.x] = gi;
   }
   // sqrt(R^-1 rloc) and R^-1 rloc d; with vertical localisation rloc is
   // scaled by the weight of |ln p_cube - ln p_obs| / vert_dist
   // (letkf_yb.F90:223-243)
-  template <int KP, bool VERT>
-  __device__ __forceinline__ void store_weights(WSmem<KP>& sm, int buf, const dk_batched_args& a,
+  template <bool VERT, class SM>
+  __device__ __forceinline__ void store_weights(SM& sm, int buf, const dk_batched_args& a,
                                                 double clnp) const {
     const int l = threadIdx.x;
     double rv = rl;
@@ -157,8 +161,6 @@ struct Wave {
   static constexpr int NJ = KP / 16;   // column sets per lane (and row blocks)
   static constexpr int NQ = NJ * 4;    // rows per column per lane
   static constexpr int NP = NJ / 2;    // column pairs per lane group
-  static constexpr int LDV = WSmem<KP>::LDV;
-
   double X[NJ][NQ];  // working matrix (see header)
   bool patB;         // current row pattern
 
@@ -171,7 +173,8 @@ struct Wave {
   // Yb rows I*16 + c of the 8 obs of half h in buffer buf (binned rows are
   // KP values apart and zero past k, so every row is in bounds and the I
   // offsets are constants of one address per obs)
-  __device__ __forceinline__ void load_rows(const dk_batched_args& a, const WSmem<KP>& sm,
+  template <class SM>
+  __device__ __forceinline__ void load_rows(const dk_batched_args& a, const SM& sm,
                                             int buf, int h, double (&raw)[8][NJ]) const {
     const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
 #pragma unroll
@@ -183,8 +186,8 @@ struct Wave {
   }
 
   // Accumulate half h of the staged block: C on MFMA, b in FP64
-  template <typename AccD, typename AccH>
-  __device__ __forceinline__ void gram_half(const WSmem<KP>& sm, int buf, int h,
+  template <typename AccD, typename AccH, class SM>
+  __device__ __forceinline__ void gram_half(const SM& sm, int buf, int h,
                                             const double (&raw)[8][NJ], double scale,
                                             double (&bacc)[NJ], AccD& accd, AccH& acch) const {
     const int g = threadIdx.x >> 4;
@@ -241,9 +244,8 @@ struct Wave {
   // two 32-obs halves.  Rows past k read row k - 1 (in bounds) and are masked
   // once at the end (a row of Yhat only reaches the row and column of C it
   // indexes).
-  template <bool VERT>
-  __device__ void gram(const dk_batched_args& a, int64_t off, int p, double clnp,
-                       WSmem<KP>& sm) {
+  template <bool VERT, class SM>
+  __device__ void gram(const dk_batched_args& a, int64_t off, int p, double clnp, SM& sm) {
     const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
     const int k = a.k;
     constexpr bool kF64 = GRAM == DK_GRAM_FP64;
@@ -289,8 +291,8 @@ struct Wave {
       ObsMeta m;
       m.load_index(a, off, p, 0);
       m.load_obs<VERT>(a);
-      m.store_index<KP>(sm, 0);
-      m.store_weights<KP, VERT>(sm, 0, a, clnp);
+      m.store_index(sm, 0);
+      m.store_weights<VERT>(sm, 0, a, clnp);
       __syncthreads();
       double rawA[8][NJ], rawB[8][NJ];
       load_rows(a, sm, 0, 0, rawA);
@@ -302,12 +304,12 @@ struct Wave {
         __builtin_amdgcn_sched_barrier(0);
         gram_half(sm, buf, 0, rawA, scale, bacc, accd, acch);
         n.load_obs<VERT>(a);
-        n.store_index<KP>(sm, buf ^ 1);
+        n.store_index(sm, buf ^ 1);
         __syncthreads();
         load_rows(a, sm, buf ^ 1, 0, rawA);
         __builtin_amdgcn_sched_barrier(0);
         if (base + 32 < p) gram_half(sm, buf, 1, rawB, scale, bacc, accd, acch);
-        n.store_weights<KP, VERT>(sm, buf ^ 1, a, clnp);
+        n.store_weights<VERT>(sm, buf ^ 1, a, clnp);
         __syncthreads();
       }
     }
@@ -357,8 +359,10 @@ struct Wave {
       }
   }
 
-  // X <- M V with V from LDS (the basis of the previous column).
-  __device__ void warm(const WSmem<KP>& sm) {
+  // X <- M V with V from LDS (the basis of the previous analysis).
+  template <class SM>
+  __device__ void warm(const SM& sm) {
+    constexpr int LDV = SM::LDV;
     const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
     double4_t d[NJ][NJ];
 #pragma unroll
@@ -730,12 +734,12 @@ __device__ __forceinline__ void update_rows(int k, int nrow, const double* __res
 // not when it was not finite (its transform is NaN) or its eigensolve hit the
 // sweep cap (its basis is not orthonormal, and A = M V_prev would then not
 // carry the eigenvectors of M).
-template <int KP, int GRAM, int MODE>
+template <int KP, int GRAM, int MODE, bool FULLV>
 __device__ bool wave_column(const dk_batched_args& a, int64_t col, int cube, bool warm,
-                            WSmem<KP>& sm, Wave<KP, GRAM>& w) {
+                            WSmem<KP, FULLV>& sm, Wave<KP, GRAM>& w) {
   constexpr int NJ = Wave<KP, GRAM>::NJ;
   constexpr int NQ = Wave<KP, GRAM>::NQ;
-  constexpr int LDV = WSmem<KP>::LDV;
+  constexpr int LDV = WSmem<KP, FULLV>::LDV;
   constexpr bool VERT = MODE == kModeVertical;
   const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
   const int k = a.k;
@@ -775,7 +779,7 @@ __device__ bool wave_column(const dk_batched_args& a, int64_t col, int cube, boo
   const double km1 = (double)(k - 1);
   w.shift(k, km1 / a.rho);
   clk.mark(DK_PHASE_GRAM);
-  if (warm) w.warm(sm);
+  if (FULLV && warm) w.warm(sm);
   clk.mark(DK_PHASE_WARM);
   bool conv;
   const int sweeps = w.jacobi(a.tol, a.max_sweeps, a.predict != 0, &conv);
@@ -795,8 +799,21 @@ __device__ bool wave_column(const dk_batched_args& a, int64_t col, int cube, boo
   }
   bad = __any(bad);
   __syncthreads();
+
+  // trans(i, e) = W(i, e) + wbar(i): tiles D[4r + g][c] = (I*16 + 4r + g, I2*16 + c),
+  // summed over the eigenvectors v_j = a_j / lambda_j (through LDS: rows to
+  // the lanes of the MFMA operands), f_j = sqrt((k-1)/lambda_j), u_j = v_j^T b
+  // / lambda_j; FULLV: all KP at once (V stays for the next warm start),
+  // otherwise 16 at a time
+  double4_t T[NJ][NJ], wb[NJ];
 #pragma unroll
-  for (int J = 0; J < NJ; ++J) {
+  for (int I = 0; I < NJ; ++I) {
+    wb[I] = double4_t{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int I2 = 0; I2 < NJ; ++I2) T[I][I2] = double4_t{0.0, 0.0, 0.0, 0.0};
+  }
+  // eigenvector slot J to LDS, with f and u
+  auto put_v = [&](int J) {
     const double il = lam[J] > 0.0 ? 1.0 / lam[J] : 0.0;
     double ub = 0.0;
 #pragma unroll
@@ -805,7 +822,7 @@ __device__ bool wave_column(const dk_batched_args& a, int64_t col, int cube, boo
       for (int r = 0; r < 4; ++r) {
         const int i = row_of(w.patB, I, r, g);
         const double v = w.X[J][I * 4 + r] * il;
-        sm.V[i + (J * 16 + c) * LDV] = v;
+        sm.V[i + ((FULLV ? J * 16 : 0) + c) * LDV] = v;
         ub += v * sm.b[i];
       }
     ub = sum_g(ub);
@@ -813,28 +830,36 @@ __device__ bool wave_column(const dk_batched_args& a, int64_t col, int cube, boo
       sm.f[J * 16 + c] = lam[J] > 0.0 ? sqrt(km1 / lam[J]) : 0.0;
       sm.u[J * 16 + c] = ub * il;
     }
-  }
-  __syncthreads();
-
-  // trans(i, e) = W(i, e) + wbar(i): tiles D[4r + g][c] = (I*16 + 4r + g, I2*16 + c)
-  double4_t T[NJ][NJ], wb[NJ];
+  };
+  // the contraction over eigenvectors [j0, j1) (LDS column slots from 0)
+  auto mul_v = [&](int j0, int j1) {
+    for (int jb = j0; jb < j1; jb += 4) {
+      const int jj = jb + g;
+      const double fj = sm.f[jj], uj = sm.u[jj];
+      double v[NJ];
 #pragma unroll
-  for (int I = 0; I < NJ; ++I) {
-    wb[I] = double4_t{0.0, 0.0, 0.0, 0.0};
+      for (int I = 0; I < NJ; ++I) v[I] = sm.V[I * 16 + c + (jj - j0) * LDV];
 #pragma unroll
-    for (int I2 = 0; I2 < NJ; ++I2) T[I][I2] = double4_t{0.0, 0.0, 0.0, 0.0};
-  }
-  for (int j0 = 0; j0 < KP; j0 += 4) {
-    const int jj = j0 + g;
-    const double fj = sm.f[jj], uj = sm.u[jj];
-    double v[NJ];
+      for (int I = 0; I < NJ; ++I) {
+        wb[I] = mfma_f64_16x16x4(v[I], uj, wb[I]);
 #pragma unroll
-    for (int I = 0; I < NJ; ++I) v[I] = sm.V[I * 16 + c + jj * LDV];
+        for (int I2 = 0; I2 < NJ; ++I2)
+          T[I][I2] = mfma_f64_16x16x4(v[I] * fj, v[I2], T[I][I2]);
+      }
+    }
+  };
+  if constexpr (FULLV) {
 #pragma unroll
-    for (int I = 0; I < NJ; ++I) {
-      wb[I] = mfma_f64_16x16x4(v[I], uj, wb[I]);
+    for (int J = 0; J < NJ; ++J) put_v(J);
+    __syncthreads();
+    mul_v(0, KP);
+  } else {
 #pragma unroll
-      for (int I2 = 0; I2 < NJ; ++I2) T[I][I2] = mfma_f64_16x16x4(v[I] * fj, v[I2], T[I][I2]);
+    for (int J = 0; J < NJ; ++J) {
+      put_v(J);
+      __syncthreads();
+      mul_v(J * 16, J * 16 + 16);
+      __syncthreads();  // the slot is read before the next one overwrites it
     }
   }
   const double nan = __builtin_nan("");
@@ -894,12 +919,15 @@ __device__ bool wave_column(const dk_batched_args& a, int64_t col, int cube, boo
 }
 
 // The plain form: one wavefront per analysis of [aid0, aid0 + gridDim.x) in
-// grid order, every eigensolve cold (no chains), transform to HBM.
+// grid order, every eigensolve cold (no chains), transform to HBM.  Two
+// wavefronts a SIMD (256 registers each: the few spills cost less than the
+// latency a second wavefront hides; config 2: 210 -> 198 ms a sweep).
 template <int KP>
-__global__ __launch_bounds__(64, 1) void k_letkf_plain(dk_batched_args a, int32_t aid0) {
-  __shared__ WSmem<KP> sm;
+__global__ __launch_bounds__(64, 2) void k_letkf_plain(dk_batched_args a, int32_t aid0) {
+  __shared__ WSmem<KP, false> sm;
   Wave<KP, DK_GRAM_FP64> w;
-  wave_column<KP, DK_GRAM_FP64, kModeTransOut>(a, (int64_t)aid0 + blockIdx.x, 0, false, sm, w);
+  wave_column<KP, DK_GRAM_FP64, kModeTransOut, false>(a, (int64_t)aid0 + blockIdx.x, 0, false,
+                                                       sm, w);
 }
 
 // First longitude of stripe g of n groups (groups of whole longitudes).
@@ -907,9 +935,11 @@ __device__ __forceinline__ int stripe_lo(int g, int n, int nlon) {
   return (int)(((int64_t)g * nlon) / n);
 }
 
+// One wavefront per SIMD: the fused analysis needs about 490 registers
+// (at 256, two waves a SIMD, it spills 300-700 of them and runs slower).
 template <int KP, int GRAM, int MODE>
 __global__ __launch_bounds__(64, 1) void k_letkf_wave(dk_batched_args a) {
-  __shared__ WSmem<KP> sm;
+  __shared__ WSmem<KP, true> sm;
   Wave<KP, GRAM> w;
   const int lane = threadIdx.x;
   // one queue (ngroup 0) or ngroup group queues, own group first
@@ -955,10 +985,10 @@ __global__ __launch_bounds__(64, 1) void k_letkf_wave(dk_batched_args a) {
         // of the previous column
         const bool down = ((ilat - lat0) & 1) != 0;
         for (int t = 0; t < a.ncube; ++t)
-          prev_ok = wave_column<KP, GRAM, MODE>(a, col, down ? a.ncube - 1 - t : t,
-                                                a.warm_start && prev_ok, sm, w);
+          prev_ok = wave_column<KP, GRAM, MODE, true>(a, col, down ? a.ncube - 1 - t : t,
+                                                      a.warm_start && prev_ok, sm, w);
       } else {
-        prev_ok = wave_column<KP, GRAM, MODE>(a, col, 0, a.warm_start && prev_ok, sm, w);
+        prev_ok = wave_column<KP, GRAM, MODE, true>(a, col, 0, a.warm_start && prev_ok, sm, w);
       }
     }
   }
