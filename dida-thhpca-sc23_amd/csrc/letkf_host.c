@@ -474,6 +474,12 @@ int dida_letkf_set_grid(dida_letkf_ctx *c, const float *grid_lat, const float *g
 static int alloc_vertical_sort(dida_letkf_ctx *c)
 {
     if (c->prm.vert_dist <= 0.0 || c->capacity <= 0 || c->vkey_in) return DIDA_OK;
+    /* rocPRIM's segmented sort counts entries and segments in 32 bits */
+    if (c->capacity > (int64_t)UINT32_MAX || c->nsel > (int64_t)UINT32_MAX)
+        return set_err(DIDA_ERR_UNSUPPORTED,
+                       "vertical localisation: a selection workspace of %lld entries is above "
+                       "2^32 - 1 (pass a smaller capacity to run the sweep in pieces)",
+                       (long long)c->capacity);
     const size_t cap = (size_t)c->capacity;
     size_t tb = 0;
     if (dk_seg_sort_lnp(NULL, &tb, c->nsel, c->capacity, c->offset, NULL, NULL, NULL, NULL, NULL,
@@ -889,7 +895,9 @@ static int core_launch(dida_letkf_ctx *c, dk_core_args *ca, hipStream_t s)
         return c->prm.nanals <= 64 ? dk_letkf_plain_wave(ca, s) : dk_letkf_core_f64(ca, s);
     dk_chain_args ch;
     ch.chain_len = c->chain_len;
-    ch.nchain = (ca->nanal + c->chain_len - 1) / c->chain_len;
+    /* chains on multiples of chain_len from analysis 0 (a first partial one
+     * when aid0 is inside a chain) */
+    ch.nchain = (ca->aid0 % c->chain_len + ca->nanal + c->chain_len - 1) / c->chain_len;
     ch.head = c->queue_head;
     ch.vscr = c->vscr;
     ch.phase_clk = NULL;
@@ -956,14 +964,17 @@ static int plain_transform(dida_letkf_ctx *c, int64_t a0, int64_t a1, int64_t ba
     ua.xa32 = c->xa32;
     ua.rtps = c->prm.rtps;
     ua.rtps_alpha = c->prm.rtps_alpha;
-    /* chained analyses: launches start at multiples of the chain length, so
-     * every chain is the same whichever runs and launches a sweep is cut in */
-    const int64_t chunk = c->chained && c->chunk >= c->chain_len
-                              ? c->chunk / c->chain_len * c->chain_len
-                              : c->chunk;
+    /* chained analyses: launches end at multiples of the chain length, so
+     * every chain is the same whichever launches a run is cut in */
+    const int64_t L = c->chained ? c->chain_len : 1;
+    const int64_t chunk = c->chunk >= L ? c->chunk / L * L : c->chunk;
+#define LAUNCH_END(c0)                                                               \
+    (chunk >= L ? ((c0) / L * L + chunk < a1 ? (c0) / L * L + chunk : a1)              \
+                : ((c0) + chunk < a1 ? (c0) + chunk : a1))
     if (c->prm.cube_n > 1) {
-        for (int64_t c0 = a0; c0 < a1; c0 += chunk) {
-            int32_t n = (int32_t)((a1 - c0) < chunk ? (a1 - c0) : chunk);
+        for (int64_t c0 = a0, c1; c0 < a1; c0 = c1) {
+            c1 = LAUNCH_END(c0);
+            int32_t n = (int32_t)(c1 - c0);
             ca.aid0 = (int32_t)c0;
             ca.nanal = n;
             ca.trans = c->trans_all + c0 * k * k;
@@ -976,8 +987,9 @@ static int plain_transform(dida_letkf_ctx *c, int64_t a0, int64_t a1, int64_t ba
         }
         return DIDA_OK;
     }
-    for (int64_t c0 = a0; c0 < a1; c0 += chunk) {
-        int32_t n = (int32_t)((a1 - c0) < chunk ? (a1 - c0) : chunk);
+    for (int64_t c0 = a0, c1; c0 < a1; c0 = c1) {
+        c1 = LAUNCH_END(c0);
+        int32_t n = (int32_t)(c1 - c0);
         ca.aid0 = (int32_t)c0;
         ca.nanal = n;
         HIP_TRY(core_launch(c, &ca, s));
@@ -988,6 +1000,7 @@ static int plain_transform(dida_letkf_ctx *c, int64_t a0, int64_t a1, int64_t ba
         else HIP_TRY(dk_update_f64(&ua, s));
         tmark(c, DIDA_STAGE_UPDATE, s);
     }
+#undef LAUNCH_END
     return DIDA_OK;
 }
 
@@ -1023,7 +1036,9 @@ static int sweep_plain_chunked(dida_letkf_ctx *c, dk_select_args *sa, const doub
         }
         int64_t a1 = lo;
         if (c->chained && a1 < n) {
-            /* end the run at a chain boundary when a whole chain fits */
+            /* end the run at a chain boundary when one is in reach (chains
+             * are counted from analysis 0, so a run that ends inside a chain
+             * is continued by the next one all the same) */
             const int64_t al = a1 / c->chain_len * c->chain_len;
             if (al > a0) a1 = al;
         }

@@ -707,6 +707,17 @@ __global__ __launch_bounds__(kThreads) void k_letkf_core_f64(dk_core_args a) {
   analysis_f64<KP, 6, DK_GRAM_FP64>(a, aid, sm, nullptr, nullptr, nullptr);
 }
 
+// Analyses [c0, c1) of chain q of a launch over [aid0, aid0 + nanal): chains
+// start at multiples of chain_len counted from analysis 0, so a launch that
+// starts inside a chain runs the rest of it as its first chain, and how a
+// sweep is cut into launches does not change its chains.
+__device__ __forceinline__ void chain_range(const dk_core_args& a, int chain_len, int q, int* c0,
+                                            int* c1) {
+  const int s0 = a.aid0 - a.aid0 % chain_len + q * chain_len;
+  *c0 = max(s0, a.aid0);
+  *c1 = min(s0 + chain_len, a.aid0 + a.nanal);
+}
+
 // Chains of chain_len consecutive analyses drained from a work counter by
 // persistent workgroups; every analysis after the first of a chain starts its
 // eigensolve from the previous one's eigenvectors (a per-workgroup KP x KP
@@ -722,8 +733,8 @@ __global__ __launch_bounds__(kThreads) void k_letkf_chain_f64(dk_core_args a, dk
     const int item = item_s;
     __syncthreads();
     if (item >= ch.nchain) break;
-    const int c0 = a.aid0 + (ch.order ? ch.order[item] : item) * ch.chain_len;
-    const int c1 = min(c0 + ch.chain_len, a.aid0 + a.nanal);
+    int c0, c1;
+    chain_range(a, ch.chain_len, ch.order ? ch.order[item] : item, &c0, &c1);
     bool prev_ok = false;  // whether the previous analysis may warm-start this one
     for (int aid = c0; aid < c1; ++aid)
       prev_ok = analysis_f64<KP, 4, GRAM>(a, aid, sm, prev_ok ? vs : nullptr,
@@ -735,8 +746,8 @@ __global__ void k_chain_costs(dk_core_args a, int32_t chain_len, int32_t nchain,
                               int32_t* id) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nchain) return;
-  const int c0 = a.aid0 + q * chain_len;
-  const int c1 = min(c0 + chain_len, a.aid0 + a.nanal);
+  int c0, c1;
+  chain_range(a, chain_len, q, &c0, &c1);
   double cost = 0.0;
   for (int aid = c0; aid < c1; ++aid) {
     const int p = a.nobsl[aid];

@@ -35,7 +35,8 @@ def _check(oracle, data, params, xa, axa, cube_n=1, overlap=0, tol=1e-10):
         assert e_xa <= tol, var
 
 
-@pytest.mark.parametrize("case", ["k100_chain5", "k128_capped", "k100_cubes", "k128_chunks"])
+@pytest.mark.parametrize("case", ["k100_chain5", "k128_capped", "k100_cubes", "k128_chunks",
+                                  "k100_chunks_in_chain"])
 def test_chained_batched_k128(cuda, dl, oracle, case):
     torch = cuda
     print(f"chained batched {case}", flush=True)
@@ -45,7 +46,7 @@ def test_chained_batched_k128(cuda, dl, oracle, case):
     data = dl.synth_generate(cfg)
     p = dict(nanals=k, nrow=7, nlat=cfg.nlat, nlon=cfg.nlon, local_dist=1200.0,
              inflation_factor=1.05, bin_deg=4.0, form=dl.FORM_BATCHED, gram=dl.GRAM_FP64,
-             chain_len=5)
+             chain_len=40 if case == "k100_chunks_in_chain" else 5)
     cap, cube = 0, (1, 0)
     if case == "k128_capped":
         p["max_obs"] = 200
@@ -54,12 +55,16 @@ def test_chained_batched_k128(cuda, dl, oracle, case):
         p.update(cube_n=2, overlap=1)
     params = dl.default_params(**p)
     obs = dl.DeviceObs(data["lat"], data["lon"], data["yb"], data["dep"], data["rinv"])
-    if case == "k128_chunks":
+    if case.endswith("chunks") or case.endswith("in_chain"):
         probe = dl.Engine(params, nobs_max=cfg.nobs)
         probe.set_grid(data["grid_lat"], data["grid_lon"])
         x, ax = torch.empty_like(data["xb"]), torch.empty_like(data["axb"])
         probe.sweep(obs, data["xb"], data["axb"], x, ax)
-        cap = probe.stats()["nobsl_sum"] // 3 + 1
+        pst = probe.stats()
+        # k128_chunks: a third of the lists (runs of whole chains);
+        # in_chain: a third of one 40-analysis chain's lists (runs end inside chains)
+        cap = pst["nobsl_sum"] // 3 + 1 if case == "k128_chunks" else \
+            max(pst["nobsl_sum"] // 60, pst["nobsl_max"]) + 1
         probe.close()
     eng = dl.Engine(params, nobs_max=cfg.nobs, capacity=cap)
     eng.set_grid(data["grid_lat"], data["grid_lon"])
@@ -70,9 +75,16 @@ def test_chained_batched_k128(cuda, dl, oracle, case):
     assert st["status"] == 0 and st["nonconverged"] == 0, st
     if case == "k128_capped":
         assert st["nobsl_capped"] > 0
-    if cap:
+    if case == "k128_chunks":
         # runs start at chain boundaries: bit-identical to one whole-workspace sweep
         assert torch.equal(xa, x) and torch.equal(axa, ax)
+    elif cap:
+        # a chain split between runs keeps its analyses (chains are counted
+        # from analysis 0) but its second part starts cold: the same within
+        # the eigensolver's tolerance
+        inc, inc1 = (axa - data["axb"]).cpu().numpy(), (ax - data["axb"]).cpu().numpy()
+        assert _rel_err(inc, inc1) <= 1e-10
+        assert _rel_err(xa.cpu().numpy(), x.cpu().numpy()) <= 1e-10
     _check(oracle, data, params, xa, axa, *cube)
     if case == "k100_chain5":
         # the plain form's cold starts take more sweeps
