@@ -95,23 +95,17 @@ __device__ __forceinline__ double rsq_nr(double x) {
   y = y * fma(-hx * y, y, 1.5);
   return y;
 }
-// One Newton step each (relative error about 2^-45): enough for a rotation
-// angle, whose error only delays convergence.
-__device__ __forceinline__ double rsq_nr1(double x) {
-  const double y = __builtin_amdgcn_rsq(x);
-  return y * fma(-0.5 * x * y, y, 1.5);
-}
-__device__ __forceinline__ double rcp_nr1(double x) {
-  const double y = __builtin_amdgcn_rcp(x);
-  return fma(y, fma(-x, y, 1.0), y);
-}
 
 // Scaled one-sided Jacobi rotation of a column pair (p, q) stored as
 // sp x_p, sq x_q, from their squared norms al, be and gs = x_p . x_q:
 // t = tan(theta), gm = gamma = gs sp sq, cs = cos(theta), ic = 1/cos(theta).
 // A pair is rotated while gamma^2 > tol2 al be (t = 0, cs = ic = 1 below);
 // *big records gamma^2 > tolp al be (the next sweep would still rotate).
-// t = sign(d e) |e| / (|d| + sqrt(d^2 + e^2)), d = be - al, e = 2 gamma.
+// t = sign(d e) |e| / (|d| + sqrt(d^2 + e^2)), d = be - al, e = 2 gamma,
+// from the hardware estimates of 1/sqrt and 1/x: an error of t (about 2^-22
+// relative) only leaves a rotated pair that far from orthogonal, which the
+// next sweep removes, while cs (two Newton steps) keeps every applied
+// rotation orthogonal to a few ulp.
 __device__ __forceinline__ void rotation_angle(double gs, double al, double be, double sp,
                                                double sq, double tol2, double tolp,
                                                bool& rotated, bool& big, double& t, double& gm,
@@ -123,8 +117,8 @@ __device__ __forceinline__ void rotation_angle(double gs, double al, double be, 
   rotated |= need;
   const double d = be - al, e = 2.0 * gm;
   const double r2 = fma(d, d, e * e);
-  const double den = fabs(d) + r2 * rsq_nr1(r2);
-  t = fabs(e) * rcp_nr1(den);
+  const double den = fma(r2, __builtin_amdgcn_rsq(r2), fabs(d));
+  t = fabs(e) * __builtin_amdgcn_rcp(den);
   if ((d < 0.0) != (e < 0.0) && d != 0.0) t = -t;
   t = need ? t : 0.0;
   const double u = fma(t, t, 1.0);

@@ -436,8 +436,8 @@ struct Wave {
     // t = sign(d e) |e| / (|d| + sqrt(d^2 + e^2)), d = beta - alpha, e = 2 gamma
     const double d = be - al, e = 2.0 * gm;
     const double r2 = fma(d, d, e * e);
-    const double den = fabs(d) + r2 * rsq_nr1(r2);
-    double t = fabs(e) * rcp_nr1(den);
+    const double den = fma(r2, __builtin_amdgcn_rsq(r2), fabs(d));
+    double t = fabs(e) * __builtin_amdgcn_rcp(den);
     if ((d < 0.0) != (e < 0.0) && d != 0.0) t = -t;
     t = need ? t : 0.0;
     const double u = fma(t, t, 1.0);
