@@ -29,10 +29,12 @@ typedef struct dk_bin_geom {
  * formula precomputed (cos/sin of the latitude in the reference's radians
  * and the longitude in the reference's radians) and the unit vector
  * (cx cos yr, cx sin yr, sx) rounded to FP32, used to decide the candidates
- * away from the support radius. */
+ * away from the support radius, and in FP64, for the weights of the obs
+ * surely inside it. */
 typedef struct dk_obs_binned {
     const double *cx, *sx, *yr;   /* [nobs] */
     const float *u4;              /* [nobs * 4]: unit vector x, y, z, 0 */
+    const double *ux, *uy, *uz;   /* [nobs]: unit vector in FP64 */
     const double *yb;             /* [nobs * k] ensemble fastest */
     const double *dep;            /* [nobs] */
     const float *rinv;            /* [nobs] */
@@ -91,8 +93,9 @@ int dk_bin_scan(int32_t nbins, const int32_t *count, int32_t *bin_start, void *s
  * yb_b and/or rounded to FP32 into yb32_b (either may be NULL). */
 int dk_obs_gather(int32_t nobs, int32_t k, int32_t ldy, const int32_t *perm, const double *lat,
                   const double *lon, const double *yb, const double *dep, const float *rinv,
-                  double *cx, double *sx, double *yr, float *u4, double *yb_b, float *yb32_b,
-                  double *dep_b, float *rinv_b, void *stream);
+                  double *cx, double *sx, double *yr, float *u4, double *ux, double *uy,
+                  double *uz, double *yb_b, float *yb32_b, double *dep_b, float *rinv_b,
+                  void *stream);
 /* Selection passes: count (capped at max_obs) and fill (entries with gc > 0
  * and perm < thr, in bin order) by tiles of columns (DK_SELECT_TILE_*), one
  * wavefront a tile reading each candidate once for its columns; cap (thr of every

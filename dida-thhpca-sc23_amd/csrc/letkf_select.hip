@@ -91,7 +91,9 @@ __global__ void k_obs_gather(int32_t nobs, const int32_t* __restrict__ perm,
                              const double* __restrict__ dep, const float* __restrict__ rinv,
                              double* __restrict__ cx, double* __restrict__ sx,
                              double* __restrict__ yr, float4* __restrict__ u4,
-                             double* __restrict__ dep_b, float* __restrict__ rinv_b) {
+                             double* __restrict__ ux, double* __restrict__ uy,
+                             double* __restrict__ uz, double* __restrict__ dep_b,
+                             float* __restrict__ rinv_b) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nobs) return;
   int s = perm[i];
@@ -100,7 +102,11 @@ __global__ void k_obs_gather(int32_t nobs, const int32_t* __restrict__ perm,
   cx[i] = c1;
   sx[i] = s1;
   yr[i] = y1;
-  u4[i] = make_float4((float)(c1 * cos(y1)), (float)(c1 * sin(y1)), (float)s1, 0.0f);
+  const double vx = c1 * cos(y1), vy = c1 * sin(y1);
+  u4[i] = make_float4((float)vx, (float)vy, (float)s1, 0.0f);
+  ux[i] = vx;
+  uy[i] = vy;
+  uz[i] = s1;
   dep_b[i] = dep[s];
   rinv_b[i] = rinv[s];
 }
@@ -202,6 +208,7 @@ __device__ __forceinline__ void for_region_ranges(const dk_select_args& a, const
 // The column's geometry for the candidate test.
 struct ColGeo {
   double cx2, sx2, y2r;  // cos / sin of the latitude, longitude (reference radians)
+  double ux, uy, uz;     // unit vector
   float vx, vy, vz;      // unit vector, FP32
 };
 
@@ -211,9 +218,12 @@ __device__ __forceinline__ ColGeo col_geo(double clat, double clon) {
   q.cx2 = cos(x2r);
   q.sx2 = sin(x2r);
   q.y2r = deg2rad_ref(clon);
-  q.vx = (float)(q.cx2 * cos(q.y2r));
-  q.vy = (float)(q.cx2 * sin(q.y2r));
-  q.vz = (float)q.sx2;
+  q.ux = q.cx2 * cos(q.y2r);
+  q.uy = q.cx2 * sin(q.y2r);
+  q.uz = q.sx2;
+  q.vx = (float)q.ux;
+  q.vy = (float)q.uy;
+  q.vz = (float)q.uz;
   return q;
 }
 
@@ -247,6 +257,41 @@ __device__ __forceinline__ bool exact_keep(const dk_select_args& a, const ColGeo
   return gc > 0.0;
 }
 
+// Central angle (rad) between unit vectors p and q from their difference d
+// = p - q: |d| = 2 sin(theta / 2), so theta = 2 asin(|d| / 2), summed as the
+// Maclaurin series of asin, x sum_k (2k)! / (4^k (k!)^2 (2k + 1)) x^(2k).
+// For theta <= 0.4 (x^2 < 0.04) thirteen terms leave a relative error below
+// 1e-18; unlike acos(p . q) it keeps its digits at short distances.
+constexpr double kChordMaxDeg = 22.9;  // 0.4 rad
+__device__ __forceinline__ double chord_angle(double dx, double dy, double dz) {
+  const double c2 = fma(dx, dx, fma(dy, dy, dz * dz));
+  const double x2 = 0.25 * c2;
+  double s = 0.006447210311889649;      // 676039 / 104857600
+  s = fma(s, x2, 0.0073125258735988454);  // 88179 / 12058624
+  s = fma(s, x2, 0.008390335809616815);   // 46189 / 5505024
+  s = fma(s, x2, 0.009761609529194078);   // 12155 / 1245184
+  s = fma(s, x2, 0.011551800896139705);   // 6435 / 557056
+  s = fma(s, x2, 0.01396484375);          // 143 / 10240
+  s = fma(s, x2, 0.017352764423076924);   // 231 / 13312
+  s = fma(s, x2, 0.022372159090909092);   // 63 / 2816
+  s = fma(s, x2, 0.030381944444444444);   // 35 / 1152
+  s = fma(s, x2, 0.044642857142857144);   // 5 / 112
+  s = fma(s, x2, 0.075);                  // 3 / 40
+  s = fma(s, x2, 0.16666666666666666);    // 1 / 6
+  s = fma(s, x2, 1.0);
+  return sqrt(c2) * s;                    // 2 x s
+}
+
+// The weight of binned obs l surely inside the support (its keep decided by
+// the FP32 test): the Gaspari-Cohn weight of r = a theta / L with theta from
+// the FP64 unit vectors (agrees with the reference formula to about 1e-13,
+// the rounding of its acos near 1).
+__device__ __forceinline__ double inner_weight(const dk_select_args& a, const ColGeo& q, int l,
+                                               double r_per_rad) {
+  const double th = chord_angle(a.obs.ux[l] - q.ux, a.obs.uy[l] - q.uy, a.obs.uz[l] - q.uz);
+  return gc_weight(th * r_per_rad, a.gc_filter);
+}
+
 // Whether binned obs l is kept (gc > 0), without its weight.
 __device__ __forceinline__ bool keep_candidate(const dk_select_args& a, const ColGeo& q, int l,
                                                const float4& u) {
@@ -262,15 +307,16 @@ __device__ __forceinline__ bool keep_candidate(const dk_select_args& a, const Co
 constexpr int kNT = 16;
 
 struct TileCols {
-  double cx2[kNT], sx2[kNT], y2r[kNT];
+  double cx2[kNT], sx2[kNT], y2r[kNT], ux[kNT], uy[kNT], uz[kNT];
   int64_t out[kNT];
   float vx[kNT], vy[kNT], vz[kNT];
   int32_t limit[kNT], thr[kNT], cnt[kNT];
 };
 
 // Fill pass: per column, the candidates that passed the FP32 test, in scan
-// order, waiting for a full wavefront of exact tests (up to 63 waiting plus
-// the 64 of one chunk).
+// order, waiting for a full wavefront of weights and exact tests (up to 63
+// waiting plus the 64 of one chunk): l, or ~l for a candidate in the shell
+// that the reference formula decides.
 template <int NTT>
 struct TileBuf {
   int32_t l[NTT][128];
@@ -350,6 +396,9 @@ __global__ __launch_bounds__(256) void k_select_tiles(dk_select_args a) {
     tc.cx2[lane] = q.cx2;
     tc.sx2[lane] = q.sx2;
     tc.y2r[lane] = q.y2r;
+    tc.ux[lane] = q.ux;
+    tc.uy[lane] = q.uy;
+    tc.uz[lane] = q.uz;
     tc.vx[lane] = q.vx;
     tc.vy[lane] = q.vy;
     tc.vz[lane] = q.vz;
@@ -365,16 +414,34 @@ __global__ __launch_bounds__(256) void k_select_tiles(dk_select_args a) {
   // columns still counting / filling (bit j)
   uint32_t open = (uint32_t)__ballot(lane < kNTT && tc.limit[lane < kNTT ? lane : 0] > 0);
   const float flo = far_lo32(a), nhi = near_hi32(a);
-  // Fill: the exact test (and weight) of the first n buffered candidates of
-  // column j, one per lane; kept ones are written in buffer (scan) order.
+  // weights of the candidates surely inside from the chord angle when the
+  // support is within its range
+  const bool chord = kFill && a.theta_deg <= kChordMaxDeg;
+  const double r_per_rad = kEarthA / a.local_dist;
+  // Fill: the weight (and, in the shell, the exact test) of the first n
+  // buffered candidates of column j, one per lane; kept ones are written in
+  // buffer (scan) order.
   auto flush = [&](int j, int n) {
     ColGeo q;
     q.cx2 = tc.cx2[j];
     q.sx2 = tc.sx2[j];
     q.y2r = tc.y2r[j];
-    const int l = lane < n ? tb.l[j][lane] : 0;
+    q.ux = tc.ux[j];
+    q.uy = tc.uy[j];
+    q.uz = tc.uz[j];
+    const int lb = lane < n ? tb.l[j][lane] : 0;
+    const bool shell = lb < 0;
+    const int l = shell ? ~lb : lb;
     double gc = 0.0;
-    const bool keep = lane < n && exact_keep(a, q, l, gc);
+    bool keep = false;
+    if (lane < n) {
+      if (shell || !chord) {
+        keep = exact_keep(a, q, l, gc);
+      } else {
+        gc = inner_weight(a, q, l, r_per_rad);
+        keep = true;
+      }
+    }
     const uint64_t bm = __ballot(keep);
     const int c0 = tc.cnt[j], lim = tc.limit[j];
     if (keep) {
@@ -413,7 +480,7 @@ __global__ __launch_bounds__(256) void k_select_tiles(dk_select_args a) {
             if (cand)
               tb.l[j][nb0 + (int)__builtin_amdgcn_mbcnt_hi(
                                 (uint32_t)(bm >> 32),
-                                __builtin_amdgcn_mbcnt_lo((uint32_t)bm, 0u))] = l;
+                                __builtin_amdgcn_mbcnt_lo((uint32_t)bm, 0u))] = d > nhi ? l : ~l;
             int nb = nb0 + __popcll(bm);
             wave_sync_lds();
             if (nb >= kWave) {
@@ -599,12 +666,12 @@ extern "C" int dk_obs_gather(int32_t nobs, int32_t k, int32_t ldy, const int32_t
                              const double* lat,
                              const double* lon, const double* yb, const double* dep,
                              const float* rinv, double* cx, double* sx, double* yr, float* u4,
-                             double* yb_b, float* yb32_b, double* dep_b, float* rinv_b,
-                             void* stream) {
+                             double* ux, double* uy, double* uz, double* yb_b, float* yb32_b,
+                             double* dep_b, float* rinv_b, void* stream) {
   if (nobs <= 0) return 0;
   hipStream_t s = (hipStream_t)stream;
   k_obs_gather<<<(nobs + 255) / 256, 256, 0, s>>>(nobs, perm, lat, lon, dep, rinv, cx, sx, yr,
-                                                  (float4*)u4, dep_b, rinv_b);
+                                                  (float4*)u4, ux, uy, uz, dep_b, rinv_b);
   const int64_t total = (int64_t)nobs * ldy;
   const unsigned blocks = (unsigned)((total + 255) / 256);
   if (yb_b) k_obs_gather_yb<double><<<blocks, 256, 0, s>>>(nobs, k, ldy, perm, yb, yb_b);
