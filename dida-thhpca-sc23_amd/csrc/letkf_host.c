@@ -174,6 +174,13 @@ struct dida_letkf_ctx {
     int64_t *offset;
     void *scan_tmp;
     size_t scan_tmp_bytes;
+    /* count / fill tiles, largest candidate count first (sized for the
+     * smallest tile shape) */
+    int32_t ntile_max;
+    uint32_t *tile_key, *tile_key_sorted;
+    int32_t *tile_id, *tile_order;
+    void *tile_tmp;
+    size_t tile_tmp_bytes;
     int32_t *csr_idx;
     double *csr_w;
     int64_t capacity;
@@ -254,7 +261,8 @@ int dida_letkf_destroy(dida_letkf_ctx *c)
                     c->obs_lnp, c->cube_lnp, c->cube_row_off, c->cube_rows, c->ca_lat,
                     c->ca_lon, c->trans_all, c->vkey_in, c->vkey_out, c->vpos_in,
                     c->vpos_out, c->vidx, c->vw, c->vsort_tmp, c->vscr, c->line_cnt,
-                    c->chain_col0, c->chain_ncol, c->line_off};
+                    c->chain_col0, c->chain_ncol, c->line_off, c->tile_key,
+                    c->tile_key_sorted, c->tile_id, c->tile_order, c->tile_tmp};
     for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; ++i) dfree(ptrs[i]);
     free(c);
     return DIDA_OK;
@@ -408,6 +416,17 @@ int dida_letkf_create(dida_letkf_ctx **out, const dida_letkf_params *prm, int32_
     DMALLOC(c->offset, (size_t)(c->nsel + 1) * sizeof(int64_t));
     dk_scan_offsets(NULL, &c->scan_tmp_bytes, NULL, NULL, (int32_t)c->nsel, NULL);
     DMALLOC(c->scan_tmp, c->scan_tmp_bytes);
+    c->ntile_max = (int32_t)dk_select_ntiles(c->nalat, c->nalon, DK_SELECT_TILE_4X1);
+    DMALLOC(c->tile_key, (size_t)c->ntile_max * sizeof(uint32_t));
+    DMALLOC(c->tile_key_sorted, (size_t)c->ntile_max * sizeof(uint32_t));
+    DMALLOC(c->tile_id, (size_t)c->ntile_max * sizeof(int32_t));
+    DMALLOC(c->tile_order, (size_t)c->ntile_max * sizeof(int32_t));
+    if (dk_select_tile_order(NULL, c->ntile_max, NULL, NULL, NULL, NULL, NULL, &c->tile_tmp_bytes,
+                             NULL)) {
+        dida_letkf_destroy(c);
+        return set_err(DIDA_ERR_HIP, "tile sort size query failed");
+    }
+    DMALLOC(c->tile_tmp, c->tile_tmp_bytes);
     DMALLOC(c->flags, 4 * sizeof(int32_t));
     c->capacity = sel_capacity;
     c->user_capacity = sel_capacity > 0;
@@ -664,6 +683,17 @@ static int sweep_select_count(dida_letkf_ctx *c, const dida_letkf_obs *o, hipStr
     sa->tile = c->sel_tile;
     sa->perm_bits = 1;
     while (sa->perm_bits < 31 && ((int64_t)1 << sa->perm_bits) < (int64_t)o->nobs) ++sa->perm_bits;
+    /* the tiles of the count and fill passes, largest first */
+    {
+        const int64_t nt = dk_select_ntiles(sa->nlat, sa->nlon, sa->tile);
+        size_t tb = c->tile_tmp_bytes;
+        sa->tile_order = NULL;
+        if (nt <= c->ntile_max) {
+            HIP_TRY(dk_select_tile_order(sa, (int32_t)nt, c->tile_key, c->tile_id,
+                                         c->tile_key_sorted, c->tile_order, c->tile_tmp, &tb, s));
+            sa->tile_order = c->tile_order;
+        }
+    }
     HIP_TRY(dk_select_count(sa, s));
     /* columns capped at MAX_OBS: the caller-order threshold of what they keep */
     HIP_TRY(dk_select_cap(sa, s));

@@ -69,6 +69,9 @@ typedef struct dk_select_args {
     int32_t *thr;
     int32_t perm_bits;
     int32_t tile;        /* count and fill: DK_SELECT_TILE_* columns per wavefront */
+    /* count and fill over all columns: wavefront w of the launch takes tile
+     * tile_order[w] (largest candidate count first); NULL: grid order */
+    const int32_t *tile_order;
 } dk_select_args;
 
 /* Tile shapes (latitudes x longitudes of the analysis grid) of the count and
@@ -102,6 +105,15 @@ int dk_obs_gather(int32_t nobs, int32_t k, int32_t ldy, const int32_t *perm, con
  * column; a column whose count reached max_obs scans all its candidates for
  * the caller-order threshold by a radix select over perm), one wavefront a
  * column. */
+/* Tiles of the count and fill passes (shape a->tile) over the analysis grid. */
+int64_t dk_select_ntiles(int32_t nlat, int32_t nlon, int32_t tile);
+/* order[0 .. ntiles) = the tiles by the obs count of their candidate bins,
+ * largest first (so that the longest tiles do not start last); key, id and
+ * key_sorted hold ntiles entries; tmp as dk_sort_pairs (query with tmp NULL:
+ * *tmp_bytes for ntiles). */
+int dk_select_tile_order(const dk_select_args *a, int32_t ntiles, uint32_t *key, int32_t *id,
+                         uint32_t *key_sorted, int32_t *order, void *tmp, size_t *tmp_bytes,
+                         void *stream);
 int dk_select_count(const dk_select_args *a, void *stream);
 int dk_select_cap(const dk_select_args *a, void *stream);
 int dk_select_fill(const dk_select_args *a, void *stream);

@@ -361,8 +361,14 @@ __global__ __launch_bounds__(256) void k_select_tiles(dk_select_args a) {
   TileCols& tc = tc_all[wave];
   auto& tb = tb_all[wave];
   const int ntj = (a.nlat + kTJ - 1) / kTJ;
-  const int ti0 = kFill && a.ncols > 0 ? (int)(a.col0 / a.nlat) / kTI : 0;
-  const int64_t tile = (int64_t)ti0 * ntj + (int64_t)blockIdx.x * 4 + wave;
+  const bool run = kFill && a.ncols > 0;
+  const int ti0 = run ? (int)(a.col0 / a.nlat) / kTI : 0;
+  int64_t tile = (int64_t)ti0 * ntj + (int64_t)blockIdx.x * 4 + wave;
+  if (a.tile_order && !run) {
+    const int64_t ntiles = (int64_t)ntj * ((a.nlon + kTI - 1) / kTI);
+    if (tile >= ntiles) return;
+    tile = a.tile_order[tile];
+  }
   const int tj = (int)(tile % ntj), ti = (int)(tile / ntj);
   const int64_t c_lo = kFill && a.ncols > 0 ? a.col0 : 0;
   const int64_t c_hi = kFill && a.ncols > 0 ? a.col0 + a.ncols : (int64_t)a.nlat * a.nlon;
@@ -528,6 +534,51 @@ __global__ __launch_bounds__(256) void k_select_tiles(dk_select_args a) {
     const int c = tc.cnt[lane];
     a.nobsl[col] = c < a.max_obs ? c : a.max_obs;
   }
+}
+
+// Sort key of every tile of TJ x TI columns: the obs count of the bins its
+// wavefront scans (the union of its columns' candidate regions, formed as
+// the tile kernel forms it), largest first.
+template <int TJ, int TI>
+__global__ void k_tile_cost(dk_select_args a, int32_t ntiles, uint32_t* __restrict__ key,
+                            int32_t* __restrict__ id) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntiles) return;
+  const int ntj = (a.nlat + TJ - 1) / TJ;
+  const int tj = t % ntj, ti = t / ntj;
+  CandRegion reg;
+  reg.b0 = 1 << 30;
+  reg.b1 = -1;
+  reg.full = false;
+  reg.lo_deg = 1e300;
+  reg.hi_deg = -1e300;
+  double ref = 0.0;
+  bool first = true;
+  for (int ii = 0; ii < TI; ++ii)
+    for (int jj = 0; jj < TJ; ++jj) {
+      const int ilat = tj * TJ + jj, ilon = ti * TI + ii;
+      if (ilat >= a.nlat || ilon >= a.nlon) continue;
+      const double clat = (double)a.grid_lat[ilat];
+      double cw = wrap360((double)a.grid_lon[ilon]);
+      if (first) ref = cw;
+      first = false;
+      if (cw - ref > 180.0) cw -= 360.0;
+      if (cw - ref < -180.0) cw += 360.0;
+      const CandRegion rc = column_region(a, clat, cw);
+      reg.b0 = min(reg.b0, rc.b0);
+      reg.b1 = max(reg.b1, rc.b1);
+      reg.full = reg.full || rc.full;
+      reg.lo_deg = fmin(reg.lo_deg, rc.lo_deg);
+      reg.hi_deg = fmax(reg.hi_deg, rc.hi_deg);
+    }
+  int64_t n = 0;
+  if (!first)
+    for_region_ranges(a, reg, [&](int l0, int l1) -> bool {
+      n += l1 - l0;
+      return true;
+    });
+  key[t] = 0xFFFFFFFFu - (uint32_t)(n < 0xFFFFFFFE ? n : 0xFFFFFFFE);
+  id[t] = t;
 }
 
 // LDS writes of this wave visible to its later LDS reads.
@@ -721,6 +772,36 @@ static int select_tiles(const dk_select_args* a, hipStream_t s) {
       break;
   }
   return (int)hipGetLastError();
+}
+
+static void tile_shape(int32_t tile, int* tj, int* ti) {
+  *tj = 4;
+  *ti = tile == DK_SELECT_TILE_4X1 ? 1 : tile == DK_SELECT_TILE_4X4 ? 4 : 2;
+}
+
+extern "C" int64_t dk_select_ntiles(int32_t nlat, int32_t nlon, int32_t tile) {
+  int tj, ti;
+  tile_shape(tile, &tj, &ti);
+  return (int64_t)((nlat + tj - 1) / tj) * ((nlon + ti - 1) / ti);
+}
+
+extern "C" int dk_select_tile_order(const dk_select_args* a, int32_t ntiles, uint32_t* key,
+                                    int32_t* id, uint32_t* key_sorted, int32_t* order, void* tmp,
+                                    size_t* tmp_bytes, void* stream) {
+  if (tmp == nullptr)
+    return dk_sort_pairs_u32(nullptr, tmp_bytes, key, key_sorted, id, order, ntiles, 32, stream);
+  if (ntiles <= 0) return 0;
+  if (dk_select_ntiles(a->nlat, a->nlon, a->tile) != ntiles) return (int)hipErrorInvalidValue;
+  hipStream_t s = (hipStream_t)stream;
+  const unsigned blocks = (unsigned)((ntiles + 255) / 256);
+  switch (a->tile) {
+    case DK_SELECT_TILE_4X1: k_tile_cost<4, 1><<<blocks, 256, 0, s>>>(*a, ntiles, key, id); break;
+    case DK_SELECT_TILE_4X4: k_tile_cost<4, 4><<<blocks, 256, 0, s>>>(*a, ntiles, key, id); break;
+    default: k_tile_cost<4, 2><<<blocks, 256, 0, s>>>(*a, ntiles, key, id); break;
+  }
+  const int e = (int)hipGetLastError();
+  if (e) return e;
+  return dk_sort_pairs_u32(tmp, tmp_bytes, key, key_sorted, id, order, ntiles, 32, stream);
 }
 
 extern "C" int dk_select_count(const dk_select_args* a, void* stream) {
