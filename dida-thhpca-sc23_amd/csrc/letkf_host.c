@@ -165,7 +165,7 @@ struct dida_letkf_ctx {
     double *cx, *sx, *yr, *yb_b, *dep_b;
     float *u4;           /* FP32 unit vector of every binned obs (x, y, z, 0) */
     double *u3;          /* FP64 unit vector of every binned obs: x, y, z blocks of nobs_max */
-    float *yb32_b;       /* chained half-precision Grams: binned Yb in FP32 (no yb_b) */
+    float *yb32_b;       /* batched half-precision Grams: binned Yb in FP32 (no yb_b) */
     int32_t ldy;         /* row stride of yb_b: k (plain), the padded k (batched) */
     float *rinv_b;
     /* selection */
@@ -404,7 +404,9 @@ int dida_letkf_create(dida_letkf_ctx **out, const dida_letkf_params *prm, int32_
      * read binned Yb rows of 32 or 64 values, zero past k */
     c->ldy = k <= 64 && !c->chained ? (k <= 32 ? 32 : 64) : k;
     if (c->chained && prm->gram != DIDA_GRAM_FP64) c->ldy = (k + 3) & ~3; /* float4 rows */
-    if (c->chained && prm->gram != DIDA_GRAM_FP64)
+    /* the split half-precision Grams read FP32 rows (their hi + lo parts
+     * hold at most 22 bits) */
+    if (prm->form == DIDA_FORM_BATCHED && prm->gram != DIDA_GRAM_FP64)
         DMALLOC(c->yb32_b, no * (size_t)c->ldy * sizeof(float));
     else
         DMALLOC(c->yb_b, no * (size_t)c->ldy * sizeof(double));
@@ -813,6 +815,7 @@ static int sweep_batched(dida_letkf_ctx *c, const double *xb, const double *axb,
     ba.idx = c->csr_idx;
     ba.rloc = c->csr_w;
     ba.yb = c->yb_b;
+    ba.yb32 = c->yb32_b;
     ba.ldy = c->ldy;
     ba.rinv = c->rinv_b;
     ba.dep = c->dep_b;
@@ -871,7 +874,8 @@ static int sweep_batched(dida_letkf_ctx *c, const double *xb, const double *axb,
         }
     }
     if (c->prm.gram == DIDA_GRAM_FP16X3)
-        HIP_TRY(dk_obs_ymax(c->last_nobs_pending, ba.k, c->ldy, c->yb_b, c->rinv_b, c->ymax, s));
+        HIP_TRY(dk_obs_ymax32(c->last_nobs_pending, ba.k, c->ldy, c->yb32_b, c->rinv_b, c->ymax,
+                              s));
     ba.ngroup = c->ngroup;
     if (c->bucketed) {
         /* chains of one obs-count bucket, built from this sweep's counts */

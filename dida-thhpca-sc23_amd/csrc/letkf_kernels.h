@@ -236,7 +236,9 @@ typedef struct dk_batched_args {
     const int32_t *nobsl;
     const int32_t *idx;     /* CSR binned obs index */
     const double *rloc;
-    const double *yb;       /* binned obs rows of ldy values, zero past k */
+    const double *yb;       /* binned obs rows of ldy values, zero past k (FP64 Gram) */
+    const float *yb32;      /* the rows rounded to FP32 (split half-precision Grams,
+                               whose hi + lo parts hold 16 to 22 bits) */
     const float *rinv;
     const double *dep;
     const double *ymax;     /* per binned obs: max |yb| sqrt(R^-1) (FP16 scaling) */

@@ -26,6 +26,7 @@
 //   update    Xa = trans^T Xb' + xbar_b, mean, re-centre, RTPS (xb_2_xa,
 //             letkf_compute.F90:261-299) on FP64 MFMA.
 #include <cstring>
+#include <type_traits>
 
 #include "letkf_device.h"
 #include "letkf_kernels.h"
@@ -163,6 +164,8 @@ struct Wave {
   static constexpr int NP = NJ / 2;    // column pairs per lane group
   double X[NJ][NQ];  // working matrix (see header)
   bool patB;         // current row pattern
+  // Yb rows as read: FP64 for the FP64 Gram, FP32 for the half-precision ones
+  typedef typename std::conditional<GRAM == DK_GRAM_FP64, double, float>::type RawT;
 
   // obs e (< 8) of half h of a 64-obs block in the Gram MFMA operand layout:
   // FP64 16x16x4 k-steps of 4 obs (obs 4e + g), half 16x16x32 (obs 8g + e)
@@ -175,11 +178,14 @@ struct Wave {
   // offsets are constants of one address per obs)
   template <class SM>
   __device__ __forceinline__ void load_rows(const dk_batched_args& a, const SM& sm,
-                                            int buf, int h, double (&raw)[8][NJ]) const {
+                                            int buf, int h, RawT (&raw)[8][NJ]) const {
     const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
+    const RawT* yb;
+    if constexpr (GRAM == DK_GRAM_FP64) yb = a.yb;
+    else yb = a.yb32;
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
-      const double* y = a.yb + (int64_t)sm.ogi[buf][half_obs(h, e, g)] * KP + c;
+      const RawT* y = yb + (int64_t)sm.ogi[buf][half_obs(h, e, g)] * KP + c;
 #pragma unroll
       for (int I = 0; I < NJ; ++I) raw[e][I] = y[I * 16];
     }
@@ -188,7 +194,7 @@ struct Wave {
   // Accumulate half h of the staged block: C on MFMA, b in FP64
   template <typename AccD, typename AccH, class SM>
   __device__ __forceinline__ void gram_half(const SM& sm, int buf, int h,
-                                            const double (&raw)[8][NJ], double scale,
+                                            const RawT (&raw)[8][NJ], double scale,
                                             double (&bacc)[NJ], AccD& accd, AccH& acch) const {
     const int g = threadIdx.x >> 4;
     if constexpr (GRAM == DK_GRAM_FP64) {
@@ -219,8 +225,8 @@ struct Wave {
         const float rs = (float)(rr * scale);
 #pragma unroll
         for (int I = 0; I < NJ; ++I) {
-          bacc[I] = fma(raw[e][I], w, bacc[I]);
-          const float xs = (float)raw[e][I] * rs;
+          bacc[I] = fma((double)raw[e][I], w, bacc[I]);
+          const float xs = raw[e][I] * rs;
           const helt hv = (helt)xs;
           hi[I][e] = hv;
           if (kSplit) lo[I][e] = (helt)(xs - (float)hv);
@@ -294,7 +300,7 @@ struct Wave {
       m.store_index(sm, 0);
       m.store_weights<VERT>(sm, 0, a, clnp);
       __syncthreads();
-      double rawA[8][NJ], rawB[8][NJ];
+      RawT rawA[8][NJ], rawB[8][NJ];
       load_rows(a, sm, 0, 0, rawA);
       for (int base = 0; base < p; base += 64) {
         const int buf = (base >> 6) & 1;
@@ -1240,6 +1246,7 @@ extern "C" int dk_letkf_wave(const dk_batched_args* a, int nblocks, void* stream
   if (a->nchain <= 0) return 0;
   if (a->k < 2 || a->k > 64) return (int)hipErrorInvalidValue;
   if (a->ldy != (a->k <= 32 ? 32 : 64)) return (int)hipErrorInvalidValue;
+  if (a->gram == DK_GRAM_FP64 ? !a->yb : !a->yb32) return (int)hipErrorInvalidValue;
   hipStream_t s = (hipStream_t)stream;
   if (a->k <= 32) return wdispatch<32>(a, nblocks, s);
   return wdispatch<64>(a, nblocks, s);
