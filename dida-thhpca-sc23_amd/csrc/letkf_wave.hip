@@ -290,7 +290,46 @@ struct Wave {
 #pragma unroll
         for (int J = 0; J < NJ; ++J) acch[I][J] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
     }
-    if (p > 0) {
+    if (p > 0 && !kF64) {
+      // half-precision Grams (FP32 rows, half the registers of FP64 ones):
+      // the rows of block b + 1 are issued while block b computes, and the
+      // indices and weights of block b + 2 are staged in block b's LDS
+      // buffer once it has been read (loads past p read obs 0, in bounds)
+      ObsMeta m0, m1;
+      m0.load_index(a, off, p, 0);
+      m1.load_index(a, off, p, 64);
+      m0.load_obs<VERT>(a);
+      m1.load_obs<VERT>(a);
+      m0.store_index(sm, 0);
+      m1.store_index(sm, 1);
+      m0.store_weights<VERT>(sm, 0, a, clnp);
+      m1.store_weights<VERT>(sm, 1, a, clnp);
+      __syncthreads();
+      RawT A0[8][NJ], A1[8][NJ], B0[8][NJ], B1[8][NJ];
+      load_rows(a, sm, 0, 0, A0);
+      load_rows(a, sm, 0, 1, A1);
+      auto step = [&](int base, RawT (&c0)[8][NJ], RawT (&c1)[8][NJ], RawT (&n0)[8][NJ],
+                      RawT (&n1)[8][NJ]) {
+        const int buf = (base >> 6) & 1;
+        ObsMeta n;
+        n.load_index(a, off, p, base + 128);
+        load_rows(a, sm, buf ^ 1, 0, n0);
+        load_rows(a, sm, buf ^ 1, 1, n1);
+        __builtin_amdgcn_sched_barrier(0);
+        gram_half(sm, buf, 0, c0, scale, bacc, accd, acch);
+        n.load_obs<VERT>(a);
+        if (base + 32 < p) gram_half(sm, buf, 1, c1, scale, bacc, accd, acch);
+        __syncthreads();  // block b's buffer has been read
+        n.store_index(sm, buf);
+        n.store_weights<VERT>(sm, buf, a, clnp);
+        __syncthreads();
+      };
+      for (int base = 0; base < p; base += 128) {
+        step(base, A0, A1, B0, B1);
+        if (base + 64 >= p) break;
+        step(base + 64, B0, B1, A0, A1);
+      }
+    } else if (p > 0) {
       // software pipeline over 32-obs halves: the rows of a half are issued
       // while the previous half computes; the next block's indices are
       // staged in the other LDS buffer during the current block
