@@ -82,7 +82,7 @@ static int cube_step(const dida_letkf_params *p)
 static double jacobi_tol_of(const dida_letkf_params *p)
 {
     if (p->jacobi_tol > 0.0) return p->jacobi_tol;
-    return p->gram == DIDA_GRAM_FP64 ? 1e-13 : 1e-9;
+    return p->gram == DIDA_GRAM_FP64 ? 1e-13 : 1e-8;
 }
 
 static int predict_of(const dida_letkf_params *p)

@@ -69,7 +69,7 @@ typedef struct dida_letkf_params {
     int32_t gram;               /* DIDA_GRAM_* */
     int32_t jacobi_max_sweeps;  /* eigensolver sweep cap */
     double jacobi_tol;          /* rotate while |a_p.a_q| > tol |a_p||a_q|
-                                   (0: 1e-13 with an FP64 Gram, 1e-9 otherwise) */
+                                   (0: 1e-13 with an FP64 Gram, 1e-8 otherwise) */
     double bin_deg;             /* obs bin size in degrees (0: automatic) */
     int32_t chain_len;          /* batched form: adjacent columns per work item,
                                    each warm-starting the next one's eigensolve
