@@ -163,14 +163,17 @@ def kernel_roofline(cfg, st, head, kernel_ms, config, scale, layout=None):
     flops = half + fp64
     peak = flops / t_peak / 1e12
     achieved = flops / (kernel_ms / 1e3) / 1e12 if kernel_ms > 0 else None
-    # bytes the kernel must move: obs rows gathered per analysis p (8k + 24)
-    # (mostly served from L2, neighbouring columns share obs) and, fused, the
-    # update's 2kn f64 + 2n f64 per column
-    gathered = p_sum * (8.0 * k + 24.0)
+    # bytes the kernel must move: obs rows gathered per analysis p (8k + 24,
+    # 4k + 24 with the FP32 rows of the split half-precision Grams; mostly
+    # served from L2, neighbouring columns share obs) and, fused, the update's
+    # 2kn f64 + 2n f64 per column
+    row_bytes = 8.0 if head.endswith("fp64") or not fused else 4.0
+    gathered = p_sum * (row_bytes * k + 24.0)
     upd = ncol * (2.0 * k * n * 8 + 2.0 * n * 8) if fused else 0.0
     traffic = measured_traffic(head, config, scale, layout)
     return {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
             "frac": achieved / peak if achieved else None, "traffic": traffic,
+            "traffic_key": traffic_key(head, config, scale, layout) if layout else None,
             "kernel": "k_letkf_wave (fused Gram, eigensolve, transform, update)" if fused
             else ("k_letkf_chain_f64 (chained Gram, warm-started eigensolve, transform)"
                   if head != "plain_fp64" else "k_letkf_core_f64 (Gram, eigensolve, transform)"),
@@ -191,9 +194,18 @@ def measured_traffic(head, config, scale, layout=None):
         return None
     with open(path) as f:
         table = json.load(f)
-    e = table.get(f"{head}/config{config}/scale{scale:g}/chain{int(layout['chain_len'])}"
-                  f"/tiles{int(layout.get('xcd_tiles', 0))}")
+    e = table.get(traffic_key(head, config, scale, layout))
     return e["hbm_bytes_per_launch"] if e else None
+
+
+def traffic_key(head, config, scale, layout):
+    """Entry of a measured layout: form, workload, chain length, XCD tiles
+    and, for per-bucket chains, the chain length of every bucket."""
+    key = (f"{head}/config{config}/scale{scale:g}/chain{int(layout['chain_len'])}"
+           f"/tiles{int(layout.get('xcd_tiles', 0))}")
+    if layout.get("bucketed"):
+        key += "/buckets" + "-".join(str(int(b)) for b in layout["bucket_chain"])
+    return key
 
 
 def time_obs_operator(dl, torch, data, cfg, reps=5):
@@ -594,6 +606,10 @@ def main():
     ap.add_argument("--state-f32", action="store_true",
                     help="Xb' and Xa' stored in FP32 (dida_letkf_sweep_f32; plain form, or "
                          "batched with k > 64); skips the FP64-state extras")
+    ap.add_argument("--layout", default="",
+                    help="pin the batched forms' layout instead of autotuning: JSON with "
+                         "chain_len, xcd_tiles, bin_deg and (per-bucket chains) "
+                         "bucket_chain, as the autotuner reports it")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device copy (counter scale for profiles)")
     args = ap.parse_args()
@@ -631,6 +647,9 @@ def main():
         sbytes = 4
         args.no_vertical = args.no_cubes = args.no_struct = args.no_domains = True
 
+    pinned = json.loads(args.layout) if args.layout else None
+    if pinned is not None:
+        pinned["bucketed"] = int(any(int(b) > 0 for b in pinned.get("bucket_chain", [])))
     forms = {"plain_fp64": (dl.FORM_PLAIN, dl.GRAM_FP64, {}),
              "batched_fp64": (dl.FORM_BATCHED, dl.GRAM_FP64, {}),
              "batched_fp16x3": (dl.FORM_BATCHED, dl.GRAM_FP16X3, {}),
@@ -644,6 +663,12 @@ def main():
             extra["jacobi_tol"] = args.jacobi_tol
         if args.chain_len > 0 and form == dl.FORM_BATCHED:
             extra["chain_len"] = args.chain_len
+        if pinned and form == dl.FORM_BATCHED:
+            extra.update(chain_len=int(pinned["chain_len"]),
+                         xcd_tiles=int(pinned.get("xcd_tiles", 0)),
+                         bin_deg=float(pinned.get("bin_deg", 0.0)))
+            if pinned.get("bucketed"):
+                extra["bucket_chain"] = [int(b) for b in pinned["bucket_chain"]]
     cpu, cpu_cols, cpu_out = None, None, None
     if not args.no_cpu:
         # first, so that every GPU form's output is compared with it on the
@@ -657,7 +682,8 @@ def main():
         log(f"timing {name}")
         r = time_form(dl, torch, data, cfg, dict(prm, **extra), form, gram, args.steps,
                       args.warmup,
-                      tune=form == dl.FORM_BATCHED and not args.no_tune and not args.state_f32)
+                      tune=form == dl.FORM_BATCHED and not args.no_tune and not args.state_f32
+                      and pinned is None)
         if r["tuning"]:
             log(f"  autotuned layout {r['tuning']}")
         log(f"  {r['ms']:.1f} ms/step, stages {r['stages']}")
@@ -692,7 +718,7 @@ def main():
     ms = r["ms"]
     value = ncol / (ms / 1e3)
 
-    layout = r["tuning"]
+    layout = r["tuning"] or pinned
     if layout is None and head != "plain_fp64":
         layout = {"chain_len": args.chain_len or (32 if cfg.k > 64 else 8),
                   "xcd_tiles": 0}   # library defaults
