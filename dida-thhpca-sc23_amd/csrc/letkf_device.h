@@ -135,12 +135,16 @@ __device__ __forceinline__ void rotation_angle(double gs, double al, double be, 
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef _Float16 f16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef _Float16 f16x2_t __attribute__((ext_vector_type(2)));
 
 template <int GRAM>
 struct H16;
 template <>
 struct H16<DK_GRAM_BF16X3> {
   typedef bf16x8_t vec;
+  typedef bf16x2_t vec2;  // two elements converted by one instruction
   typedef __bf16 elt;
   static __device__ __forceinline__ f32x4_t mfma(vec a, vec b, f32x4_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
@@ -149,6 +153,7 @@ struct H16<DK_GRAM_BF16X3> {
 template <>
 struct H16<DK_GRAM_FP16X3> {
   typedef f16x8_t vec;
+  typedef f16x2_t vec2;
   typedef _Float16 elt;
   static __device__ __forceinline__ f32x4_t mfma(vec a, vec b, f32x4_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);

@@ -215,23 +215,38 @@ struct Wave {
       }
     } else {
       typedef typename H16<GRAM>::vec hvec;
-      typedef typename H16<GRAM>::elt helt;
+      typedef typename H16<GRAM>::vec2 hvec2;
       constexpr bool kSplit = GRAM != DK_GRAM_BF16;
       hvec hi[NJ], lo[NJ];
+      // b: the half's 8 products a row summed in FP32, then added in FP64
+      // (relative error ~1e-7, against the split Gram's ~1e-6)
+      float bh[NJ];
 #pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const int ol = half_obs(h, e, g);
-        const double rr = sm.orr[buf][ol], w = sm.ow[buf][ol];
-        const float rs = (float)(rr * scale);
+      for (int I = 0; I < NJ; ++I) bh[I] = 0.0f;
+      // two obs at a time, so one instruction converts both elements of a pair
+#pragma unroll
+      for (int e = 0; e < 8; e += 2) {
+        const int o0 = half_obs(h, e, g), o1 = half_obs(h, e + 1, g);
+        const f32x2_t rs = {(float)(sm.orr[buf][o0] * scale), (float)(sm.orr[buf][o1] * scale)};
+        const float w0 = (float)sm.ow[buf][o0], w1 = (float)sm.ow[buf][o1];
 #pragma unroll
         for (int I = 0; I < NJ; ++I) {
-          bacc[I] = fma((double)raw[e][I], w, bacc[I]);
-          const float xs = raw[e][I] * rs;
-          const helt hv = (helt)xs;
-          hi[I][e] = hv;
-          if (kSplit) lo[I][e] = (helt)(xs - (float)hv);
+          bh[I] = fmaf(raw[e][I], w0, bh[I]);
+          bh[I] = fmaf(raw[e + 1][I], w1, bh[I]);
+          const f32x2_t xs = f32x2_t{raw[e][I], raw[e + 1][I]} * rs;
+          const hvec2 hv = __builtin_convertvector(xs, hvec2);
+          hi[I][e] = hv[0];
+          hi[I][e + 1] = hv[1];
+          if (kSplit) {
+            const hvec2 lv = __builtin_convertvector(xs - __builtin_convertvector(hv, f32x2_t),
+                                                     hvec2);
+            lo[I][e] = lv[0];
+            lo[I][e + 1] = lv[1];
+          }
         }
       }
+#pragma unroll
+      for (int I = 0; I < NJ; ++I) bacc[I] += (double)bh[I];
 #pragma unroll
       for (int I = 0; I < NJ; ++I)
 #pragma unroll
