@@ -307,14 +307,17 @@ struct Wave {
     }
     if (p > 0 && !kF64) {
       // half-precision Grams (FP32 rows, half the registers of FP64 ones):
-      // the rows of block b + 1 are issued while block b computes, and the
-      // indices and weights of block b + 2 are staged in block b's LDS
-      // buffer once it has been read (loads past p read obs 0, in bounds)
-      ObsMeta m0, m1;
+      // the rows of block b + 1 are issued while block b computes; the
+      // metadata of block b + 3 is loaded during block b and staged in the
+      // LDS buffer of its parity at the end of block b + 1, so its gathers
+      // have a whole block to arrive (loads past p read obs 0, in bounds)
+      ObsMeta m0, m1, MA, MB;
       m0.load_index(a, off, p, 0);
       m1.load_index(a, off, p, 64);
+      MA.load_index(a, off, p, 128);
       m0.load_obs<VERT>(a);
       m1.load_obs<VERT>(a);
+      MA.load_obs<VERT>(a);
       m0.store_index(sm, 0);
       m1.store_index(sm, 1);
       m0.store_weights<VERT>(sm, 0, a, clnp);
@@ -323,26 +326,27 @@ struct Wave {
       RawT A0[8][NJ], A1[8][NJ], B0[8][NJ], B1[8][NJ];
       load_rows(a, sm, 0, 0, A0);
       load_rows(a, sm, 0, 1, A1);
+      // block b (at base): its rows in c0/c1, block b + 1 staged in LDS,
+      // block b + 2's metadata in `cur` (loaded), block b + 3's into `nxt`
       auto step = [&](int base, RawT (&c0)[8][NJ], RawT (&c1)[8][NJ], RawT (&n0)[8][NJ],
-                      RawT (&n1)[8][NJ]) {
+                      RawT (&n1)[8][NJ], ObsMeta& cur, ObsMeta& nxt) {
         const int buf = (base >> 6) & 1;
-        ObsMeta n;
-        n.load_index(a, off, p, base + 128);
+        nxt.load_index(a, off, p, base + 192);
         load_rows(a, sm, buf ^ 1, 0, n0);
         load_rows(a, sm, buf ^ 1, 1, n1);
         __builtin_amdgcn_sched_barrier(0);
         gram_half(sm, buf, 0, c0, scale, bacc, accd, acch);
-        n.load_obs<VERT>(a);
+        nxt.load_obs<VERT>(a);
         if (base + 32 < p) gram_half(sm, buf, 1, c1, scale, bacc, accd, acch);
         __syncthreads();  // block b's buffer has been read
-        n.store_index(sm, buf);
-        n.store_weights<VERT>(sm, buf, a, clnp);
+        cur.store_index(sm, buf);
+        cur.store_weights<VERT>(sm, buf, a, clnp);
         __syncthreads();
       };
       for (int base = 0; base < p; base += 128) {
-        step(base, A0, A1, B0, B1);
+        step(base, A0, A1, B0, B1, MA, MB);
         if (base + 64 >= p) break;
-        step(base + 64, B0, B1, A0, A1);
+        step(base + 64, B0, B1, A0, A1, MB, MA);
       }
     } else if (p > 0) {
       // software pipeline over 32-obs halves: the rows of a half are issued
