@@ -49,6 +49,7 @@ struct WSmem {
   int32_t ogi[2][64];  // staged obs blocks (two buffers): binned index,
   double orr[2][64];   // sqrt(R^-1 rloc),
   double ow[2][64];    // R^-1 rloc dep
+  float ct[FULLV ? 16 * 17 : 1];  // a 16 x 16 Gram tile being transposed
 };
 
 __device__ __forceinline__ int rowA(int I, int r, int g) { return I * 16 + 4 * g + r; }
@@ -247,10 +248,11 @@ struct Wave {
       }
 #pragma unroll
       for (int I = 0; I < NJ; ++I) bacc[I] += (double)bh[I];
+      // C is symmetric: the tiles I >= J only
 #pragma unroll
       for (int I = 0; I < NJ; ++I)
 #pragma unroll
-        for (int J = 0; J < NJ; ++J) {
+        for (int J = 0; J <= I; ++J) {
           acch[I][J] = H16<GRAM>::mfma(hi[I], hi[J], acch[I][J]);
           if (kSplit) {
             acch[I][J] = H16<GRAM>::mfma(hi[I], lo[J], acch[I][J]);
@@ -390,16 +392,32 @@ struct Wave {
           }
       patB = true;
     } else {
-      // D[4g + r][c] of tile (I, J) = C(I*16 + 4g + r, J*16 + c)
+      // D[4g + r][c] of tile (I, J) = C(I*16 + 4g + r, J*16 + c), formed for
+      // I >= J; tile (I, J) of I < J is the transpose of tile (J, I), passed
+      // through LDS (written as element (4g + r, c), read as (c, 4g + r))
 #pragma unroll
       for (int I = 0; I < NJ; ++I)
 #pragma unroll
-        for (int J = 0; J < NJ; ++J)
+        for (int J = 0; J <= I; ++J)
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             const bool in = I * 16 + 4 * g + r < k && J * 16 + c < k;
             X[J][I * 4 + r] = in ? (double)acch[I][J][r] * inv2 : 0.0;
           }
+#pragma unroll
+      for (int J = 1; J < NJ; ++J)
+#pragma unroll
+        for (int I = 0; I < J; ++I) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) sm.ct[(4 * g + r) * 17 + c] = acch[J][I][r];
+          __syncthreads();
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const bool in = I * 16 + 4 * g + r < k && J * 16 + c < k;
+            X[J][I * 4 + r] = in ? (double)sm.ct[c * 17 + 4 * g + r] * inv2 : 0.0;
+          }
+          __syncthreads();  // read before the next tile overwrites it
+        }
       patB = false;
     }
     // b: lanes hold partials of rows I*16 + c over their obs; sum over g
