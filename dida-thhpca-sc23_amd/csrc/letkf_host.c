@@ -165,6 +165,7 @@ struct dida_letkf_ctx {
     double *cx, *sx, *yr, *yb_b, *dep_b;
     float *u4;           /* FP32 unit vector of every binned obs (x, y, z, 0) */
     double *u3;          /* FP64 unit vector of every binned obs: x, y, z blocks of nobs_max */
+    int32_t *iperm;      /* binned position of every caller obs index */
     float *yb32_b;       /* batched half-precision Grams: binned Yb in FP32 (no yb_b) */
     int32_t ldy;         /* row stride of yb_b: k (plain), the padded k (batched) */
     float *rinv_b;
@@ -253,7 +254,7 @@ int dida_letkf_destroy(dida_letkf_ctx *c)
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
     void *ptrs[] = {c->key, c->key_sorted, c->iota, c->perm, c->bin_count, c->bin_start,
-                    c->sort_tmp, c->cx, c->sx, c->yr, c->u4, c->u3, c->yb_b,
+                    c->sort_tmp, c->cx, c->sx, c->yr, c->u4, c->u3, c->iperm, c->yb_b,
                     c->yb32_b, c->dep_b, c->rinv_b,
                     c->nobsl, c->thr, c->offset, c->scan_tmp, c->csr_idx, c->csr_w, c->flags,
                     c->trans, c->info, c->chain_key, c->chain_key_sorted, c->chain_id,
@@ -399,6 +400,7 @@ int dida_letkf_create(dida_letkf_ctx **out, const dida_letkf_params *prm, int32_
     DMALLOC(c->yr, no * sizeof(double));
     DMALLOC(c->u4, no * 4 * sizeof(float));
     DMALLOC(c->u3, no * 3 * sizeof(double));
+    DMALLOC(c->iperm, no * sizeof(int32_t));
     c->chained = prm->form == DIDA_FORM_BATCHED && k > 64;
     /* the one-wavefront kernels (batched form, and the plain form with k <= 64)
      * read binned Yb rows of 32 or 64 values, zero past k */
@@ -642,7 +644,8 @@ static int sweep_select_count(dida_letkf_ctx *c, const dida_letkf_obs *o, hipStr
     HIP_TRY(dk_bin_scan(c->geom.nbins, c->bin_count, c->bin_start, s));
     HIP_TRY(dk_obs_gather(o->nobs, k, c->ldy, c->perm, o->lat, o->lon, o->yb, o->dep, o->rinv,
                           c->cx, c->sx, c->yr, c->u4, c->u3, c->u3 + c->nobs_max,
-                          c->u3 + 2 * c->nobs_max, c->yb_b, c->yb32_b, c->dep_b, c->rinv_b, s));
+                          c->u3 + 2 * c->nobs_max, c->iperm, c->yb_b, c->yb32_b, c->dep_b,
+                          c->rinv_b, s));
     if (c->ncube > 0) HIP_TRY(dk_obs_lnp(o->nobs, c->perm, o->ph, c->obs_lnp, s));
     if (c->chained && c->prm.gram == DIDA_GRAM_FP16X3)
         HIP_TRY(dk_obs_ymax32(o->nobs, k, c->ldy, c->yb32_b, c->rinv_b, c->ymax, s));
@@ -681,6 +684,8 @@ static int sweep_select_count(dida_letkf_ctx *c, const dida_letkf_obs *o, hipStr
     sa->offset = c->offset;
     sa->overflow = c->flags;
     sa->perm = c->perm;
+    sa->iperm = c->iperm;
+    sa->nobs = o->nobs;
     sa->thr = c->thr;
     sa->tile = c->sel_tile;
     sa->perm_bits = 1;

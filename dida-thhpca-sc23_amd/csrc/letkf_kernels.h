@@ -66,8 +66,10 @@ typedef struct dk_select_args {
      * input) = one past the largest caller index a column capped at max_obs
      * keeps, INT32_MAX for a column that keeps every obs with gc > 0 */
     const int32_t *perm;
+    const int32_t *iperm; /* [nobs]: binned position of caller index i (cap pass) */
     int32_t *thr;
     int32_t perm_bits;
+    int32_t nobs;
     int32_t tile;        /* count and fill: DK_SELECT_TILE_* columns per wavefront */
     /* count and fill over all columns: wavefront w of the launch takes tile
      * tile_order[w] (largest candidate count first); NULL: grid order */
@@ -97,8 +99,8 @@ int dk_bin_scan(int32_t nbins, const int32_t *count, int32_t *bin_start, void *s
 int dk_obs_gather(int32_t nobs, int32_t k, int32_t ldy, const int32_t *perm, const double *lat,
                   const double *lon, const double *yb, const double *dep, const float *rinv,
                   double *cx, double *sx, double *yr, float *u4, double *ux, double *uy,
-                  double *uz, double *yb_b, float *yb32_b, double *dep_b, float *rinv_b,
-                  void *stream);
+                  double *uz, int32_t *iperm, double *yb_b, float *yb32_b, double *dep_b,
+                  float *rinv_b, void *stream);
 /* Selection passes: count (capped at max_obs) and fill (entries with gc > 0
  * and perm < thr, in bin order) by tiles of columns (DK_SELECT_TILE_*), one
  * wavefront a tile reading each candidate once for its columns; cap (thr of every

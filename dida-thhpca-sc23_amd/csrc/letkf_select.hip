@@ -92,11 +92,12 @@ __global__ void k_obs_gather(int32_t nobs, const int32_t* __restrict__ perm,
                              double* __restrict__ cx, double* __restrict__ sx,
                              double* __restrict__ yr, float4* __restrict__ u4,
                              double* __restrict__ ux, double* __restrict__ uy,
-                             double* __restrict__ uz, double* __restrict__ dep_b,
-                             float* __restrict__ rinv_b) {
+                             double* __restrict__ uz, int32_t* __restrict__ iperm,
+                             double* __restrict__ dep_b, float* __restrict__ rinv_b) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nobs) return;
   int s = perm[i];
+  iperm[s] = i;
   double x1r = deg2rad_ref(lat[s]);
   const double c1 = cos(x1r), s1 = sin(x1r), y1 = deg2rad_ref(lon[s]);
   cx[i] = c1;
@@ -590,12 +591,18 @@ __device__ __forceinline__ void wave_lds_sync() {
 
 constexpr int kCapDigit = 11;               // bits of caller index per radix level
 constexpr int kCapBins = 1 << kCapDigit;
+constexpr int kCapEnumBits = 12;            // digit ranges of up to 2^12 caller indices
+                                            // are enumerated instead of rescanned
 
 // thr[col] of every column: INT32_MAX unless the count pass stopped at
 // max_obs; then, if more than max_obs obs are kept, one past the caller index
 // of the max_obs-th smallest kept one.  Level by level, the kept obs whose
 // caller index lies in the open range [lo, lo + 2^width) are counted per
-// 11-bit digit, and the digit holding rank `rank` narrows the range.
+// 11-bit digit, and the digit holding rank `rank` narrows the range; once
+// the range holds at most 2^12 caller indices they are tested one by one in
+// caller order (the inverse permutation gives their binned positions; obs in
+// the polar bin are never candidates), which replaces a rescan of the
+// column's candidates.
 __global__ __launch_bounds__(256) void k_select_cap(dk_select_args a) {
   __shared__ uint32_t hist_all[4][kCapBins];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -679,6 +686,32 @@ __global__ __launch_bounds__(256) void k_select_cap(dk_select_args a) {
       thr = (int32_t)(lo + 1u);
       break;
     }
+    if (a.iperm && width <= kCapEnumBits) {
+      const uint32_t n = 1u << width;
+      const int32_t polar0 = a.obs.bin_start[a.geom.nbins];
+      int cum = 0;  // kept obs of the range before this chunk
+      for (uint32_t base = 0; base < n; base += kWave) {
+        const uint32_t ci = lo + base + (uint32_t)lane;
+        bool keep = false;
+        if (base + (uint32_t)lane < n && ci < (uint32_t)a.nobs) {
+          const int l = a.iperm[ci];
+          keep = l < polar0 &&
+                 keep_candidate(a, q, l, reinterpret_cast<const float4*>(a.obs.u4)[l]);
+        }
+        const uint64_t km = __ballot(keep);
+        const int cnt = __popcll(km);
+        if (cum + cnt >= rank) {
+          // the lane holding the (rank - cum)-th kept obs of the chunk
+          const int below = (int)__builtin_amdgcn_mbcnt_hi(
+              (uint32_t)(km >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)km, 0u));
+          const uint64_t hit = __ballot(keep && below == rank - cum - 1);
+          thr = (int32_t)(lo + base + (uint32_t)(__ffsll((unsigned long long)hit) - 1) + 1u);
+          break;
+        }
+        cum += cnt;
+      }
+      break;
+    }
     wave_lds_sync();  // the bins are read before the next level clears them
   }
   if (lane == 0) a.thr[col] = thr;
@@ -717,12 +750,12 @@ extern "C" int dk_obs_gather(int32_t nobs, int32_t k, int32_t ldy, const int32_t
                              const double* lat,
                              const double* lon, const double* yb, const double* dep,
                              const float* rinv, double* cx, double* sx, double* yr, float* u4,
-                             double* ux, double* uy, double* uz, double* yb_b, float* yb32_b,
-                             double* dep_b, float* rinv_b, void* stream) {
+                             double* ux, double* uy, double* uz, int32_t* iperm, double* yb_b,
+                             float* yb32_b, double* dep_b, float* rinv_b, void* stream) {
   if (nobs <= 0) return 0;
   hipStream_t s = (hipStream_t)stream;
   k_obs_gather<<<(nobs + 255) / 256, 256, 0, s>>>(nobs, perm, lat, lon, dep, rinv, cx, sx, yr,
-                                                  (float4*)u4, ux, uy, uz, dep_b, rinv_b);
+                                                  (float4*)u4, ux, uy, uz, iperm, dep_b, rinv_b);
   const int64_t total = (int64_t)nobs * ldy;
   const unsigned blocks = (unsigned)((total + 255) / 256);
   if (yb_b) k_obs_gather_yb<double><<<blocks, 256, 0, s>>>(nobs, k, ldy, perm, yb, yb_b);
