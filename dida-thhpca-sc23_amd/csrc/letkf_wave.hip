@@ -49,7 +49,7 @@ struct WSmem {
   int32_t ogi[2][64];  // staged obs blocks (two buffers): binned index,
   double orr[2][64];   // sqrt(R^-1 rloc),
   double ow[2][64];    // R^-1 rloc dep
-  float ct[FULLV ? 16 * 17 : 1];  // a 16 x 16 Gram tile being transposed
+  double ct[16 * 17];  // a 16 x 16 tile being transposed (Gram, W), or wbar
 };
 
 __device__ __forceinline__ int rowA(int I, int r, int g) { return I * 16 + 4 * g + r; }
@@ -882,15 +882,17 @@ __device__ bool wave_column(const dk_batched_args& a, int64_t col, int cube, boo
   bad = __any(bad);
   __syncthreads();
 
-  // trans(i, e) = W(i, e) + wbar(i): tiles D[4r + g][c] = (I*16 + 4r + g, I2*16 + c),
-  // summed over the eigenvectors v_j = a_j / lambda_j (through LDS: rows to
-  // the lanes of the MFMA operands), f_j = sqrt((k-1)/lambda_j), u_j = v_j^T b
-  // / lambda_j; FULLV: all KP at once (V stays for the next warm start),
-  // otherwise 16 at a time
-  double4_t T[NJ][NJ], wb[NJ];
+  // trans(i, e) = W(i, e) + wbar(i): tiles D[4r + g][c] = (I*16 + 4r + g, I2*16 + c).
+  // W = sum_j f_j v_j v_j^T on FP64 MFMA (the tiles on and below the
+  // diagonal), wbar = sum_j u_j v_j on the VALU, over the eigenvectors
+  // v_j = a_j / lambda_j (through LDS: rows to the lanes of the MFMA
+  // operands), f_j = sqrt((k-1)/lambda_j), u_j = v_j^T b / lambda_j; FULLV:
+  // all KP at once (V stays for the next warm start), otherwise 16 at a time
+  double4_t T[NJ][NJ];
+  double wp[NJ];  // partials of wbar(I*16 + c) over this lane's eigenvectors
 #pragma unroll
   for (int I = 0; I < NJ; ++I) {
-    wb[I] = double4_t{0.0, 0.0, 0.0, 0.0};
+    wp[I] = 0.0;
 #pragma unroll
     for (int I2 = 0; I2 < NJ; ++I2) T[I][I2] = double4_t{0.0, 0.0, 0.0, 0.0};
   }
@@ -923,9 +925,10 @@ __device__ bool wave_column(const dk_batched_args& a, int64_t col, int cube, boo
       for (int I = 0; I < NJ; ++I) v[I] = sm.V[I * 16 + c + (jj - j0) * LDV];
 #pragma unroll
       for (int I = 0; I < NJ; ++I) {
-        wb[I] = mfma_f64_16x16x4(v[I], uj, wb[I]);
+        wp[I] = fma(v[I], uj, wp[I]);
+        // W is symmetric: the tiles I2 <= I only
 #pragma unroll
-        for (int I2 = 0; I2 < NJ; ++I2)
+        for (int I2 = 0; I2 <= I; ++I2)
           T[I][I2] = mfma_f64_16x16x4(v[I] * fj, v[I2], T[I][I2]);
       }
     }
@@ -944,6 +947,34 @@ __device__ bool wave_column(const dk_batched_args& a, int64_t col, int cube, boo
       __syncthreads();  // the slot is read before the next one overwrites it
     }
   }
+  // wbar(i) = sum_j v_j(i) u_j: the partials summed over the lane groups,
+  // then through LDS to the rows of the tile layout (wb[I][r] = wbar(I*16 +
+  // 4r + g))
+  double4_t wb[NJ];
+#pragma unroll
+  for (int I = 0; I < NJ; ++I) wp[I] = sum_g(wp[I]);
+  if (g == 0)
+#pragma unroll
+    for (int I = 0; I < NJ; ++I) sm.ct[I * 16 + c] = wp[I];
+  __syncthreads();
+#pragma unroll
+  for (int I = 0; I < NJ; ++I)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) wb[I][r] = sm.ct[I * 16 + 4 * r + g];
+  __syncthreads();
+  // the tiles above the diagonal: transposes of the ones below, through LDS
+  // (tile (I2, I) written as element (4r + g, c), read as (c, 4r + g))
+#pragma unroll
+  for (int I2 = 1; I2 < NJ; ++I2)
+#pragma unroll
+    for (int I = 0; I < I2; ++I) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) sm.ct[(4 * r + g) * 17 + c] = T[I2][I][r];
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < 4; ++r) T[I][I2][r] = sm.ct[c * 17 + 4 * r + g];
+      __syncthreads();  // read before the next tile overwrites it
+    }
   const double nan = __builtin_nan("");
 #pragma unroll
   for (int I = 0; I < NJ; ++I)
