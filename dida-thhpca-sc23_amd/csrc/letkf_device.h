@@ -87,12 +87,14 @@ __device__ __forceinline__ double bperm_d(int addr, double v) {
 }
 
 // 1 / sqrt(x) for positive normal x: the hardware estimate (relative error
-// about 2^-23) refined by two Newton steps, y (3 - x y^2) / 2, to a few ulp.
+// about 2^-23) refined by NR Newton steps, y (3 - x y^2) / 2: two reach a
+// few ulp, one about 2^-44.
+template <int NR = 2>
 __device__ __forceinline__ double rsq_nr(double x) {
   double y = __builtin_amdgcn_rsq(x);
   const double hx = 0.5 * x;
-  y = y * fma(-hx * y, y, 1.5);
-  y = y * fma(-hx * y, y, 1.5);
+#pragma unroll
+  for (int i = 0; i < NR; ++i) y = y * fma(-hx * y, y, 1.5);
   return y;
 }
 
@@ -101,11 +103,14 @@ __device__ __forceinline__ double rsq_nr(double x) {
 // t = tan(theta), gm = gamma = gs sp sq, cs = cos(theta), ic = 1/cos(theta).
 // A pair is rotated while gamma^2 > tol2 al be (t = 0, cs = ic = 1 below);
 // *big records gamma^2 > tolp al be (the next sweep would still rotate).
-// t = sign(d e) |e| / (|d| + sqrt(d^2 + e^2)), d = be - al, e = 2 gamma,
-// from the hardware estimates of 1/sqrt and 1/x: an error of t (about 2^-22
-// relative) only leaves a rotated pair that far from orthogonal, which the
-// next sweep removes, while cs (two Newton steps) keeps every applied
-// rotation orthogonal to a few ulp.
+// t = sign(d e) |e| / (|d| + sqrt(d^2 + e^2)), d = be - al, e = 2 gamma
+// (for d = 0 either sign zeroes gamma), from the hardware estimates of
+// 1/sqrt and 1/x: an error of t (about 2^-22 relative) only leaves a rotated
+// pair that far from orthogonal, which the next sweep removes, while cs keeps
+// every applied rotation orthogonal to the accuracy of its NR Newton steps
+// (two: a few ulp; one, for the split half-precision Grams whose own error
+// is some 1e-6: about 2^-44).
+template <int NR = 2>
 __device__ __forceinline__ void rotation_angle(double gs, double al, double be, double sp,
                                                double sq, double tol2, double tolp,
                                                bool& rotated, bool& big, double& t, double& gm,
@@ -118,11 +123,10 @@ __device__ __forceinline__ void rotation_angle(double gs, double al, double be, 
   const double d = be - al, e = 2.0 * gm;
   const double r2 = fma(d, d, e * e);
   const double den = fma(r2, __builtin_amdgcn_rsq(r2), fabs(d));
-  t = fabs(e) * __builtin_amdgcn_rcp(den);
-  if ((d < 0.0) != (e < 0.0) && d != 0.0) t = -t;
+  t = copysign(fabs(e) * __builtin_amdgcn_rcp(den), d * e);
   t = need ? t : 0.0;
   const double u = fma(t, t, 1.0);
-  cs = rsq_nr(u);
+  cs = rsq_nr<NR>(u);
   ic = u * cs;
   cs = need ? cs : 1.0;
   ic = need ? ic : 1.0;

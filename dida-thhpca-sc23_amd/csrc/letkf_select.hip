@@ -307,11 +307,11 @@ __device__ __forceinline__ bool keep_candidate(const dk_select_args& a, const Co
 // wavefront a tile, four tiles a workgroup; TJ x TI <= kNT.
 constexpr int kNT = 16;
 
+// Geometry and output offset of a tile's columns, read by the weight and
+// exact-test code (once per buffered wavefront of candidates).
 struct TileCols {
   double cx2[kNT], sx2[kNT], y2r[kNT], ux[kNT], uy[kNT], uz[kNT];
   int64_t out[kNT];
-  float vx[kNT], vy[kNT], vz[kNT];
-  int32_t limit[kNT], thr[kNT], cnt[kNT];
 };
 
 // Fill pass: per column, the candidates that passed the FP32 test, in scan
@@ -321,7 +321,6 @@ struct TileCols {
 template <int NTT>
 struct TileBuf {
   int32_t l[NTT][128];
-  int32_t n[NTT];
 };
 
 __device__ __forceinline__ double wave_min(double v) {
@@ -339,6 +338,16 @@ __device__ __forceinline__ int wave_min_i(int v) {
 __device__ __forceinline__ int wave_max_i(int v) {
   for (int s = 32; s >= 1; s >>= 1) v = max(v, __shfl_xor(v, s, kWave));
   return v;
+}
+
+// Per-column state held lane-wise (lane j holds column j's): the value of
+// lane j (j wave-uniform) as a scalar, and v with lane j set to x.
+__device__ __forceinline__ int lane_get(int v, int j) { return __builtin_amdgcn_readlane(v, j); }
+__device__ __forceinline__ float lane_get(float v, int j) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+}
+__device__ __forceinline__ int lane_set(int v, int j, int x) {
+  return (int)(threadIdx.x & 63) == j ? x : v;
 }
 
 // Count (kFill false: nobsl[col] = min(kept, max_obs)) or fill (kept obs with
@@ -398,59 +407,75 @@ __global__ __launch_bounds__(256) void k_select_tiles(dk_select_args a) {
   reg.full = __any(mine && rc.full);
   reg.lo_deg = wave_min(mine ? rc.lo_deg : 1e300);
   reg.hi_deg = wave_max(mine ? rc.hi_deg : -1e300);
+  const ColGeo q = col_geo(clat, clon);
   if (lane < kNTT) {
-    const ColGeo q = col_geo(clat, clon);
     tc.cx2[lane] = q.cx2;
     tc.sx2[lane] = q.sx2;
     tc.y2r[lane] = q.y2r;
     tc.ux[lane] = q.ux;
     tc.uy[lane] = q.uy;
     tc.uz[lane] = q.uz;
-    tc.vx[lane] = q.vx;
-    tc.vy[lane] = q.vy;
-    tc.vz[lane] = q.vz;
-    tc.limit[lane] = !mine ? 0 : (kFill ? a.nobsl[col] : a.max_obs);
-    tc.thr[lane] = kFill && mine ? a.thr[col] : INT32_MAX;
     tc.out[lane] = kFill && mine ? a.offset[col] - a.base : 0;
-    tc.cnt[lane] = 0;
-    if constexpr (kFill) tb.n[lane] = 0;
     if (kFill && mine && a.offset[col] - a.base + a.nobsl[col] > a.capacity) *a.overflow = 1;
   }
+  // lane-wise column state: FP32 unit vector, how many obs the column takes
+  // (0 for lanes that are not columns of this run), caller-index threshold,
+  // kept so far, candidates waiting in the buffer
+  const float cvx = q.vx, cvy = q.vy, cvz = q.vz;
+  const int climit = mine ? (kFill ? a.nobsl[col] : a.max_obs) : 0;
+  const int cthr = kFill && mine ? a.thr[col] : INT32_MAX;
+  int ccnt = 0, cnb = 0;
   wave_sync_lds();
-  const bool capped = __any(lane < kNTT && tc.thr[lane < kNTT ? lane : 0] != INT32_MAX);
+  const bool capped = __any(cthr != INT32_MAX);
   // columns still counting / filling (bit j)
-  uint32_t open = (uint32_t)__ballot(lane < kNTT && tc.limit[lane < kNTT ? lane : 0] > 0);
+  uint32_t open = (uint32_t)__ballot(climit > 0);
   const float flo = far_lo32(a), nhi = near_hi32(a);
   // weights of the candidates surely inside from the chord angle when the
   // support is within its range
   const bool chord = kFill && a.theta_deg <= kChordMaxDeg;
   const double r_per_rad = kEarthA / a.local_dist;
+  // The exact test of the lanes' candidates l (shell: those the FP32 test
+  // leaves open) against column j.
+  auto exact_j = [&](int j, int l, bool shell, double& gc) -> bool {
+    bool keep = false;
+    gc = 0.0;
+    if (shell) {
+      ColGeo g;
+      g.cx2 = tc.cx2[j];
+      g.sx2 = tc.sx2[j];
+      g.y2r = tc.y2r[j];
+      keep = exact_keep(a, g, l, gc);
+    }
+    return keep;
+  };
   // Fill: the weight (and, in the shell, the exact test) of the first n
   // buffered candidates of column j, one per lane; kept ones are written in
   // buffer (scan) order.
   auto flush = [&](int j, int n) {
-    ColGeo q;
-    q.cx2 = tc.cx2[j];
-    q.sx2 = tc.sx2[j];
-    q.y2r = tc.y2r[j];
-    q.ux = tc.ux[j];
-    q.uy = tc.uy[j];
-    q.uz = tc.uz[j];
     const int lb = lane < n ? tb.l[j][lane] : 0;
-    const bool shell = lb < 0;
-    const int l = shell ? ~lb : lb;
+    const bool live = lane < n;
+    const bool shell = live && (lb < 0 || !chord);
+    const int l = lb < 0 ? ~lb : lb;
     double gc = 0.0;
     bool keep = false;
-    if (lane < n) {
-      if (shell || !chord) {
-        keep = exact_keep(a, q, l, gc);
-      } else {
-        gc = inner_weight(a, q, l, r_per_rad);
-        keep = true;
+    if (live && !shell) {
+      ColGeo g;
+      g.ux = tc.ux[j];
+      g.uy = tc.uy[j];
+      g.uz = tc.uz[j];
+      gc = inner_weight(a, g, l, r_per_rad);
+      keep = true;
+    }
+    if (__any(shell)) {
+      double ge;
+      const bool ke = exact_j(j, l, shell, ge);
+      if (shell) {
+        keep = ke;
+        gc = ge;
       }
     }
     const uint64_t bm = __ballot(keep);
-    const int c0 = tc.cnt[j], lim = tc.limit[j];
+    const int c0 = lane_get(ccnt, j), lim = lane_get(climit, j);
     if (keep) {
       const int pos = c0 + (int)__builtin_amdgcn_mbcnt_hi(
                                (uint32_t)(bm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bm, 0u));
@@ -461,8 +486,7 @@ __global__ __launch_bounds__(256) void k_select_tiles(dk_select_args a) {
       }
     }
     const int c1 = c0 + __popcll(bm);
-    wave_sync_lds();
-    if (lane == 0) tc.cnt[j] = c1;
+    ccnt = lane_set(ccnt, j, c1);
     if (c1 >= lim) open &= ~(1u << j);
   };
   if (open) {
@@ -475,22 +499,22 @@ __global__ __launch_bounds__(256) void k_select_tiles(dk_select_args a) {
         const int32_t pv = kFill && capped && valid ? a.perm[l] : 0;
         for (uint32_t m = open; m; m &= m - 1) {
           const int j = __builtin_ctz(m);
-          const float d = dot32(u, tc.vx[j], tc.vy[j], tc.vz[j]);
+          const float d = dot32(u, lane_get(cvx, j), lane_get(cvy, j), lane_get(cvz, j));
           if constexpr (kFill) {
             // a capped column's caller-index threshold first, then the FP32
             // test; survivors wait in the column's buffer
-            const int32_t thr = tc.thr[j];
+            const int32_t thr = lane_get(cthr, j);
             const bool cand = valid && d >= flo && (thr == INT32_MAX || pv < thr);
             const uint64_t bm = __ballot(cand);
             if (!bm) continue;
-            const int nb0 = tb.n[j];
+            const int nb0 = lane_get(cnb, j);
             if (cand)
               tb.l[j][nb0 + (int)__builtin_amdgcn_mbcnt_hi(
                                 (uint32_t)(bm >> 32),
                                 __builtin_amdgcn_mbcnt_lo((uint32_t)bm, 0u))] = d > nhi ? l : ~l;
             int nb = nb0 + __popcll(bm);
-            wave_sync_lds();
             if (nb >= kWave) {
+              wave_sync_lds();
               flush(j, kWave);
               // the rest moves to the front of the buffer
               const int rest = nb - kWave;
@@ -499,42 +523,37 @@ __global__ __launch_bounds__(256) void k_select_tiles(dk_select_args a) {
               if (lane < rest) tb.l[j][lane] = v;
               nb = rest;
             }
-            if (lane == 0) tb.n[j] = nb;
-            wave_sync_lds();
+            cnb = lane_set(cnb, j, nb);
           } else {
             const bool cand = valid && d >= flo;
             if (!__any(cand)) continue;
-            ColGeo q;
-            q.cx2 = tc.cx2[j];
-            q.sx2 = tc.sx2[j];
-            q.y2r = tc.y2r[j];
-            double gc;
-            const bool keep = cand && (d > nhi || exact_keep(a, q, l, gc));
-            const int c1 = tc.cnt[j] + __popcll(__ballot(keep));
-            wave_sync_lds();
-            if (lane == 0) tc.cnt[j] = c1;
-            if (c1 >= tc.limit[j]) open &= ~(1u << j);
+            bool keep = cand && d > nhi;
+            const bool shell = cand && !keep;
+            if (__any(shell)) {
+              double gc;
+              const bool ke = exact_j(j, l, shell, gc);
+              keep = keep || ke;
+            }
+            const int c1 = lane_get(ccnt, j) + __popcll(__ballot(keep));
+            ccnt = lane_set(ccnt, j, c1);
+            if (c1 >= lane_get(climit, j)) open &= ~(1u << j);
           }
         }
-        wave_sync_lds();
         if (!open) return false;
       }
       return true;
     });
     if constexpr (kFill) {
       // the candidates still waiting
+      wave_sync_lds();
       for (uint32_t m = open; m; m &= m - 1) {
         const int j = __builtin_ctz(m);
-        const int nb = tb.n[j];
+        const int nb = lane_get(cnb, j);
         if (nb > 0) flush(j, nb);
       }
     }
   }
-  wave_sync_lds();
-  if (!kFill && mine) {
-    const int c = tc.cnt[lane];
-    a.nobsl[col] = c < a.max_obs ? c : a.max_obs;
-  }
+  if (!kFill && mine) a.nobsl[col] = ccnt < a.max_obs ? ccnt : a.max_obs;
 }
 
 // Sort key of every tile of TJ x TI columns: the obs count of the bins its

@@ -163,6 +163,9 @@ struct Wave {
   static constexpr int NJ = KP / 16;   // column sets per lane (and row blocks)
   static constexpr int NQ = NJ * 4;    // rows per column per lane
   static constexpr int NP = NJ / 2;    // column pairs per lane group
+  // Newton steps of the rotation cosine: one for the split half-precision
+  // Grams (about 2^-44, far below their own error), two for the FP64 Gram
+  static constexpr int kNewton = GRAM == DK_GRAM_FP64 ? 2 : 1;
   double X[NJ][NQ];  // working matrix (see header)
   bool patB;         // current row pattern
   // Yb rows as read: FP64 for the FP64 Gram, FP32 for the half-precision ones
@@ -502,6 +505,21 @@ struct Wave {
     return s0 + s1;
   }
 
+  // The dot products of the two pairs of a round, their FMAs interleaved.
+  template <int P0, int Q0, int P1, int Q1>
+  __device__ __forceinline__ void dot2(double& r0, double& r1) const {
+    double a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0;
+#pragma unroll
+    for (int q = 0; q < NQ; q += 2) {
+      a0 = fma(X[P0][q], X[Q0][q], a0);
+      b0 = fma(X[P1][q], X[Q1][q], b0);
+      a1 = fma(X[P0][q + 1], X[Q0][q + 1], a1);
+      b1 = fma(X[P1][q + 1], X[Q1][q + 1], b1);
+    }
+    r0 = a0 + a1;
+    r1 = b0 + b1;
+  }
+
   // Rotate columns P, Q given gs = X[P] . X[Q] (summed over the wave).  A
   // pair is rotated while gamma^2 > tol^2 alpha beta; *big records a pair with
   // gamma^2 > tol alpha beta (the next sweep would still rotate, the test of
@@ -519,11 +537,10 @@ struct Wave {
     const double d = be - al, e = 2.0 * gm;
     const double r2 = fma(d, d, e * e);
     const double den = fma(r2, __builtin_amdgcn_rsq(r2), fabs(d));
-    double t = fabs(e) * __builtin_amdgcn_rcp(den);
-    if ((d < 0.0) != (e < 0.0) && d != 0.0) t = -t;
+    double t = copysign(fabs(e) * __builtin_amdgcn_rcp(den), d * e);
     t = need ? t : 0.0;
     const double u = fma(t, t, 1.0);
-    double cs = rsq_nr(u);  // cos = 1 / sqrt(1 + t^2)
+    double cs = rsq_nr<kNewton>(u);  // cos = 1 / sqrt(1 + t^2)
     double ic = u * cs;     // 1 / cos
     cs = need ? cs : 1.0;
     ic = need ? ic : 1.0;
@@ -568,10 +585,12 @@ struct Wave {
   template <int T0, int B0, int T1, int B1>
   __device__ __forceinline__ void round(double tol2, double tolp, bool& rotated, bool& big) {
     if constexpr (NP == 2) {
-      const double gs = sum_g_pair(dot<T0, B0>(), dot<T1, B1>());
+      double d0, d1;
+      dot2<T0, B0, T1, B1>(d0, d1);
+      const double gs = sum_g_pair(d0, d1);
       const bool second = threadIdx.x >= 32;
       double t, gm, cs, ic;
-      rotation_angle(gs, second ? nrm[T1] : nrm[T0], second ? nrm[B1] : nrm[B0],
+      rotation_angle<kNewton>(gs, second ? nrm[T1] : nrm[T0], second ? nrm[B1] : nrm[B0],
                      second ? sc[T1] : sc[T0], second ? sc[B1] : sc[B0], tol2, tolp, rotated,
                      big, t, gm, cs, ic);
       double t0, t1, gm0, gm1, cs0, cs1, ic0, ic1;
