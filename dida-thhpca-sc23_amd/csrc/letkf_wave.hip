@@ -493,6 +493,15 @@ struct Wave {
   // so c >= 1/sqrt(2) and a sweep (KP - 1 rotations per column) changes a
   // scale by at most 2^((KP - 1) / 2).
   double nrm[NJ], sc[NJ], isc[NJ];
+  // With two pairs a lane (NP = 2) the state lives with the pair that is
+  // rotated: lane groups 0-1 hold the top tp and bottom bt of the first pair
+  // of the round, groups 2-3 those of the second.  A round then broadcasts
+  // only the two FMA factors of each pair, and between rounds the state
+  // follows the pairing (nrm, sc, isc above hold it between sweeps).
+  struct ColSt {
+    double nrm, sc, isc;
+  };
+  ColSt tp, bt;
 
   template <int P, int Q>
   __device__ __forceinline__ double dot() const {
@@ -579,29 +588,118 @@ struct Wave {
     isc[Q] *= ic;
   }
 
+  // x_P' = x_P - ta x_Q, x_Q' = x_Q + tb x_P over the lane's rows.
+  template <int P, int Q>
+  __device__ __forceinline__ void rotate_x(double ta, double tb) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const double x = X[P][q], y = X[Q][q];
+      X[P][q] = fma(-ta, y, x);
+      X[Q][q] = fma(tb, x, y);
+    }
+  }
+
   // One round: pairs (T0, B0) and, with two pairs a lane, (T1, B1).  With two
-  // pairs, lane groups 0-1 compute the angle of the first and groups 2-3 that
-  // of the second, and the angles are then broadcast.
+  // pairs, lane groups 0-1 compute the angle of the first from their state
+  // (tp, bt) and groups 2-3 that of the second; each half updates its state
+  // and the FMA factors of both pairs are then broadcast.
   template <int T0, int B0, int T1, int B1>
   __device__ __forceinline__ void round(double tol2, double tolp, bool& rotated, bool& big) {
     if constexpr (NP == 2) {
       double d0, d1;
       dot2<T0, B0, T1, B1>(d0, d1);
       const double gs = sum_g_pair(d0, d1);
-      const bool second = threadIdx.x >= 32;
       double t, gm, cs, ic;
-      rotation_angle<kNewton>(gs, second ? nrm[T1] : nrm[T0], second ? nrm[B1] : nrm[B0],
-                     second ? sc[T1] : sc[T0], second ? sc[B1] : sc[B0], tol2, tolp, rotated,
-                     big, t, gm, cs, ic);
-      double t0, t1, gm0, gm1, cs0, cs1, ic0, ic1;
-      split_g(t, t0, t1);
-      split_g(gm, gm0, gm1);
-      split_g(cs, cs0, cs1);
-      split_g(ic, ic0, ic1);
-      apply<T0, B0>(t0, gm0, cs0, ic0);
-      apply<T1, B1>(t1, gm1, cs1, ic1);
+      rotation_angle<kNewton>(gs, tp.nrm, bt.nrm, tp.sc, bt.sc, tol2, tolp, rotated, big, t, gm,
+                              cs, ic);
+      const double ta = t * (bt.sc * tp.isc);
+      const double tb = t * (tp.sc * bt.isc);
+      tp.nrm = fma(-t, gm, tp.nrm);
+      bt.nrm = fma(t, gm, bt.nrm);
+      tp.sc *= cs;
+      bt.sc *= cs;
+      tp.isc *= ic;
+      bt.isc *= ic;
+      double ta0, ta1, tb0, tb1;
+      split_g(ta, ta0, ta1);
+      split_g(tb, tb0, tb1);
+      rotate_x<T0, B0>(ta0, tb0);
+      rotate_x<T1, B1>(ta1, tb1);
     } else {
       rot<T0, B0>(sum_g(dot<T0, B0>()), tol2, tolp, rotated, big);
+    }
+  }
+
+  // NP = 2 state moves.  The bottoms' state trades lane halves (the pairing
+  // (0, 1), (2, 3) to (0, 3), (2, 1) and back).
+  __device__ __forceinline__ void swap_bt() {
+    const bool second = threadIdx.x >= 32;
+    double a, b;
+    split_g(bt.nrm, a, b);
+    bt.nrm = second ? a : b;
+    split_g(bt.sc, a, b);
+    bt.sc = second ? a : b;
+    split_g(bt.isc, a, b);
+    bt.isc = second ? a : b;
+  }
+  // x and y trade halves crosswise: x keeps groups 0-1 and takes y's groups
+  // 0-1 into groups 2-3; y takes x's groups 2-3 into groups 0-1 and keeps
+  // groups 2-3 (one v_permlane32_swap of the two registers).
+  static __device__ __forceinline__ void cross(double& x, double& y) {
+    const auto a = __builtin_amdgcn_permlane32_swap(__double2loint(x), __double2loint(y), false,
+                                                    false);
+    const auto b = __builtin_amdgcn_permlane32_swap(__double2hiint(x), __double2hiint(y), false,
+                                                    false);
+    x = __hiloint2double(b[0], a[0]);
+    y = __hiloint2double(b[1], a[1]);
+  }
+  // From the pairing (0, 3), (2, 1) to (0, 2), (3, 1): groups 0-1 keep their
+  // top (column 0) and take the top of groups 2-3 (column 2) as bottom;
+  // groups 2-3 take the bottom of groups 0-1 (column 3) as top and keep their
+  // bottom (column 1).
+  __device__ __forceinline__ void cross_tb() {
+    cross(tp.nrm, bt.nrm);
+    cross(tp.sc, bt.sc);
+    cross(tp.isc, bt.isc);
+  }
+  __device__ __forceinline__ void bt_xor1() {
+    bt.nrm = swz_xor1(bt.nrm);
+    bt.sc = swz_xor1(bt.sc);
+    bt.isc = swz_xor1(bt.isc);
+  }
+  __device__ __forceinline__ void bt_perm(int addr) {
+    bt.nrm = bperm_d(addr, bt.nrm);
+    bt.sc = bperm_d(addr, bt.sc);
+    bt.isc = bperm_d(addr, bt.isc);
+  }
+  __device__ __forceinline__ void tb_xswap(int addr, bool first) {
+    double w = bperm_d(addr, first ? bt.nrm : tp.nrm);
+    bt.nrm = first ? w : bt.nrm;
+    tp.nrm = first ? tp.nrm : w;
+    w = bperm_d(addr, first ? bt.sc : tp.sc);
+    bt.sc = first ? w : bt.sc;
+    tp.sc = first ? tp.sc : w;
+    w = bperm_d(addr, first ? bt.isc : tp.isc);
+    bt.isc = first ? w : bt.isc;
+    tp.isc = first ? tp.isc : w;
+  }
+  template <int J>
+  __device__ __forceinline__ void x_xor1() {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) X[J][q] = swz_xor1(X[J][q]);
+  }
+  template <int J>
+  __device__ __forceinline__ void x_perm(int addr) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) X[J][q] = bperm_d(addr, X[J][q]);
+  }
+  template <int T, int B>
+  __device__ __forceinline__ void x_xswap(int addr, bool first) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const double w = bperm_d(addr, first ? X[B][q] : X[T][q]);
+      X[B][q] = first ? w : X[B][q];
+      X[T][q] = first ? X[T][q] : w;
     }
   }
 
@@ -684,29 +782,44 @@ struct Wave {
       for (int J = 0; J < NJ; ++J) nrm[J] = colnorm(J);
       bool rotated = false, big = false;
       if constexpr (NP == 2) {
+        // groups 0-1: pair (0, 1), groups 2-3: pair (2, 3)
+        const bool second = lane >= 32;
+        tp.nrm = second ? nrm[2] : nrm[0];
+        bt.nrm = second ? nrm[3] : nrm[1];
+        tp.sc = bt.sc = tp.isc = bt.isc = 1.0;
         for (int G = KP / 2; G >= 4; G >>= 1) {
           for (int r = 0; r < G; r += 4) {
             round<0, 1, 2, 3>(tol2, tolp, rotated, big);
+            swap_bt();
             round<0, 3, 2, 1>(tol2, tolp, rotated, big);
-            move_xor1<1>();
-            move_xor1<3>();
+            x_xor1<1>();
+            x_xor1<3>();
+            bt_xor1();
             round<0, 3, 2, 1>(tol2, tolp, rotated, big);
+            swap_bt();
             round<0, 1, 2, 3>(tol2, tolp, rotated, big);
             if (r + 4 < G) {
               const int addr = (lane ^ (((r + 4) & -(r + 4)) >> 1)) << 2;
-              move_perm<1>(addr);
-              move_perm<3>(addr);
+              x_perm<1>(addr);
+              x_perm<3>(addr);
+              bt_perm(addr);
             }
           }
           const int d = G >> 2;
           const int addr = (lane ^ d) << 2;
           const bool first = (c & d) == 0;
-          xswap<0, 1>(addr, first);
-          xswap<2, 3>(addr, first);
+          x_xswap<0, 1>(addr, first);
+          x_xswap<2, 3>(addr, first);
+          tb_xswap(addr, first);
         }
         round<0, 1, 2, 3>(tol2, tolp, rotated, big);
+        swap_bt();
         round<0, 3, 2, 1>(tol2, tolp, rotated, big);
+        cross_tb();
         round<0, 2, 3, 1>(tol2, tolp, rotated, big);
+        // groups 0-1 hold columns 0 and 2, groups 2-3 columns 3 and 1
+        split_g(tp.sc, sc[0], sc[3]);
+        split_g(bt.sc, sc[2], sc[1]);
       } else {
         for (int G = KP / 2; G >= 1; G >>= 1) {
           for (int r = 0; r < G; ++r) {
