@@ -262,7 +262,8 @@ __device__ __forceinline__ bool exact_keep(const dk_select_args& a, const ColGeo
 // = p - q: |d| = 2 sin(theta / 2), so theta = 2 asin(|d| / 2), summed as the
 // Maclaurin series of asin, x sum_k (2k)! / (4^k (k!)^2 (2k + 1)) x^(2k).
 // For theta <= 0.4 (x^2 < 0.04) thirteen terms leave a relative error below
-// 1e-18; unlike acos(p . q) it keeps its digits at short distances.
+// 1e-18; unlike acos(p . q) it keeps its digits at short distances.  |d| is
+// c2 / sqrt(c2) from the Newton-refined reciprocal square root (a few ulp).
 constexpr double kChordMaxDeg = 22.9;  // 0.4 rad
 __device__ __forceinline__ double chord_angle(double dx, double dy, double dz) {
   const double c2 = fma(dx, dx, fma(dy, dy, dz * dz));
@@ -280,7 +281,33 @@ __device__ __forceinline__ double chord_angle(double dx, double dy, double dz) {
   s = fma(s, x2, 0.075);                  // 3 / 40
   s = fma(s, x2, 0.16666666666666666);    // 1 / 6
   s = fma(s, x2, 1.0);
-  return sqrt(c2) * s;                    // 2 x s
+  const double c = c2 > 0.0 ? c2 * rsq_nr(c2) : 0.0;  // |d|
+  return c * s;                           // 2 x s
+}
+
+// gc_weight for a candidate surely inside the support (r <= 2): the same
+// polynomials, with r / 12 as a product and 0.6666667 / r from the
+// Newton-refined reciprocal (agrees with gc_weight to a few ulp).
+__device__ __forceinline__ double rcp_nr(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  y = fma(fma(-x, y, 1.0), y, y);
+  y = fma(fma(-x, y, 1.0), y, y);
+  return y;
+}
+__device__ __forceinline__ double gc_weight_inner(double ri, int gc_filter) {
+  if (!gc_filter || ri > 2.0) return ri > 2.0 ? 0.0 : 1.0;
+  if (ri > 1.0) {
+    double rj = fma(ri, 1.0 / 12.0, -0.5);
+    rj = fma(rj, ri, 0.625);
+    rj = fma(rj, ri, kGc53);
+    return fma(fma(rj, ri, -5.0), ri, 4.0) - kGc23 * rcp_nr(ri);
+  }
+  if (ri > 0.0) {
+    const double rk = ri * ri;
+    double rj = fma(fma(-0.25, ri, 0.5), ri, 0.625);
+    return fma(fma(rj, ri, -kGc53), rk, 1.0);
+  }
+  return 1.0;
 }
 
 // The weight of binned obs l surely inside the support (its keep decided by
@@ -290,7 +317,7 @@ __device__ __forceinline__ double chord_angle(double dx, double dy, double dz) {
 __device__ __forceinline__ double inner_weight(const dk_select_args& a, const ColGeo& q, int l,
                                                double r_per_rad) {
   const double th = chord_angle(a.obs.ux[l] - q.ux, a.obs.uy[l] - q.uy, a.obs.uz[l] - q.uz);
-  return gc_weight(th * r_per_rad, a.gc_filter);
+  return gc_weight_inner(th * r_per_rad, a.gc_filter);
 }
 
 // Whether binned obs l is kept (gc > 0), without its weight.
