@@ -156,7 +156,7 @@ struct dida_letkf_ctx {
     int end_bit;
     int32_t nbins_max;   /* bin arrays are sized for the finest bins tried */
     double bin_deg;
-    double theta_deg, cos_far, cos_near;
+    double theta_deg, cos_far, cos_near, cos_keep;
     /* binning */
     uint32_t *key, *key_sorted;
     int32_t *iota, *perm, *bin_count, *bin_start;
@@ -354,6 +354,9 @@ int dida_letkf_create(dida_letkf_ctx **out, const dida_letkf_params *prm, int32_
     c->cos_far = (theta_rad * (1.0 + 1e-7) < pi) ? cos(theta_rad * (1.0 + 1e-7)) : -2.0;
     /* candidates closer than theta (1 - 1e-7) have gc > 0 (r < 2) for sure */
     c->cos_near = (theta_rad * (1.0 - 1e-7) < pi) ? cos(theta_rad * (1.0 - 1e-7)) : -1.0;
+    /* gc > 0 exactly when r = d / L <= 2 (gc(2) is about 1.4e-6 with the
+     * reference's REAL(4) constants), that is central angle <= 2 L / a */
+    c->cos_keep = theta_rad < pi ? cos(theta_rad) : -2.0;
     /* bins: the given size, or theta / 4; arrays sized for the finest size
      * the autotuner tries (theta / 8) */
     const double bd = prm->bin_deg > 0.0 ? prm->bin_deg : c->theta_deg / 4.0;
@@ -678,6 +681,7 @@ static int sweep_select_count(dida_letkf_ctx *c, const dida_letkf_obs *o, hipStr
     sa->theta_deg = c->theta_deg;
     sa->cos_far = c->cos_far;
     sa->cos_near = c->cos_near;
+    sa->cos_keep = c->cos_keep;
     sa->gc_filter = c->prm.gc_filter;
     sa->max_obs = c->prm.max_obs;
     sa->nobsl = c->nobsl;

@@ -50,6 +50,7 @@ typedef struct dk_select_args {
     double theta_deg;    /* angular radius of the support plus margin, deg */
     double cos_far;      /* central-angle cosine below which gc = 0 surely */
     double cos_near;     /* central-angle cosine above which gc > 0 surely */
+    double cos_keep;     /* cos(2 L / a): kept iff the central-angle cosine >= it */
     int32_t gc_filter;
     int32_t max_obs;
     int32_t *nobsl;      /* [ncol] count pass output / fill pass input */

@@ -244,18 +244,17 @@ __device__ __forceinline__ float dot32(const float4& u, float vx, float vy, floa
   return fmaf(u.z, vz, fmaf(u.y, vy, u.x * vx));
 }
 
-// The reference's test of binned obs l (letkf_yb.F90:84-96): gc > 0 with gc
-// the Gaspari-Cohn weight of the distance in the reference's formula.
-__device__ __forceinline__ bool exact_keep(const dk_select_args& a, const ColGeo& q, int l,
+// The reference's weight of binned obs l (letkf_yb.F90:84-96): gc of the
+// distance in the reference's formula (0 beyond the support).
+__device__ __forceinline__ void ref_weight(const dk_select_args& a, const ColGeo& q, int l,
                                            double& gc) {
 #pragma clang fp contract(off)
   gc = 0.0;
   const double v = central_cos(a.obs.cx[l], a.obs.sx[l], a.obs.yr[l], q.cx2, q.sx2, q.y2r);
-  if (!(v >= a.cos_far)) return false;
+  if (!(v >= a.cos_far)) return;
   const double d = dist_from_cos(v);
   const double r = d * 1.0 / a.local_dist;
   gc = gc_weight(r, a.gc_filter);
-  return gc > 0.0;
 }
 
 // Central angle (rad) between unit vectors p and q from their difference d
@@ -320,14 +319,26 @@ __device__ __forceinline__ double inner_weight(const dk_select_args& a, const Co
   return gc_weight_inner(th * r_per_rad, a.gc_filter);
 }
 
+// The test of a candidate in the shell that the FP32 test leaves open: the
+// central-angle cosine from the FP64 unit vectors against cos(2 L / a).  In
+// the reference's radians the dot product of the unit vectors is the
+// reference's cos(x1) cos(x2) cos(y1 - y2) + sin(x1) sin(x2) up to rounding
+// (a few 1e-16), and its gc > 0 holds exactly when r = d / L <= 2 (gc(2) is
+// about 1.4e-6 with its REAL(4) constants), so the two tests can differ only
+// for an obs within about 1e-15 of the support edge in cosine.
+__device__ __forceinline__ bool shell_keep(const dk_select_args& a, double qx, double qy,
+                                           double qz, int l) {
+  const double v = fma(a.obs.ux[l], qx, fma(a.obs.uy[l], qy, a.obs.uz[l] * qz));
+  return v >= a.cos_keep;
+}
+
 // Whether binned obs l is kept (gc > 0), without its weight.
 __device__ __forceinline__ bool keep_candidate(const dk_select_args& a, const ColGeo& q, int l,
                                                const float4& u) {
   const float d = dot32(u, q.vx, q.vy, q.vz);
   if (d < far_lo32(a)) return false;
   if (d > near_hi32(a)) return true;
-  double gc;
-  return exact_keep(a, q, l, gc);
+  return shell_keep(a, q.ux, q.uy, q.uz, l);
 }
 
 // Tiles of TJ x TI analysis columns (latitude-fastest within the tile), one
@@ -388,7 +399,9 @@ __device__ __forceinline__ void wave_sync_lds() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <bool kFill, int TJ, int TI>
+// kChord: weights from the chord angle (a support within kChordMaxDeg); else
+// from the reference's formula, evaluated for every kept obs.
+template <bool kFill, int TJ, int TI, bool kChord>
 __global__ __launch_bounds__(256) void k_select_tiles(dk_select_args a) {
   constexpr int kTJ = TJ, kTI = TI, kNTT = TJ * TI;
   static_assert(kNTT <= kNT, "tile larger than its column state");
@@ -457,23 +470,10 @@ __global__ __launch_bounds__(256) void k_select_tiles(dk_select_args a) {
   // columns still counting / filling (bit j)
   uint32_t open = (uint32_t)__ballot(climit > 0);
   const float flo = far_lo32(a), nhi = near_hi32(a);
-  // weights of the candidates surely inside from the chord angle when the
-  // support is within its range
-  const bool chord = kFill && a.theta_deg <= kChordMaxDeg;
   const double r_per_rad = kEarthA / a.local_dist;
-  // The exact test of the lanes' candidates l (shell: those the FP32 test
-  // leaves open) against column j.
-  auto exact_j = [&](int j, int l, bool shell, double& gc) -> bool {
-    bool keep = false;
-    gc = 0.0;
-    if (shell) {
-      ColGeo g;
-      g.cx2 = tc.cx2[j];
-      g.sx2 = tc.sx2[j];
-      g.y2r = tc.y2r[j];
-      keep = exact_keep(a, g, l, gc);
-    }
-    return keep;
+  // The test of the lanes' shell candidates l against column j.
+  auto shell_j = [&](int j, int l, bool shell) -> bool {
+    return shell && shell_keep(a, tc.ux[j], tc.uy[j], tc.uz[j], l);
   };
   // Fill: the weight (and, in the shell, the exact test) of the first n
   // buffered candidates of column j, one per lane; kept ones are written in
@@ -481,24 +481,23 @@ __global__ __launch_bounds__(256) void k_select_tiles(dk_select_args a) {
   auto flush = [&](int j, int n) {
     const int lb = lane < n ? tb.l[j][lane] : 0;
     const bool live = lane < n;
-    const bool shell = live && (lb < 0 || !chord);
+    const bool shell = live && lb < 0;
     const int l = lb < 0 ? ~lb : lb;
+    bool keep = live && !shell;
+    if (__any(shell)) keep = keep || shell_j(j, l, shell);
     double gc = 0.0;
-    bool keep = false;
-    if (live && !shell) {
+    if (keep) {
       ColGeo g;
-      g.ux = tc.ux[j];
-      g.uy = tc.uy[j];
-      g.uz = tc.uz[j];
-      gc = inner_weight(a, g, l, r_per_rad);
-      keep = true;
-    }
-    if (__any(shell)) {
-      double ge;
-      const bool ke = exact_j(j, l, shell, ge);
-      if (shell) {
-        keep = ke;
-        gc = ge;
+      if constexpr (kChord) {
+        g.ux = tc.ux[j];
+        g.uy = tc.uy[j];
+        g.uz = tc.uz[j];
+        gc = inner_weight(a, g, l, r_per_rad);
+      } else {
+        g.cx2 = tc.cx2[j];
+        g.sx2 = tc.sx2[j];
+        g.y2r = tc.y2r[j];
+        ref_weight(a, g, l, gc);
       }
     }
     const uint64_t bm = __ballot(keep);
@@ -556,11 +555,7 @@ __global__ __launch_bounds__(256) void k_select_tiles(dk_select_args a) {
             if (!__any(cand)) continue;
             bool keep = cand && d > nhi;
             const bool shell = cand && !keep;
-            if (__any(shell)) {
-              double gc;
-              const bool ke = exact_j(j, l, shell, gc);
-              keep = keep || ke;
-            }
+            if (__any(shell)) keep = keep || shell_j(j, l, shell);
             const int c1 = lane_get(ccnt, j) + __popcll(__ballot(keep));
             ccnt = lane_set(ccnt, j, c1);
             if (c1 >= lane_get(climit, j)) open &= ~(1u << j);
@@ -639,6 +634,7 @@ constexpr int kCapDigit = 11;               // bits of caller index per radix le
 constexpr int kCapBins = 1 << kCapDigit;
 constexpr int kCapEnumBits = 12;            // digit ranges of up to 2^12 caller indices
                                             // are enumerated instead of rescanned
+constexpr int kCapUnroll = 4;               // candidate chunks in flight
 
 // thr[col] of every column: INT32_MAX unless the count pass stopped at
 // max_obs; then, if more than max_obs obs are kept, one past the caller index
@@ -678,17 +674,25 @@ __global__ __launch_bounds__(256) void k_select_cap(dk_select_args a) {
     const uint32_t span = width >= 31 ? 0x80000000u : (1u << width);
     int total = 0;
     for_region_ranges(a, reg, [&](int l0, int l1) -> bool {
-      for (int base = l0; base < l1; base += kWave) {
-        const int l = base + lane;
-        bool keep = false;
-        uint32_t rel = 0;
-        if (l < l1) {
-          rel = (uint32_t)a.perm[l] - lo;
-          keep = rel < span &&
-                 keep_candidate(a, q, l, reinterpret_cast<const float4*>(a.obs.u4)[l]);
+      // kCapUnroll chunks a step, their loads issued together
+      for (int base = l0; base < l1; base += kCapUnroll * kWave) {
+        uint32_t rel[kCapUnroll];
+        float4 u[kCapUnroll];
+#pragma unroll
+        for (int k = 0; k < kCapUnroll; ++k) {
+          const int l = base + k * kWave + lane;
+          const bool valid = l < l1;
+          rel[k] = valid ? (uint32_t)a.perm[l] - lo : span;
+          u[k] = valid ? reinterpret_cast<const float4*>(a.obs.u4)[l]
+                       : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         }
-        if (keep) atomicAdd(&hist[rel >> shift], 1u);
-        total += __popcll(__ballot(keep));
+#pragma unroll
+        for (int k = 0; k < kCapUnroll; ++k) {
+          const int l = base + k * kWave + lane;
+          const bool keep = rel[k] < span && keep_candidate(a, q, l, u[k]);
+          if (keep) atomicAdd(&hist[rel[k] >> shift], 1u);
+          total += __popcll(__ballot(keep));
+        }
       }
       return true;
     });
@@ -837,18 +841,30 @@ static unsigned tile_blocks(const dk_select_args* a, bool fill, int tj, int ti) 
   return (unsigned)(((ti1 - ti0) * ntj + 3) / 4);
 }
 
-template <bool kFill>
-static int select_tiles(const dk_select_args* a, hipStream_t s) {
+template <bool kFill, bool kChord>
+static void select_tiles_launch(const dk_select_args* a, hipStream_t s) {
   switch (a->tile) {
     case DK_SELECT_TILE_4X1:
-      k_select_tiles<kFill, 4, 1><<<tile_blocks(a, kFill, 4, 1), 256, 0, s>>>(*a);
+      k_select_tiles<kFill, 4, 1, kChord><<<tile_blocks(a, kFill, 4, 1), 256, 0, s>>>(*a);
       break;
     case DK_SELECT_TILE_4X4:
-      k_select_tiles<kFill, 4, 4><<<tile_blocks(a, kFill, 4, 4), 256, 0, s>>>(*a);
+      k_select_tiles<kFill, 4, 4, kChord><<<tile_blocks(a, kFill, 4, 4), 256, 0, s>>>(*a);
       break;
     default:
-      k_select_tiles<kFill, 4, 2><<<tile_blocks(a, kFill, 4, 2), 256, 0, s>>>(*a);
+      k_select_tiles<kFill, 4, 2, kChord><<<tile_blocks(a, kFill, 4, 2), 256, 0, s>>>(*a);
       break;
+  }
+}
+
+// The count pass needs no weights; the fill takes them from the chord angle
+// when the support is within its range.
+template <bool kFill>
+static int select_tiles(const dk_select_args* a, hipStream_t s) {
+  if constexpr (!kFill) {
+    select_tiles_launch<false, true>(a, s);
+  } else {
+    if (a->theta_deg <= kChordMaxDeg) select_tiles_launch<true, true>(a, s);
+    else select_tiles_launch<true, false>(a, s);
   }
   return (int)hipGetLastError();
 }
