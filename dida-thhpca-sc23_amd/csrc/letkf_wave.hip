@@ -1337,28 +1337,89 @@ __device__ __forceinline__ int bucket_of(int p, const dk_chain_build_args& a) {
   return b;
 }
 
-// One thread per longitude: the chains of its latitudes (runs of one bucket,
-// at most the bucket's length), counted (kWrite false) or written.
+// One wavefront per longitude: the chains of its latitudes, counted (kWrite
+// false) or written.  A chain is a piece of a run of columns of one bucket:
+// the run is cut every len[bucket] columns from its start.  Lane t holds
+// latitudes [t per, (t + 1) per): the run start entering each lane comes
+// from a max-scan of the lanes' last run starts, a chain starts where the
+// offset from its run start is a multiple of the length, and a chain ends at
+// the next chain start (a suffix min-scan of the lanes' first ones).
 template <bool kWrite>
-__global__ void k_chain_build(dk_chain_build_args a) {
-  const int ilon = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_chain_build(dk_chain_build_args a) {
+  const int lane = threadIdx.x & 63;
+  const int ilon = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
   if (ilon >= a.nlon) return;
-  const int32_t* p = a.nobsl + (int64_t)ilon * a.nlat;
-  const int64_t out = kWrite ? a.line_off[ilon] : 0;
-  int cnt = 0;
-  for (int i = 0; i < a.nlat;) {
-    const int b = bucket_of(p[i], a);
-    const int len = a.len[b] > 0 ? a.len[b] : 1;
-    int j = i + 1;
-    while (j < a.nlat && j - i < len && bucket_of(p[j], a) == b) ++j;
-    if (kWrite) {
-      a.chain_col0[out + cnt] = ilon * a.nlat + i;
-      a.chain_ncol[out + cnt] = j - i;
-    }
-    ++cnt;
-    i = j;
+  const int n = a.nlat;
+  const int32_t* p = a.nobsl + (int64_t)ilon * n;
+  const int per = (n + kWave - 1) / kWave;
+  const int i0 = min(lane * per, n), i1 = min(i0 + per, n);
+  const int prevb = i0 > 0 && i0 < n ? bucket_of(p[i0 - 1], a) : -1;
+  // the lane's last run start (or -1), then the one entering the lane
+  int last = -1;
+  int b = prevb;
+  for (int i = i0; i < i1; ++i) {
+    const int bi = bucket_of(p[i], a);
+    if (bi != b) last = i;
+    b = bi;
   }
-  if (!kWrite) a.line_cnt[ilon] = cnt;
+  int inc = last;
+  for (int o = 1; o < kWave; o <<= 1) {
+    const int t = __shfl_up(inc, o, kWave);
+    if (lane >= o) inc = max(inc, t);
+  }
+  int rs = __shfl_up(inc, 1, kWave);
+  if (lane == 0) rs = -1;
+  // chain starts of the lane: count and first
+  int cnt = 0, first = n;
+  {
+    int r = rs;
+    b = prevb;
+    for (int i = i0; i < i1; ++i) {
+      const int bi = bucket_of(p[i], a);
+      if (bi != b) r = i;
+      b = bi;
+      const int len = a.len[bi] > 0 ? a.len[bi] : 1;
+      if ((i - r) % len == 0) {
+        first = min(first, i);
+        ++cnt;
+      }
+    }
+  }
+  int before = cnt;  // inclusive scan of the counts
+  for (int o = 1; o < kWave; o <<= 1) {
+    const int t = __shfl_up(before, o, kWave);
+    if (lane >= o) before += t;
+  }
+  const int total = __shfl(before, kWave - 1, kWave);
+  before -= cnt;
+  if (!kWrite) {
+    if (lane == 0) a.line_cnt[ilon] = total;
+    return;
+  }
+  // the first chain start above the lane (n if none)
+  int nx = first;
+  for (int o = 1; o < kWave; o <<= 1) {
+    const int t = __shfl_down(nx, o, kWave);
+    if (lane + o < kWave) nx = min(nx, t);
+  }
+  nx = __shfl_down(nx, 1, kWave);
+  if (lane == kWave - 1) nx = n;
+  const int64_t out = a.line_off[ilon] + before;
+  int r = rs, prev = -1, k = 0;
+  b = prevb;
+  for (int i = i0; i < i1; ++i) {
+    const int bi = bucket_of(p[i], a);
+    if (bi != b) r = i;
+    b = bi;
+    const int len = a.len[bi] > 0 ? a.len[bi] : 1;
+    if ((i - r) % len == 0) {
+      if (prev >= 0) a.chain_ncol[out + k - 1] = i - prev;
+      a.chain_col0[out + k] = ilon * n + i;
+      prev = i;
+      ++k;
+    }
+  }
+  if (prev >= 0) a.chain_ncol[out + k - 1] = nx - prev;
 }
 
 // Grouped queue: chain (ilon, latchain) of stripe g goes to position
@@ -1469,9 +1530,9 @@ extern "C" int dk_chain_groups(const dk_batched_args* a, int32_t* order, void* s
 extern "C" int dk_chain_build(const dk_chain_build_args* a, int write, void* stream) {
   if (a->nlon <= 0) return 0;
   if (a->nbucket < 1 || a->nbucket > DK_NBUCKET) return (int)hipErrorInvalidValue;
-  const unsigned nb = (unsigned)((a->nlon + 127) / 128);
-  if (write) k_chain_build<true><<<nb, 128, 0, (hipStream_t)stream>>>(*a);
-  else k_chain_build<false><<<nb, 128, 0, (hipStream_t)stream>>>(*a);
+  const unsigned nb = (unsigned)((a->nlon + 3) / 4);
+  if (write) k_chain_build<true><<<nb, 256, 0, (hipStream_t)stream>>>(*a);
+  else k_chain_build<false><<<nb, 256, 0, (hipStream_t)stream>>>(*a);
   return (int)hipGetLastError();
 }
 
