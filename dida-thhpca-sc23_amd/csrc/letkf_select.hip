@@ -646,16 +646,18 @@ constexpr int kCapUnroll = 4;               // candidate chunks in flight
 // the polar bin are never candidates), which replaces a rescan of the
 // column's candidates.
 __global__ __launch_bounds__(256) void k_select_cap(dk_select_args a) {
-  __shared__ uint32_t hist_all[4][kCapBins];
+  // one workgroup a column: the four wavefronts share the candidate scan
+  // (chunks dealt round-robin) and one histogram; wavefront 0 picks the
+  // digit and enumerates
+  __shared__ uint32_t hist[kCapBins];
+  __shared__ int part[4];
+  __shared__ int sel[3];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int64_t ncol = (int64_t)a.nlat * a.nlon;
-  const int64_t col = (int64_t)blockIdx.x * 4 + wave;
-  if (col >= ncol) return;
+  const int64_t col = blockIdx.x;
   if (a.max_obs <= 0 || a.nobsl[col] < a.max_obs) {
-    if (lane == 0) a.thr[col] = INT32_MAX;
+    if (threadIdx.x == 0) a.thr[col] = INT32_MAX;
     return;
   }
-  uint32_t* hist = hist_all[wave];
   const int ilat = (int)(col % a.nlat), ilon = (int)(col / a.nlat);
   const double clat = (double)a.grid_lat[ilat];
   const double clon = (double)a.grid_lon[ilon];
@@ -669,13 +671,14 @@ __global__ __launch_bounds__(256) void k_select_cap(dk_select_args a) {
     const int dbits = width < kCapDigit ? width : kCapDigit;
     const int shift = width - dbits;
     const int nb = 1 << dbits;
-    for (int i = lane; i < nb; i += kWave) hist[i] = 0u;
-    wave_lds_sync();
+    for (int i = (int)threadIdx.x; i < nb; i += 256) hist[i] = 0u;
+    __syncthreads();
     const uint32_t span = width >= 31 ? 0x80000000u : (1u << width);
     int total = 0;
     for_region_ranges(a, reg, [&](int l0, int l1) -> bool {
       // kCapUnroll chunks a step, their loads issued together
-      for (int base = l0; base < l1; base += kCapUnroll * kWave) {
+      for (int base = l0 + wave * kCapUnroll * kWave; base < l1;
+           base += 4 * kCapUnroll * kWave) {
         uint32_t rel[kCapUnroll];
         float4 u[kCapUnroll];
 #pragma unroll
@@ -696,75 +699,82 @@ __global__ __launch_bounds__(256) void k_select_cap(dk_select_args a) {
       }
       return true;
     });
-    wave_lds_sync();
+    if (lane == 0) part[wave] = total;
+    __syncthreads();
+    total = part[0] + part[1] + part[2] + part[3];
     if (level == 0 && total <= a.max_obs) break;  // every kept obs fits
     // the digit whose cumulative count reaches rank: bins split over lanes,
     // a wave scan of the lane sums, then the owning lane walks its bins
-    const int per = (nb + kWave - 1) / kWave;
-    const int b0 = lane * per;
-    int sum = 0;
-    for (int i = 0; i < per; ++i)
-      if (b0 + i < nb) sum += (int)hist[b0 + i];
-    int inc = sum;
-    for (int o = 1; o < kWave; o <<= 1) {
-      const int t = __shfl_up(inc, o, kWave);
-      if (lane >= o) inc += t;
-    }
-    const int exc = inc - sum;
-    const uint64_t m = __ballot(exc < rank && rank <= inc);
-    if (m == 0) break;  // cannot happen with consistent passes: keep bin order
-    const int src = __ffsll((unsigned long long)m) - 1;
-    int dsel = 0, before = 0;
-    if (lane == src) {
-      int cum = exc;
-      for (int i = 0; i < per; ++i) {
-        const int h = b0 + i < nb ? (int)hist[b0 + i] : 0;
-        if (cum + h >= rank) {
-          dsel = b0 + i;
-          before = cum;
-          break;
+    if (wave == 0) {
+      const int per = (nb + kWave - 1) / kWave;
+      const int b0 = lane * per;
+      int sum = 0;
+      for (int i = 0; i < per; ++i)
+        if (b0 + i < nb) sum += (int)hist[b0 + i];
+      int inc = sum;
+      for (int o = 1; o < kWave; o <<= 1) {
+        const int t = __shfl_up(inc, o, kWave);
+        if (lane >= o) inc += t;
+      }
+      const int exc = inc - sum;
+      const uint64_t m = __ballot(exc < rank && rank <= inc);
+      const int src = m ? __ffsll((unsigned long long)m) - 1 : 0;
+      if (lane == src) {
+        int cum = exc, dsel = 0, before = 0;
+        for (int i = 0; i < per; ++i) {
+          const int h = b0 + i < nb ? (int)hist[b0 + i] : 0;
+          if (cum + h >= rank) {
+            dsel = b0 + i;
+            before = cum;
+            break;
+          }
+          cum += h;
         }
-        cum += h;
+        sel[0] = dsel;
+        sel[1] = before;
+        sel[2] = m != 0;  // cannot fail with consistent passes
       }
     }
-    dsel = __shfl(dsel, src, kWave);
-    before = __shfl(before, src, kWave);
-    lo += (uint32_t)dsel << shift;
-    rank -= before;
+    __syncthreads();
+    if (!sel[2]) break;
+    lo += (uint32_t)sel[0] << shift;
+    rank -= sel[1];
     width = shift;
     if (width == 0) {
       thr = (int32_t)(lo + 1u);
       break;
     }
     if (a.iperm && width <= kCapEnumBits) {
-      const uint32_t n = 1u << width;
-      const int32_t polar0 = a.obs.bin_start[a.geom.nbins];
-      int cum = 0;  // kept obs of the range before this chunk
-      for (uint32_t base = 0; base < n; base += kWave) {
-        const uint32_t ci = lo + base + (uint32_t)lane;
-        bool keep = false;
-        if (base + (uint32_t)lane < n && ci < (uint32_t)a.nobs) {
-          const int l = a.iperm[ci];
-          keep = l < polar0 &&
-                 keep_candidate(a, q, l, reinterpret_cast<const float4*>(a.obs.u4)[l]);
+      if (wave == 0) {
+        const uint32_t n = 1u << width;
+        const int32_t polar0 = a.obs.bin_start[a.geom.nbins];
+        int cum = 0;  // kept obs of the range before this chunk
+        for (uint32_t base = 0; base < n; base += kWave) {
+          const uint32_t ci = lo + base + (uint32_t)lane;
+          bool keep = false;
+          if (base + (uint32_t)lane < n && ci < (uint32_t)a.nobs) {
+            const int l = a.iperm[ci];
+            keep = l < polar0 &&
+                   keep_candidate(a, q, l, reinterpret_cast<const float4*>(a.obs.u4)[l]);
+          }
+          const uint64_t km = __ballot(keep);
+          const int cnt = __popcll(km);
+          if (cum + cnt >= rank) {
+            // the lane holding the (rank - cum)-th kept obs of the chunk
+            const int below = (int)__builtin_amdgcn_mbcnt_hi(
+                (uint32_t)(km >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)km, 0u));
+            const uint64_t hit = __ballot(keep && below == rank - cum - 1);
+            thr = (int32_t)(lo + base + (uint32_t)(__ffsll((unsigned long long)hit) - 1) + 1u);
+            break;
+          }
+          cum += cnt;
         }
-        const uint64_t km = __ballot(keep);
-        const int cnt = __popcll(km);
-        if (cum + cnt >= rank) {
-          // the lane holding the (rank - cum)-th kept obs of the chunk
-          const int below = (int)__builtin_amdgcn_mbcnt_hi(
-              (uint32_t)(km >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)km, 0u));
-          const uint64_t hit = __ballot(keep && below == rank - cum - 1);
-          thr = (int32_t)(lo + base + (uint32_t)(__ffsll((unsigned long long)hit) - 1) + 1u);
-          break;
-        }
-        cum += cnt;
       }
       break;
     }
-    wave_lds_sync();  // the bins are read before the next level clears them
+    __syncthreads();  // sel and the bins are read before the next level clears them
   }
-  if (lane == 0) a.thr[col] = thr;
+  if (threadIdx.x == 0) a.thr[col] = thr;
 }
 
 }  // namespace
@@ -909,7 +919,7 @@ extern "C" int dk_select_cap(const dk_select_args* a, void* stream) {
   const int64_t ncol = (int64_t)a->nlat * a->nlon;
   if (ncol <= 0) return 0;
   if (a->perm_bits < 1 || a->perm_bits > 31) return (int)hipErrorInvalidValue;
-  k_select_cap<<<(unsigned)((ncol + 3) / 4), 256, 0, (hipStream_t)stream>>>(*a);
+  k_select_cap<<<(unsigned)ncol, 256, 0, (hipStream_t)stream>>>(*a);
   return (int)hipGetLastError();
 }
 
