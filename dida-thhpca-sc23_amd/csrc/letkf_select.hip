@@ -1,10 +1,12 @@
 // Device-side local observation selection (gen_yb_dis, letkf_yb.F90:3-133).
 //
 // Observations are binned on a regular lat/lon lattice (stable sort by bin
-// id), then one wavefront per grid column visits the candidate bins in
-// ascending bin id, evaluates the reference's distance and Gaspari-Cohn
-// weight per candidate, and compacts the kept obs with a ballot.  Three
-// passes: count (capped at MAX_OBS), cap and fill into a CSR list.
+// id), then one wavefront per tile of grid columns visits the candidate bins
+// in ascending bin id, decides per candidate whether the reference's
+// Gaspari-Cohn weight is above zero (an FP32 unit-vector test, the thin shell
+// it leaves open by the FP64 unit-vector cosine), takes the weight of kept
+// ones, and compacts them with a ballot.  Three passes: count (capped at
+// MAX_OBS), cap and fill into a CSR list.
 //
 // MAX_OBS truncation.  gen_yb_dis appends the kept obs in its scan order and
 // stops at MAX_OBS (letkf_yb.F90:98-115, letkf.F90:155); over one global
@@ -233,7 +235,7 @@ __device__ __forceinline__ ColGeo col_geo(double clat, double clon) {
 // products and the sums), so outside [cos_far - 1e-6, cos_near + 1e-6] it
 // decides a candidate as the reference formula would (candidates surely
 // beyond the support are rejected, those surely inside it have gc > 0); the
-// thin shell between is decided by the reference formula itself.
+// thin shell between is decided by shell_keep.
 __device__ __forceinline__ float far_lo32(const dk_select_args& a) {
   return (float)(a.cos_far - 1e-6);
 }
@@ -353,9 +355,9 @@ struct TileCols {
 };
 
 // Fill pass: per column, the candidates that passed the FP32 test, in scan
-// order, waiting for a full wavefront of weights and exact tests (up to 63
+// order, waiting for a full wavefront of weights and shell tests (up to 63
 // waiting plus the 64 of one chunk): l, or ~l for a candidate in the shell
-// that the reference formula decides.
+// that shell_keep decides.
 template <int NTT>
 struct TileBuf {
   int32_t l[NTT][128];
@@ -475,7 +477,7 @@ __global__ __launch_bounds__(256) void k_select_tiles(dk_select_args a) {
   auto shell_j = [&](int j, int l, bool shell) -> bool {
     return shell && shell_keep(a, tc.ux[j], tc.uy[j], tc.uz[j], l);
   };
-  // Fill: the weight (and, in the shell, the exact test) of the first n
+  // Fill: the weight (and, in the shell, the FP64 test) of the first n
   // buffered candidates of column j, one per lane; kept ones are written in
   // buffer (scan) order.
   auto flush = [&](int j, int n) {
