@@ -1239,6 +1239,28 @@ static int timed_sweep(dida_letkf_ctx *c, const dida_letkf_obs *o, const double 
     return DIDA_OK;
 }
 
+/* Best of two sweeps' binning and selection stages alone, from the stage
+ * marks (ms). */
+static int timed_select(dida_letkf_ctx *c, const dida_letkf_obs *o, const double *xb,
+                        const double *axb, double *xa, double *axa, hipStream_t s, double *ms)
+{
+    *ms = 1e300;
+    const int timing = c->timing;
+    c->timing = 1;
+    int rc = DIDA_OK;
+    for (int rep = 0; rep < 2 && !rc; ++rep) {
+        rc = dida_letkf_sweep(c, o, xb, axb, xa, axa, s);
+        if (rc < 0) break;
+        rc = DIDA_OK;
+        double st[DIDA_NSTAGES];
+        rc = dida_letkf_stage_ms(c, st);
+        if (!rc && st[DIDA_STAGE_BIN] + st[DIDA_STAGE_SELECT] < *ms)
+            *ms = st[DIDA_STAGE_BIN] + st[DIDA_STAGE_SELECT];
+    }
+    c->timing = timing;
+    return rc;
+}
+
 int dida_letkf_autotune(dida_letkf_ctx *c, const dida_letkf_obs *o, const double *xb,
                         const double *axb, double *xa, double *axa, void *stream,
                         dida_letkf_tuning *out)
@@ -1268,9 +1290,24 @@ int dida_letkf_autotune(dida_letkf_ctx *c, const dida_letkf_obs *o, const double
     int best_bk = bucketed0;
     int32_t best_bl[DK_NBUCKET];
     memcpy(best_bl, bl0, sizeof best_bl);
-    rc = timed_sweep(c, o, xb, axb, xa, axa, s, e0, e1, &best_ms);
+    /* 0. the tile of columns of the selection passes, judged by the binning
+     * and selection stages alone (the whole sweep varies between runs by
+     * more than the tiles differ); the initial time is taken with it */
+    double best_sel = 1e300;
+    int32_t best_tile = c->sel_tile;
+    for (int32_t tt = DK_SELECT_TILE_4X2; tt <= DK_SELECT_TILE_4X4 && !rc; ++tt) {
+        c->sel_tile = tt;
+        double ms = 0.0;
+        rc = timed_select(c, o, xb, axb, xa, axa, s, &ms);
+        if (!rc && ms < best_sel) {
+            best_sel = ms;
+            best_tile = tt;
+        }
+    }
+    c->sel_tile = best_tile;
+    if (!rc) rc = timed_sweep(c, o, xb, axb, xa, axa, s, e0, e1, &best_ms);
     out->ms_initial = best_ms;
-    out->ncand = 1;
+    out->ncand = 4;
     const int batched = c->prm.form == DIDA_FORM_BATCHED;
     /* one candidate: time it and keep it if faster */
 #define TRY_LAYOUT(BD, CL, TL, BL)                                                       \
@@ -1289,21 +1326,6 @@ int dida_letkf_autotune(dida_letkf_ctx *c, const dida_letkf_obs *o, const double
             if (BL) memcpy(best_bl, (BL), sizeof best_bl);                               \
         }                                                                                \
     } while (0)
-    /* 0. the tile of columns of the selection passes */
-    const int32_t tile0 = c->sel_tile;
-    int32_t best_tile = tile0;
-    for (int32_t tt = DK_SELECT_TILE_4X2; tt <= DK_SELECT_TILE_4X4 && !rc; ++tt) {
-        if (tt == tile0) continue;
-        c->sel_tile = tt;
-        double ms = 0.0;
-        rc = timed_sweep(c, o, xb, axb, xa, axa, s, e0, e1, &ms);
-        out->ncand++;
-        if (!rc && ms < best_ms) {
-            best_ms = ms;
-            best_tile = tt;
-        }
-    }
-    c->sel_tile = best_tile;
     /* 1. the bin size (a capped column keeps the same obs at every size) */
     const double bins[3] = {c->theta_deg / 2.0, c->theta_deg / 4.0, c->theta_deg / 8.0};
     for (int ib = 0; ib < 3 && !rc; ++ib)

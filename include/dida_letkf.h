@@ -293,7 +293,9 @@ typedef struct dida_letkf_tuning {
     int32_t chain_len;          /* kept (batched form; 0 for the plain form) */
     int32_t ncand;              /* layouts timed */
     double bin_deg;             /* kept */
-    double ms_initial;          /* sweep time with the layout on entry */
+    double ms_initial;          /* sweep time with the layout on entry and the kept
+                                   selection tile (chosen first, by the binning and
+                                   selection stages alone) */
     double ms_best;             /* sweep time with the kept layout */
     int32_t xcd_tiles;          /* kept queue order (batched form) */
     int32_t bucketed;           /* 1: chain length per obs-count bucket (below) */
