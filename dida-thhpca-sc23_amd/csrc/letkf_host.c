@@ -411,7 +411,17 @@ int dida_letkf_create(dida_letkf_ctx **out, const dida_letkf_params *prm, int32_
     if (c->chained && prm->gram != DIDA_GRAM_FP64) c->ldy = (k + 3) & ~3; /* float4 rows */
     /* the split half-precision Grams read FP32 rows (their hi + lo parts
      * hold at most 22 bits) */
-    if (prm->form == DIDA_FORM_BATCHED && prm->gram != DIDA_GRAM_FP64)
+    const size_t row_bytes = prm->form == DIDA_FORM_BATCHED && prm->gram != DIDA_GRAM_FP64
+                                 ? sizeof(float) : sizeof(double);
+    /* the one-wavefront kernels address the rows with 32-bit byte offsets */
+    if (k <= 64 && !c->chained && no * (size_t)c->ldy * row_bytes > 0xFFFFFFFFull) {
+        dida_letkf_destroy(c);
+        return set_err(DIDA_ERR_UNSUPPORTED,
+                       "nobs_max = %lld: the binned rows exceed 4 GiB (at most %lld obs at k = %d)",
+                       (long long)no, (long long)(0xFFFFFFFFull / ((size_t)c->ldy * row_bytes)),
+                       k);
+    }
+    if (row_bytes == sizeof(float))
         DMALLOC(c->yb32_b, no * (size_t)c->ldy * sizeof(float));
     else
         DMALLOC(c->yb_b, no * (size_t)c->ldy * sizeof(double));

@@ -156,6 +156,20 @@ struct ObsMeta {
     sm.orr[buf][l] = rr;
     sm.ow[buf][l] = rr * (rr * dp);
   }
+  // The same for the split half-precision Grams, in FP32 and with the
+  // analysis's power-of-two scale applied to sqrt(R^-1 rloc), in the first
+  // half of the same LDS arrays (read as FP32 only by those Grams).
+  template <bool VERT, class SM>
+  __device__ __forceinline__ void store_weights32(SM& sm, int buf, const dk_batched_args& a,
+                                                  double clnp, double scale) const {
+    const int l = threadIdx.x;
+    double rv = rl;
+    if constexpr (VERT) rv = rl * gc_weight(fabs(clnp - lp) / a.vert_dist, a.gc_vert);
+    const double r2 = ok ? ri * rv : 0.0;
+    const double rr = sqrt(r2);
+    reinterpret_cast<float*>(sm.orr[buf])[l] = (float)(rr * scale);
+    reinterpret_cast<float*>(sm.ow[buf])[l] = (float)(rr * (rr * dp));
+  }
 };
 
 template <int KP, int GRAM>
@@ -178,20 +192,34 @@ struct Wave {
   }
 
   // Yb rows I*16 + c of the 8 obs of half h in buffer buf (binned rows are
-  // KP values apart and zero past k, so every row is in bounds and the I
-  // offsets are constants of one address per obs)
+  // KP values apart and zero past k, so every row is in bounds).  Buffer
+  // loads with a 32-bit byte offset a lane and obs, the I offsets in the
+  // instruction (the host keeps the rows below 4 GiB): one add per obs
+  // instead of 64-bit address arithmetic.
+  static constexpr int kRowShift = (KP == 64 ? 6 : 5) + (sizeof(RawT) == 8 ? 3 : 2);
   template <class SM>
   __device__ __forceinline__ void load_rows(const dk_batched_args& a, const SM& sm,
                                             int buf, int h, RawT (&raw)[8][NJ]) const {
     const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
-    const RawT* yb;
+    const void* yb;
     if constexpr (GRAM == DK_GRAM_FP64) yb = a.yb;
     else yb = a.yb32;
+    const __amdgpu_buffer_rsrc_t rows =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(yb), (short)0, 0xFFFFFFFF, 0x00020000);
+    const uint32_t lane_off = (uint32_t)c * (uint32_t)sizeof(RawT);
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
-      const RawT* y = yb + (int64_t)sm.ogi[buf][half_obs(h, e, g)] * KP + c;
+      const uint32_t o = ((uint32_t)sm.ogi[buf][half_obs(h, e, g)] << kRowShift) + lane_off;
 #pragma unroll
-      for (int I = 0; I < NJ; ++I) raw[e][I] = y[I * 16];
+      for (int I = 0; I < NJ; ++I) {
+        if constexpr (sizeof(RawT) == 8) {
+          raw[e][I] = __longlong_as_double((long long)__builtin_amdgcn_raw_buffer_load_b64(
+              rows, o + (uint32_t)(I * 16 * 8), 0, 0));
+        } else {
+          raw[e][I] = __int_as_float((int)__builtin_amdgcn_raw_buffer_load_b32(
+              rows, o + (uint32_t)(I * 16 * 4), 0, 0));
+        }
+      }
     }
   }
 
@@ -231,8 +259,10 @@ struct Wave {
 #pragma unroll
       for (int e = 0; e < 8; e += 2) {
         const int o0 = half_obs(h, e, g), o1 = half_obs(h, e + 1, g);
-        const f32x2_t rs = {(float)(sm.orr[buf][o0] * scale), (float)(sm.orr[buf][o1] * scale)};
-        const float w0 = (float)sm.ow[buf][o0], w1 = (float)sm.ow[buf][o1];
+        const float* orr = reinterpret_cast<const float*>(sm.orr[buf]);
+        const float* ow = reinterpret_cast<const float*>(sm.ow[buf]);
+        const f32x2_t rs = {orr[o0], orr[o1]};
+        const float w0 = ow[o0], w1 = ow[o1];
 #pragma unroll
         for (int I = 0; I < NJ; ++I) {
           bh[I] = fmaf(raw[e][I], w0, bh[I]);
@@ -325,8 +355,8 @@ struct Wave {
       MA.load_obs<VERT>(a);
       m0.store_index(sm, 0);
       m1.store_index(sm, 1);
-      m0.store_weights<VERT>(sm, 0, a, clnp);
-      m1.store_weights<VERT>(sm, 1, a, clnp);
+      m0.store_weights32<VERT>(sm, 0, a, clnp, scale);
+      m1.store_weights32<VERT>(sm, 1, a, clnp, scale);
       __syncthreads();
       RawT A0[8][NJ], A1[8][NJ], B0[8][NJ], B1[8][NJ];
       load_rows(a, sm, 0, 0, A0);
@@ -345,7 +375,7 @@ struct Wave {
         if (base + 32 < p) gram_half(sm, buf, 1, c1, scale, bacc, accd, acch);
         __syncthreads();  // block b's buffer has been read
         cur.store_index(sm, buf);
-        cur.store_weights<VERT>(sm, buf, a, clnp);
+        cur.store_weights32<VERT>(sm, buf, a, clnp, scale);
         __syncthreads();
       };
       for (int base = 0; base < p; base += 128) {
