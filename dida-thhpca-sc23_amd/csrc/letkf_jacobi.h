@@ -195,8 +195,16 @@ __device__ __forceinline__ double lane_xor(double v) {
 struct Jacobi128 {
   static constexpr int NR = 16;  // rows per thread
   double X[4][NR];               // column slots of the group
-  double nrm[4], sc[4], isc[4];  // squared norms, scales, inverse scales
+  double nrm[4], sc[4], isc[4];  // squared norms, scales, inverse scales (between sweeps)
   int id[4];                     // column index in A
+  // Within a sweep the state lives with the pair that is rotated: lanes 0-3
+  // of a group hold the top tp and bottom bt of the round's first pair,
+  // lanes 4-7 those of the second (row_half_mirror, lane i <-> 7 - i, trades
+  // them between the halves).
+  struct ColSt {
+    double nrm, sc, isc;
+  };
+  ColSt tp, bt;
 
   template <int P, int Q>
   __device__ __forceinline__ double dot() const {
@@ -226,10 +234,20 @@ struct Jacobi128 {
     isc[P] *= ic;
     isc[Q] *= ic;
   }
+  template <int P, int Q>
+  __device__ __forceinline__ void rotate_x(double ta, double tb) {
+#pragma unroll
+    for (int u = 0; u < NR; ++u) {
+      const double x = X[P][u], y = X[Q][u];
+      X[P][u] = fma(-ta, y, x);
+      X[Q][u] = fma(tb, x, y);
+    }
+  }
   // Lanes 0-3 of a group take the angle of the first pair, lanes 4-7 that
   // of the second: the quad sums of both dot products are exchanged across
   // the halves (row_half_mirror pairs lane i with 7 - i), each half finishes
-  // its own pair's sum and angle, and the angles come back the same way.
+  // its own pair's sum and angle from its state, updates that state, and the
+  // two FMA factors of each pair come back the same way.
   template <int T0, int B0, int T1, int B1>
   __device__ __forceinline__ void round(double tol2, double tolp, bool& rotated, bool& big) {
     double x = dot<T0, B0>(), y = dot<T1, B1>();
@@ -240,23 +258,62 @@ struct Jacobi128 {
     const bool second = (threadIdx.x & 4) != 0;
     const double gs = (second ? y : x) + dpp_d<0x141>(second ? x : y);
     double t, gm, cs, ic;
-    rotation_angle(gs, second ? nrm[T1] : nrm[T0], second ? nrm[B1] : nrm[B0],
-                   second ? sc[T1] : sc[T0], second ? sc[B1] : sc[B0], tol2, tolp, rotated, big,
-                   t, gm, cs, ic);
-    const double to = dpp_d<0x141>(t), gmo = dpp_d<0x141>(gm);
-    const double cso = dpp_d<0x141>(cs), ico = dpp_d<0x141>(ic);
-    apply<T0, B0>(second ? to : t, second ? gmo : gm, second ? cso : cs, second ? ico : ic);
-    apply<T1, B1>(second ? t : to, second ? gm : gmo, second ? cs : cso, second ? ic : ico);
+    rotation_angle(gs, tp.nrm, bt.nrm, tp.sc, bt.sc, tol2, tolp, rotated, big, t, gm, cs, ic);
+    const double ta = t * (bt.sc * tp.isc), tb = t * (tp.sc * bt.isc);
+    tp.nrm = fma(-t, gm, tp.nrm);
+    bt.nrm = fma(t, gm, bt.nrm);
+    tp.sc *= cs;
+    bt.sc *= cs;
+    tp.isc *= ic;
+    bt.isc *= ic;
+    const double tao = dpp_d<0x141>(ta), tbo = dpp_d<0x141>(tb);
+    rotate_x<T0, B0>(second ? tao : ta, second ? tbo : tb);
+    rotate_x<T1, B1>(second ? ta : tao, second ? tb : tbo);
+  }
+  // the bottoms' state trades halves (pairing (0, 1), (2, 3) <-> (0, 3), (2, 1))
+  __device__ __forceinline__ void swap_bt() {
+    bt.nrm = dpp_d<0x141>(bt.nrm);
+    bt.sc = dpp_d<0x141>(bt.sc);
+    bt.isc = dpp_d<0x141>(bt.isc);
+  }
+  // (0, 3), (2, 1) -> (0, 2), (3, 1): lanes 0-3 keep their top (column 0) and
+  // take the top of lanes 4-7 (column 2) as bottom; lanes 4-7 take the bottom
+  // of lanes 0-3 (column 3) as top and keep their bottom (column 1)
+  __device__ __forceinline__ void cross_tb() {
+    const bool second = (threadIdx.x & 4) != 0;
+    const ColSt mt{dpp_d<0x141>(tp.nrm), dpp_d<0x141>(tp.sc), dpp_d<0x141>(tp.isc)};
+    const ColSt mb{dpp_d<0x141>(bt.nrm), dpp_d<0x141>(bt.sc), dpp_d<0x141>(bt.isc)};
+    if (second) tp = mb;
+    else bt = mt;
+  }
+  // the pair state at a sweep's start (lanes 0-3: slots 0, 1; 4-7: 2, 3)
+  __device__ __forceinline__ void state_start() {
+    const bool second = (threadIdx.x & 4) != 0;
+    tp = ColSt{second ? nrm[2] : nrm[0], 1.0, 1.0};
+    bt = ColSt{second ? nrm[3] : nrm[1], 1.0, 1.0};
+  }
+  // the scales of all four slots at a sweep's end (lanes 0-3 hold columns
+  // 0 and 2, lanes 4-7 columns 3 and 1)
+  __device__ __forceinline__ void state_end() {
+    const bool second = (threadIdx.x & 4) != 0;
+    const double mt = dpp_d<0x141>(tp.sc), mb = dpp_d<0x141>(bt.sc);
+    sc[0] = second ? mt : tp.sc;
+    sc[1] = second ? bt.sc : mb;
+    sc[2] = second ? mb : bt.sc;
+    sc[3] = second ? tp.sc : mt;
   }
   // slot J takes slot J of the group V / 8 groups away (lanes l ^ V)
   template <int J, int V>
   __device__ __forceinline__ void move_lane() {
 #pragma unroll
     for (int u = 0; u < NR; ++u) X[J][u] = lane_xor<V>(X[J][u]);
-    nrm[J] = lane_xor<V>(nrm[J]);
-    sc[J] = lane_xor<V>(sc[J]);
-    isc[J] = lane_xor<V>(isc[J]);
     id[J] = (int)lane_xor_u<V>((unsigned)id[J]);
+  }
+  template <int V>
+  __device__ __forceinline__ void move_bt_lane() {
+    bt.nrm = lane_xor<V>(bt.nrm);
+    bt.sc = lane_xor<V>(bt.sc);
+    bt.isc = lane_xor<V>(bt.isc);
   }
   // the column of slot J (q: 0 or 1 of the group's two exchange places) to
   // the exchange area, and back from group src
@@ -264,44 +321,58 @@ struct Jacobi128 {
   __device__ __forceinline__ void put(JacobiXbuf& xb, int gp, int sub, int q) const {
 #pragma unroll
     for (int u = 0; u < NR; ++u) xb.col[(2 * gp + q) * JacobiXbuf::LD + 8 * u + sub] = X[J][u];
-    if (sub == 0) {
-      xb.nrm[2 * gp + q] = nrm[J];
-      xb.sc[2 * gp + q] = sc[J];
-      xb.isc[2 * gp + q] = isc[J];
-      xb.id[2 * gp + q] = id[J];
-    }
+    if (sub == 0) xb.id[2 * gp + q] = id[J];
   }
   template <int J>
   __device__ __forceinline__ void get(const JacobiXbuf& xb, int src, int sub, int q) {
 #pragma unroll
     for (int u = 0; u < NR; ++u) X[J][u] = xb.col[(2 * src + q) * JacobiXbuf::LD + 8 * u + sub];
-    nrm[J] = xb.nrm[2 * src + q];
-    sc[J] = xb.sc[2 * src + q];
-    isc[J] = xb.isc[2 * src + q];
     id[J] = xb.id[2 * src + q];
   }
-  // slots 1 and 3 from group gp ^ v
+  // a half's state c to exchange place q of the group (lanes 0 and 4 write)
+  __device__ __forceinline__ static void put_st(JacobiXbuf& xb, int gp, int sub, int q,
+                                                const ColSt& c) {
+    if ((sub & 3) == 0) {
+      xb.nrm[2 * gp + q] = c.nrm;
+      xb.sc[2 * gp + q] = c.sc;
+      xb.isc[2 * gp + q] = c.isc;
+    }
+  }
+  __device__ __forceinline__ static void get_st(const JacobiXbuf& xb, int src, int q, ColSt& c) {
+    c.nrm = xb.nrm[2 * src + q];
+    c.sc = xb.sc[2 * src + q];
+    c.isc = xb.isc[2 * src + q];
+  }
+  // slots 1 and 3 from group gp ^ v with their state (v >= 8, through LDS:
+  // in the pairing (0, 1), (2, 3), where the bottom state of lanes 0-3 is
+  // slot 1's and of lanes 4-7 slot 3's)
   __device__ __forceinline__ void move_bottoms(JacobiXbuf& xb, int gp, int sub, int v) {
     if (v == 1) {
       move_lane<1, 8>();
       move_lane<3, 8>();
+      move_bt_lane<8>();
     } else if (v == 2) {
       move_lane<1, 16>();
       move_lane<3, 16>();
+      move_bt_lane<16>();
     } else if (v == 4) {
       move_lane<1, 32>();
       move_lane<3, 32>();
+      move_bt_lane<32>();
     } else {
+      const int qh = sub >> 2;  // the half's exchange place: slot 1 or 3
       put<1>(xb, gp, sub, 0);
       put<3>(xb, gp, sub, 1);
+      put_st(xb, gp, sub, qh, bt);
       __syncthreads();
       get<1>(xb, gp ^ v, sub, 0);
       get<3>(xb, gp ^ v, sub, 1);
+      get_st(xb, gp ^ v, qh, bt);
       __syncthreads();
     }
   }
   // first: slot B takes the partner's slot T; otherwise slot T takes the
-  // partner's slot B (partner: lanes l ^ V)
+  // partner's slot B (partner: lanes l ^ V); the pair's state alike
   template <int T, int B, int V>
   __device__ __forceinline__ void xswap_lane(bool first) {
 #pragma unroll
@@ -310,46 +381,57 @@ struct Jacobi128 {
       X[B][u] = first ? w : X[B][u];
       X[T][u] = first ? X[T][u] : w;
     }
-    double w = lane_xor<V>(first ? nrm[B] : nrm[T]);
-    nrm[B] = first ? w : nrm[B];
-    nrm[T] = first ? nrm[T] : w;
-    w = lane_xor<V>(first ? sc[B] : sc[T]);
-    sc[B] = first ? w : sc[B];
-    sc[T] = first ? sc[T] : w;
-    w = lane_xor<V>(first ? isc[B] : isc[T]);
-    isc[B] = first ? w : isc[B];
-    isc[T] = first ? isc[T] : w;
     const int wi = (int)lane_xor_u<V>((unsigned)(first ? id[B] : id[T]));
     id[B] = first ? wi : id[B];
     id[T] = first ? id[T] : wi;
   }
-  // (0, 1) and (2, 3) swapped with group gp ^ d
+  template <int V>
+  __device__ __forceinline__ void xswap_st_lane(bool first) {
+    double w = lane_xor<V>(first ? bt.nrm : tp.nrm);
+    bt.nrm = first ? w : bt.nrm;
+    tp.nrm = first ? tp.nrm : w;
+    w = lane_xor<V>(first ? bt.sc : tp.sc);
+    bt.sc = first ? w : bt.sc;
+    tp.sc = first ? tp.sc : w;
+    w = lane_xor<V>(first ? bt.isc : tp.isc);
+    bt.isc = first ? w : bt.isc;
+    tp.isc = first ? tp.isc : w;
+  }
+  // (0, 1) and (2, 3) swapped with group gp ^ d, in the pairing (0, 1), (2, 3)
   __device__ __forceinline__ void xswap(JacobiXbuf& xb, int gp, int sub, int d) {
     const bool first = (gp & d) == 0;
     if (d == 1) {
       xswap_lane<0, 1, 8>(first);
       xswap_lane<2, 3, 8>(first);
+      xswap_st_lane<8>(first);
     } else if (d == 2) {
       xswap_lane<0, 1, 16>(first);
       xswap_lane<2, 3, 16>(first);
+      xswap_st_lane<16>(first);
     } else if (d == 4) {
       xswap_lane<0, 1, 32>(first);
       xswap_lane<2, 3, 32>(first);
+      xswap_st_lane<32>(first);
     } else {
+      const int qh = sub >> 2;  // the half's pair: (0, 1) or (2, 3)
       if (first) {
         put<1>(xb, gp, sub, 0);
         put<3>(xb, gp, sub, 1);
+        put_st(xb, gp, sub, qh, bt);
       } else {
         put<0>(xb, gp, sub, 0);
         put<2>(xb, gp, sub, 1);
+        put_st(xb, gp, sub, qh, tp);
       }
       __syncthreads();
       if (first) {
         get<1>(xb, gp ^ d, sub, 0);
         get<3>(xb, gp ^ d, sub, 1);
+        get_st(xb, gp ^ d, qh, bt);
       } else {
         get<0>(xb, gp ^ d, sub, 0);
         get<2>(xb, gp ^ d, sub, 1);
+        get_st(xb, gp ^ d, qh, tp);
       }
       __syncthreads();
     }
@@ -403,21 +485,27 @@ __device__ int jacobi_regs128(double* A, int* flag, double tol, int max_sweeps, 
       }
       j.nrm[J] = oct_sum(a0 + a1);
     }
+    j.state_start();
     bool rotated = false, big = false;
     for (int G = 64; G >= 4; G >>= 1) {
       for (int r = 0; r < G; r += 4) {
         j.round<0, 1, 2, 3>(tol2, tolp, rotated, big);
+        j.swap_bt();
         j.round<0, 3, 2, 1>(tol2, tolp, rotated, big);
-        j.move_bottoms(xb, gp, sub, 1);
+        j.move_bottoms(xb, gp, sub, 1);  // lanes l ^ 8: the bottom state moves along
         j.round<0, 3, 2, 1>(tol2, tolp, rotated, big);
+        j.swap_bt();
         j.round<0, 1, 2, 3>(tol2, tolp, rotated, big);
         if (r + 4 < G) j.move_bottoms(xb, gp, sub, ((r + 4) & -(r + 4)) >> 1);
       }
       j.xswap(xb, gp, sub, G >> 2);
     }
     j.round<0, 1, 2, 3>(tol2, tolp, rotated, big);
+    j.swap_bt();
     j.round<0, 3, 2, 1>(tol2, tolp, rotated, big);
+    j.cross_tb();
     j.round<0, 2, 3, 1>(tol2, tolp, rotated, big);
+    j.state_end();
     const bool any_rot = __any(rotated), any_big = __any(big);
     if (lane == 0) {
       if (any_rot) flag[sweep & 1] = 1;
