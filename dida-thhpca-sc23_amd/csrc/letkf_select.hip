@@ -266,21 +266,29 @@ __device__ __forceinline__ void ref_weight(const dk_select_args& a, const ColGeo
 // 1e-18; unlike acos(p . q) it keeps its digits at short distances.  |d| is
 // c2 / sqrt(c2) from the Newton-refined reciprocal square root (a few ulp).
 constexpr double kChordMaxDeg = 22.9;  // 0.4 rad
+// a * b + c as the three-operand FP64 FMA: left to itself the compiler takes
+// the two-operand form, whose accumulator is the addend, and copies each
+// loop-invariant series coefficient into it first.
+__device__ __forceinline__ double fma3(double a, double b, double c) {
+  double r;
+  asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  return r;
+}
 __device__ __forceinline__ double chord_angle(double dx, double dy, double dz) {
   const double c2 = fma(dx, dx, fma(dy, dy, dz * dz));
   const double x2 = 0.25 * c2;
-  double s = 0.006447210311889649;      // 676039 / 104857600
-  s = fma(s, x2, 0.0073125258735988454);  // 88179 / 12058624
-  s = fma(s, x2, 0.008390335809616815);   // 46189 / 5505024
-  s = fma(s, x2, 0.009761609529194078);   // 12155 / 1245184
-  s = fma(s, x2, 0.011551800896139705);   // 6435 / 557056
-  s = fma(s, x2, 0.01396484375);          // 143 / 10240
-  s = fma(s, x2, 0.017352764423076924);   // 231 / 13312
-  s = fma(s, x2, 0.022372159090909092);   // 63 / 2816
-  s = fma(s, x2, 0.030381944444444444);   // 35 / 1152
-  s = fma(s, x2, 0.044642857142857144);   // 5 / 112
-  s = fma(s, x2, 0.075);                  // 3 / 40
-  s = fma(s, x2, 0.16666666666666666);    // 1 / 6
+  double s = 0.006447210311889649;         // 676039 / 104857600
+  s = fma3(s, x2, 0.0073125258735988454);  // 88179 / 12058624
+  s = fma3(s, x2, 0.008390335809616815);   // 46189 / 5505024
+  s = fma3(s, x2, 0.009761609529194078);   // 12155 / 1245184
+  s = fma3(s, x2, 0.011551800896139705);   // 6435 / 557056
+  s = fma3(s, x2, 0.01396484375);          // 143 / 10240
+  s = fma3(s, x2, 0.017352764423076924);   // 231 / 13312
+  s = fma3(s, x2, 0.022372159090909092);   // 63 / 2816
+  s = fma3(s, x2, 0.030381944444444444);   // 35 / 1152
+  s = fma3(s, x2, 0.044642857142857144);   // 5 / 112
+  s = fma3(s, x2, 0.075);                  // 3 / 40
+  s = fma3(s, x2, 0.16666666666666666);    // 1 / 6
   s = fma(s, x2, 1.0);
   const double c = c2 > 0.0 ? c2 * rsq_nr(c2) : 0.0;  // |d|
   return c * s;                           // 2 x s
@@ -352,6 +360,7 @@ constexpr int kNT = 16;
 struct TileCols {
   double cx2[kNT], sx2[kNT], y2r[kNT], ux[kNT], uy[kNT], uz[kNT];
   int64_t out[kNT];
+  float4 v[kNT];  // FP32 unit vector (read by the scan as one broadcast)
 };
 
 // Fill pass: per column, the candidates that passed the FP32 test, in scan
@@ -388,6 +397,29 @@ __device__ __forceinline__ float lane_get(float v, int j) {
 }
 __device__ __forceinline__ int lane_set(int v, int j, int x) {
   return (int)(threadIdx.x & 63) == j ? x : v;
+}
+// Ballot and any-lane of a condition as a lane mask, without widening it to
+// an int first (HIP's __any and __ballot take an int).
+__device__ __forceinline__ uint64_t ballot(bool b) { return __builtin_amdgcn_ballot_w64(b); }
+__device__ __forceinline__ bool any_lane(bool b) { return __builtin_amdgcn_ballot_w64(b) != 0; }
+// A wave-uniform mask in scalar registers: the compiler cannot always prove
+// a ballot-derived value uniform after divergent stores, and would run the
+// loops over it with exec masks and vector bit scans.
+__device__ __forceinline__ uint32_t uniform_u32(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+}
+__device__ __forceinline__ int64_t uniform_i64(int64_t v) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)v >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+using u32x2 = unsigned int __attribute__((__vector_size__(2 * sizeof(unsigned int))));
+// Raw buffer over p (32-bit byte offsets, no range clamp below 4 GiB).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* p) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, 0xFFFFFFFF, 0x00020000);
+}
+__device__ __forceinline__ double buf_f64(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  return __longlong_as_double((long long)__builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
 }
 
 // Count (kFill false: nobsl[col] = min(kept, max_obs)) or fill (kept obs with
@@ -429,7 +461,7 @@ __global__ __launch_bounds__(256) void k_select_tiles(dk_select_args a) {
   const int ilat = tj * kTJ + jj, ilon = ti * kTI + ii;
   const int64_t col = (int64_t)ilat + (int64_t)a.nlat * ilon;
   const bool mine = lane < kNTT && ilat < a.nlat && ilon < a.nlon && col >= c_lo && col < c_hi;
-  if (!__any(mine)) return;
+  if (!any_lane(mine)) return;
   double clat = 0.0, clon = 0.0;
   if (mine) {
     clat = (double)a.grid_lat[ilat];
@@ -437,7 +469,7 @@ __global__ __launch_bounds__(256) void k_select_tiles(dk_select_args a) {
   }
   // the union of the columns' regions, longitudes unwrapped around the first
   // column's
-  const int src = __ffsll((unsigned long long)__ballot(mine)) - 1;
+  const int src = __ffsll((unsigned long long)ballot(mine)) - 1;
   const double ref = wrap360(__shfl(clon, src, kWave));
   double cw = wrap360(clon);
   if (cw - ref > 180.0) cw -= 360.0;
@@ -446,7 +478,7 @@ __global__ __launch_bounds__(256) void k_select_tiles(dk_select_args a) {
   CandRegion reg;
   reg.b0 = wave_min_i(mine ? rc.b0 : 1 << 30);
   reg.b1 = wave_max_i(mine ? rc.b1 : -1);
-  reg.full = __any(mine && rc.full);
+  reg.full = any_lane(mine && rc.full);
   reg.lo_deg = wave_min(mine ? rc.lo_deg : 1e300);
   reg.hi_deg = wave_max(mine ? rc.hi_deg : -1e300);
   const ColGeo q = col_geo(clat, clon);
@@ -458,21 +490,23 @@ __global__ __launch_bounds__(256) void k_select_tiles(dk_select_args a) {
     tc.uy[lane] = q.uy;
     tc.uz[lane] = q.uz;
     tc.out[lane] = kFill && mine ? a.offset[col] - a.base : 0;
+    tc.v[lane] = make_float4(q.vx, q.vy, q.vz, 0.0f);
     if (kFill && mine && a.offset[col] - a.base + a.nobsl[col] > a.capacity) *a.overflow = 1;
   }
-  // lane-wise column state: FP32 unit vector, how many obs the column takes
-  // (0 for lanes that are not columns of this run), caller-index threshold,
-  // kept so far, candidates waiting in the buffer
-  const float cvx = q.vx, cvy = q.vy, cvz = q.vz;
+  // lane-wise column state: how many obs the column takes (0 for lanes that
+  // are not columns of this run), caller-index threshold, kept so far,
+  // candidates waiting in the buffer
   const int climit = mine ? (kFill ? a.nobsl[col] : a.max_obs) : 0;
   const int cthr = kFill && mine ? a.thr[col] : INT32_MAX;
   int ccnt = 0, cnb = 0;
   wave_sync_lds();
-  const bool capped = __any(cthr != INT32_MAX);
+  const bool capped = any_lane(cthr != INT32_MAX);
   // columns still counting / filling (bit j)
-  uint32_t open = (uint32_t)__ballot(climit > 0);
+  uint32_t open = uniform_u32((uint32_t)ballot(climit > 0));
   const float flo = far_lo32(a), nhi = near_hi32(a);
   const double r_per_rad = kEarthA / a.local_dist;
+  const __amdgpu_buffer_rsrc_t rux = buf_rsrc(a.obs.ux), ruy = buf_rsrc(a.obs.uy),
+                               ruz = buf_rsrc(a.obs.uz);
   // The test of the lanes' shell candidates l against column j.
   auto shell_j = [&](int j, int l, bool shell) -> bool {
     return shell && shell_keep(a, tc.ux[j], tc.uy[j], tc.uz[j], l);
@@ -481,36 +515,54 @@ __global__ __launch_bounds__(256) void k_select_tiles(dk_select_args a) {
   // buffered candidates of column j, one per lane; kept ones are written in
   // buffer (scan) order.
   auto flush = [&](int j, int n) {
-    const int lb = lane < n ? tb.l[j][lane] : 0;
     const bool live = lane < n;
-    const bool shell = live && lb < 0;
-    const int l = lb < 0 ? ~lb : lb;
-    bool keep = live && !shell;
-    if (__any(shell)) keep = keep || shell_j(j, l, shell);
-    double gc = 0.0;
-    if (keep) {
-      ColGeo g;
-      if constexpr (kChord) {
-        g.ux = tc.ux[j];
-        g.uy = tc.uy[j];
-        g.uz = tc.uz[j];
-        gc = inner_weight(a, g, l, r_per_rad);
-      } else {
-        g.cx2 = tc.cx2[j];
-        g.sx2 = tc.sx2[j];
-        g.y2r = tc.y2r[j];
-        ref_weight(a, g, l, gc);
+    const int lb = live ? tb.l[j][lane] : 0;
+    const int l = lb < 0 ? ~lb : lb;  // lanes past n: obs 0, never stored
+    const uint64_t mlive = n >= kWave ? ~0ull : (1ull << n) - 1;
+    const uint64_t mshell = ballot(lb < 0) & mlive;
+    // kept: the candidates surely inside, and those of the shell that pass
+    uint64_t bm = mlive & ~mshell;
+    bool keep = live & (lb >= 0);
+    double gc;
+    if constexpr (kChord) {
+      // each candidate's FP64 unit vector serves the shell test and the
+      // weight (binned index < nobs < 2^29: 32-bit byte offsets)
+      const uint32_t off = (uint32_t)l << 3;
+      const double px = buf_f64(rux, off), py = buf_f64(ruy, off), pz = buf_f64(ruz, off);
+      const double qx = tc.ux[j], qy = tc.uy[j], qz = tc.uz[j];
+      if (mshell) {
+        // shell_keep's cosine
+        const bool sk = live & (lb < 0) & (fma(px, qx, fma(py, qy, pz * qz)) >= a.cos_keep);
+        bm |= ballot(sk);
+        keep = keep | sk;
       }
+      gc = gc_weight_inner(chord_angle(px - qx, py - qy, pz - qz) * r_per_rad, a.gc_filter);
+    } else {
+      if (mshell) {
+        const bool sk = live & (lb < 0) && shell_keep(a, tc.ux[j], tc.uy[j], tc.uz[j], l);
+        bm |= ballot(sk);
+        keep = keep | sk;
+      }
+      ColGeo g;
+      g.cx2 = tc.cx2[j];
+      g.sx2 = tc.sx2[j];
+      g.y2r = tc.y2r[j];
+      ref_weight(a, g, l, gc);
     }
-    const uint64_t bm = __ballot(keep);
     const int c0 = lane_get(ccnt, j), lim = lane_get(climit, j);
+    // the column's run of the lists, entries past the workspace dropped
+    // (the overflow flag is set); 32-bit offsets from a scalar base
+    const int64_t oj = uniform_i64(tc.out[j]);
+    const int64_t room = a.capacity - oj;
+    const int lim_w = room < (int64_t)lim ? (room > 0 ? (int)room : 0) : lim;
     if (keep) {
       const int pos = c0 + (int)__builtin_amdgcn_mbcnt_hi(
                                (uint32_t)(bm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bm, 0u));
-      const int64_t o = tc.out[j] + pos;
-      if (pos < lim && o < a.capacity) {
-        a.csr_idx[o] = l;
-        a.csr_w[o] = gc;
+      if (pos < lim_w) {
+        __builtin_amdgcn_raw_buffer_store_b32((uint32_t)l, buf_rsrc(a.csr_idx + oj),
+                                              (uint32_t)pos * 4u, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, gc),
+                                              buf_rsrc(a.csr_w + oj), (uint32_t)pos * 8u, 0, 0);
       }
     }
     const int c1 = c0 + __popcll(bm);
@@ -525,16 +577,21 @@ __global__ __launch_bounds__(256) void k_select_tiles(dk_select_args a) {
         const float4 u = valid ? reinterpret_cast<const float4*>(a.obs.u4)[l]
                                : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         const int32_t pv = kFill && capped && valid ? a.perm[l] : 0;
-        for (uint32_t m = open; m; m &= m - 1) {
+        // the FP32 test's bounds, out of reach for lanes past the range
+        // (their d is 0), so that each test is one compare
+        const float flo_l = valid ? flo : 3.0f, nhi_l = valid ? nhi : 3.0f;
+        for (uint32_t m = uniform_u32(open); m; m &= m - 1) {
           const int j = __builtin_ctz(m);
-          const float d = dot32(u, lane_get(cvx, j), lane_get(cvy, j), lane_get(cvz, j));
+          const float4 cv = tc.v[j];
+          const float d = dot32(u, cv.x, cv.y, cv.z);
           if constexpr (kFill) {
-            // a capped column's caller-index threshold first, then the FP32
-            // test; survivors wait in the column's buffer
-            const int32_t thr = lane_get(cthr, j);
-            const bool cand = valid && d >= flo && (thr == INT32_MAX || pv < thr);
-            const uint64_t bm = __ballot(cand);
+            // a capped column's caller-index threshold (INT32_MAX, above
+            // every caller index, for the others), then the FP32 test;
+            // survivors wait in the column's buffer
+            uint64_t bm = ballot(d >= flo_l);
+            if (capped) bm &= ballot(pv < lane_get(cthr, j));
             if (!bm) continue;
+            const bool cand = (d >= flo_l) & (!capped | (pv < lane_get(cthr, j)));
             const int nb0 = lane_get(cnb, j);
             if (cand)
               tb.l[j][nb0 + (int)__builtin_amdgcn_mbcnt_hi(
@@ -553,12 +610,12 @@ __global__ __launch_bounds__(256) void k_select_tiles(dk_select_args a) {
             }
             cnb = lane_set(cnb, j, nb);
           } else {
-            const bool cand = valid && d >= flo;
-            if (!__any(cand)) continue;
-            bool keep = cand && d > nhi;
-            const bool shell = cand && !keep;
-            if (__any(shell)) keep = keep || shell_j(j, l, shell);
-            const int c1 = lane_get(ccnt, j) + __popcll(__ballot(keep));
+            const bool cand = d >= flo_l;
+            if (!any_lane(cand)) continue;
+            bool keep = d > nhi_l;
+            const bool shell = cand & !keep;
+            if (any_lane(shell)) keep = keep || shell_j(j, l, shell);
+            const int c1 = lane_get(ccnt, j) + __popcll(ballot(keep));
             ccnt = lane_set(ccnt, j, c1);
             if (c1 >= lane_get(climit, j)) open &= ~(1u << j);
           }
@@ -570,7 +627,7 @@ __global__ __launch_bounds__(256) void k_select_tiles(dk_select_args a) {
     if constexpr (kFill) {
       // the candidates still waiting
       wave_sync_lds();
-      for (uint32_t m = open; m; m &= m - 1) {
+      for (uint32_t m = uniform_u32(open); m; m &= m - 1) {
         const int j = __builtin_ctz(m);
         const int nb = lane_get(cnb, j);
         if (nb > 0) flush(j, nb);
@@ -652,8 +709,7 @@ __global__ __launch_bounds__(256) void k_select_cap(dk_select_args a) {
   // (chunks dealt round-robin) and one histogram; wavefront 0 picks the
   // digit and enumerates
   __shared__ uint32_t hist[kCapBins];
-  __shared__ int part[4];
-  __shared__ int sel[3];
+  __shared__ int sel[4];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t col = blockIdx.x;
   if (a.max_obs <= 0 || a.nobsl[col] < a.max_obs) {
@@ -665,6 +721,7 @@ __global__ __launch_bounds__(256) void k_select_cap(dk_select_args a) {
   const double clon = (double)a.grid_lon[ilon];
   const ColGeo q = col_geo(clat, clon);
   const CandRegion reg = column_region(a, clat, wrap360(clon));
+  const float flo = far_lo32(a), nhi = near_hi32(a);
   int width = a.perm_bits;  // caller indices < 2^perm_bits
   uint32_t lo = 0;
   int rank = a.max_obs;
@@ -676,35 +733,38 @@ __global__ __launch_bounds__(256) void k_select_cap(dk_select_args a) {
     for (int i = (int)threadIdx.x; i < nb; i += 256) hist[i] = 0u;
     __syncthreads();
     const uint32_t span = width >= 31 ? 0x80000000u : (1u << width);
-    int total = 0;
     for_region_ranges(a, reg, [&](int l0, int l1) -> bool {
       // kCapUnroll chunks a step, their loads issued together
       for (int base = l0 + wave * kCapUnroll * kWave; base < l1;
            base += 4 * kCapUnroll * kWave) {
-        uint32_t rel[kCapUnroll];
+        // indices clamped into the range, so that the loads need no branch
+        // and all issue before the first use
+        int32_t pv[kCapUnroll];
         float4 u[kCapUnroll];
 #pragma unroll
         for (int k = 0; k < kCapUnroll; ++k) {
           const int l = base + k * kWave + lane;
-          const bool valid = l < l1;
-          rel[k] = valid ? (uint32_t)a.perm[l] - lo : span;
-          u[k] = valid ? reinterpret_cast<const float4*>(a.obs.u4)[l]
-                       : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+          const int lc = l < l1 ? l : l1 - 1;
+          pv[k] = a.perm[lc];
+          u[k] = reinterpret_cast<const float4*>(a.obs.u4)[lc];
         }
 #pragma unroll
         for (int k = 0; k < kCapUnroll; ++k) {
           const int l = base + k * kWave + lane;
-          const bool keep = rel[k] < span && keep_candidate(a, q, l, u[k]);
-          if (keep) atomicAdd(&hist[rel[k] >> shift], 1u);
-          total += __popcll(__ballot(keep));
+          // keep_candidate's tests as single compares; lanes past the range
+          // or outside the caller-index range have rel = span
+          const uint32_t rel = l < l1 ? (uint32_t)pv[k] - lo : span;
+          const float d = dot32(u[k], q.vx, q.vy, q.vz);
+          const bool in = rel < span;
+          bool keep = in & (d > nhi);
+          if (ballot(in) & ballot(d >= flo) & ~ballot(d > nhi))
+            keep = keep || ((in & (d >= flo) & !(d > nhi)) && shell_keep(a, q.ux, q.uy, q.uz, l));
+          if (keep) atomicAdd(&hist[rel >> shift], 1u);
         }
       }
       return true;
     });
-    if (lane == 0) part[wave] = total;
     __syncthreads();
-    total = part[0] + part[1] + part[2] + part[3];
-    if (level == 0 && total <= a.max_obs) break;  // every kept obs fits
     // the digit whose cumulative count reaches rank: bins split over lanes,
     // a wave scan of the lane sums, then the owning lane walks its bins
     if (wave == 0) {
@@ -719,7 +779,10 @@ __global__ __launch_bounds__(256) void k_select_cap(dk_select_args a) {
         if (lane >= o) inc += t;
       }
       const int exc = inc - sum;
-      const uint64_t m = __ballot(exc < rank && rank <= inc);
+      // the level's kept obs; at the first level all of them, and a column
+      // whose kept obs all fit needs no threshold
+      if (lane == kWave - 1) sel[3] = inc;
+      const uint64_t m = ballot(exc < rank && rank <= inc);
       const int src = m ? __ffsll((unsigned long long)m) - 1 : 0;
       if (lane == src) {
         int cum = exc, dsel = 0, before = 0;
@@ -738,6 +801,7 @@ __global__ __launch_bounds__(256) void k_select_cap(dk_select_args a) {
       }
     }
     __syncthreads();
+    if (level == 0 && sel[3] <= a.max_obs) break;  // every kept obs fits
     if (!sel[2]) break;
     lo += (uint32_t)sel[0] << shift;
     rank -= sel[1];
@@ -759,13 +823,13 @@ __global__ __launch_bounds__(256) void k_select_cap(dk_select_args a) {
             keep = l < polar0 &&
                    keep_candidate(a, q, l, reinterpret_cast<const float4*>(a.obs.u4)[l]);
           }
-          const uint64_t km = __ballot(keep);
+          const uint64_t km = ballot(keep);
           const int cnt = __popcll(km);
           if (cum + cnt >= rank) {
             // the lane holding the (rank - cum)-th kept obs of the chunk
             const int below = (int)__builtin_amdgcn_mbcnt_hi(
                 (uint32_t)(km >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)km, 0u));
-            const uint64_t hit = __ballot(keep && below == rank - cum - 1);
+            const uint64_t hit = ballot(keep && below == rank - cum - 1);
             thr = (int32_t)(lo + base + (uint32_t)(__ffsll((unsigned long long)hit) - 1) + 1u);
             break;
           }
@@ -928,6 +992,8 @@ extern "C" int dk_select_cap(const dk_select_args* a, void* stream) {
 extern "C" int dk_select_fill(const dk_select_args* a, void* stream) {
   const int64_t ncol = a->ncols > 0 ? a->ncols : (int64_t)a->nlat * a->nlon;
   if (ncol <= 0) return 0;
+  // the fill's 32-bit byte offsets (binned obs, a column's entries)
+  if (a->nobs >= (1 << 29)) return (int)hipErrorInvalidValue;
   return select_tiles<true>(a, (hipStream_t)stream);
 }
 
