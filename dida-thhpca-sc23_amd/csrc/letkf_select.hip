@@ -570,15 +570,26 @@ __global__ __launch_bounds__(256) void k_select_tiles(dk_select_args a) {
     if (c1 >= lim) open &= ~(1u << j);
   };
   if (open) {
+    const float4* __restrict__ u4 = reinterpret_cast<const float4*>(a.obs.u4);
     for_region_ranges(a, reg, [&](int l0, int l1) -> bool {
+      // (ranges of empty bins come too: nothing to load)
+      if (l0 >= l1) return true;
+      // each chunk's unit vectors (and caller indices) are loaded while the
+      // one before is tested; indices clamped into the range need no branch
+      float4 u_next = u4[l0 + lane < l1 ? l0 + lane : l1 - 1];
+      int32_t pv_next = kFill && capped ? a.perm[l0 + lane < l1 ? l0 + lane : l1 - 1] : 0;
       for (int base = l0; base < l1; base += kWave) {
         const int l = base + lane;
         const bool valid = l < l1;
-        const float4 u = valid ? reinterpret_cast<const float4*>(a.obs.u4)[l]
-                               : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        const int32_t pv = kFill && capped && valid ? a.perm[l] : 0;
-        // the FP32 test's bounds, out of reach for lanes past the range
-        // (their d is 0), so that each test is one compare
+        const float4 u = u_next;
+        const int32_t pv = pv_next;
+        if (base + kWave < l1) {
+          const int ln = l + kWave < l1 ? l + kWave : l1 - 1;
+          u_next = u4[ln];
+          if (kFill && capped) pv_next = a.perm[ln];
+        }
+        // the FP32 test's bounds, out of reach for lanes past the range, so
+        // that each test is one compare
         const float flo_l = valid ? flo : 3.0f, nhi_l = valid ? nhi : 3.0f;
         for (uint32_t m = uniform_u32(open); m; m &= m - 1) {
           const int j = __builtin_ctz(m);
