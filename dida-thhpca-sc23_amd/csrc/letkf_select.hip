@@ -262,36 +262,46 @@ __device__ __forceinline__ void ref_weight(const dk_select_args& a, const ColGeo
 // Central angle (rad) between unit vectors p and q from their difference d
 // = p - q: |d| = 2 sin(theta / 2), so theta = 2 asin(|d| / 2), summed as the
 // Maclaurin series of asin, x sum_k (2k)! / (4^k (k!)^2 (2k + 1)) x^(2k).
-// For theta <= 0.4 (x^2 < 0.04) thirteen terms leave a relative error below
-// 1e-18; unlike acos(p . q) it keeps its digits at short distances.  |d| is
+// kTerms = 13 for theta <= 0.4 (x^2 < 0.04), 8 for theta <= 0.197 (x^2 <
+// 0.0097): the first term left out is below 1e-18 of the sum either way.
+// Unlike acos(p . q) it keeps its digits at short distances.  |d| is
 // c2 / sqrt(c2) from the Newton-refined reciprocal square root (a few ulp).
-constexpr double kChordMaxDeg = 22.9;  // 0.4 rad
-// a * b + c as the three-operand FP64 FMA: left to itself the compiler takes
-// the two-operand form, whose accumulator is the addend, and copies each
-// loop-invariant series coefficient into it first.
+constexpr double kChordMaxDeg = 22.9;       // 0.4 rad
+constexpr double kChordShortMaxDeg = 11.3;  // 0.197 rad
+// a * b + c as the three-operand FP64 FMA with c in scalar registers: left to
+// itself the compiler takes the two-operand form, whose accumulator is the
+// addend, and copies each loop-invariant series coefficient into it first.
 __device__ __forceinline__ double fma3(double a, double b, double c) {
   double r;
-  asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "s"(c));
   return r;
 }
+template <int kTerms>
 __device__ __forceinline__ double chord_angle(double dx, double dy, double dz) {
+  static_assert(kTerms == 13 || kTerms == 8, "13 or 8 terms");
   const double c2 = fma(dx, dx, fma(dy, dy, dz * dz));
   const double x2 = 0.25 * c2;
-  double s = 0.006447210311889649;         // 676039 / 104857600
-  s = fma3(s, x2, 0.0073125258735988454);  // 88179 / 12058624
-  s = fma3(s, x2, 0.008390335809616815);   // 46189 / 5505024
-  s = fma3(s, x2, 0.009761609529194078);   // 12155 / 1245184
-  s = fma3(s, x2, 0.011551800896139705);   // 6435 / 557056
-  s = fma3(s, x2, 0.01396484375);          // 143 / 10240
-  s = fma3(s, x2, 0.017352764423076924);   // 231 / 13312
-  s = fma3(s, x2, 0.022372159090909092);   // 63 / 2816
-  s = fma3(s, x2, 0.030381944444444444);   // 35 / 1152
-  s = fma3(s, x2, 0.044642857142857144);   // 5 / 112
-  s = fma3(s, x2, 0.075);                  // 3 / 40
-  s = fma3(s, x2, 0.16666666666666666);    // 1 / 6
+  double s;
+  if constexpr (kTerms == 13) {
+    s = fma(0.006447210311889649,          // 676039 / 104857600
+            x2, 0.0073125258735988454);    // 88179 / 12058624
+    s = fma3(s, x2, 0.008390335809616815);   // 46189 / 5505024
+    s = fma3(s, x2, 0.009761609529194078);   // 12155 / 1245184
+    s = fma3(s, x2, 0.011551800896139705);   // 6435 / 557056
+    s = fma3(s, x2, 0.01396484375);          // 143 / 10240
+    s = fma3(s, x2, 0.017352764423076924);   // 231 / 13312
+  } else {
+    s = fma(0.01396484375,                 // 143 / 10240
+            x2, 0.017352764423076924);     // 231 / 13312
+  }
+  s = fma3(s, x2, 0.022372159090909092);     // 63 / 2816
+  s = fma3(s, x2, 0.030381944444444444);     // 35 / 1152
+  s = fma3(s, x2, 0.044642857142857144);     // 5 / 112
+  s = fma3(s, x2, 0.075);                    // 3 / 40
+  s = fma3(s, x2, 0.16666666666666666);      // 1 / 6
   s = fma(s, x2, 1.0);
   const double c = c2 > 0.0 ? c2 * rsq_nr(c2) : 0.0;  // |d|
-  return c * s;                           // 2 x s
+  return c * s;                              // 2 x s
 }
 
 // gc_weight for a candidate surely inside the support (r <= 2): the same
@@ -317,16 +327,6 @@ __device__ __forceinline__ double gc_weight_inner(double ri, int gc_filter) {
     return fma(fma(rj, ri, -kGc53), rk, 1.0);
   }
   return 1.0;
-}
-
-// The weight of binned obs l surely inside the support (its keep decided by
-// the FP32 test): the Gaspari-Cohn weight of r = a theta / L with theta from
-// the FP64 unit vectors (agrees with the reference formula to about 1e-13,
-// the rounding of its acos near 1).
-__device__ __forceinline__ double inner_weight(const dk_select_args& a, const ColGeo& q, int l,
-                                               double r_per_rad) {
-  const double th = chord_angle(a.obs.ux[l] - q.ux, a.obs.uy[l] - q.uy, a.obs.uz[l] - q.uz);
-  return gc_weight_inner(th * r_per_rad, a.gc_filter);
 }
 
 // The test of a candidate in the shell that the FP32 test leaves open: the
@@ -433,9 +433,11 @@ __device__ __forceinline__ void wave_sync_lds() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// kChord: weights from the chord angle (a support within kChordMaxDeg); else
+// kTerms (fill): weights from the chord angle, its series of 13 or 8 terms
+// by the support (kChordMaxDeg, kChordShortMaxDeg), agreeing with the
+// reference formula to about 1e-13 (the rounding of its acos near 1); 0:
 // from the reference's formula, evaluated for every kept obs.
-template <bool kFill, int TJ, int TI, bool kChord>
+template <bool kFill, int TJ, int TI, int kTerms>
 __global__ __launch_bounds__(256) void k_select_tiles(dk_select_args a) {
   constexpr int kTJ = TJ, kTI = TI, kNTT = TJ * TI;
   static_assert(kNTT <= kNT, "tile larger than its column state");
@@ -524,7 +526,7 @@ __global__ __launch_bounds__(256) void k_select_tiles(dk_select_args a) {
     uint64_t bm = mlive & ~mshell;
     bool keep = live & (lb >= 0);
     double gc;
-    if constexpr (kChord) {
+    if constexpr (kTerms != 0) {
       // each candidate's FP64 unit vector serves the shell test and the
       // weight (binned index < nobs < 2^29: 32-bit byte offsets)
       const uint32_t off = (uint32_t)l << 3;
@@ -536,7 +538,8 @@ __global__ __launch_bounds__(256) void k_select_tiles(dk_select_args a) {
         bm |= ballot(sk);
         keep = keep | sk;
       }
-      gc = gc_weight_inner(chord_angle(px - qx, py - qy, pz - qz) * r_per_rad, a.gc_filter);
+      gc = gc_weight_inner(chord_angle<kTerms>(px - qx, py - qy, pz - qz) * r_per_rad,
+                           a.gc_filter);
     } else {
       if (mshell) {
         const bool sk = live & (lb < 0) && shell_keep(a, tc.ux[j], tc.uy[j], tc.uz[j], l);
@@ -928,30 +931,32 @@ static unsigned tile_blocks(const dk_select_args* a, bool fill, int tj, int ti) 
   return (unsigned)(((ti1 - ti0) * ntj + 3) / 4);
 }
 
-template <bool kFill, bool kChord>
+template <bool kFill, int kTerms>
 static void select_tiles_launch(const dk_select_args* a, hipStream_t s) {
   switch (a->tile) {
     case DK_SELECT_TILE_4X1:
-      k_select_tiles<kFill, 4, 1, kChord><<<tile_blocks(a, kFill, 4, 1), 256, 0, s>>>(*a);
+      k_select_tiles<kFill, 4, 1, kTerms><<<tile_blocks(a, kFill, 4, 1), 256, 0, s>>>(*a);
       break;
     case DK_SELECT_TILE_4X4:
-      k_select_tiles<kFill, 4, 4, kChord><<<tile_blocks(a, kFill, 4, 4), 256, 0, s>>>(*a);
+      k_select_tiles<kFill, 4, 4, kTerms><<<tile_blocks(a, kFill, 4, 4), 256, 0, s>>>(*a);
       break;
     default:
-      k_select_tiles<kFill, 4, 2, kChord><<<tile_blocks(a, kFill, 4, 2), 256, 0, s>>>(*a);
+      k_select_tiles<kFill, 4, 2, kTerms><<<tile_blocks(a, kFill, 4, 2), 256, 0, s>>>(*a);
       break;
   }
 }
 
 // The count pass needs no weights; the fill takes them from the chord angle
-// when the support is within its range.
+// when the support is within its range, with the shorter series when that
+// suffices.
 template <bool kFill>
 static int select_tiles(const dk_select_args* a, hipStream_t s) {
   if constexpr (!kFill) {
-    select_tiles_launch<false, true>(a, s);
+    select_tiles_launch<false, 13>(a, s);
   } else {
-    if (a->theta_deg <= kChordMaxDeg) select_tiles_launch<true, true>(a, s);
-    else select_tiles_launch<true, false>(a, s);
+    if (a->theta_deg <= kChordShortMaxDeg) select_tiles_launch<true, 8>(a, s);
+    else if (a->theta_deg <= kChordMaxDeg) select_tiles_launch<true, 13>(a, s);
+    else select_tiles_launch<true, 0>(a, s);
   }
   return (int)hipGetLastError();
 }

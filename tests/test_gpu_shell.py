@@ -7,7 +7,7 @@ test leaves open and that the FP64 unit-vector cosine decides against
 cos(2L/a).  The device lists must equal the oracle's (the reference formula,
 letkf_yb.F90:84-96 with latlon_dist_r8, pro_info.F90:157) entry for entry,
 with weights within 1e-12, at supports inside the chord-angle range (L =
-1000 km) and beyond it (L = 1500 km).
+600 km, the shorter series, and 1000 km) and beyond it (L = 1500 km).
 """
 import numpy as np
 import pytest
@@ -30,7 +30,7 @@ def _ring(clat, clon, theta, beta):
     return x2 / s, np.mod(y2 / s, 360.0)
 
 
-@pytest.mark.parametrize("local_dist", [1000.0, 1500.0])
+@pytest.mark.parametrize("local_dist", [600.0, 1000.0, 1500.0])
 def test_selection_at_the_support_edge_matches_oracle(cuda, dl, oracle, local_dist):
     torch = cuda
     cfg = dl.synth_cfg(18, 36, 2, 16, 3200, seed=0xD1DA0030)

@@ -62,13 +62,18 @@ def _check_selection(oracle, dbg, perm, bin_start, nband, nlonbin, ob, prm, bin_
         assert np.abs(dbg["csr_w"][off:off + p] - w).max(initial=0.0) <= 1e-12, col
 
 
-@pytest.mark.parametrize("case", ["uniform", "clustered_capped", "k128"])
+@pytest.mark.parametrize("case", ["uniform", "clustered_capped", "k128", "short_support"])
 def test_sweep_matches_oracle(cuda, dl, oracle, case):
     torch = cuda
     bin_deg = 3.0
     if case == "uniform":
         cfg = dl.synth_cfg(36, 72, 4, 20, 2500, seed=0xD1DA0000)
         over = dict(local_dist=1000.0, inflation_factor=1.1, bin_deg=bin_deg)
+    elif case == "short_support":
+        # 2L/a below 0.197 rad: the weights' shorter arcsine series
+        cfg = dl.synth_cfg(36, 72, 3, 16, 4000, seed=0xD1DA0005, cluster_frac=0.5,
+                           nclusters=2, cluster_radius_km=500.0)
+        over = dict(local_dist=600.0, inflation_factor=1.0, bin_deg=bin_deg)
     elif case == "k128":
         cfg = dl.synth_cfg(12, 24, 2, 128, 1500, seed=0xD1DA0004, cluster_frac=0.5,
                            nclusters=2, cluster_radius_km=800.0, rscale=(0.5, 3.0))
