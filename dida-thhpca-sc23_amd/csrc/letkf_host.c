@@ -335,6 +335,10 @@ int dida_letkf_create(dida_letkf_ctx **out, const dida_letkf_params *prm, int32_
     int rc = check_params(prm);
     if (rc) return rc;
     if (nobs_max < 0 || sel_capacity < 0) return set_err(DIDA_ERR_ARG, "negative size");
+    /* the selection's fill addresses the binned obs with 32-bit byte offsets */
+    if (nobs_max >= (1 << 29))
+        return set_err(DIDA_ERR_UNSUPPORTED, "nobs_max = %d: at most %d obs", (int)nobs_max,
+                       (1 << 29) - 1);
     dida_letkf_ctx *c = (dida_letkf_ctx *)calloc(1, sizeof *c);
     if (!c) return set_err(DIDA_ERR_NOMEM, "calloc");
     c->prm = *prm;

@@ -225,9 +225,9 @@ int dida_letkf_core_batch_dev_noalloc(int32_t batch, int32_t max_obs, int32_t na
  * update for every column of the grid, the per-cube loop of
  * letkf_compute_nooverlap (letkf_compute.F90:12-126) with cube_n = 1.
  *
- * nobs_max bounds the observation count of later sweeps (for k <= 64 the
- * binned rows, nobs_max x 32 or 64 values of 4 or 8 bytes, must stay below
- * 4 GiB, else DIDA_ERR_UNSUPPORTED); sel_capacity is the
+ * nobs_max bounds the observation count of later sweeps (below 2^29; for
+ * k <= 64 the binned rows, nobs_max x 32 or 64 values of 4 or 8 bytes, must
+ * also stay below 4 GiB; else DIDA_ERR_UNSUPPORTED); sel_capacity is the
  * number of (column, obs) selection entries to reserve (0: sized by the
  * sweeps).  Every eager sweep reads its number of selection entries back once
  * (one stream synchronisation, after the count pass).  Sized by the sweeps,

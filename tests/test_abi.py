@@ -57,6 +57,10 @@ def test_params_default_and_errors(dl):
                             gram=dl.GRAM_BF16X3, vert_dist=0.5)
     rc = dl.lib().dida_letkf_create(ctypes.byref(h), ctypes.byref(bad), 10, 0)
     assert rc == -5 and b"vertical" in dl.lib().dida_letkf_last_error()
+    # the selection addresses the binned obs with 32-bit byte offsets
+    ok = dl.default_params(nanals=8, nrow=3, nlat=2, nlon=2)
+    rc = dl.lib().dida_letkf_create(ctypes.byref(h), ctypes.byref(ok), 1 << 29, 0)
+    assert rc == -5 and b"at most 536870911 obs" in dl.lib().dida_letkf_last_error()
 
 
 def test_struct_layouts_match_header(dl, tmp_path):
