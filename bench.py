@@ -208,6 +208,32 @@ def traffic_key(head, config, scale, layout):
     return key
 
 
+def time_host_core(dl, cfg, batch=512, p=2100, reps=3):
+    """letkf_core_batch_ (the reference's batched core, host arrays in and out)
+    on a batch of analyses of the workload's k and its mean local obs count,
+    timed on the host clock: the copies of hxens, R^-1, d and the weights to
+    the device and of the transforms back are inside the call, so this is the
+    rate a caller holding host arrays sees (synthetic values)."""
+    import time
+    import numpy as np
+    k = cfg.k
+    rng = np.random.default_rng(11)
+    hx = rng.standard_normal((batch, p, k))
+    ri = rng.uniform(0.5, 2.0, (batch, p)).astype(np.float32)
+    dep = rng.standard_normal((batch, p))
+    rl = rng.uniform(0.0, 1.0, (batch, p))
+    nb = np.full(batch, p, dtype=np.int32)
+    dl.letkf_core_batch(nb, hx, ri, dep, rl, 1.0)  # warm-up (device buffers)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dl.letkf_core_batch(nb, hx, ri, dep, rl, 1.0)
+    ms = (time.perf_counter() - t0) / reps * 1e3
+    nbytes = batch * p * (8 * k + 4 + 8 + 8) + batch * (k * k + k) * 8
+    return {"batch": batch, "nobsl": p, "k": k, "ms_per_call": ms,
+            "analyses_per_s": batch / (ms / 1e3), "host_bytes_per_call": nbytes,
+            "host_GBps": nbytes / (ms / 1e3) / 1e9}
+
+
 def time_obs_operator(dl, torch, data, cfg, reps=5):
     """H(Xb) + centring for one record per synthetic obs position (one level,
     the temperature variable, sigma levels), timed with HIP events.  Bytes
@@ -610,6 +636,8 @@ def main():
                     help="pin the batched forms' layout instead of autotuning: JSON with "
                          "chain_len, xcd_tiles, bin_deg and (per-bucket chains) "
                          "bucket_chain, as the autotuner reports it")
+    ap.add_argument("--no-host-core", action="store_true",
+                    help="skip timing letkf_core_batch_ on host arrays")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device copy (counter scale for profiles)")
     args = ap.parse_args()
@@ -760,10 +788,18 @@ def main():
         log("timing twelve regional domains in one graph")
         domains = time_domains(dl, torch)
     obsop = None
+    host_core = None
     if not args.state_f32:
         log("timing the observation operator")
         obsop = time_obs_operator(dl, torch, data, cfg)
         log(f"  {obsop}")
+        if not args.no_host_core:
+            log("timing letkf_core_batch_ on host arrays")
+            try:
+                host_core = time_host_core(dl, cfg)
+            except Exception as exc:  # the headline line must still print
+                host_core = {"error": str(exc)}
+            log(f"  {host_core}")
     line = {
         "metric": METRIC, "value": value, "unit": "local analyses/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
@@ -790,7 +826,8 @@ def main():
                    "oracle_sample_columns": len(cpu_cols) if cpu_cols is not None else None,
                    "speedup_vs_plain_fp64": plain_ms / r["ms"] if plain_ms else None},
         "roofline": roofline, "cpu_baseline": cpu, "forms": out_forms,
-        "obs_operator": obsop, "vertical_localisation": vertical, "horizontal_cubes": cubes,
+        "obs_operator": obsop, "host_core_batch": host_core,
+        "vertical_localisation": vertical, "horizontal_cubes": cubes,
         "coupler_layout": struct, "regional_domains": domains,
     }
     print(json.dumps(line), flush=True)
