@@ -159,7 +159,7 @@ struct dida_letkf_ctx {
     double theta_deg, cos_far, cos_near, cos_keep;
     /* binning */
     uint32_t *key, *key_sorted;
-    int32_t *iota, *perm, *bin_count, *bin_start;
+    int32_t *iota, *perm, *bin_start;
     void *sort_tmp;
     size_t sort_tmp_bytes;
     double *cx, *sx, *yr, *yb_b, *dep_b;
@@ -253,7 +253,7 @@ int dida_letkf_destroy(dida_letkf_ctx *c)
         if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
-    void *ptrs[] = {c->key, c->key_sorted, c->iota, c->perm, c->bin_count, c->bin_start,
+    void *ptrs[] = {c->key, c->key_sorted, c->iota, c->perm, c->bin_start,
                     c->sort_tmp, c->cx, c->sx, c->yr, c->u4, c->u3, c->iperm, c->yb_b,
                     c->yb32_b, c->dep_b, c->rinv_b,
                     c->nobsl, c->thr, c->offset, c->scan_tmp, c->csr_idx, c->csr_w, c->flags,
@@ -393,7 +393,6 @@ int dida_letkf_create(dida_letkf_ctx **out, const dida_letkf_params *prm, int32_
     DMALLOC(c->key_sorted, no * sizeof(uint32_t));
     DMALLOC(c->iota, no * sizeof(int32_t));
     DMALLOC(c->perm, no * sizeof(int32_t));
-    DMALLOC(c->bin_count, (size_t)(c->nbins_max + 1) * sizeof(int32_t));
     DMALLOC(c->bin_start, (size_t)(c->nbins_max + 2) * sizeof(int32_t));
     c->sort_tmp_bytes = 0;
     if (dk_sort_pairs_u32(NULL, &c->sort_tmp_bytes, c->key, c->key_sorted, c->iota, c->perm,
@@ -650,15 +649,14 @@ static int sweep_select_count(dida_letkf_ctx *c, const dida_letkf_obs *o, hipStr
     c->last_nobs_pending = o->nobs;
     c->nmark = 0;
     tmark(c, -1, s);
-    HIP_TRY(dk_fill_i32(c->bin_count, 0, c->geom.nbins + 1, s));
     HIP_TRY(dk_fill_i32(c->flags, 0, 4, s));
-    HIP_TRY(dk_bin_keys(o->nobs, o->lat, o->lon, c->geom, c->key, c->iota, c->bin_count, s));
+    HIP_TRY(dk_bin_keys(o->nobs, o->lat, o->lon, c->geom, c->key, c->iota, s));
     if (o->nobs > 0) {
         size_t tb = c->sort_tmp_bytes;
         HIP_TRY(dk_sort_pairs_u32(c->sort_tmp, &tb, c->key, c->key_sorted, c->iota, c->perm,
                                   o->nobs, c->end_bit, s));
     }
-    HIP_TRY(dk_bin_scan(c->geom.nbins, c->bin_count, c->bin_start, s));
+    HIP_TRY(dk_bin_starts(c->geom.nbins, o->nobs, c->key_sorted, c->bin_start, s));
     HIP_TRY(dk_obs_gather(o->nobs, k, c->ldy, c->perm, o->lat, o->lon, o->yb, o->dep, o->rinv,
                           c->cx, c->sx, c->yr, c->u4, c->u3, c->u3 + c->nobs_max,
                           c->u3 + 2 * c->nobs_max, c->iperm, c->yb_b, c->yb32_b, c->dep_b,

@@ -86,14 +86,16 @@ typedef struct dk_select_args {
 /* Fill n int32 with v (a kernel, so that captured graphs hold no memset). */
 int dk_fill_i32(int32_t *p, int32_t v, int64_t n, void *stream);
 
-/* Stage 1: keys, counts per bin and identity values for the sort. */
+/* Stage 1: keys and identity values for the sort. */
 int dk_bin_keys(int32_t nobs, const double *lat, const double *lon, dk_bin_geom g,
-                uint32_t *key, int32_t *iota, int32_t *bin_count, void *stream);
+                uint32_t *key, int32_t *iota, void *stream);
 /* Stable radix sort of (key, iota) by key.  tmp == NULL: *tmp_bytes is set. */
 int dk_sort_pairs_u32(void *tmp, size_t *tmp_bytes, const uint32_t *kin, uint32_t *kout,
                       const int32_t *vin, int32_t *vout, int32_t n, int end_bit, void *stream);
-/* Exclusive scan of nbins + 1 counts into bin_start[nbins + 2] (one block). */
-int dk_bin_scan(int32_t nbins, const int32_t *count, int32_t *bin_start, void *stream);
+/* bin_start[b] = number of obs with key < b, b = 0 .. nbins + 1, from the
+ * nobs sorted keys (unread when nobs is 0). */
+int dk_bin_starts(int32_t nbins, int32_t nobs, const uint32_t *key_sorted, int32_t *bin_start,
+                  void *stream);
 /* Gather obs into binned order and precompute the distance factors.  yb
  * rows are copied into rows of ldy >= k values, zero past k: in FP64 into
  * yb_b and/or rounded to FP32 into yb32_b (either may be NULL). */

@@ -41,8 +41,7 @@ __global__ void k_fill_i32(int32_t* __restrict__ p, int32_t v, int64_t n) {
 
 __global__ void k_bin_keys(int32_t nobs, const double* __restrict__ lat,
                            const double* __restrict__ lon, dk_bin_geom g,
-                           uint32_t* __restrict__ key, int32_t* __restrict__ iota,
-                           int32_t* __restrict__ bin_count) {
+                           uint32_t* __restrict__ key, int32_t* __restrict__ iota) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nobs) return;
   double la = lat[i];
@@ -58,34 +57,22 @@ __global__ void k_bin_keys(int32_t nobs, const double* __restrict__ lat,
   }
   key[i] = kk;
   iota[i] = i;
-  atomicAdd(&bin_count[kk], 1);
 }
 
-// One block of 1024 threads scans nbins + 1 counts in chunks.
-__global__ __launch_bounds__(1024) void k_bin_scan(int32_t nbins, const int32_t* __restrict__ cnt,
-                                                   int32_t* __restrict__ start) {
-  __shared__ int32_t s[1024];
-  __shared__ int32_t carry;
-  if (threadIdx.x == 0) carry = 0;
-  __syncthreads();
-  const int n = nbins + 1;
-  for (int base = 0; base < n; base += 1024) {
-    int i = base + threadIdx.x;
-    int v = i < n ? cnt[i] : 0;
-    s[threadIdx.x] = v;
-    __syncthreads();
-    for (int d = 1; d < 1024; d <<= 1) {
-      int t = (int)threadIdx.x >= d ? s[threadIdx.x - d] : 0;
-      __syncthreads();
-      s[threadIdx.x] += t;
-      __syncthreads();
-    }
-    if (i < n) start[i] = carry + s[threadIdx.x] - v;
-    __syncthreads();
-    if (threadIdx.x == 1023) carry += s[1023];
-    __syncthreads();
+// bin_start[b] = the number of obs whose key is below b, b = 0 .. nbins + 1
+// (bin nbins holds the obs beyond obs_polar): a lower-bound search of the
+// sorted keys, one thread a bin.
+__global__ void k_bin_starts(int32_t nbins, int32_t nobs, const uint32_t* __restrict__ key_sorted,
+                             int32_t* __restrict__ bin_start) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b > nbins + 1) return;
+  int lo = 0, hi = nobs;
+  while (lo < hi) {
+    const int mid = lo + ((hi - lo) >> 1);
+    if (key_sorted[mid] < (uint32_t)b) lo = mid + 1;
+    else hi = mid;
   }
-  if (threadIdx.x == 0) start[n] = carry;
+  bin_start[b] = lo;
 }
 
 __global__ void k_obs_gather(int32_t nobs, const int32_t* __restrict__ perm,
@@ -114,19 +101,20 @@ __global__ void k_obs_gather(int32_t nobs, const int32_t* __restrict__ perm,
   rinv_b[i] = rinv[s];
 }
 
-// Binned rows of ldy >= k values, zero past k.
 // Yb rows in bin order (row stride ldy, zero past k), in FP64 or rounded to
-// FP32 (the half-precision Gram of the chained form splits FP32 values).
+// FP32 (the half-precision Gram of the chained form splits FP32 values): a
+// wavefront a row, four rows a workgroup.
 template <typename T>
-__global__ void k_obs_gather_yb(int32_t nobs, int32_t k, int32_t ldy,
-                                const int32_t* __restrict__ perm, const double* __restrict__ yb,
-                                T* __restrict__ yb_b) {
-  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int64_t total = (int64_t)nobs * ldy;
-  if (t >= total) return;
-  int64_t i = t / ldy;
-  int e = (int)(t - i * ldy);
-  yb_b[t] = e < k ? (T)yb[(int64_t)perm[i] * k + e] : (T)0;
+__global__ __launch_bounds__(256) void k_obs_gather_yb(int32_t nobs, int32_t k, int32_t ldy,
+                                                       const int32_t* __restrict__ perm,
+                                                       const double* __restrict__ yb,
+                                                       T* __restrict__ yb_b) {
+  const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= nobs) return;
+  const int lane = threadIdx.x & 63;
+  const double* __restrict__ src = yb + (int64_t)perm[i] * k;
+  T* __restrict__ dst = yb_b + i * ldy;
+  for (int e = lane; e < ldy; e += kWave) dst[e] = e < k ? (T)src[e] : (T)0;
 }
 
 template <typename T>
@@ -866,10 +854,9 @@ extern "C" int dk_fill_i32(int32_t* p, int32_t v, int64_t n, void* stream) {
 }
 
 extern "C" int dk_bin_keys(int32_t nobs, const double* lat, const double* lon, dk_bin_geom g,
-                           uint32_t* key, int32_t* iota, int32_t* bin_count, void* stream) {
+                           uint32_t* key, int32_t* iota, void* stream) {
   if (nobs <= 0) return 0;
-  k_bin_keys<<<(nobs + 255) / 256, 256, 0, (hipStream_t)stream>>>(nobs, lat, lon, g, key, iota,
-                                                                   bin_count);
+  k_bin_keys<<<(nobs + 255) / 256, 256, 0, (hipStream_t)stream>>>(nobs, lat, lon, g, key, iota);
   return (int)hipGetLastError();
 }
 
@@ -880,9 +867,11 @@ extern "C" int dk_sort_pairs_u32(void* tmp, size_t* tmp_bytes, const uint32_t* k
                                         end_bit, (hipStream_t)stream, false);
 }
 
-extern "C" int dk_bin_scan(int32_t nbins, const int32_t* count, int32_t* bin_start,
-                           void* stream) {
-  k_bin_scan<<<1, 1024, 0, (hipStream_t)stream>>>(nbins, count, bin_start);
+extern "C" int dk_bin_starts(int32_t nbins, int32_t nobs, const uint32_t* key_sorted,
+                             int32_t* bin_start, void* stream) {
+  if (nbins < 0 || nobs < 0) return (int)hipErrorInvalidValue;
+  k_bin_starts<<<(unsigned)((nbins + 2 + 255) / 256), 256, 0, (hipStream_t)stream>>>(
+      nbins, nobs, key_sorted, bin_start);
   return (int)hipGetLastError();
 }
 
@@ -896,8 +885,7 @@ extern "C" int dk_obs_gather(int32_t nobs, int32_t k, int32_t ldy, const int32_t
   hipStream_t s = (hipStream_t)stream;
   k_obs_gather<<<(nobs + 255) / 256, 256, 0, s>>>(nobs, perm, lat, lon, dep, rinv, cx, sx, yr,
                                                   (float4*)u4, ux, uy, uz, iperm, dep_b, rinv_b);
-  const int64_t total = (int64_t)nobs * ldy;
-  const unsigned blocks = (unsigned)((total + 255) / 256);
+  const unsigned blocks = (unsigned)((nobs + 3) / 4);
   if (yb_b) k_obs_gather_yb<double><<<blocks, 256, 0, s>>>(nobs, k, ldy, perm, yb, yb_b);
   if (yb32_b) k_obs_gather_yb<float><<<blocks, 256, 0, s>>>(nobs, k, ldy, perm, yb, yb32_b);
   return (int)hipGetLastError();
